@@ -1,0 +1,175 @@
+// axis_hip.hpp -- C++20 host API of the MI355X AXIS regrid library.
+//
+// Mirrors the reference's solver API (libs/axis):
+//   WeightGenerator::generate(src, dst, RegridConfig) -> InterpolationMatrix
+//                                           weight_generator.hpp:60-62
+//   InterpolationMatrix (COO views, CSR row_ptr/col_idx/values, frac/area)
+//                                           interpolation_matrix.hpp:61-372
+//   apply / batch_apply (layout_left fields) apply.hpp:61-119, 275-343
+//   check_conservation                      conservation.hpp:95-97
+// with the same argument meaning and error behaviour (InvalidArgument thrown
+// before any output is written, RuntimeError for unmapped cells / HIP
+// failures).  Meshes and matrices live in HBM; host arrays are copied in once
+// at construction.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace axis_hip {
+
+// regrid_config.hpp:24-89
+enum class InterpolationMethod : std::uint8_t { Bilinear, NearestNeighbor, Bicubic, Patch, Conservative1stOrder, Conservative2ndOrder };
+enum class NormType : std::uint8_t { DstArea, FracArea };
+enum class LineType : std::uint8_t { Cartesian, GreatCircle };
+enum class UnmappedAction : std::uint8_t { Error, Ignore };
+enum class ExtrapolationAction : std::uint8_t { None, NearestWet };
+// enums.hpp:73-80
+enum class CoordinateSystem : std::uint8_t { SphericalDeg, SphericalRad, Cartesian3D };
+
+struct RegridConfig {
+    InterpolationMethod method = InterpolationMethod::Bilinear;
+    NormType norm_type = NormType::DstArea;
+    LineType line_type = LineType::GreatCircle;
+    UnmappedAction unmapped = UnmappedAction::Ignore;
+    bool use_limiter = false;
+    ExtrapolationAction extrap_method = ExtrapolationAction::NearestWet;
+};
+
+/// UnstructuredMesh (unstructured_mesh.hpp:41-220): node_coords [n_nodes, 2]
+/// layout_left (lon block, then lat block), CSR cell->node connectivity,
+/// optional per-cell areas and mask.  A host copy is kept for the few
+/// libm-exact host steps; the device copy feeds the kernels.
+class Mesh {
+   public:
+    Mesh(const double *node_coords, index_t n_nodes, index_t ndim, const index_t *conn_offsets, index_t n_cells, const index_t *conn_indices,
+         CoordinateSystem cs, const double *areas, const std::int32_t *mask);
+    index_t n_nodes() const { return n_nodes_; }
+    index_t n_cells() const { return n_cells_; }
+    CoordinateSystem coord_system() const { return cs_; }
+    bool has_areas() const { return !h_areas.empty(); }
+    bool has_mask() const { return !h_mask.empty(); }
+    int max_cell_vertices() const { return max_nv_; }
+
+    // host copies
+    std::vector<double> h_coords;  // [2 * n_nodes] lon block then lat block
+    std::vector<index_t> h_offsets, h_indices;
+    std::vector<double> h_areas;
+    std::vector<std::int32_t> h_mask;
+    // device copies
+    DeviceBuffer<double> d_coords;
+    DeviceBuffer<index_t> d_offsets, d_indices;
+    DeviceBuffer<double> d_areas;
+    DeviceBuffer<std::int32_t> d_mask;
+
+   private:
+    index_t n_nodes_ = 0, n_cells_ = 0;
+    CoordinateSystem cs_ = CoordinateSystem::SphericalDeg;
+    int max_nv_ = 0;
+};
+
+/// InterpolationMatrix, always held in CSR form sorted by (row, col)
+/// (the layout to_csr() produces, interpolation_matrix.hpp:180-255), in HBM.
+class InterpolationMatrix {
+   public:
+    index_t n_src = 0, n_dst = 0, nnz = 0;
+    DeviceBuffer<index_t> row_ptr;   // [n_dst + 1]
+    DeviceBuffer<index_t> col;       // [nnz]
+    DeviceBuffer<double> val;        // [nnz]
+    DeviceBuffer<std::int32_t> col32;  // [nnz] int32 column copy streamed by the apply kernels
+    DeviceBuffer<float> val_hi, val_lo;  // [nnz] FP64 weights split into FP32 hi + lo
+    DeviceBuffer<double> frac_a, area_a;  // [n_src]
+    DeviceBuffer<double> frac_b, area_b;  // [n_dst]
+    int max_row_nnz = 0;
+    int n_long_rows = 0;                 // rows the apply hands to the wave-per-row kernel
+    DeviceBuffer<std::int32_t> long_rows;
+    // diagnostics of the generation that produced it
+    index_t n_candidates = 0, n_host_arbitrated = 0, n_cap_borderline = 0;
+    double gen_ms = 0.0;
+
+    /// Builds the FP32 hi/lo split and int32 columns used by the mixed-precision apply.
+    void prepare_apply(hipStream_t s);
+};
+
+struct ConservationReport {
+    double src_integral = 0, dst_integral = 0, absolute_error = 0, relative_error = 0;
+};
+
+enum class FieldType : int { F64 = 0, F32 = 1, BF16 = 2 };
+enum class ApplyMode : int { Plain = 0, Mixed = 1 };
+
+class WeightGenerator {
+   public:
+    static std::unique_ptr<InterpolationMatrix> generate(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, hipStream_t s = nullptr);
+};
+
+/// Builds a matrix from COO triplets (the reference's InterpolationMatrix
+/// constructor followed by to_csr()); duplicates are kept.
+std::unique_ptr<InterpolationMatrix> matrix_from_coo(const index_t *rows, const index_t *cols, const double *vals, index_t nnz, index_t n_src,
+                                                     index_t n_dst, const double *frac_a, const double *frac_b, const double *area_a,
+                                                     const double *area_b);
+
+/// Y = A X for device-resident layout_left fields X [n_src, k], Y [n_dst, k].
+void batch_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t n_src, index_t n_dst, index_t k,
+                 ApplyMode mode, hipStream_t s);
+
+ConservationReport check_conservation(const InterpolationMatrix &A, const double *d_src, const double *d_dst, NormType norm, hipStream_t s);
+
+/// Times the candidate launch shapes (fields per lane x rows per lane) of
+/// batch_apply on the given device fields and caches the fastest for this
+/// (n_dst, nnz, k, types, mode).  Returns the chosen field tile and rows per lane.
+std::pair<int, int> autotune_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t k, ApplyMode mode,
+                                   hipStream_t s);
+
+// ---------------------------------------------------------------- internal
+namespace detail {
+
+/// Per-mesh data derived inside one generate() call (never cached across calls).
+struct MeshGeom {
+    index_t n_cells = 0, n_nodes = 0;
+    std::vector<double> h_xyz;          // node xyz [3 * n_nodes] (libm, reference bits)
+    DeviceBuffer<double> d_xyz;         // same on the device
+    DeviceBuffer<double> d_area;        // [n_cells]
+    DeviceBuffer<std::int32_t> d_flags; // bit0 degenerate, bit1 masked, bit2 area<=0
+    DeviceBuffer<double> d_cap;         // [4 * n_cells] centroid xyz + angular radius
+    DeviceBuffer<float> d_box;          // [6 * n_cells] float AABB lo xyz, hi xyz
+    // regular-grid node trig cache (trig_cache.hpp:179-267)
+    bool regular = false;
+    index_t ni = 0, nj = 0;
+    double lon_min = 0, delta_lon = 0, lat_min = 0, delta_lat = 0;
+    std::vector<double> h_trig;   // [2*(ni+1) + 2*(nj+1)]: sin_lon, cos_lon, sin_lat, cos_lat
+    DeviceBuffer<double> d_trig;
+};
+
+struct GridAxes {
+    bool all_quads = false;
+    std::vector<double> ulon, ulat;  // tolerance-unique sorted node coordinates
+};
+
+GridAxes unique_axes(const Mesh &m, hipStream_t s);
+bool regular_from_axes(const Mesh &m, const GridAxes &ax, MeshGeom &g);
+bool rectilinear_from_axes(const Mesh &m, const GridAxes &ax);
+
+void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s);
+
+// scan / sort primitives (scan.hip, sort.hip)
+void exclusive_scan_i64(const index_t *in, index_t *out, index_t n, hipStream_t s);  // out[n] = total
+void radix_sort_pairs_u32(std::uint32_t *keys, std::uint32_t *vals, std::size_t n, int bits, hipStream_t s);
+void radix_sort_keys_u64(std::uint64_t *keys, std::size_t n, hipStream_t s);
+void radix_sort_pairs_u64(std::uint64_t *keys, std::uint32_t *vals, std::size_t n, int bits, hipStream_t s);
+
+// LBVH broad phase (lbvh.hip): candidate CSR of src cells whose float boxes
+// intersect each dst cell's box.
+void lbvh_candidates(const float *src_boxes, index_t n_src, const float *dst_boxes, index_t n_dst, DeviceBuffer<index_t> &cand_off,
+                     DeviceBuffer<std::int32_t> &cand_src, hipStream_t s);
+
+std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, hipStream_t s);
+std::unique_ptr<InterpolationMatrix> generate_bilinear_rect(const Mesh &src, const Mesh &dst, const GridAxes &src_axes, hipStream_t s);
+void check_longitude_ranges(const Mesh &src, const Mesh &dst, bool structured_fast_path, hipStream_t s);
+
+}  // namespace detail
+}  // namespace axis_hip

@@ -1,0 +1,353 @@
+// c_abi.cpp -- extern "C" boundary of libaxis_hip.so (declared in include/axis_hip.h).
+//
+// Conventions of the reference interop layer (libs/axis/src/fortran/
+// axis_c_interop.cpp:25-62, handle_registry.hpp:16-100): every entry point
+// catches everything and returns AXIS_SUCCESS (0) / AXIS_ERROR (-1); handles are
+// int tokens from a mutex-guarded registry, token 0 is invalid and tokens are
+// never reused.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <variant>
+
+#include "../../include/axis_hip.h"
+#include "axis_hip.hpp"
+
+namespace {
+
+using namespace axis_hip;
+
+thread_local std::string g_last_error;
+
+struct Registry {
+    std::mutex mu;
+    int next = 1;
+    std::unordered_map<int, std::variant<std::shared_ptr<Mesh>, std::shared_ptr<InterpolationMatrix>>> items;
+    static Registry &get() {
+        static Registry r;
+        return r;
+    }
+    template <class T>
+    int add(std::shared_ptr<T> p) {
+        std::lock_guard<std::mutex> g(mu);
+        int h = next++;
+        items.emplace(h, std::move(p));
+        return h;
+    }
+    template <class T>
+    std::shared_ptr<T> find(int h) {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = items.find(h);
+        if (it == items.end()) throw InvalidArgument("invalid handle " + std::to_string(h));
+        auto *p = std::get_if<std::shared_ptr<T>>(&it->second);
+        if (!p) throw InvalidArgument("handle " + std::to_string(h) + " has the wrong kind");
+        return *p;
+    }
+    template <class T>
+    void release(int h) {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = items.find(h);
+        if (it == items.end() || !std::holds_alternative<std::shared_ptr<T>>(it->second))
+            throw InvalidArgument("invalid handle " + std::to_string(h));
+        items.erase(it);
+    }
+};
+
+template <class F>
+int guarded(F &&f) {
+    try {
+        f();
+        return AXIS_SUCCESS;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+    } catch (...) {
+        g_last_error = "unknown error";
+    }
+    return AXIS_ERROR;
+}
+
+bool is_device_ptr(const void *p) {
+    if (!p) return false;
+    hipPointerAttribute_t attr{};
+    hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();  // plain pageable host memory
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice;
+}
+
+// Stages a host array into device memory when needed.
+template <class T>
+struct DevView {
+    const T *ptr = nullptr;
+    DeviceBuffer<T> own;
+    DevView(const T *p, index_t n, hipStream_t s) {
+        if (is_device_ptr(p) || n == 0) {
+            ptr = p;
+        } else {
+            own.resize(n);
+            own.upload(p, n, s);
+            ptr = own.get();
+        }
+    }
+};
+
+FieldType ftype(int code) {
+    switch (code) {
+        case AXIS_F64: return FieldType::F64;
+        case AXIS_F32: return FieldType::F32;
+        case AXIS_BF16: return FieldType::BF16;
+        default: throw InvalidArgument("unknown field dtype code " + std::to_string(code));
+    }
+}
+
+ApplyMode amode(int code) {
+    if (code == AXIS_APPLY_PLAIN) return ApplyMode::Plain;
+    if (code == AXIS_APPLY_MIXED) return ApplyMode::Mixed;
+    throw InvalidArgument("unknown apply mode " + std::to_string(code));
+}
+
+template <class T>
+void copy_out(T *dst, const DeviceBuffer<T> &src, index_t n) {
+    if (!dst || n <= 0) return;
+    AXIS_HIP_CHECK(hipMemcpy(dst, src.get(), sizeof(T) * n, hipMemcpyDefault));
+}
+
+void check_device() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        throw RuntimeError("no HIP device available: libaxis_hip runs on the GPU only");
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int axis_init_c(void) {
+    return guarded([] { check_device(); });
+}
+
+const char *axis_last_error_c(void) { return g_last_error.c_str(); }
+
+int axis_mesh_create_c(const double *node_coords, int64_t n_nodes, int64_t ndim, const int64_t *conn_offsets, int64_t n_cells,
+                       const int64_t *conn_indices, int coord_sys, const double *cell_areas, const int32_t *cell_mask, int *mesh_handle) {
+    return guarded([&] {
+        if (!mesh_handle) throw InvalidArgument("null mesh_handle");
+        if (coord_sys < 0 || coord_sys > 2) throw InvalidArgument("unknown coordinate system");
+        check_device();
+        auto m = std::make_shared<Mesh>(node_coords, n_nodes, ndim, conn_offsets, n_cells, conn_indices, static_cast<CoordinateSystem>(coord_sys),
+                                        cell_areas, cell_mask);
+        *mesh_handle = Registry::get().add(std::move(m));
+    });
+}
+
+int axis_mesh_regular_latlon_c(double min_x, double max_x, double min_y, double max_y, double r_x, double r_y, int *mesh_handle) {
+    return guarded([&] {
+        if (!mesh_handle) throw InvalidArgument("null mesh_handle");
+        // validate_bbox / validate_resolution, rule_generator.cpp:110-111
+        if (!(max_x > min_x) || !(max_y > min_y)) throw InvalidArgument("RuleGenerator: invalid bounding box");
+        if (!(r_x > 0.0) || !(r_y > 0.0)) throw InvalidArgument("RuleGenerator: resolution must be positive");
+        const auto ni = static_cast<index_t>(std::floor((max_x - min_x) / r_x));
+        const auto nj = static_cast<index_t>(std::floor((max_y - min_y) / r_y));
+        if (ni == 0 || nj == 0) throw InvalidArgument("RuleGenerator: RegularLatLon resolution produces zero cells");
+        const index_t nn = (ni + 1) * (nj + 1), nc = ni * nj;
+        std::vector<double> coords(2 * nn);
+        for (index_t idx = 0; idx < nn; ++idx) {
+            const index_t i = idx % (ni + 1), j = idx / (ni + 1);
+            coords[idx] = min_x + static_cast<double>(i) * r_x;
+            coords[nn + idx] = min_y + static_cast<double>(j) * r_y;
+        }
+        std::vector<index_t> off(nc + 1), ind(4 * nc);
+        for (index_t c = 0; c < nc; ++c) {
+            const index_t i = c % ni, j = c / ni, st = ni + 1;
+            off[c] = 4 * c;
+            ind[4 * c + 0] = j * st + i;
+            ind[4 * c + 1] = j * st + i + 1;
+            ind[4 * c + 2] = (j + 1) * st + i + 1;
+            ind[4 * c + 3] = (j + 1) * st + i;
+        }
+        off[nc] = 4 * nc;
+        check_device();
+        auto m = std::make_shared<Mesh>(coords.data(), nn, 2, off.data(), nc, ind.data(), CoordinateSystem::SphericalDeg, nullptr, nullptr);
+        *mesh_handle = Registry::get().add(std::move(m));
+    });
+}
+
+int axis_mesh_info_c(int mesh_handle, int64_t *n_nodes, int64_t *n_cells) {
+    return guarded([&] {
+        auto m = Registry::get().find<Mesh>(mesh_handle);
+        if (n_nodes) *n_nodes = m->n_nodes();
+        if (n_cells) *n_cells = m->n_cells();
+    });
+}
+
+int axis_generate_weights_config_c(int src_handle, int dst_handle, const axis_regrid_config *config, int *matrix_handle) {
+    return guarded([&] {
+        if (!config || !matrix_handle) throw InvalidArgument("null argument");
+        auto src = Registry::get().find<Mesh>(src_handle);
+        auto dst = Registry::get().find<Mesh>(dst_handle);
+        RegridConfig cfg;
+        if (config->method < 0 || config->method > 5 || config->norm_type < 0 || config->norm_type > 1 || config->line_type < 0 ||
+            config->line_type > 1 || config->unmapped < 0 || config->unmapped > 1 || config->extrap_method < 0 || config->extrap_method > 1)
+            throw InvalidArgument("RegridConfig field out of range");
+        cfg.method = static_cast<InterpolationMethod>(config->method);
+        cfg.norm_type = static_cast<NormType>(config->norm_type);
+        cfg.line_type = static_cast<LineType>(config->line_type);
+        cfg.unmapped = static_cast<UnmappedAction>(config->unmapped);
+        cfg.use_limiter = config->use_limiter != 0;
+        cfg.extrap_method = static_cast<ExtrapolationAction>(config->extrap_method);
+        std::shared_ptr<InterpolationMatrix> M = WeightGenerator::generate(*src, *dst, cfg, nullptr);
+        *matrix_handle = Registry::get().add(std::move(M));
+    });
+}
+
+int axis_generate_weights_c(int src_handle, int dst_handle, int method, int *matrix_handle) {
+    axis_regrid_config cfg{0, 0, 1, 1, 0, 1};
+    switch (method) {  // axis_c_interop.cpp:126-140
+        case AXIS_METHOD_BILINEAR: cfg.method = 0; break;
+        case AXIS_METHOD_NEAREST: cfg.method = 1; break;
+        case AXIS_METHOD_CONSERVATIVE1: cfg.method = 4; break;
+        default: g_last_error = "unknown method code " + std::to_string(method); return AXIS_ERROR;
+    }
+    return axis_generate_weights_config_c(src_handle, dst_handle, &cfg, matrix_handle);
+}
+
+int axis_matrix_from_coo_c(const int64_t *rows, const int64_t *cols, const double *values, int64_t nnz, int64_t n_src, int64_t n_dst,
+                           const double *frac_a, const double *frac_b, const double *area_a, const double *area_b, int *matrix_handle) {
+    return guarded([&] {
+        if (!matrix_handle || (nnz > 0 && (!rows || !cols || !values))) throw InvalidArgument("null argument");
+        check_device();
+        std::shared_ptr<InterpolationMatrix> M = matrix_from_coo(rows, cols, values, nnz, n_src, n_dst, frac_a, frac_b, area_a, area_b);
+        *matrix_handle = Registry::get().add(std::move(M));
+    });
+}
+
+int axis_matrix_to_csr_c(int matrix_handle) {
+    return guarded([&] { (void)Registry::get().find<InterpolationMatrix>(matrix_handle); });
+}
+
+int axis_matrix_info_c(int matrix_handle, int64_t *n_src, int64_t *n_dst, int64_t *nnz) {
+    return guarded([&] {
+        auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
+        if (n_src) *n_src = M->n_src;
+        if (n_dst) *n_dst = M->n_dst;
+        if (nnz) *nnz = M->nnz;
+    });
+}
+
+int axis_matrix_csr_c(int matrix_handle, int64_t *row_ptr, int64_t *col_idx, double *values) {
+    return guarded([&] {
+        auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
+        copy_out(row_ptr, M->row_ptr, M->n_dst + 1);
+        copy_out(col_idx, M->col, M->nnz);
+        copy_out(values, M->val, M->nnz);
+    });
+}
+
+int axis_matrix_fractions_c(int matrix_handle, double *frac_a, double *frac_b, double *area_a, double *area_b) {
+    return guarded([&] {
+        auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
+        copy_out(frac_a, M->frac_a, M->n_src);
+        copy_out(frac_b, M->frac_b, M->n_dst);
+        copy_out(area_a, M->area_a, M->n_src);
+        copy_out(area_b, M->area_b, M->n_dst);
+    });
+}
+
+int axis_matrix_stats_c(int matrix_handle, int64_t *out4, double *gen_ms) {
+    return guarded([&] {
+        auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
+        if (out4) {
+            out4[0] = M->n_candidates;
+            out4[1] = M->n_host_arbitrated;
+            out4[2] = M->n_cap_borderline;
+            out4[3] = M->max_row_nnz;
+        }
+        if (gen_ms) *gen_ms = M->gen_ms;
+    });
+}
+
+int axis_apply_c(int matrix_handle, const double *src, double *dst, int n_src, int n_dst) {
+    return guarded([&] {
+        auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
+        // apply.hpp:63-71: validation before dst is touched
+        if (static_cast<index_t>(n_src) != M->n_src)
+            throw InvalidArgument("axis::solver::apply: src.extent(0) (" + std::to_string(n_src) + ") != matrix.n_src() (" +
+                                  std::to_string(M->n_src) + ")");
+        if (static_cast<index_t>(n_dst) != M->n_dst)
+            throw InvalidArgument("axis::solver::apply: dst.extent(0) (" + std::to_string(n_dst) + ") != matrix.n_dst() (" +
+                                  std::to_string(M->n_dst) + ")");
+        hipStream_t s = nullptr;
+        DevView<double> x(src, n_src, s);
+        const bool dev_out = is_device_ptr(dst);
+        DeviceBuffer<double> ybuf;
+        double *y = dst;
+        if (!dev_out && n_dst > 0) {
+            ybuf.resize(n_dst);
+            y = ybuf.get();
+        }
+        batch_apply(*M, x.ptr, FieldType::F64, y, FieldType::F64, n_src, n_dst, 1, ApplyMode::Plain, s);
+        if (!dev_out && n_dst > 0) ybuf.download(dst, n_dst, s);
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+int axis_batch_apply_async_c(int matrix_handle, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n_src, int64_t n_dst,
+                             int64_t n_fields, int mode, void *stream) {
+    return guarded([&] {
+        auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
+        batch_apply(*M, src, ftype(src_dtype), dst, ftype(dst_dtype), n_src, n_dst, n_fields, amode(mode), static_cast<hipStream_t>(stream));
+    });
+}
+
+int axis_batch_apply_c(int matrix_handle, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n_src, int64_t n_dst,
+                       int64_t n_fields, int mode, void *stream) {
+    return guarded([&] {
+        auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
+        auto s = static_cast<hipStream_t>(stream);
+        batch_apply(*M, src, ftype(src_dtype), dst, ftype(dst_dtype), n_src, n_dst, n_fields, amode(mode), s);
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+int axis_autotune_apply_c(int matrix_handle, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n_fields, int mode, void *stream,
+                          int *fields_per_tile, int *rows_per_lane) {
+    return guarded([&] {
+        auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
+        auto r = autotune_apply(*M, src, ftype(src_dtype), dst, ftype(dst_dtype), n_fields, amode(mode), static_cast<hipStream_t>(stream));
+        if (fields_per_tile) *fields_per_tile = r.first;
+        if (rows_per_lane) *rows_per_lane = r.second;
+    });
+}
+
+int axis_check_conservation_c(int matrix_handle, const double *src, const double *dst, int norm_type, double *out4) {
+    return guarded([&] {
+        if (!out4) throw InvalidArgument("null out4");
+        if (norm_type < 0 || norm_type > 1) throw InvalidArgument("unknown norm type");
+        auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
+        hipStream_t s = nullptr;
+        DevView<double> x(src, M->n_src, s), y(dst, M->n_dst, s);
+        ConservationReport r = check_conservation(*M, x.ptr, y.ptr, static_cast<NormType>(norm_type), s);
+        out4[0] = r.src_integral;
+        out4[1] = r.dst_integral;
+        out4[2] = r.absolute_error;
+        out4[3] = r.relative_error;
+    });
+}
+
+int axis_destroy_mesh_c(int handle) {
+    return guarded([&] { Registry::get().release<Mesh>(handle); });
+}
+
+int axis_destroy_matrix_c(int handle) {
+    return guarded([&] { Registry::get().release<InterpolationMatrix>(handle); });
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+// common.hpp -- HIP plumbing shared by every translation unit of libaxis_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <utility>
+
+namespace axis_hip {
+
+using index_t = std::int64_t;  // reference libs/axis/include/axis/types.hpp:90
+
+/// Error raised for invalid arguments (maps to std::invalid_argument in the reference).
+struct InvalidArgument : std::invalid_argument {
+    using std::invalid_argument::invalid_argument;
+};
+/// Error raised for runtime failures: HIP errors, unmapped cells (std::runtime_error in the reference).
+struct RuntimeError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+[[noreturn]] inline void hip_fail(hipError_t e, const char *what, const char *file, int line) {
+    throw RuntimeError(std::string("HIP error ") + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ") in " + what + " at " + file + ":" +
+                       std::to_string(line));
+}
+
+#define AXIS_HIP_CHECK(expr)                                                      \
+    do {                                                                          \
+        hipError_t _e = (expr);                                                   \
+        if (_e != hipSuccess) ::axis_hip::hip_fail(_e, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+#define AXIS_LAUNCH_CHECK() AXIS_HIP_CHECK(hipGetLastError())
+
+/// Owning device allocation (hipMalloc / hipFree), movable, not copyable.
+template <class T>
+class DeviceBuffer {
+   public:
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(std::size_t n) { resize(n); }
+    ~DeviceBuffer() { reset(); }
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    DeviceBuffer(DeviceBuffer &&o) noexcept : p_(std::exchange(o.p_, nullptr)), n_(std::exchange(o.n_, 0)) {}
+    DeviceBuffer &operator=(DeviceBuffer &&o) noexcept {
+        if (this != &o) {
+            reset();
+            p_ = std::exchange(o.p_, nullptr);
+            n_ = std::exchange(o.n_, 0);
+        }
+        return *this;
+    }
+    void resize(std::size_t n) {
+        reset();
+        if (n) AXIS_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&p_), n * sizeof(T)));
+        n_ = n;
+    }
+    void reset() {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    T *get() const { return p_; }
+    std::size_t size() const { return n_; }
+    std::size_t bytes() const { return n_ * sizeof(T); }
+    void upload(const T *h, std::size_t n, hipStream_t s) {
+        if (n > n_) throw InvalidArgument("DeviceBuffer::upload overflow");
+        if (n) AXIS_HIP_CHECK(hipMemcpyAsync(p_, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void download(T *h, std::size_t n, hipStream_t s) const {
+        if (n > n_) throw InvalidArgument("DeviceBuffer::download overflow");
+        if (n) AXIS_HIP_CHECK(hipMemcpyAsync(h, p_, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+
+   private:
+    T *p_ = nullptr;
+    std::size_t n_ = 0;
+};
+
+inline unsigned grid_for(std::size_t n, unsigned block) {
+    std::size_t g = (n + block - 1) / block;
+    return static_cast<unsigned>(g ? g : 1);
+}
+
+}  // namespace axis_hip

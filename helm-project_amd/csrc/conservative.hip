@@ -1,0 +1,546 @@
+// conservative.hip -- first-order conservative weights on great circles.
+//
+// Restates the reference host path generate_conservative /
+// generate_conservative_impl<3> (libs/axis/src/solver/weight_generator.cpp:
+// 3246-3302, 3304-3770) as a GPU pipeline:
+//
+//   detection ........ unique_axes / regular_from_axes        (mesh.hip)
+//   per-cell pass .... areas, degenerate flags, caps, boxes   (mesh.hip)
+//   broad phase ...... LBVH over float boxes                  (lbvh.hip)
+//   overlap pass ..... one lane per (dst, candidate) pair: cap filter,
+//                      great-circle Sutherland-Hodgman clip with the subject
+//                      polygon in lane-private LDS columns, FP64 fan area
+//   arbitration ...... the few pairs whose outcome hinges on libm ulps
+//                      (cap-test ties, atan2 cancellation, >CAP vertices)
+//                      are recomputed on the host with the reference's libm
+//   CSR assembly ..... nonzero compaction, (row, col) radix sort, row_ptr,
+//                      weights w = overlap / area_dst, frac_b (column order),
+//                      FracArea scaling, frac_a (row order), clamps
+//
+// The sparsity pattern equals the reference's bit for bit: an entry exists
+// iff the clipped area is > 0, and that decision depends only on + - * /
+// sqrt (identical on both sides under -ffp-contract=off) except where the
+// arbitration hands it to host libm.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <cmath>
+#include <limits>
+#include <string>
+
+#include "axis_hip.hpp"
+#include "geometry.hpp"
+#include "host_util.hpp"
+
+namespace axis_hip::detail {
+
+using geo::PolyRef;
+using geo::V3;
+
+namespace {
+
+constexpr int kWave = 64;
+
+// Relative band around the cap-test threshold inside which the GPU's acos
+// could disagree with host libm; such pairs are clipped and re-decided.
+constexpr double kCapTieRel = 1.0e-12;
+// Fan areas whose terms cancel below this fraction of their magnitude sum
+// are recomputed with host atan2.
+constexpr double kCancelRel = 1.0e-3;
+
+struct Side {
+    const index_t *off;
+    const index_t *ind;
+    const double *xyz;
+    const double *trig;  // non-null for a regular lat-lon mesh
+    index_t ni, nlon, nlat;
+    const std::int32_t *flags;
+    const double *cap;
+    const double *area;
+};
+
+__device__ inline V3 trig_node(const Side &m, index_t a, index_t b) {
+    const double *slon = m.trig, *clon = m.trig + m.nlon, *slat = m.trig + 2 * m.nlon, *clat = m.trig + 2 * m.nlon + m.nlat;
+    double cl = clat[b];
+    return V3{cl * clon[a], cl * slon[a], slat[b]};
+}
+
+// vertex k of cell c: trig cache corners BL, BR, TR, TL
+// (weight_generator.cpp:483-520) or mesh nodes
+__device__ inline V3 cell_vertex(const Side &m, index_t c, int k) {
+    if (m.trig) {
+        index_t ci = c % m.ni, cj = c / m.ni;
+        index_t a = (k == 1 || k == 2) ? ci + 1 : ci;
+        index_t b = (k >= 2) ? cj + 1 : cj;
+        return trig_node(m, a, b);
+    }
+    index_t nd = m.ind[m.off[c] + k];
+    return V3{m.xyz[3 * nd], m.xyz[3 * nd + 1], m.xyz[3 * nd + 2]};
+}
+
+__device__ inline int cell_nverts(const Side &m, index_t c) {
+    if (m.trig) return 4;
+    index_t nv = m.off[c + 1] - m.off[c];
+    return static_cast<int>(nv < 32 ? nv : 32);
+}
+
+template <int CAP>
+__global__ void __launch_bounds__(kWave) overlap_kernel(index_t n_pairs, const std::int32_t *pair_row, const std::int32_t *pair_src, Side src,
+                                                        Side dst, double *overlap, std::int32_t *arb, int *n_arb, int *n_tie) {
+    __shared__ double buf[2][3][CAP][kWave];
+    const int lane = threadIdx.x;
+    const index_t p = static_cast<index_t>(blockIdx.x) * kWave + lane;
+    if (p >= n_pairs) return;
+    const index_t j = pair_row[p];
+    const index_t i = pair_src[p];
+    double ov = 0.0;
+    bool to_host = false;
+    // masked / degenerate / area <= 0 (weight_generator.cpp:3618-3634, 3650-3660)
+    if (((dst.flags[j] | src.flags[i]) & 7) == 0) {
+        // spherical cap early reject (spherical_cap_filter.hpp:190-197)
+        const double *cs = src.cap + 4 * i, *cd = dst.cap + 4 * j;
+        double c = cs[0] * cd[0] + cs[1] * cd[1] + cs[2] * cd[2];
+        c = fmin(1.0, fmax(-1.0, c));
+        const double ang = acos(c);
+        const double sum_r = cs[3] + cd[3];
+        const double tie = kCapTieRel * sum_r + 1.0e-300;
+        if (ang <= sum_r + tie) {
+            if (ang > sum_r - tie) {
+                to_host = true;
+                atomicAdd(n_tie, 1);
+            }
+            PolyRef A{&buf[0][0][0][lane], &buf[0][1][0][lane], &buf[0][2][0][lane], kWave};
+            PolyRef B{&buf[1][0][0][lane], &buf[1][1][0][lane], &buf[1][2][0][lane], kWave};
+            const int ns = cell_nverts(src, i);
+            bool overflow = ns > CAP;
+            if (!overflow) {
+                for (int k = 0; k < ns; ++k) A.set(k, cell_vertex(src, i, k));
+                const int nc = cell_nverts(dst, j);
+                auto clipv = [&](int k) { return cell_vertex(dst, j, k); };
+                int n = geo::clip_polygon(A, ns, B, clipv, nc, CAP, &overflow);
+                if (!overflow && n >= 3) {
+                    double abs_sum;
+                    bool zero_num;
+                    ov = geo::polygon_area(A, n, &abs_sum, &zero_num);
+                    if (!zero_num && ov < kCancelRel * abs_sum) to_host = true;
+                }
+            }
+            if (overflow) to_host = true;
+        }
+    }
+    overlap[p] = ov;
+    if (to_host) {
+        int k = atomicAdd(n_arb, 1);
+        arb[k] = static_cast<std::int32_t>(p);
+    }
+}
+
+__global__ void fill_rows_kernel(const index_t *cand_off, index_t n_dst, std::int32_t *pair_row) {
+    index_t j = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n_dst) return;
+    for (index_t e = cand_off[j]; e < cand_off[j + 1]; ++e) pair_row[e] = static_cast<std::int32_t>(j);
+}
+
+__global__ void scatter_overlaps_kernel(const std::int32_t *idx, const double *vals, int n, double *overlap) {
+    int k = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n) overlap[idx[k]] = vals[k];
+}
+
+__global__ void nz_flag_kernel(const double *overlap, index_t n, index_t *flag) {
+    index_t p = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p < n) flag[p] = overlap[p] > 0.0 ? 1 : 0;
+}
+
+// compact nonzero pairs, keyed (major << minor_bits | minor)
+__global__ void compact_kernel(const double *overlap, const index_t *pos, index_t n, const std::int32_t *pair_row, const std::int32_t *pair_src,
+                               int col_bits, std::uint64_t *key, std::uint32_t *perm, double *ov_out) {
+    index_t p = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n || !(overlap[p] > 0.0)) return;
+    index_t q = pos[p];
+    key[q] = (static_cast<std::uint64_t>(pair_row[p]) << col_bits) | static_cast<std::uint64_t>(static_cast<std::uint32_t>(pair_src[p]));
+    perm[q] = static_cast<std::uint32_t>(q);
+    ov_out[q] = overlap[p];
+}
+
+__global__ void unpack_sorted_kernel(const std::uint64_t *key, const std::uint32_t *perm, const double *ov_in, index_t nnz, int col_bits,
+                                     index_t *col, double *ov, std::int32_t *row32, index_t *row_count) {
+    index_t e = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    std::uint64_t k = key[e];
+    index_t r = static_cast<index_t>(k >> col_bits);
+    col[e] = static_cast<index_t>(k & ((std::uint64_t(1) << col_bits) - 1));
+    ov[e] = ov_in[perm[e]];
+    row32[e] = static_cast<std::int32_t>(r);
+    atomicAdd(reinterpret_cast<unsigned long long *>(row_count + r), 1ull);
+}
+
+// per row: w = ov / area_dst (>= 0), frac_b = Σ ov / area_dst in column order,
+// FracArea division by the unclamped frac_b, clamp (weight_generator.cpp:3682-3724)
+__global__ void row_weights_kernel(const index_t *row_ptr, index_t n_dst, const double *area_dst, const double *ov, double *w, double *frac_b,
+                                   int frac_area, int *max_row) {
+    index_t r = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= n_dst) return;
+    const index_t b = row_ptr[r], e = row_ptr[r + 1];
+    const double a = area_dst[r];
+    double fb = 0.0;
+    for (index_t k = b; k < e; ++k) {
+        double x = ov[k] / a;
+        w[k] = fmax(x, 0.0);
+        fb += x;
+    }
+    if (frac_area && fb > 0.0)
+        for (index_t k = b; k < e; ++k) w[k] /= fb;
+    frac_b[r] = fmin(fb, 1.0);
+    atomicMax(max_row, static_cast<int>(e - b));
+}
+
+__global__ void transpose_keys_kernel(const index_t *col, const std::int32_t *row32, index_t nnz, int row_bits, std::uint64_t *key,
+                                      std::uint32_t *perm) {
+    index_t e = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    key[e] = (static_cast<std::uint64_t>(col[e]) << row_bits) | static_cast<std::uint64_t>(static_cast<std::uint32_t>(row32[e]));
+    perm[e] = static_cast<std::uint32_t>(e);
+}
+
+// frac_a: Σ ov / area_src over each source column's entries in row order.
+// keys are sorted by (col, row); a thread owns the run that starts at its
+// position.
+__global__ void frac_a_kernel(const std::uint64_t *key, const std::uint32_t *perm, index_t nnz, int row_bits, const double *ov,
+                              const double *area_src, double *frac_a) {
+    index_t e = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    index_t c = static_cast<index_t>(key[e] >> row_bits);
+    if (e > 0 && static_cast<index_t>(key[e - 1] >> row_bits) == c) return;
+    const double a = area_src[c];
+    double s = 0.0;
+    for (index_t k = e; k < nnz && static_cast<index_t>(key[k] >> row_bits) == c; ++k) s += ov[perm[k]] / a;
+    frac_a[c] = fmin(s, 1.0);
+}
+
+// first unmapped destination under UnmappedAction::Error
+__global__ void unmapped_kernel(const index_t *row_ptr, index_t n_dst, const std::int32_t *dflags, unsigned long long *first) {
+    index_t r = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= n_dst) return;
+    int f = dflags[r];
+    if (f & 3) return;  // masked or degenerate: skipped by the reference
+    if ((f & 4) || row_ptr[r + 1] == row_ptr[r]) atomicMin(first, static_cast<unsigned long long>(r));
+}
+
+int bits_for(index_t n) {
+    int b = 1;
+    while ((index_t(1) << b) < n) ++b;
+    return b;
+}
+
+// ------------------------------------------------------------ host replay
+struct HostSide {
+    const Mesh *m;
+    const MeshGeom *g;
+    V3 vertex(index_t c, int k) const {
+        if (g->regular) {
+            const index_t nl = g->ni + 1, na = g->nj + 1;
+            const double *slon = g->h_trig.data(), *clon = slon + nl, *slat = clon + nl, *clat = slat + na;
+            index_t ci = c % g->ni, cj = c / g->ni;
+            index_t a = (k == 1 || k == 2) ? ci + 1 : ci;
+            index_t b = (k >= 2) ? cj + 1 : cj;
+            return V3{clat[b] * clon[a], clat[b] * slon[a], slat[b]};
+        }
+        index_t nd = m->h_indices[m->h_offsets[c] + k];
+        return V3{g->h_xyz[3 * nd], g->h_xyz[3 * nd + 1], g->h_xyz[3 * nd + 2]};
+    }
+    int nverts(index_t c) const {
+        if (g->regular) return 4;
+        return static_cast<int>(std::min<index_t>(m->h_offsets[c + 1] - m->h_offsets[c], 32));
+    }
+    // precompute_cap_data for one cell with host libm (spherical_cap_filter.hpp:85-167)
+    void cap(index_t c, V3 &cen, double &rad) const {
+        const index_t b = m->h_offsets[c], e = m->h_offsets[c + 1];
+        cen = V3{0, 0, 0};
+        rad = 0.0;
+        if (e - b < 1) return;
+        double sx = 0, sy = 0, sz = 0;
+        for (index_t k = b; k < e; ++k) {
+            index_t nd = m->h_indices[k];
+            sx += g->h_xyz[3 * nd];
+            sy += g->h_xyz[3 * nd + 1];
+            sz += g->h_xyz[3 * nd + 2];
+        }
+        double inv = 1.0 / static_cast<double>(e - b);
+        cen = geo::normalize(V3{sx * inv, sy * inv, sz * inv});
+        for (index_t k = b; k < e; ++k) {
+            index_t nd = m->h_indices[k];
+            double d = geo::dot(cen, V3{g->h_xyz[3 * nd], g->h_xyz[3 * nd + 1], g->h_xyz[3 * nd + 2]});
+            d = std::fmin(1.0, std::fmax(-1.0, d));
+            double ang = std::acos(d);
+            if (ang > rad) rad = ang;
+        }
+    }
+};
+
+// The reference's inner-loop body for one (dst j, src i) pair with host
+// libm: cap test, clip with SphericalPolygon<32> capacity, fan area.
+double host_pair_overlap(const HostSide &S, const HostSide &D, index_t j, index_t i) {
+    V3 cs, cd;
+    double rs, rd;
+    S.cap(i, cs, rs);
+    D.cap(j, cd, rd);
+    double c = geo::dot(cs, cd);
+    c = std::fmin(1.0, std::fmax(-1.0, c));
+    if (std::acos(c) > (rs + rd)) return 0.0;
+    double ax[32], ay[32], az[32], bx[32], by[32], bz[32];
+    PolyRef A{ax, ay, az, 1}, B{bx, by, bz, 1};
+    const int ns = S.nverts(i);
+    for (int k = 0; k < ns; ++k) A.set(k, S.vertex(i, k));
+    const int nc = D.nverts(j);
+    bool overflow = false;
+    auto clipv = [&](int k) { return D.vertex(j, k); };
+    int n = geo::clip_polygon(A, ns, B, clipv, nc, 32, &overflow);
+    double abs_sum;
+    bool zero_num;
+    return geo::polygon_area(A, n, &abs_sum, &zero_num);
+}
+
+// WeightGenerator::generate's longitude-disjoint check, weight_generator.cpp:1898-1992
+__global__ void lon_range_kernel(const index_t *off, const index_t *ind, const double *lon, index_t n_cells, unsigned long long *mn,
+                                 unsigned long long *mx) {
+    index_t c = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= n_cells) return;
+    index_t b = off[c], e = off[c + 1];
+    if (e == b) return;
+    double s = 0.0;
+    for (index_t k = b; k < e; ++k) s += lon[ind[k]];
+    double cx = s / static_cast<double>(e - b);
+    std::uint64_t bits = static_cast<std::uint64_t>(__double_as_longlong(cx));
+    std::uint64_t key = (bits >> 63) ? ~bits : (bits | 0x8000000000000000ull);
+    atomicMin(mn, static_cast<unsigned long long>(key));
+    atomicMax(mx, static_cast<unsigned long long>(key));
+}
+
+double key_to_double(unsigned long long k) {
+    std::uint64_t b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~static_cast<std::uint64_t>(k);
+    double d;
+    std::memcpy(&d, &b, sizeof d);
+    return d;
+}
+
+bool lon_range(const Mesh &m, double &lo, double &hi, hipStream_t s) {
+    DeviceBuffer<unsigned long long> r(2);
+    unsigned long long init[2] = {~0ull, 0ull};
+    AXIS_HIP_CHECK(hipMemcpyAsync(r.get(), init, sizeof init, hipMemcpyHostToDevice, s));
+    if (m.n_cells() > 0) {
+        lon_range_kernel<<<grid_for(m.n_cells(), 256), 256, 0, s>>>(m.d_offsets.get(), m.d_indices.get(), m.d_coords.get(), m.n_cells(), r.get(),
+                                                                    r.get() + 1);
+        AXIS_LAUNCH_CHECK();
+    }
+    unsigned long long h[2];
+    AXIS_HIP_CHECK(hipMemcpyAsync(h, r.get(), sizeof h, hipMemcpyDeviceToHost, s));
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    if (h[0] == ~0ull) {
+        lo = std::numeric_limits<double>::max();
+        hi = -std::numeric_limits<double>::max();
+        return false;
+    }
+    lo = key_to_double(h[0]);
+    hi = key_to_double(h[1]);
+    return true;
+}
+
+}  // namespace
+
+void check_longitude_ranges(const Mesh &src, const Mesh &dst, bool structured_fast_path, hipStream_t s) {
+    const auto cs = src.coord_system();
+    if (cs != CoordinateSystem::SphericalDeg && cs != CoordinateSystem::SphericalRad) return;
+    double s0, s1, d0, d1;
+    lon_range(src, s0, s1, s);
+    lon_range(dst, d0, d1, s);
+    bool disjoint = (s1 < d0) || (d1 < s0);
+    if (!disjoint) return;
+    const double period = (cs == CoordinateSystem::SphericalDeg) ? 360.0 : (2.0 * kPi);
+    double shift = std::round((0.5 * (s0 + s1) - 0.5 * (d0 + d1)) / period) * period;
+    bool overlaps = !(s1 < d0 + shift || d1 + shift < s0);
+    if (overlaps && !structured_fast_path)
+        throw InvalidArgument("WeightGenerator::generate: Spherical longitude coordinate range mismatch: source [" + std::to_string(s0) + ", " +
+                              std::to_string(s1) + "] and destination [" + std::to_string(d0) + ", " + std::to_string(d1) +
+                              "] are disjoint but would overlap after a shift of " + std::to_string(shift));
+}
+
+std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, hipStream_t s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const index_t n_src = src.n_cells(), n_dst = dst.n_cells();
+    if (n_src >= (index_t(1) << 31) || n_dst >= (index_t(1) << 31)) throw InvalidArgument("generate_conservative: more than 2^31 cells");
+    MeshGeom sg, dg;
+    GridAxes sax = unique_axes(src, s), dax = unique_axes(dst, s);
+    const bool s_reg = regular_from_axes(src, sax, sg), d_reg = regular_from_axes(dst, dax, dg);
+    if (s_reg && d_reg)
+        throw InvalidArgument("generate_conservative: both meshes are regular lat-lon grids; the reference routes this pair to its analytic "
+                              "rectangle path, which this library does not provide");
+    if (rectilinear_from_axes(src, sax) && rectilinear_from_axes(dst, dax))
+        throw InvalidArgument("generate_conservative: both meshes are rectilinear; the reference routes this pair to its analytic "
+                              "rectilinear path, which this library does not provide");
+    if (cfg.line_type != LineType::GreatCircle) throw InvalidArgument("generate_conservative: only LineType::GreatCircle is supported");
+    prepare_geometry(src, sg, s);
+    prepare_geometry(dst, dg, s);
+
+    DeviceBuffer<index_t> cand_off;
+    DeviceBuffer<std::int32_t> cand_src;
+    lbvh_candidates(sg.d_box.get(), n_src, dg.d_box.get(), n_dst, cand_off, cand_src, s);
+    index_t n_pairs = 0;
+    AXIS_HIP_CHECK(hipMemcpyAsync(&n_pairs, cand_off.get() + n_dst, sizeof(index_t), hipMemcpyDeviceToHost, s));
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+
+    auto M = std::make_unique<InterpolationMatrix>();
+    M->n_src = n_src;
+    M->n_dst = n_dst;
+    M->n_candidates = n_pairs;
+
+    DeviceBuffer<std::int32_t> pair_row(std::max<index_t>(n_pairs, 1));
+    DeviceBuffer<double> overlap(std::max<index_t>(n_pairs, 1));
+    DeviceBuffer<std::int32_t> arb(std::max<index_t>(n_pairs, 1));
+    DeviceBuffer<int> counters(2);
+    AXIS_HIP_CHECK(hipMemsetAsync(counters.get(), 0, 2 * sizeof(int), s));
+    if (n_dst > 0) {
+        fill_rows_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(cand_off.get(), n_dst, pair_row.get());
+        AXIS_LAUNCH_CHECK();
+    }
+    auto side = [](const Mesh &m, const MeshGeom &g) {
+        Side x{};
+        x.off = m.d_offsets.get();
+        x.ind = m.d_indices.get();
+        x.xyz = g.d_xyz.get();
+        x.trig = g.regular ? g.d_trig.get() : nullptr;
+        x.ni = g.ni;
+        x.nlon = g.ni + 1;
+        x.nlat = g.nj + 1;
+        x.flags = g.d_flags.get();
+        x.cap = g.d_cap.get();
+        x.area = g.d_area.get();
+        return x;
+    };
+    const Side S = side(src, sg), D = side(dst, dg);
+    if (n_pairs > 0) {
+        const int nv_s = sg.regular ? 4 : std::min(src.max_cell_vertices(), 32);
+        const int nv_d = dg.regular ? 4 : std::min(dst.max_cell_vertices(), 32);
+        const int need = nv_s + nv_d;  // bound for convex cells; larger results go to the host
+        const unsigned grid = static_cast<unsigned>((n_pairs + kWave - 1) / kWave);
+        if (need <= 8)
+            overlap_kernel<8><<<grid, kWave, 0, s>>>(n_pairs, pair_row.get(), cand_src.get(), S, D, overlap.get(), arb.get(), counters.get(),
+                                                     counters.get() + 1);
+        else if (need <= 16)
+            overlap_kernel<16><<<grid, kWave, 0, s>>>(n_pairs, pair_row.get(), cand_src.get(), S, D, overlap.get(), arb.get(), counters.get(),
+                                                      counters.get() + 1);
+        else
+            overlap_kernel<24><<<grid, kWave, 0, s>>>(n_pairs, pair_row.get(), cand_src.get(), S, D, overlap.get(), arb.get(), counters.get(),
+                                                      counters.get() + 1);
+        AXIS_LAUNCH_CHECK();
+    }
+    int h_cnt[2] = {0, 0};
+    AXIS_HIP_CHECK(hipMemcpyAsync(h_cnt, counters.get(), sizeof h_cnt, hipMemcpyDeviceToHost, s));
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    M->n_host_arbitrated = h_cnt[0];
+    M->n_cap_borderline = h_cnt[1];
+    if (h_cnt[0] > 0) {
+        const int na = h_cnt[0];
+        std::vector<std::int32_t> pairs(na), rows(na), srcs(na);
+        arb.download(pairs.data(), na, s);
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+        std::sort(pairs.begin(), pairs.end());
+        std::vector<double> vals(na);
+        for (int k = 0; k < na; ++k) {
+            AXIS_HIP_CHECK(hipMemcpyAsync(&rows[k], pair_row.get() + pairs[k], sizeof(std::int32_t), hipMemcpyDeviceToHost, s));
+            AXIS_HIP_CHECK(hipMemcpyAsync(&srcs[k], cand_src.get() + pairs[k], sizeof(std::int32_t), hipMemcpyDeviceToHost, s));
+        }
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+        HostSide HS{&src, &sg}, HD{&dst, &dg};
+        parallel_range(
+            na, [&](index_t b, index_t e) {
+                for (index_t k = b; k < e; ++k) vals[k] = host_pair_overlap(HS, HD, rows[k], srcs[k]);
+            },
+            64);
+        DeviceBuffer<std::int32_t> d_idx(na);
+        DeviceBuffer<double> d_vals(na);
+        d_idx.upload(pairs.data(), na, s);
+        d_vals.upload(vals.data(), na, s);
+        scatter_overlaps_kernel<<<grid_for(na, 256), 256, 0, s>>>(d_idx.get(), d_vals.get(), na, overlap.get());
+        AXIS_LAUNCH_CHECK();
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    }
+
+    // ---- CSR assembly
+    DeviceBuffer<index_t> nzf(std::max<index_t>(n_pairs, 1)), pos(n_pairs + 1);
+    if (n_pairs > 0) {
+        nz_flag_kernel<<<grid_for(n_pairs, 256), 256, 0, s>>>(overlap.get(), n_pairs, nzf.get());
+        AXIS_LAUNCH_CHECK();
+    }
+    exclusive_scan_i64(nzf.get(), pos.get(), n_pairs, s);
+    index_t nnz = 0;
+    AXIS_HIP_CHECK(hipMemcpyAsync(&nnz, pos.get() + n_pairs, sizeof(index_t), hipMemcpyDeviceToHost, s));
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    M->nnz = nnz;
+    const int col_bits = bits_for(std::max<index_t>(n_src, 2));
+    const int row_bits = bits_for(std::max<index_t>(n_dst, 2));
+    const index_t nz1 = std::max<index_t>(nnz, 1);
+    DeviceBuffer<std::uint64_t> key(nz1);
+    DeviceBuffer<std::uint32_t> perm(nz1);
+    DeviceBuffer<double> ov_c(nz1);
+    if (nnz > 0) {
+        compact_kernel<<<grid_for(n_pairs, 256), 256, 0, s>>>(overlap.get(), pos.get(), n_pairs, pair_row.get(), cand_src.get(), col_bits, key.get(),
+                                                             perm.get(), ov_c.get());
+        AXIS_LAUNCH_CHECK();
+        radix_sort_pairs_u64(key.get(), perm.get(), nnz, col_bits + row_bits, s);
+    }
+    M->row_ptr.resize(n_dst + 1);
+    M->col.resize(nz1);
+    M->val.resize(nz1);
+    M->frac_a.resize(std::max<index_t>(n_src, 1));
+    M->frac_b.resize(std::max<index_t>(n_dst, 1));
+    M->area_a.resize(std::max<index_t>(n_src, 1));
+    M->area_b.resize(std::max<index_t>(n_dst, 1));
+    DeviceBuffer<double> ov(nz1);
+    DeviceBuffer<std::int32_t> row32(nz1);
+    DeviceBuffer<index_t> row_count(std::max<index_t>(n_dst, 1));
+    AXIS_HIP_CHECK(hipMemsetAsync(row_count.get(), 0, sizeof(index_t) * std::max<index_t>(n_dst, 1), s));
+    if (nnz > 0) {
+        unpack_sorted_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(key.get(), perm.get(), ov_c.get(), nnz, col_bits, M->col.get(), ov.get(), row32.get(),
+                                                                row_count.get());
+        AXIS_LAUNCH_CHECK();
+    }
+    exclusive_scan_i64(row_count.get(), M->row_ptr.get(), n_dst, s);
+    DeviceBuffer<int> max_row(1);
+    AXIS_HIP_CHECK(hipMemsetAsync(max_row.get(), 0, sizeof(int), s));
+    if (n_dst > 0) {
+        row_weights_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(M->row_ptr.get(), n_dst, dg.d_area.get(), ov.get(), M->val.get(), M->frac_b.get(),
+                                                               cfg.norm_type == NormType::FracArea ? 1 : 0, max_row.get());
+        AXIS_LAUNCH_CHECK();
+    }
+    AXIS_HIP_CHECK(hipMemsetAsync(M->frac_a.get(), 0, sizeof(double) * std::max<index_t>(n_src, 1), s));
+    if (nnz > 0) {
+        transpose_keys_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(M->col.get(), row32.get(), nnz, row_bits, key.get(), perm.get());
+        AXIS_LAUNCH_CHECK();
+        radix_sort_pairs_u64(key.get(), perm.get(), nnz, col_bits + row_bits, s);
+        frac_a_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(key.get(), perm.get(), nnz, row_bits, ov.get(), sg.d_area.get(), M->frac_a.get());
+        AXIS_LAUNCH_CHECK();
+    }
+    if (n_src > 0)
+        AXIS_HIP_CHECK(hipMemcpyAsync(M->area_a.get(), sg.d_area.get(), sizeof(double) * n_src, hipMemcpyDeviceToDevice, s));
+    if (n_dst > 0)
+        AXIS_HIP_CHECK(hipMemcpyAsync(M->area_b.get(), dg.d_area.get(), sizeof(double) * n_dst, hipMemcpyDeviceToDevice, s));
+    AXIS_HIP_CHECK(hipMemcpyAsync(&M->max_row_nnz, max_row.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    if (cfg.unmapped == UnmappedAction::Error && n_dst > 0) {
+        DeviceBuffer<unsigned long long> first(1);
+        unsigned long long init = ~0ull, h_first = 0;
+        AXIS_HIP_CHECK(hipMemcpyAsync(first.get(), &init, sizeof init, hipMemcpyHostToDevice, s));
+        unmapped_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(M->row_ptr.get(), n_dst, dg.d_flags.get(), first.get());
+        AXIS_LAUNCH_CHECK();
+        AXIS_HIP_CHECK(hipMemcpyAsync(&h_first, first.get(), sizeof h_first, hipMemcpyDeviceToHost, s));
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+        if (h_first != ~0ull)
+            throw RuntimeError("WeightGenerator::generate_conservative: unmapped destination cell " + std::to_string(h_first));
+    }
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    M->gen_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return M;
+}
+
+}  // namespace axis_hip::detail

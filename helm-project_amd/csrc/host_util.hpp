@@ -1,0 +1,58 @@
+// host_util.hpp -- small host-side helpers (thread pool loop, libm-exact
+// coordinate conversion shared by every host step that must reproduce the
+// reference's bits).
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <thread>
+#include <vector>
+
+#include "geometry.hpp"
+
+namespace axis_hip {
+
+/// Host threads used for the libm-exact O(n) steps: AXIS_HOST_THREADS, else
+/// hardware_concurrency capped at 16.
+inline int host_threads() {
+    static int n = [] {
+        if (const char *e = std::getenv("AXIS_HOST_THREADS")) {
+            int v = std::atoi(e);
+            if (v > 0) return v;
+        }
+        unsigned hc = std::thread::hardware_concurrency();
+        return static_cast<int>(std::clamp(hc ? hc : 1u, 1u, 16u));
+    }();
+    return n;
+}
+
+/// Runs fn(begin, end) over [0, n) split into contiguous chunks.
+template <class Fn>
+void parallel_range(std::int64_t n, Fn fn, std::int64_t min_chunk = 4096) {
+    int nt = host_threads();
+    std::int64_t chunks = std::max<std::int64_t>(1, std::min<std::int64_t>(nt, (n + min_chunk - 1) / min_chunk));
+    if (chunks <= 1) {
+        fn(std::int64_t(0), n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(chunks);
+    for (std::int64_t c = 0; c < chunks; ++c) {
+        std::int64_t b = n * c / chunks, e = n * (c + 1) / chunks;
+        th.emplace_back([=] { fn(b, e); });
+    }
+    for (auto &t : th) t.join();
+}
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kDeg2Rad = kPi / 180.0;  // folded once, as spherical_geometry.hpp:36 / spherical_cap_filter.hpp:40
+
+/// lonlat_to_xyz (spherical_geometry.hpp:80-83) with host libm.
+inline geo::V3 lonlat_to_xyz_host(double lon_rad, double lat_rad) {
+    double cl = std::cos(lat_rad);
+    return geo::V3{cl * std::cos(lon_rad), cl * std::sin(lon_rad), std::sin(lat_rad)};
+}
+
+}  // namespace axis_hip

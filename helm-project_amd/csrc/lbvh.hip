@@ -1,0 +1,251 @@
+// lbvh.hip -- GPU linear BVH broad phase over float cell boxes.
+//
+// Replaces the reference's ArborX BoundingVolumeHierarchy build + intersects
+// query (libs/axis/src/solver/weight_generator.cpp:3345-3367).  Result: for
+// every destination cell the list of source cells whose float boxes
+// intersect its box (closed intervals, as ArborX::intersects).
+//
+//   1. 30-bit Morton codes of the box centres on unit-sphere xyz
+//      (10 bits per axis over [-1.1, 1.1]^3),
+//   2. LDS radix sort of (code, cell) pairs,
+//   3. Karras radix-tree build (Karras 2012, "Maximizing parallelism in the
+//      construction of BVHs, octrees and k-d trees"), duplicate codes broken
+//      by sorted position,
+//   4. bottom-up box refit with one atomic arrival counter per internal node,
+//   5. stack traversal per query: count pass, exclusive scan, fill pass.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "axis_hip.hpp"
+
+namespace axis_hip::detail {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kStack = 64;
+constexpr std::uint32_t kLeaf = 0x80000000u;
+
+struct FBox {
+    float lo[3], hi[3];
+};
+
+__device__ inline FBox load_box(const float *b, std::int64_t i) {
+    FBox x;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        x.lo[a] = b[6 * i + a];
+        x.hi[a] = b[6 * i + 3 + a];
+    }
+    return x;
+}
+
+__device__ inline bool overlap(const FBox &a, const FBox &b) {
+    return !(a.hi[0] < b.lo[0] || b.hi[0] < a.lo[0] || a.hi[1] < b.lo[1] || b.hi[1] < a.lo[1] || a.hi[2] < b.lo[2] || b.hi[2] < a.lo[2]);
+}
+
+__device__ inline std::uint32_t spread10(std::uint32_t v) {
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ void morton_kernel(const float *boxes, std::int64_t n, std::uint32_t *codes, std::uint32_t *ids) {
+    std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    FBox b = load_box(boxes, i);
+    std::uint32_t q[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float c = 0.5f * (b.lo[a] + b.hi[a]);
+        float t = (c + 1.1f) * (1.0f / 2.2f);
+        t = fminf(fmaxf(t, 0.0f), 1.0f);
+        q[a] = static_cast<std::uint32_t>(fminf(t * 1024.0f, 1023.0f));
+    }
+    codes[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+    ids[i] = static_cast<std::uint32_t>(i);
+}
+
+// common prefix length of sorted keys i and j, ties broken by position
+__device__ inline int delta(const std::uint32_t *codes, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    std::uint32_t a = codes[i], b = codes[j];
+    if (a == b) return 32 + __clz(static_cast<std::uint32_t>(i ^ j));
+    return __clz(a ^ b);
+}
+
+// internal node i (0..n-2): children and parents
+__global__ void karras_kernel(const std::uint32_t *codes, int n, std::uint32_t *left, std::uint32_t *right, int *parent) {
+    int i = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (delta(codes, n, i, i + 1) - delta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = delta(codes, n, i, i - d);
+    int lmax = 2;
+    while (delta(codes, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(codes, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = delta(codes, n, i, j);
+    int s = 0;
+    int t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta(codes, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int gamma = i + s * d + (d < 0 ? d : 0);
+    int lo = i < j ? i : j, hi = i < j ? j : i;
+    std::uint32_t lc = (lo == gamma) ? (kLeaf | static_cast<std::uint32_t>(gamma)) : static_cast<std::uint32_t>(gamma);
+    std::uint32_t rc = (hi == gamma + 1) ? (kLeaf | static_cast<std::uint32_t>(gamma + 1)) : static_cast<std::uint32_t>(gamma + 1);
+    left[i] = lc;
+    right[i] = rc;
+    // parents: internal nodes at [0, n-1), leaves at [n-1, 2n-1)
+    parent[(lc & kLeaf) ? (n - 1 + (lc & ~kLeaf)) : lc] = i;
+    parent[(rc & kLeaf) ? (n - 1 + (rc & ~kLeaf)) : rc] = i;
+}
+
+// leaf boxes -> internal boxes, bottom-up; the second child to arrive at a
+// node merges the two child boxes and climbs on
+__global__ void refit_kernel(const float *boxes, const std::uint32_t *ids, int n, const std::uint32_t *left, const std::uint32_t *right,
+                             const int *parent, float *node_box, int *arrive) {
+    int k = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int p = parent[n - 1 + k];
+    while (p >= 0) {
+        __threadfence();
+        if (atomicAdd(&arrive[p], 1) == 0) return;
+        __threadfence();
+        FBox b;
+        std::uint32_t c[2] = {left[p], right[p]};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            b.lo[a] = INFINITY;
+            b.hi[a] = -INFINITY;
+        }
+        for (int h = 0; h < 2; ++h) {
+            FBox x;
+            if (c[h] & kLeaf) {
+                x = load_box(boxes, ids[c[h] & ~kLeaf]);
+            } else {
+                // written by another wave, possibly on another XCD: coherent loads
+                float *nb = node_box + 6 * static_cast<std::int64_t>(c[h]);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    x.lo[a] = __hip_atomic_load(nb + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    x.hi[a] = __hip_atomic_load(nb + 3 + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                b.lo[a] = fminf(b.lo[a], x.lo[a]);
+                b.hi[a] = fmaxf(b.hi[a], x.hi[a]);
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            node_box[6 * static_cast<std::int64_t>(p) + a] = b.lo[a];
+            node_box[6 * static_cast<std::int64_t>(p) + 3 + a] = b.hi[a];
+        }
+        p = parent[p];
+    }
+}
+
+// Traversal; kFill == false counts, kFill == true writes.  Stack overflow
+// (never expected: depth <= 62 with 30-bit codes + position tie-break) sets
+// *err so the host reports it instead of returning a short list.
+template <bool kFill>
+__global__ void __launch_bounds__(kBlock) query_kernel(const float *src_boxes, const std::uint32_t *ids, int n_src, const std::uint32_t *left,
+                                                       const std::uint32_t *right, const float *node_box, const float *dst_boxes,
+                                                       std::int64_t n_dst, std::int64_t *count, const std::int64_t *offsets, std::int32_t *out,
+                                                       int *err) {
+    std::int64_t j = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n_dst) return;
+    FBox q = load_box(dst_boxes, j);
+    std::int64_t cnt = 0;
+    std::int64_t w = kFill ? offsets[j] : 0;
+    if (n_src == 1) {
+        if (overlap(q, load_box(src_boxes, ids[0]))) {
+            if (kFill) out[w] = static_cast<std::int32_t>(ids[0]);
+            ++cnt;
+        }
+    } else if (n_src > 1) {
+        std::uint32_t stack[kStack];
+        int sp = 0;
+        stack[sp++] = 0;
+        while (sp > 0) {
+            std::uint32_t nd = stack[--sp];
+            std::uint32_t c[2] = {left[nd], right[nd]};
+            for (int h = 0; h < 2; ++h) {
+                if (c[h] & kLeaf) {
+                    std::uint32_t id = ids[c[h] & ~kLeaf];
+                    if (overlap(q, load_box(src_boxes, id))) {
+                        if (kFill) out[w + cnt] = static_cast<std::int32_t>(id);
+                        ++cnt;
+                    }
+                } else if (overlap(q, load_box(node_box, c[h]))) {
+                    if (sp < kStack)
+                        stack[sp++] = c[h];
+                    else
+                        atomicOr(err, 1);
+                }
+            }
+        }
+    }
+    if (!kFill) count[j] = cnt;
+}
+
+}  // namespace
+
+void lbvh_candidates(const float *src_boxes, index_t n_src, const float *dst_boxes, index_t n_dst, DeviceBuffer<index_t> &cand_off,
+                     DeviceBuffer<std::int32_t> &cand_src, hipStream_t s) {
+    if (n_src >= (index_t(1) << 31) - 1) throw InvalidArgument("LBVH: more than 2^31 source cells");
+    const int n = static_cast<int>(n_src);
+    cand_off.resize(n_dst + 1);
+    DeviceBuffer<std::uint32_t> codes(std::max(n, 1)), ids(std::max(n, 1));
+    DeviceBuffer<std::uint32_t> left(std::max(n - 1, 1)), right(std::max(n - 1, 1));
+    DeviceBuffer<int> parent(std::max(2 * n - 1, 1)), arrive(std::max(n - 1, 1)), err(1);
+    DeviceBuffer<float> node_box(6 * std::max(n - 1, 1));
+    AXIS_HIP_CHECK(hipMemsetAsync(err.get(), 0, sizeof(int), s));
+    if (n > 0) {
+        morton_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(src_boxes, n, codes.get(), ids.get());
+        AXIS_LAUNCH_CHECK();
+        radix_sort_pairs_u32(codes.get(), ids.get(), n, 30, s);
+    }
+    if (n > 1) {
+        AXIS_HIP_CHECK(hipMemsetAsync(parent.get(), 0xFF, sizeof(int) * (2 * n - 1), s));  // root's parent = -1
+        AXIS_HIP_CHECK(hipMemsetAsync(arrive.get(), 0, sizeof(int) * (n - 1), s));
+        karras_kernel<<<grid_for(n - 1, kBlock), kBlock, 0, s>>>(codes.get(), n, left.get(), right.get(), parent.get());
+        AXIS_LAUNCH_CHECK();
+        refit_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(src_boxes, ids.get(), n, left.get(), right.get(), parent.get(), node_box.get(),
+                                                            arrive.get());
+        AXIS_LAUNCH_CHECK();
+    }
+    DeviceBuffer<index_t> count(std::max<index_t>(n_dst, 1));
+    if (n_dst > 0) {
+        query_kernel<false><<<grid_for(n_dst, kBlock), kBlock, 0, s>>>(src_boxes, ids.get(), n, left.get(), right.get(), node_box.get(), dst_boxes,
+                                                                      n_dst, count.get(), nullptr, nullptr, err.get());
+        AXIS_LAUNCH_CHECK();
+    }
+    exclusive_scan_i64(count.get(), cand_off.get(), n_dst, s);
+    index_t total = 0;
+    int h_err = 0;
+    AXIS_HIP_CHECK(hipMemcpyAsync(&total, cand_off.get() + n_dst, sizeof(index_t), hipMemcpyDeviceToHost, s));
+    AXIS_HIP_CHECK(hipMemcpyAsync(&h_err, err.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    if (h_err) throw RuntimeError("LBVH traversal stack overflow");
+    // the reference stores candidate offsets as int (weight_generator.cpp:247-254)
+    if (total >= (index_t(1) << 31)) throw RuntimeError("LBVH: candidate pair count exceeds 2^31");
+    cand_src.resize(std::max<index_t>(total, 1));
+    if (n_dst > 0 && total > 0) {
+        query_kernel<true><<<grid_for(n_dst, kBlock), kBlock, 0, s>>>(src_boxes, ids.get(), n, left.get(), right.get(), node_box.get(), dst_boxes,
+                                                                     n_dst, nullptr, cand_off.get(), cand_src.get(), err.get());
+        AXIS_LAUNCH_CHECK();
+    }
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace axis_hip::detail

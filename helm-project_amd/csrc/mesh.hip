@@ -1,0 +1,554 @@
+// mesh.hip -- mesh ingest, grid detection and per-cell geometry.
+//
+// Reference behaviour restated here (libs/axis):
+//   UnstructuredMesh ingest               topology/unstructured_mesh.hpp:41-220, mesh_factory.cpp:314-339
+//   detect_regular/rectilinear_grid       detail/regular_grid_detector.hpp:64-299
+//   build_node_trig_cache                 detail/trig_cache.hpp:210-267
+//   get_cell_areas_spherical              src/solver/weight_generator.cpp:523-552
+//   DegenerateCellHandler::classify/scan  detail/degenerate_cell_handler.hpp:115-236, 248-367
+//   precompute_cap_data                   detail/spherical_cap_filter.hpp:85-251
+//   compute_cell_aabbs_3d                 src/solver/weight_generator.cpp:198-262
+//
+// Node positions need the host libm's sin/cos bits (the clipper's sliver
+// decisions depend on them), so node xyz and the trig tables are evaluated on
+// the host and uploaded; everything per cell runs on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <cfloat>
+#include <limits>
+
+#include "axis_hip.hpp"
+#include "geometry.hpp"
+#include "host_util.hpp"
+
+namespace axis_hip {
+
+using geo::V3;
+
+Mesh::Mesh(const double *node_coords, index_t n_nodes, index_t ndim, const index_t *conn_offsets, index_t n_cells, const index_t *conn_indices,
+           CoordinateSystem cs, const double *areas, const std::int32_t *mask)
+    : n_nodes_(n_nodes), n_cells_(n_cells), cs_(cs) {
+    if (n_nodes < 0 || n_cells < 0) throw InvalidArgument("Mesh: negative size");
+    if (ndim != 2) throw InvalidArgument("Mesh: only [n_nodes, 2] (lon, lat) node coordinates are supported");
+    if (cs == CoordinateSystem::Cartesian3D) throw InvalidArgument("Mesh: Cartesian3D coordinates are not supported on the great-circle path");
+    if (n_cells > 0 && conn_offsets == nullptr) throw InvalidArgument("Mesh: null connectivity offsets");
+    h_coords.assign(node_coords, node_coords + 2 * n_nodes);
+    h_offsets.assign(conn_offsets, conn_offsets + (n_cells + 1));
+    if (h_offsets.empty()) h_offsets.push_back(0);
+    if (h_offsets[0] != 0) throw InvalidArgument("Mesh: conn_offsets[0] must be 0");
+    for (index_t c = 0; c < n_cells; ++c) {
+        index_t nv = h_offsets[c + 1] - h_offsets[c];
+        if (nv < 0) throw InvalidArgument("Mesh: conn_offsets not monotone");
+        max_nv_ = std::max<int>(max_nv_, static_cast<int>(std::min<index_t>(nv, 1 << 20)));
+    }
+    const index_t nidx = h_offsets[n_cells];
+    h_indices.assign(conn_indices, conn_indices + nidx);
+    for (index_t k = 0; k < nidx; ++k)
+        if (h_indices[k] < 0 || h_indices[k] >= n_nodes) throw InvalidArgument("Mesh: node index out of range");
+    if (areas) h_areas.assign(areas, areas + n_cells);
+    if (mask) h_mask.assign(mask, mask + n_cells);
+    hipStream_t s = nullptr;
+    d_coords.resize(h_coords.size());
+    d_coords.upload(h_coords.data(), h_coords.size(), s);
+    d_offsets.resize(h_offsets.size());
+    d_offsets.upload(h_offsets.data(), h_offsets.size(), s);
+    d_indices.resize(std::max<std::size_t>(h_indices.size(), 1));
+    d_indices.upload(h_indices.data(), h_indices.size(), s);
+    if (areas) {
+        d_areas.resize(std::max<index_t>(n_cells, 1));
+        d_areas.upload(h_areas.data(), n_cells, s);
+    }
+    if (mask) {
+        d_mask.resize(std::max<index_t>(n_cells, 1));
+        d_mask.upload(h_mask.data(), n_cells, s);
+    }
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+namespace detail {
+
+// ------------------------------------------------------------- grid axes
+namespace {
+
+__device__ inline std::uint64_t order_key(double d) {
+    std::uint64_t b = static_cast<std::uint64_t>(__double_as_longlong(d));
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ inline double from_order_key(std::uint64_t k) {
+    std::uint64_t b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+    return __longlong_as_double(static_cast<long long>(b));
+}
+
+__global__ void to_keys_kernel(const double *v, index_t n, std::uint64_t *k) {
+    index_t i = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) k[i] = order_key(v[i]);
+}
+
+// head[i] = 1 when sorted[i] starts a new tolerance class (gap > 1e-12 to the
+// previous value).
+__global__ void heads_kernel(const std::uint64_t *k, index_t n, double *vals, index_t *head) {
+    index_t i = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = from_order_key(k[i]);
+    vals[i] = v;
+    head[i] = (i == 0 || fabs(v - from_order_key(k[i - 1])) > 1.0e-12) ? 1 : 0;
+}
+
+__global__ void compact_kernel(const double *vals, const index_t *head, const index_t *pos, index_t n, double *out) {
+    index_t i = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n && head[i]) out[pos[i]] = vals[i];
+}
+
+// *bad = 1 when a class spans more than 1e-12 from its head, where the
+// reference's compare-with-last-kept loop could split it differently.
+__global__ void span_check_kernel(const double *vals, const index_t *head, const index_t *pos, index_t n, const double *u, int *bad) {
+    index_t i = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n || head[i]) return;
+    index_t cls = pos[i] - 1;
+    if (fabs(vals[i] - u[cls]) > 1.0e-12) atomicOr(bad, 1);
+}
+
+std::vector<double> host_unique(std::vector<double> s) {
+    std::sort(s.begin(), s.end());
+    std::vector<double> u;
+    if (s.empty()) return u;
+    u.push_back(s[0]);
+    for (std::size_t i = 1; i < s.size(); ++i)
+        if (std::abs(s[i] - u.back()) > 1.0e-12) u.push_back(s[i]);
+    return u;
+}
+
+// Tolerance-unique sorted values of one coordinate column (device).  Returns
+// the count; fills `out` only when `want` says so.
+index_t unique_column(const double *d_col, const double *h_col, index_t n, bool (*want)(index_t, void *), void *ctx, std::vector<double> &out,
+                      hipStream_t s) {
+    out.clear();
+    if (n == 0) return 0;
+    DeviceBuffer<std::uint64_t> keys(n);
+    to_keys_kernel<<<grid_for(n, 256), 256, 0, s>>>(d_col, n, keys.get());
+    AXIS_LAUNCH_CHECK();
+    radix_sort_keys_u64(keys.get(), static_cast<std::size_t>(n), s);
+    DeviceBuffer<double> vals(n);
+    DeviceBuffer<index_t> head(n), pos(n + 1);
+    DeviceBuffer<int> bad(1);
+    AXIS_HIP_CHECK(hipMemsetAsync(bad.get(), 0, sizeof(int), s));
+    heads_kernel<<<grid_for(n, 256), 256, 0, s>>>(keys.get(), n, vals.get(), head.get());
+    AXIS_LAUNCH_CHECK();
+    exclusive_scan_i64(head.get(), pos.get(), n, s);
+    index_t count = 0;
+    AXIS_HIP_CHECK(hipMemcpyAsync(&count, pos.get() + n, sizeof(index_t), hipMemcpyDeviceToHost, s));
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    DeviceBuffer<double> u(count);
+    compact_kernel<<<grid_for(n, 256), 256, 0, s>>>(vals.get(), head.get(), pos.get(), n, u.get());
+    AXIS_LAUNCH_CHECK();
+    span_check_kernel<<<grid_for(n, 256), 256, 0, s>>>(vals.get(), head.get(), pos.get(), n, u.get(), bad.get());
+    AXIS_LAUNCH_CHECK();
+    int h_bad = 0;
+    AXIS_HIP_CHECK(hipMemcpyAsync(&h_bad, bad.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    if (h_bad) {
+        // a tolerance class wider than 1e-12: replay the reference's sequential filter exactly
+        out = host_unique(std::vector<double>(h_col, h_col + n));
+        return static_cast<index_t>(out.size());
+    }
+    if (want(count, ctx)) {
+        out.resize(count);
+        u.download(out.data(), count, s);
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    return count;
+}
+
+}  // namespace
+
+GridAxes unique_axes(const Mesh &m, hipStream_t s) {
+    GridAxes ax;
+    const index_t n_cells = m.n_cells();
+    if (n_cells == 0) return ax;
+    ax.all_quads = true;
+    for (index_t c = 0; c < n_cells; ++c)
+        if (m.h_offsets[c + 1] - m.h_offsets[c] != 4) {
+            ax.all_quads = false;
+            break;
+        }
+    if (!ax.all_quads) return ax;
+    const index_t n = m.n_nodes();
+    // the lat count is needed to decide whether the lon values are worth downloading
+    struct Ctx {
+        index_t n_cells, other;
+    } ctx{n_cells, 0};
+    auto want_lat = [](index_t, void *) { return false; };
+    std::vector<double> tmp;
+    index_t nlat = unique_column(m.d_coords.get() + n, m.h_coords.data() + n, n, want_lat, nullptr, tmp, s);
+    ctx.other = nlat;
+    auto want_lon = [](index_t cnt, void *p) {
+        auto *c = static_cast<Ctx *>(p);
+        return cnt >= 2 && c->other >= 2 && (cnt - 1) * (c->other - 1) == c->n_cells;
+    };
+    index_t nlon = unique_column(m.d_coords.get(), m.h_coords.data(), n, want_lon, &ctx, ax.ulon, s);
+    if (nlon >= 2 && nlat >= 2 && (nlon - 1) * (nlat - 1) == n_cells) {
+        if (tmp.empty()) {
+            auto want_all = [](index_t, void *) { return true; };
+            unique_column(m.d_coords.get() + n, m.h_coords.data() + n, n, want_all, nullptr, ax.ulat, s);
+        } else {
+            ax.ulat = tmp;
+        }
+        if (ax.ulon.empty()) {
+            auto want_all = [](index_t, void *) { return true; };
+            unique_column(m.d_coords.get(), m.h_coords.data(), n, want_all, nullptr, ax.ulon, s);
+        }
+    } else {
+        ax.ulon.clear();
+        ax.ulat.clear();
+    }
+    return ax;
+}
+
+bool rectilinear_from_axes(const Mesh &m, const GridAxes &ax) {
+    if (!ax.all_quads || ax.ulon.size() < 2 || ax.ulat.size() < 2) return false;
+    return static_cast<index_t>(ax.ulon.size() - 1) * static_cast<index_t>(ax.ulat.size() - 1) == m.n_cells();
+}
+
+bool regular_from_axes(const Mesh &m, const GridAxes &ax, MeshGeom &g) {
+    g.regular = false;
+    if (!rectilinear_from_axes(m, ax)) return false;
+    auto deltas = [](const std::vector<double> &u) {
+        std::vector<double> d(u.size() - 1);
+        for (std::size_t i = 0; i + 1 < u.size(); ++i) d[i] = u[i + 1] - u[i];
+        return d;
+    };
+    auto uniform = [](const std::vector<double> &d) {
+        if (d.empty()) return false;
+        double mn = d[0], mx = d[0], sum = 0.0;
+        for (double x : d) {
+            mn = std::min(mn, x);
+            mx = std::max(mx, x);
+            sum += x;
+        }
+        if (mn <= 0.0) return false;
+        double mean = sum / static_cast<double>(d.size());
+        return (mx - mn) < 1.0e-10 * mean;
+    };
+    auto dl = deltas(ax.ulon), da = deltas(ax.ulat);
+    if (!uniform(dl) || !uniform(da)) return false;
+    double sl = 0.0, sa = 0.0;
+    for (double x : dl) sl += x;
+    for (double x : da) sa += x;
+    g.regular = true;
+    g.ni = static_cast<index_t>(dl.size());
+    g.nj = static_cast<index_t>(da.size());
+    g.lon_min = ax.ulon.front();
+    g.delta_lon = sl / static_cast<double>(g.ni);
+    g.lat_min = ax.ulat.front();
+    g.delta_lat = sa / static_cast<double>(g.nj);
+    return true;
+}
+
+// ----------------------------------------------------------- per-cell pass
+namespace {
+
+constexpr int kCellBlock = 256;
+
+__device__ inline V3 node(const double *xyz, index_t n) { return V3{xyz[3 * n], xyz[3 * n + 1], xyz[3 * n + 2]}; }
+
+// DegenerateCellHandler::classify on the first min(nv, 32) vertices.
+// Returns the classification and the fan area it used.
+__device__ int classify_cell(const double *xyz, const index_t *idx, int nv, double &fan_area) {
+    fan_area = 0.0;
+    if (nv < 3) return 1;
+    // area (spherical_clipper.hpp:112-137)
+    double total = 0.0;
+    V3 a = node(xyz, idx[0]);
+    V3 b = node(xyz, idx[1]);
+    for (int i = 1; i < nv - 1; ++i) {
+        V3 c = node(xyz, idx[i + 1]);
+        double num;
+        total += geo::fan_term(a, b, c, num);
+        b = c;
+    }
+    fan_area = fabs(total);
+    if (fan_area < 1e-25) return 1;
+    // count_unique_vertices (threshold 1e-15)
+    const double t2 = 1e-15 * 1e-15;
+    int uniq = 1;
+    for (int i = 1; i < nv; ++i) {
+        V3 vi = node(xyz, idx[i]);
+        bool dup = false;
+        for (int j = 0; j < i; ++j) {
+            V3 vj = node(xyz, idx[j]);
+            V3 d{vi.x - vj.x, vi.y - vj.y, vi.z - vj.z};
+            if (geo::length_sq(d) < t2) {
+                dup = true;
+                break;
+            }
+        }
+        if (!dup) ++uniq;
+    }
+    if (uniq < 3) return 2;
+    // is_self_intersecting
+    if (nv >= 4) {
+        for (int i = 0; i < nv; ++i) {
+            int in = (i + 1) % nv;
+            V3 pa = node(xyz, idx[i]), pb = node(xyz, idx[in]);
+            V3 nab = geo::cross(pa, pb);
+            for (int j = i + 2; j < nv; ++j) {
+                int jn = (j + 1) % nv;
+                if (jn == i) continue;
+                V3 pc = node(xyz, idx[j]), pd = node(xyz, idx[jn]);
+                double dc = geo::dot(nab, pc), dd = geo::dot(nab, pd);
+                if (dc * dd >= 0.0) continue;
+                V3 ncd = geo::cross(pc, pd);
+                double da = geo::dot(ncd, pa), db = geo::dot(ncd, pb);
+                if (da * db >= 0.0) continue;
+                return 3;
+            }
+        }
+    }
+    return 0;
+}
+
+__device__ inline float f_down(double v) { return nextafterf(__double2float_rd(v), -INFINITY); }
+__device__ inline float f_up(double v) { return nextafterf(__double2float_ru(v), INFINITY); }
+
+__global__ void __launch_bounds__(kCellBlock) cell_prep_kernel(index_t n_cells, const index_t *off, const index_t *ind, const double *xyz,
+                                                               const double *xyz_deg, const double *mesh_areas, const std::int32_t *mask,
+                                                               double *area_out, std::int32_t *flags_out, double *cap_out, float *box_out,
+                                                               index_t *recheck, int *n_recheck) {
+    index_t c = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= n_cells) return;
+    const index_t b = off[c];
+    const index_t nv_all = off[c + 1] - b;
+    const int nv = static_cast<int>(nv_all < 32 ? nv_all : 32);
+    const index_t *idx = ind + b;
+    // degenerate scan (always on the degree interpretation of the coordinates)
+    double fan_deg;
+    int cls = classify_cell(xyz_deg, idx, nv, fan_deg);
+    // cell area: mesh areas when given, else the fan of the polygon
+    double area;
+    if (mesh_areas) {
+        area = mesh_areas[c];
+    } else if (xyz_deg == xyz) {
+        area = fan_deg;
+    } else {
+        double total = 0.0;
+        if (nv >= 3) {
+            V3 a = node(xyz, idx[0]);
+            V3 bb = node(xyz, idx[1]);
+            for (int i = 1; i < nv - 1; ++i) {
+                V3 cc = node(xyz, idx[i + 1]);
+                double num;
+                total += geo::fan_term(a, bb, cc, num);
+                bb = cc;
+            }
+        }
+        area = fabs(total);
+    }
+    int flags = (cls != 0 ? 1 : 0) | ((mask && mask[c] == 0) ? 2 : 0) | (area <= 0.0 ? 4 : 0);
+    // decisions taken on atan2 bits near zero are replayed on the host
+    if ((nv >= 3 && fan_deg < 1e-20) || (!mesh_areas && area < 1e-20)) {
+        int k = atomicAdd(n_recheck, 1);
+        recheck[k] = c;
+    }
+    area_out[c] = area;
+    flags_out[c] = flags;
+    // cap data (spherical_cap_filter.hpp:85-251) over all vertices
+    V3 cen{0.0, 0.0, 0.0};
+    double rad = 0.0;
+    if (nv_all >= 1) {
+        double sx = 0, sy = 0, sz = 0;
+        for (index_t k = 0; k < nv_all; ++k) {
+            V3 p = node(xyz, idx[k]);
+            sx += p.x;
+            sy += p.y;
+            sz += p.z;
+        }
+        double inv = 1.0 / static_cast<double>(nv_all);
+        cen = geo::normalize(V3{sx * inv, sy * inv, sz * inv});
+        for (index_t k = 0; k < nv_all; ++k) {
+            double d = geo::dot(cen, node(xyz, idx[k]));
+            d = fmin(1.0, fmax(-1.0, d));
+            double ang = acos(d);
+            if (ang > rad) rad = ang;
+        }
+    }
+    cap_out[4 * c + 0] = cen.x;
+    cap_out[4 * c + 1] = cen.y;
+    cap_out[4 * c + 2] = cen.z;
+    cap_out[4 * c + 3] = rad;
+    // float AABB with the reference's cell-size-scaled dilation, rounded
+    // outward one extra float ulp so it contains the reference's box
+    double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (index_t k = 0; k < nv_all; ++k) {
+        V3 p = node(xyz, idx[k]);
+        mn[0] = fmin(mn[0], p.x);
+        mn[1] = fmin(mn[1], p.y);
+        mn[2] = fmin(mn[2], p.z);
+        mx[0] = fmax(mx[0], p.x);
+        mx[1] = fmax(mx[1], p.y);
+        mx[2] = fmax(mx[2], p.z);
+    }
+    double ext = fmax(fmax(mx[0] - mn[0], mx[1] - mn[1]), mx[2] - mn[2]);
+    double eps = fmin(fmax(0.25 * ext, 1.0e-6), 0.02);
+    for (int a = 0; a < 3; ++a) {
+        box_out[6 * c + a] = nv_all ? f_down(mn[a] - eps) : INFINITY;
+        box_out[6 * c + 3 + a] = nv_all ? f_up(mx[a] + eps) : -INFINITY;
+    }
+}
+
+// host replay of the degenerate classification for the rechecked cells
+int host_classify(const double *xyz, const index_t *idx, int nv, double &fan_area) {
+    using geo::V3;
+    auto P = [&](int i) { return V3{xyz[3 * idx[i]], xyz[3 * idx[i] + 1], xyz[3 * idx[i] + 2]}; };
+    fan_area = 0.0;
+    if (nv < 3) return 1;
+    double total = 0.0;
+    for (int i = 1; i < nv - 1; ++i) {
+        double num;
+        total += geo::fan_term(P(0), P(i), P(i + 1), num);
+    }
+    fan_area = std::fabs(total);
+    if (fan_area < 1e-25) return 1;
+    const double t2 = 1e-15 * 1e-15;
+    int uniq = 1;
+    for (int i = 1; i < nv; ++i) {
+        bool dup = false;
+        for (int j = 0; j < i; ++j) {
+            V3 a = P(i), b = P(j);
+            V3 d{a.x - b.x, a.y - b.y, a.z - b.z};
+            if (geo::length_sq(d) < t2) {
+                dup = true;
+                break;
+            }
+        }
+        if (!dup) ++uniq;
+    }
+    if (uniq < 3) return 2;
+    if (nv >= 4) {
+        for (int i = 0; i < nv; ++i) {
+            int in = (i + 1) % nv;
+            for (int j = i + 2; j < nv; ++j) {
+                int jn = (j + 1) % nv;
+                if (jn == i) continue;
+                V3 nab = geo::cross(P(i), P(in));
+                double dc = geo::dot(nab, P(j)), dd = geo::dot(nab, P(jn));
+                if (dc * dd >= 0.0) continue;
+                V3 ncd = geo::cross(P(j), P(jn));
+                double da = geo::dot(ncd, P(i)), db = geo::dot(ncd, P(in));
+                if (da * db >= 0.0) continue;
+                return 3;
+            }
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
+void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s) {
+    const index_t n_nodes = m.n_nodes(), n_cells = m.n_cells();
+    g.n_nodes = n_nodes;
+    g.n_cells = n_cells;
+    const bool deg = m.coord_system() == CoordinateSystem::SphericalDeg;
+    // node xyz with host libm (extract_cell_polygon_spherical, weight_generator.cpp:462-470)
+    g.h_xyz.resize(3 * static_cast<std::size_t>(n_nodes));
+    std::vector<double> h_xyz_deg;
+    if (!deg) h_xyz_deg.resize(3 * static_cast<std::size_t>(n_nodes));
+    const double *lon = m.h_coords.data(), *lat = m.h_coords.data() + n_nodes;
+    parallel_range(n_nodes, [&](index_t b, index_t e) {
+        for (index_t i = b; i < e; ++i) {
+            geo::V3 p = deg ? lonlat_to_xyz_host(lon[i] * kDeg2Rad, lat[i] * kDeg2Rad) : lonlat_to_xyz_host(lon[i], lat[i]);
+            g.h_xyz[3 * i] = p.x;
+            g.h_xyz[3 * i + 1] = p.y;
+            g.h_xyz[3 * i + 2] = p.z;
+            if (!deg) {
+                geo::V3 q = lonlat_to_xyz_host(lon[i] * kDeg2Rad, lat[i] * kDeg2Rad);
+                h_xyz_deg[3 * i] = q.x;
+                h_xyz_deg[3 * i + 1] = q.y;
+                h_xyz_deg[3 * i + 2] = q.z;
+            }
+        }
+    });
+    g.d_xyz.resize(std::max<std::size_t>(g.h_xyz.size(), 3));
+    g.d_xyz.upload(g.h_xyz.data(), g.h_xyz.size(), s);
+    DeviceBuffer<double> d_xyz_deg;
+    if (!deg) {
+        d_xyz_deg.resize(std::max<std::size_t>(h_xyz_deg.size(), 3));
+        d_xyz_deg.upload(h_xyz_deg.data(), h_xyz_deg.size(), s);
+    }
+    // trig cache of a regular grid (trig_cache.hpp:248-261)
+    if (g.regular) {
+        const index_t nl = g.ni + 1, na = g.nj + 1;
+        g.h_trig.resize(2 * (nl + na));
+        for (index_t i = 0; i < nl; ++i) {
+            double r = (g.lon_min + static_cast<double>(i) * g.delta_lon) * kDeg2Rad;
+            g.h_trig[i] = std::sin(r);
+            g.h_trig[nl + i] = std::cos(r);
+        }
+        for (index_t j = 0; j < na; ++j) {
+            double r = (g.lat_min + static_cast<double>(j) * g.delta_lat) * kDeg2Rad;
+            g.h_trig[2 * nl + j] = std::sin(r);
+            g.h_trig[2 * nl + na + j] = std::cos(r);
+        }
+        g.d_trig.resize(g.h_trig.size());
+        g.d_trig.upload(g.h_trig.data(), g.h_trig.size(), s);
+    }
+    const index_t nc = std::max<index_t>(n_cells, 1);
+    g.d_area.resize(nc);
+    g.d_flags.resize(nc);
+    g.d_cap.resize(4 * nc);
+    g.d_box.resize(6 * nc);
+    DeviceBuffer<index_t> recheck(nc);
+    DeviceBuffer<int> n_recheck(1);
+    AXIS_HIP_CHECK(hipMemsetAsync(n_recheck.get(), 0, sizeof(int), s));
+    if (n_cells > 0) {
+        cell_prep_kernel<<<grid_for(n_cells, kCellBlock), kCellBlock, 0, s>>>(
+            n_cells, m.d_offsets.get(), m.d_indices.get(), g.d_xyz.get(), deg ? g.d_xyz.get() : d_xyz_deg.get(),
+            m.has_areas() ? m.d_areas.get() : nullptr, m.has_mask() ? m.d_mask.get() : nullptr, g.d_area.get(), g.d_flags.get(), g.d_cap.get(),
+            g.d_box.get(), recheck.get(), n_recheck.get());
+        AXIS_LAUNCH_CHECK();
+    }
+    int h_n = 0;
+    AXIS_HIP_CHECK(hipMemcpyAsync(&h_n, n_recheck.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    if (h_n > 0) {
+        // replay near-zero-area cells with host libm (the reference's bits)
+        std::vector<index_t> cells(h_n);
+        recheck.download(cells.data(), h_n, s);
+        std::vector<double> area(h_n);
+        std::vector<std::int32_t> flags(h_n);
+        DeviceBuffer<double> tmp_area(h_n);
+        DeviceBuffer<std::int32_t> tmp_flags(h_n);
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+        const double *xyz_deg = deg ? g.h_xyz.data() : h_xyz_deg.data();
+        for (int k = 0; k < h_n; ++k) {
+            index_t c = cells[k];
+            index_t b = m.h_offsets[c];
+            int nv = static_cast<int>(std::min<index_t>(m.h_offsets[c + 1] - b, 32));
+            double fan_deg;
+            int cls = host_classify(xyz_deg, m.h_indices.data() + b, nv, fan_deg);
+            double a;
+            if (m.has_areas())
+                a = m.h_areas[c];
+            else if (deg)
+                a = fan_deg;
+            else {
+                double dummy;
+                host_classify(g.h_xyz.data(), m.h_indices.data() + b, nv, dummy);
+                a = dummy;
+            }
+            area[k] = a;
+            flags[k] = (cls != 0 ? 1 : 0) | ((m.has_mask() && m.h_mask[c] == 0) ? 2 : 0) | (a <= 0.0 ? 4 : 0);
+        }
+        for (int k = 0; k < h_n; ++k) {
+            AXIS_HIP_CHECK(hipMemcpyAsync(g.d_area.get() + cells[k], &area[k], sizeof(double), hipMemcpyHostToDevice, s));
+            AXIS_HIP_CHECK(hipMemcpyAsync(g.d_flags.get() + cells[k], &flags[k], sizeof(std::int32_t), hipMemcpyHostToDevice, s));
+        }
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    }
+}
+
+}  // namespace detail
+}  // namespace axis_hip

@@ -1,0 +1,207 @@
+// primitives.hip -- device-wide exclusive scan and LSD radix sort (gfx950,
+// 64-wide wavefronts).  Used by the LBVH (Morton sort), the CSR assembly
+// (row counts -> row_ptr) and the grid detector (sorted node coordinates).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "axis_hip.hpp"
+
+namespace axis_hip::detail {
+
+namespace {
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+// Block-wide inclusive scan of one value per thread through LDS (wave scans
+// with DPP-backed shuffles, then a scan of the four wave totals).
+__device__ inline index_t block_inclusive_scan(index_t v, index_t *lds_waves, index_t &block_total) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        index_t t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    if (lane == 63) lds_waves[wave] = v;
+    __syncthreads();
+    index_t prefix = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kScanBlock / 64; ++w) {
+        index_t s = lds_waves[w];
+        if (w < wave) prefix += s;
+        total += s;
+    }
+    __syncthreads();
+    block_total = total;
+    return v + prefix;
+}
+
+__global__ void __launch_bounds__(kScanBlock) scan_reduce_kernel(const index_t *in, index_t n, index_t *block_sums) {
+    __shared__ index_t lds[kScanBlock / 64];
+    const index_t base = static_cast<index_t>(blockIdx.x) * kScanTile;
+    index_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        index_t i = base + static_cast<index_t>(k) * kScanBlock + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    index_t total;
+    block_inclusive_scan(s, lds, total);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+// Each thread owns kScanItems consecutive elements.
+__global__ void __launch_bounds__(kScanBlock) scan_down_kernel(const index_t *in, index_t n, const index_t *block_offsets, index_t *out) {
+    __shared__ index_t lds[kScanBlock / 64];
+    const index_t base = static_cast<index_t>(blockIdx.x) * kScanTile + static_cast<index_t>(threadIdx.x) * kScanItems;
+    index_t v[kScanItems];
+    index_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        index_t i = base + k;
+        v[k] = (i < n) ? in[i] : 0;
+        s += v[k];
+    }
+    index_t total;
+    index_t incl = block_inclusive_scan(s, lds, total);
+    index_t run = block_offsets[blockIdx.x] + incl - s;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        index_t i = base + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanBlock - 1) out[n] = run;
+}
+
+}  // namespace
+
+void exclusive_scan_i64(const index_t *in, index_t *out, index_t n, hipStream_t s) {
+    if (n <= 0) {
+        AXIS_HIP_CHECK(hipMemsetAsync(out, 0, sizeof(index_t), s));
+        return;
+    }
+    const index_t nb = (n + kScanTile - 1) / kScanTile;
+    DeviceBuffer<index_t> sums(nb), offs(nb + 1);
+    scan_reduce_kernel<<<static_cast<unsigned>(nb), kScanBlock, 0, s>>>(in, n, sums.get());
+    AXIS_LAUNCH_CHECK();
+    if (nb == 1) {
+        AXIS_HIP_CHECK(hipMemsetAsync(offs.get(), 0, sizeof(index_t), s));
+    } else {
+        exclusive_scan_i64(sums.get(), offs.get(), nb, s);
+    }
+    scan_down_kernel<<<static_cast<unsigned>(nb), kScanBlock, 0, s>>>(in, n, offs.get(), out);
+    AXIS_LAUNCH_CHECK();
+    // the temporaries are freed only after the kernels that read them ran
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// --------------------------------------------------------------- radix sort
+namespace {
+
+constexpr int kSortBlock = 256;
+constexpr int kSortWaves = kSortBlock / 64;
+constexpr int kSortRounds = 16;
+constexpr int kSortTile = kSortBlock * kSortRounds;
+
+template <class K>
+__global__ void __launch_bounds__(kSortBlock) radix_hist_kernel(const K *keys, std::size_t n, int shift, index_t *hist, unsigned nblocks) {
+    __shared__ unsigned cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kSortTile;
+    for (int r = 0; r < kSortRounds; ++r) {
+        std::size_t i = base + static_cast<std::size_t>(r) * kSortBlock + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[static_cast<unsigned>((keys[i] >> shift) & 0xFF)], 1u);
+    }
+    __syncthreads();
+    hist[static_cast<std::size_t>(threadIdx.x) * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Stable scatter: keys are taken in index order, 256 per round; a key's rank
+// among equal digits inside its wave comes from 8 ballots (one per digit bit),
+// across waves from per-wave digit counts in LDS.
+template <class K, bool kVals>
+__global__ void __launch_bounds__(kSortBlock) radix_scatter_kernel(const K *keys_in, const std::uint32_t *vals_in, K *keys_out,
+                                                                   std::uint32_t *vals_out, std::size_t n, int shift, const index_t *offsets,
+                                                                   unsigned nblocks) {
+    __shared__ unsigned run[256];
+    __shared__ unsigned wcnt[kSortWaves][256];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    run[threadIdx.x] = static_cast<unsigned>(offsets[static_cast<std::size_t>(threadIdx.x) * nblocks + blockIdx.x]);
+    const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kSortTile;
+    const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    for (int r = 0; r < kSortRounds; ++r) {
+        for (int w = 0; w < kSortWaves; ++w) wcnt[w][threadIdx.x] = 0;
+        __syncthreads();
+        const std::size_t i = base + static_cast<std::size_t>(r) * kSortBlock + threadIdx.x;
+        const bool valid = i < n;
+        K key = valid ? keys_in[i] : K(0);
+        const unsigned digit = static_cast<unsigned>((key >> shift) & 0xFF);
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            unsigned long long bal = __ballot((digit >> b) & 1u);
+            peers &= ((digit >> b) & 1u) ? bal : ~bal;
+        }
+        const unsigned rank = __popcll(peers & lt);
+        if (valid && rank == 0) wcnt[wave][digit] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            unsigned before = 0;
+            for (int w = 0; w < wave; ++w) before += wcnt[w][digit];
+            const unsigned pos = run[digit] + before + rank;
+            keys_out[pos] = key;
+            if constexpr (kVals) vals_out[pos] = vals_in[i];
+        }
+        __syncthreads();
+        unsigned add = 0;
+        for (int w = 0; w < kSortWaves; ++w) add += wcnt[w][threadIdx.x];
+        run[threadIdx.x] += add;
+        __syncthreads();
+    }
+}
+
+template <class K, bool kVals>
+void radix_sort_impl(K *keys, std::uint32_t *vals, std::size_t n, int bits, hipStream_t s) {
+    if (n <= 1) return;
+    if (n >= (std::size_t(1) << 32)) throw InvalidArgument("radix sort: more than 2^32 keys");
+    const unsigned nb = static_cast<unsigned>((n + kSortTile - 1) / kSortTile);
+    DeviceBuffer<K> k2(n);
+    DeviceBuffer<std::uint32_t> v2(kVals ? n : 0);
+    DeviceBuffer<index_t> hist(static_cast<std::size_t>(256) * nb), offs(static_cast<std::size_t>(256) * nb + 1);
+    K *kin = keys, *kout = k2.get();
+    std::uint32_t *vin = vals, *vout = v2.get();
+    int passes = (bits + 7) / 8;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        radix_hist_kernel<K><<<nb, kSortBlock, 0, s>>>(kin, n, shift, hist.get(), nb);
+        AXIS_LAUNCH_CHECK();
+        exclusive_scan_i64(hist.get(), offs.get(), static_cast<index_t>(256) * nb, s);
+        radix_scatter_kernel<K, kVals><<<nb, kSortBlock, 0, s>>>(kin, vin, kout, vout, n, shift, offs.get(), nb);
+        AXIS_LAUNCH_CHECK();
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    if (kin != keys) {
+        AXIS_HIP_CHECK(hipMemcpyAsync(keys, kin, n * sizeof(K), hipMemcpyDeviceToDevice, s));
+        if constexpr (kVals) AXIS_HIP_CHECK(hipMemcpyAsync(vals, vin, n * sizeof(std::uint32_t), hipMemcpyDeviceToDevice, s));
+    }
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+void radix_sort_pairs_u32(std::uint32_t *keys, std::uint32_t *vals, std::size_t n, int bits, hipStream_t s) {
+    radix_sort_impl<std::uint32_t, true>(keys, vals, n, bits, s);
+}
+void radix_sort_keys_u64(std::uint64_t *keys, std::size_t n, hipStream_t s) { radix_sort_impl<std::uint64_t, false>(keys, nullptr, n, 64, s); }
+void radix_sort_pairs_u64(std::uint64_t *keys, std::uint32_t *vals, std::size_t n, int bits, hipStream_t s) {
+    radix_sort_impl<std::uint64_t, true>(keys, vals, n, bits, s);
+}
+
+}  // namespace axis_hip::detail

@@ -1,0 +1,147 @@
+/* axis_hip.h -- C ABI of the MI355X-native AXIS regrid library (libaxis_hip.so).
+ *
+ * Drop-in for the reference's flat C interface
+ * (libs/axis/src/fortran/axis_c_interop.cpp, bound from Fortran by
+ * libs/axis/fortran/axis_mod.f90): integer handles (token 0 invalid,
+ * monotonic, never recycled), AXIS_SUCCESS / AXIS_ERROR return codes, no
+ * exception crosses the boundary.  Calls are synchronous unless they take a
+ * stream.  Every entry point below cites the reference interface it replaces;
+ * entry points without a reference counterpart say so.
+ *
+ * Meshes and weight matrices live in GPU memory behind their handles.  Field
+ * pointers passed to the apply entry points may be host or device memory where
+ * stated.
+ */
+#ifndef AXIS_HIP_H
+#define AXIS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AXIS_SUCCESS 0 /* axis_c_interop.cpp:37 */
+#define AXIS_ERROR -1  /* axis_c_interop.cpp:40 */
+
+/* InterpolationMethod codes of axis_generate_weights_c (axis_c_interop.cpp:126-140) */
+#define AXIS_METHOD_BILINEAR 0
+#define AXIS_METHOD_NEAREST 1
+#define AXIS_METHOD_CONSERVATIVE1 2
+
+/* CoordinateSystem (libs/axis/include/axis/topology/enums.hpp:73-80) */
+#define AXIS_COORD_SPHERICAL_DEG 0
+#define AXIS_COORD_SPHERICAL_RAD 1
+#define AXIS_COORD_CARTESIAN3D 2
+
+/* field storage types and apply modes (no reference counterpart: the
+ * reference applies FP64 fields only) */
+#define AXIS_F64 0
+#define AXIS_F32 1
+#define AXIS_BF16 2
+#define AXIS_APPLY_PLAIN 0 /* FP64 fields, FP64 weights and accumulation */
+#define AXIS_APPLY_MIXED 1 /* FP32/BF16 fields, FP32 hi+lo weights, compensated FP32 accumulation */
+
+/* RegridConfig (libs/axis/include/axis/solver/regrid_config.hpp:80-89);
+ * field values are the enums' declaration order. */
+typedef struct axis_regrid_config {
+    int method;        /* InterpolationMethod: 0 Bilinear .. 4 Conservative1stOrder, 5 Conservative2ndOrder */
+    int norm_type;     /* NormType: 0 DstArea, 1 FracArea */
+    int line_type;     /* LineType: 0 Cartesian, 1 GreatCircle */
+    int unmapped;      /* UnmappedAction: 0 Error, 1 Ignore */
+    int use_limiter;   /* unused by first-order methods */
+    int extrap_method; /* ExtrapolationAction: 0 None, 1 NearestWet */
+} axis_regrid_config;
+
+/* axis_init_c (axis_c_interop.cpp:88-90): idempotent runtime initialisation. */
+int axis_init_c(void);
+
+/* Message of the last AXIS_ERROR returned on this thread (no reference
+ * counterpart: the reference drops the exception text at the boundary). */
+const char *axis_last_error_c(void);
+
+/* Mesh from UGRID-style buffers: node_coords [n_nodes, ndim] column-major
+ * (all lon, then all lat), CSR cell->node connectivity, optional cell areas
+ * and mask (NULL when absent).  Replaces axis_mesh_from_descriptor_c
+ * (axis_c_interop.cpp:105-109) for the UGRID descriptor path
+ * (mesh_factory.cpp:314-339 build_from_ugrid); the arrays are copied. */
+int axis_mesh_create_c(const double *node_coords, int64_t n_nodes, int64_t ndim, const int64_t *conn_offsets, int64_t n_cells,
+                       const int64_t *conn_indices, int coord_sys, const double *cell_areas, const int32_t *cell_mask, int *mesh_handle);
+
+/* Regular lat-lon mesh by rules: RuleGenerator RegularLatLon
+ * (rule_generator.cpp:105-169, reached through MeshFactory::from_rules). */
+int axis_mesh_regular_latlon_c(double min_x, double max_x, double min_y, double max_y, double r_x, double r_y, int *mesh_handle);
+
+/* Sizes of a mesh (UnstructuredMesh::n_nodes / n_cells, unstructured_mesh.hpp:86-97). */
+int axis_mesh_info_c(int mesh_handle, int64_t *n_nodes, int64_t *n_cells);
+
+/* axis_generate_weights_c (axis_c_interop.cpp:117-145): method 0 Bilinear,
+ * 2 Conservative1stOrder with the default RegridConfig otherwise. */
+int axis_generate_weights_c(int src_handle, int dst_handle, int method, int *matrix_handle);
+
+/* WeightGenerator::generate with a full RegridConfig (weight_generator.hpp:60-62). */
+int axis_generate_weights_config_c(int src_handle, int dst_handle, const axis_regrid_config *config, int *matrix_handle);
+
+/* InterpolationMatrix from COO triplets (interpolation_matrix.hpp:80-92);
+ * the matrix is stored in to_csr() order, duplicates kept.  frac/area arrays
+ * may be NULL (filled with 1). */
+int axis_matrix_from_coo_c(const int64_t *rows, const int64_t *cols, const double *values, int64_t nnz, int64_t n_src, int64_t n_dst,
+                           const double *frac_a, const double *frac_b, const double *area_a, const double *area_b, int *matrix_handle);
+
+/* InterpolationMatrix::to_csr (interpolation_matrix.hpp:180-255).  Matrices
+ * here are always held in CSR form, so this only validates the handle. */
+int axis_matrix_to_csr_c(int matrix_handle);
+
+/* n_src(), n_dst(), nnz() (interpolation_matrix.hpp:98-112). */
+int axis_matrix_info_c(int matrix_handle, int64_t *n_src, int64_t *n_dst, int64_t *nnz);
+
+/* row_ptr() / col_idx() / csr_values() (interpolation_matrix.hpp:262-295):
+ * copies into caller buffers (host or device) of n_dst+1, nnz, nnz entries. */
+int axis_matrix_csr_c(int matrix_handle, int64_t *row_ptr, int64_t *col_idx, double *values);
+
+/* frac_a() / frac_b() / area_a() / area_b() (interpolation_matrix.hpp:139-161);
+ * any pointer may be NULL. */
+int axis_matrix_fractions_c(int matrix_handle, double *frac_a, double *frac_b, double *area_a, double *area_b);
+
+/* Generation diagnostics (no reference counterpart): out[0] candidate pairs,
+ * out[1] pairs re-decided on the host, out[2] cap-test ties, out[3] longest
+ * row; *gen_ms wall time of the generate call. */
+int axis_matrix_stats_c(int matrix_handle, int64_t *out4, double *gen_ms);
+
+/* axis_apply_c (axis_c_interop.cpp:217-236): dst = S * src, FP64.  src/dst may
+ * be host or device memory. */
+int axis_apply_c(int matrix_handle, const double *src, double *dst, int n_src, int n_dst);
+
+/* batch_apply (apply.hpp:275-343, :336) on device-resident layout_left fields
+ * src [n_src, n_fields], dst [n_dst, n_fields] of the given storage types,
+ * enqueued on `stream` (a hipStream_t, NULL = default stream) and waited for
+ * before returning.  mode AXIS_APPLY_PLAIN needs F64/F64; AXIS_APPLY_MIXED
+ * takes F32 or BF16 src and F32 or F64 dst. */
+int axis_batch_apply_c(int matrix_handle, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n_src, int64_t n_dst,
+                       int64_t n_fields, int mode, void *stream);
+
+/* Same as axis_batch_apply_c but returns once the kernels are enqueued (no
+ * reference counterpart: the reference apply always fences). */
+int axis_batch_apply_async_c(int matrix_handle, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n_src, int64_t n_dst,
+                             int64_t n_fields, int mode, void *stream);
+
+/* Autotuner (no reference counterpart): times the launch shapes of
+ * axis_batch_apply_c on these device fields, caches the fastest for this
+ * (matrix size, n_fields, types, mode) and reports it. */
+int axis_autotune_apply_c(int matrix_handle, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n_fields, int mode, void *stream,
+                          int *fields_per_tile, int *rows_per_lane);
+
+/* check_conservation (conservation.hpp:95-97, conservation.cpp:82-103) on
+ * FP64 fields (host or device): out4 = {src_integral, dst_integral,
+ * absolute_error, relative_error}. */
+int axis_check_conservation_c(int matrix_handle, const double *src, const double *dst, int norm_type, double *out4);
+
+/* axis_destroy_mesh_c / axis_destroy_matrix_c (axis_c_interop.cpp:241-250). */
+int axis_destroy_mesh_c(int handle);
+int axis_destroy_matrix_c(int handle);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AXIS_HIP_H */

@@ -1,0 +1,88 @@
+"""CPU checks of the boundary: the C-ABI library loads and exports every
+symbol include/axis_hip.h declares (no compute calls: no GPU here), the
+ctypes binding declares the same set, and the host-side generators hold their
+invariants."""
+import ctypes
+import pathlib
+import re
+
+import numpy as np
+import pytest
+
+from helm_project_amd import axis, fields, grids
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+
+
+def header_functions():
+    text = (ROOT / "include" / "axis_hip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(axis_\w+_c)\s*\(", text)))
+
+
+def test_library_exports_header_symbols():
+    if not axis.LIB_PATH.exists():
+        pytest.fail(f"{axis.LIB_PATH} not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(str(axis.LIB_PATH))
+    names = header_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_binding_covers_header():
+    assert sorted(axis.exported_symbols()) == header_functions()
+
+
+def test_binding_loads_without_gpu():
+    L = axis.lib()
+    assert L.axis_last_error_c() is not None
+
+
+def test_cubed_sphere_topology():
+    for n in (1, 2, 5, 24):
+        m = grids.cubed_sphere(n)
+        assert grids.n_cells(m) == 6 * n * n
+        assert grids.n_nodes(m) == 6 * n * n + 2
+        # every edge is shared by exactly two cells, in opposite directions
+        idx = m["indices"].reshape(-1, 4)
+        e = np.stack([idx, np.roll(idx, -1, axis=1)], axis=-1).reshape(-1, 2)
+        fwd = {tuple(x) for x in e}
+        assert len(fwd) == e.shape[0]
+        assert all((b, a) in fwd for a, b in fwd)
+
+
+def test_latlon_layout():
+    m = grids.latlon_deg(2.0)
+    assert grids.n_cells(m) == 16200 and grids.n_nodes(m) == 181 * 91
+    c = m["coords"]
+    i0 = m["indices"][:4]
+    assert list(c[i0, 0]) == [0.0, 2.0, 2.0, 0.0] and list(c[i0, 1]) == [-90.0, -90.0, -88.0, -88.0]
+
+
+def test_rng_reproducible():
+    a = fields.normal(1001, 10)
+    b = fields.normal(1001, 10)
+    assert np.array_equal(a, b)
+    z = fields.normal(5, 200000)
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
+
+
+def test_harmonics_orthonormal():
+    """Y_lm orthonormality by quadrature on a fine lat-lon grid."""
+    nlat, nlon = 120, 240
+    lat = (np.arange(nlat) + 0.5) * np.pi / nlat - np.pi / 2
+    lon = np.arange(nlon) * 2 * np.pi / nlon
+    LA, LO = np.meshgrid(lat, lon, indexing="ij")
+    p = np.stack([np.cos(LA) * np.cos(LO), np.cos(LA) * np.sin(LO), np.sin(LA)], -1).reshape(-1, 3)
+    w = (np.cos(LA) * (np.pi / nlat) * (2 * np.pi / nlon)).reshape(-1)
+    lmax = 4
+    basis = []
+    for l in range(lmax + 1):
+        for m in range(-l, l + 1):
+            c = np.zeros((lmax + 1, 2 * lmax + 1))
+            c[l, lmax + m] = 1.0
+            basis.append(fields.harmonic_sum(p, c))
+    B = np.array(basis)
+    G = (B * w) @ B.T
+    assert np.allclose(G, np.eye(B.shape[0]), atol=2e-3)
