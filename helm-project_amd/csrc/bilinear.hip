@@ -89,6 +89,8 @@ std::unique_ptr<InterpolationMatrix> generate_bilinear_rect(const Mesh &src, con
     const auto t0 = std::chrono::steady_clock::now();
     const index_t n_src = src.n_cells(), n_dst = dst.n_cells();
     const int ni = static_cast<int>(ax.ulon.size()) - 1, nj = static_cast<int>(ax.ulat.size()) - 1;
+    // the reference reads centre i + 1 unconditionally, which needs two cells per axis
+    if (ni < 2 || nj < 2) throw InvalidArgument("generate_bilinear: the rectilinear source needs at least 2 cells along each axis");
     std::vector<double> clon(ni), clat(nj);
     for (int i = 0; i < ni; ++i) clon[i] = 0.5 * (ax.ulon[i] + ax.ulon[i + 1]);
     for (int j = 0; j < nj; ++j) clat[j] = 0.5 * (ax.ulat[j] + ax.ulat[j + 1]);
@@ -144,10 +146,12 @@ std::unique_ptr<InterpolationMatrix> WeightGenerator::generate(const Mesh &src, 
             const bool s_rect = detail::rectilinear_from_axes(src, sax);
             detail::check_longitude_ranges(src, dst, s_rect, s);
             detail::MeshGeom sg, dg;
-            detail::GridAxes dax = detail::unique_axes(dst, s);
-            if (detail::regular_from_axes(src, sax, sg) && detail::regular_from_axes(dst, dax, dg))
-                throw InvalidArgument("WeightGenerator::generate: both meshes are regular lat-lon grids; the reference routes bilinear "
-                                      "to its analytic regular path, which this library does not provide");
+            if (detail::regular_from_axes(src, sax, sg)) {
+                detail::GridAxes dax = detail::unique_axes(dst, s);
+                if (detail::regular_from_axes(dst, dax, dg))
+                    throw InvalidArgument("WeightGenerator::generate: both meshes are regular lat-lon grids; the reference routes "
+                                          "bilinear to its analytic regular path, which this library does not provide");
+            }
             if (!s_rect)
                 throw InvalidArgument("WeightGenerator::generate: bilinear needs a rectilinear lat-lon source here (the reference's BVH "
                                       "bilinear path is not provided)");

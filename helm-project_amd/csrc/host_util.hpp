@@ -49,10 +49,19 @@ void parallel_range(std::int64_t n, Fn fn, std::int64_t min_chunk = 4096) {
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kDeg2Rad = kPi / 180.0;  // folded once, as spherical_geometry.hpp:36 / spherical_cap_filter.hpp:40
 
+/// sin and cos of one argument through glibc's sincos.  The reference's
+/// sin/cos pairs (lonlat_to_xyz, the trig cache, the cap filter) are emitted by
+/// GCC at -O2 and above as one sincos call, and glibc's sincos differs from
+/// separate sin/cos in the last bit for ~0.1% of arguments; those bits decide
+/// sliver overlaps, so every host pair goes through sincos too.
+inline void sin_cos(double x, double &s, double &c) { ::sincos(x, &s, &c); }
+
 /// lonlat_to_xyz (spherical_geometry.hpp:80-83) with host libm.
 inline geo::V3 lonlat_to_xyz_host(double lon_rad, double lat_rad) {
-    double cl = std::cos(lat_rad);
-    return geo::V3{cl * std::cos(lon_rad), cl * std::sin(lon_rad), std::sin(lat_rad)};
+    double sl, cl, so, co;
+    sin_cos(lat_rad, sl, cl);
+    sin_cos(lon_rad, so, co);
+    return geo::V3{cl * co, cl * so, sl};
 }
 
 }  // namespace axis_hip

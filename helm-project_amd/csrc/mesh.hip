@@ -484,13 +484,11 @@ void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s) {
         g.h_trig.resize(2 * (nl + na));
         for (index_t i = 0; i < nl; ++i) {
             double r = (g.lon_min + static_cast<double>(i) * g.delta_lon) * kDeg2Rad;
-            g.h_trig[i] = std::sin(r);
-            g.h_trig[nl + i] = std::cos(r);
+            sin_cos(r, g.h_trig[i], g.h_trig[nl + i]);
         }
         for (index_t j = 0; j < na; ++j) {
             double r = (g.lat_min + static_cast<double>(j) * g.delta_lat) * kDeg2Rad;
-            g.h_trig[2 * nl + j] = std::sin(r);
-            g.h_trig[2 * nl + na + j] = std::cos(r);
+            sin_cos(r, g.h_trig[2 * nl + j], g.h_trig[2 * nl + na + j]);
         }
         g.d_trig.resize(g.h_trig.size());
         g.d_trig.upload(g.h_trig.data(), g.h_trig.size(), s);

@@ -95,6 +95,40 @@ def harmonic_sum(xyz: np.ndarray, coeffs: np.ndarray) -> np.ndarray:
     return out
 
 
+def harmonic_sum_torch(xyz, coeffs: np.ndarray):
+    """harmonic_sum for a float64 torch tensor xyz [n, 3] (same recurrence,
+    evaluated on the tensor's device)."""
+    import torch
+
+    lmax = coeffs.shape[0] - 1
+    z = torch.clamp(xyz[:, 2], -1.0, 1.0)
+    phi = torch.atan2(xyz[:, 1], xyz[:, 0])
+    s = torch.sqrt(torch.clamp(1.0 - z * z, min=0.0))
+    out = torch.zeros_like(z)
+    pmm = torch.full_like(z, math.sqrt(1.0 / (4.0 * math.pi)))
+    for m in range(lmax + 1):
+        if m > 0:
+            pmm = -math.sqrt((2.0 * m + 1.0) / (2.0 * m)) * s * pmm
+        cm, sm = torch.cos(m * phi), torch.sin(m * phi)
+        fac = math.sqrt(2.0) if m > 0 else 1.0
+        p_lm2, p_lm1 = None, pmm
+        out += float(coeffs[m, lmax + m]) * fac * p_lm1 * cm
+        if m > 0:
+            out += float(coeffs[m, lmax - m]) * fac * p_lm1 * sm
+        for l in range(m + 1, lmax + 1):
+            a = math.sqrt((4.0 * l * l - 1.0) / (l * l - m * m))
+            if p_lm2 is None:
+                p_l = a * z * p_lm1
+            else:
+                b = math.sqrt(((l - 1.0) ** 2 - m * m) / (4.0 * (l - 1.0) ** 2 - 1.0))
+                p_l = a * (z * p_lm1 - b * p_lm2)
+            p_lm2, p_lm1 = p_lm1, p_l
+            out += float(coeffs[l, lmax + m]) * fac * p_l * cm
+            if m > 0:
+                out += float(coeffs[l, lmax - m]) * fac * p_l * sm
+    return out
+
+
 def level_scales(seed: int, n: int) -> np.ndarray:
     """Per-level scale 10^U(-3, 3)."""
     return 10.0 ** (6.0 * uniform(seed, n) - 3.0)

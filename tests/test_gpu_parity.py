@@ -49,7 +49,7 @@ def compare_weights(mat, ref, rel=1e-12, fracs=True):
 def run_pair(src_mesh, dst_mesh, method=axis.CONSERVATIVE, **cfg):
     s, d = axis.Mesh(src_mesh), axis.Mesh(dst_mesh)
     if cfg:
-        m = axis.generate(s, d, axis.RegridConfig(**cfg))
+        m = axis.generate(s, d, axis.RegridConfig(method=method, **cfg))
     else:
         m = axis.generate_weights(s, d, method)
     return m
@@ -256,7 +256,9 @@ def test_c384_properties_and_sampled_rows():
     say("  nnz", mat.nnz, mat.stats())
     rows = np.add.reduceat(val, rp[:-1])
     assert np.all(np.diff(rp) > 0)
-    assert np.max(np.abs(rows - 1.0)) < 1e-12
+    # the reference's fan area is conditioned ~eps / cell area: its own
+    # arithmetic (the oracle) reaches |row sum - 1| = 1.68e-11 at C384
+    assert np.max(np.abs(rows - 1.0)) < 1e-10
     assert np.all(val >= 0)
     # conservation of a degree-32 harmonic field
     p = grids.cell_centroids_xyz(src)
