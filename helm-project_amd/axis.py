@@ -11,12 +11,14 @@ one contiguous row.  Device tensors are passed by pointer (no copy).
 from __future__ import annotations
 
 import ctypes as C
+import os
 import pathlib
 
 import numpy as np
 
 _HERE = pathlib.Path(__file__).resolve().parent
-LIB_PATH = _HERE / "lib" / "libaxis_hip.so"
+# AXIS_HIP_LIBRARY selects another build of the same library (e.g. a debug build)
+LIB_PATH = pathlib.Path(os.environ.get("AXIS_HIP_LIBRARY", _HERE / "lib" / "libaxis_hip.so"))
 
 F64, F32, BF16 = 0, 1, 2
 PLAIN, MIXED = 0, 1

@@ -7,14 +7,13 @@
 //   mixed  FP32 or BF16 fields, weights split into FP32 hi + lo, FP32
 //          TwoProduct/TwoSum-compensated accumulation, FP32 or FP64 output.
 //
-// Mapping: one lane per destination row, a tile of T fields per lane (the row's
-// (col, w) entries are loaded once and reused across the T fields), so each
-// field's output column is written by 64 consecutive rows per wave (full-line
-// stores) and the source gathers of neighbouring rows share cache lines.  Rows
-// longer than kLongRow go to a wave-per-row kernel whose lanes split the row and
-// combine partial sums with cross-lane (DPP/swizzle) reductions.  Work items are
-// (row block, field tile) pairs remapped so that each XCD walks a contiguous
-// range, keeping its source-column working set in its own L2.
+// The kernels live in apply_kernels.hpp: rows cut into segments of at most E
+// nonzeros, one lane per segment with its weights held in registers while the
+// fields stream through, long rows finished by a deterministic reduction of
+// their segment partials.  This file holds the per-matrix set-up (FP32 hi/lo
+// split, int32 columns, segments), the entry points, the autotuner over
+// (fields per step, field chunks), the conservation integrals and the COO
+// constructor.
 #include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 
@@ -23,6 +22,7 @@
 
 #include "apply_kernels.hpp"
 #include "axis_hip.hpp"
+#include "host_util.hpp"
 
 namespace axis_hip {
 
@@ -38,12 +38,6 @@ __global__ void split_kernel(const double *v, const index_t *col, index_t nnz, f
     c32[e] = static_cast<std::int32_t>(col[e]);
 }
 
-__global__ void long_rows_kernel(const index_t *row_ptr, index_t n_dst, int limit, std::int32_t *list, int *n_list) {
-    index_t r = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r >= n_dst) return;
-    if (row_ptr[r + 1] - row_ptr[r] > limit) list[atomicAdd(n_list, 1)] = static_cast<std::int32_t>(r);
-}
-
 }  // namespace
 
 void InterpolationMatrix::prepare_apply(hipStream_t s) {
@@ -57,27 +51,82 @@ void InterpolationMatrix::prepare_apply(hipStream_t s) {
         split_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(val.get(), col.get(), nnz, val_hi.get(), val_lo.get(), col32.get());
         AXIS_LAUNCH_CHECK();
     }
-    // rows handled by the wave-per-row kernel
-    DeviceBuffer<std::int32_t> tmp(std::max<index_t>(n_dst, 1));
-    DeviceBuffer<int> cnt(1);
-    AXIS_HIP_CHECK(hipMemsetAsync(cnt.get(), 0, sizeof(int), s));
-    if (n_dst > 0) {
-        long_rows_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(row_ptr.get(), n_dst, apply_detail::kLongRow, tmp.get(), cnt.get());
-        AXIS_LAUNCH_CHECK();
-    }
-    int h = 0;
-    AXIS_HIP_CHECK(hipMemcpyAsync(&h, cnt.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    // E = the smallest register segment (4, 8, 12, 16) that leaves at most
+    // 0.1% of the rows longer than one segment
+    std::vector<index_t> rp(n_dst + 1);
+    row_ptr.download(rp.data(), n_dst + 1, s);
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
-    n_long_rows = h;
-    long_rows.resize(std::max(h, 1));
-    if (h > 0) {
-        std::vector<std::int32_t> v(h);
-        tmp.download(v.data(), h, s);
-        AXIS_HIP_CHECK(hipStreamSynchronize(s));
-        std::sort(v.begin(), v.end());
-        long_rows.upload(v.data(), h, s);
-        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    index_t hist[apply_detail::kMaxE + 2] = {0};
+    for (index_t r = 0; r < n_dst; ++r) {
+        index_t len = rp[r + 1] - rp[r];
+        hist[len > apply_detail::kMaxE ? apply_detail::kMaxE + 1 : len]++;
     }
+    seg_cap = apply_detail::kMaxE;
+    for (int cap : {4, 8, 12, 16}) {
+        index_t above = 0;
+        for (int l = cap + 1; l <= apply_detail::kMaxE + 1; ++l) above += hist[l];
+        if (above * 1000 <= n_dst) {
+            seg_cap = cap;
+            break;
+        }
+    }
+    // rows are processed in index order: a destination mesh numbered along
+    // rows of cells (cubed-sphere faces, lat-lon rows) gives each wave
+    // neighbouring cells, whose source columns share cache lines, and full-line
+    // stores of Y; regrouping rows by source column was measured slower
+    std::vector<std::int32_t> order(n_dst);
+    for (index_t r = 0; r < n_dst; ++r) order[r] = static_cast<std::int32_t>(r);
+    std::vector<std::int32_t> h_row, h_len, h_slot, h_lrow, h_lslot;
+    std::vector<index_t> h_begin;
+    h_row.reserve(n_dst);
+    h_len.reserve(n_dst);
+    h_slot.reserve(n_dst);
+    h_begin.reserve(n_dst);
+    int slots = 0;
+    for (std::int32_t r : order) {
+        const index_t b = rp[r], e = rp[r + 1];
+        if (e == b) continue;  // empty rows are appended after the others
+        if (e - b <= seg_cap) {
+            h_row.push_back(r);
+            h_begin.push_back(b);
+            h_len.push_back(static_cast<std::int32_t>(e - b));
+            h_slot.push_back(-1);
+        } else {
+            h_lrow.push_back(r);
+            h_lslot.push_back(slots);
+            for (index_t q = b; q < e; q += seg_cap) {
+                h_row.push_back(r);
+                h_begin.push_back(q);
+                h_len.push_back(static_cast<std::int32_t>(std::min<index_t>(seg_cap, e - q)));
+                h_slot.push_back(slots++);
+            }
+        }
+    }
+    // empty rows: a zero-length segment writes y = 0 (the reference's spmv
+    // overwrites every destination entry)
+    for (std::int32_t r : order)
+        if (rp[r + 1] == rp[r]) {
+            h_row.push_back(r);
+            h_begin.push_back(rp[r]);
+            h_len.push_back(0);
+            h_slot.push_back(-1);
+        }
+    h_lslot.push_back(slots);
+    n_seg = static_cast<index_t>(h_row.size());
+    n_long_rows = static_cast<int>(h_lrow.size());
+    n_slots = slots;
+    auto put32 = [&](DeviceBuffer<std::int32_t> &d, const std::vector<std::int32_t> &h) {
+        d.resize(std::max<std::size_t>(h.size(), 1));
+        if (!h.empty()) d.upload(h.data(), h.size(), s);
+    };
+    put32(seg_row, h_row);
+    put32(seg_len, h_len);
+    put32(seg_slot, h_slot);
+    put32(long_rows, h_lrow);
+    put32(long_slot, h_lslot);
+    seg_begin.resize(std::max<std::size_t>(h_begin.size(), 1));
+    if (!h_begin.empty()) seg_begin.upload(h_begin.data(), h_begin.size(), s);
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 void batch_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t n_src, index_t n_dst, index_t k, ApplyMode mode,
@@ -100,20 +149,20 @@ void batch_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, F
 
 std::pair<int, int> autotune_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t k, ApplyMode mode,
                                    hipStream_t s) {
+    if (k <= 0) throw InvalidArgument("autotune_apply: no fields");
     A.prepare_apply(s);
     hipEvent_t e0, e1;
     AXIS_HIP_CHECK(hipEventCreate(&e0));
     AXIS_HIP_CHECK(hipEventCreate(&e1));
-    apply_detail::Launch best = apply_detail::heuristic(k);
+    apply_detail::Launch best = apply_detail::heuristic(A, k, mode);
     float best_ms = 1e30f;
-    const int tiles[] = {1, 2, 4, 8, 16};
-    const int rpl[] = {1, 2, 4};
-    for (int t : tiles) {
-        if (t > 1 && t / 2 >= k) continue;  // a tile no smaller one fills as well
-        for (int r : rpl) {
+    for (int fu : {1, 2, 4}) {
+        if (fu > 1 && fu / 2 >= k) continue;
+        for (int chunks : {1, 2, 4, 8, 16, 32, 64}) {
+            if (chunks > 1 && (chunks / 2) * fu >= k) continue;
             apply_detail::Launch L;
-            L.tile = t;
-            L.rows_per_lane = r;
+            L.fu = fu;
+            L.chunks = chunks;
             apply_detail::launch(A, x, xt, y, yt, k, mode, L, s);  // warm
             AXIS_HIP_CHECK(hipEventRecord(e0, s));
             const int reps = 3;
@@ -138,7 +187,7 @@ std::pair<int, int> autotune_apply(InterpolationMatrix &A, const void *x, FieldT
     // leave the output of the chosen shape in y
     apply_detail::launch(A, x, xt, y, yt, k, mode, best, s);
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
-    return {best.tile, best.rows_per_lane};
+    return {best.fu, best.chunks};
 }
 
 // ------------------------------------------------------ conservation check

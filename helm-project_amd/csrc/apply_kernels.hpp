@@ -1,6 +1,22 @@
-// apply_kernels.hpp -- device code of the sparse apply, shared by the
-// single-matrix launch (apply.hip) and the persistent multi-pair batch
-// (batch.hip).
+// apply_kernels.hpp -- device code of the sparse apply Y = A X on layout_left
+// fields, shared by the single-matrix launch (apply.hip) and the persistent
+// multi-pair batch (batch.hip).
+//
+// Design (HBM-bound: touch every byte of A, X and Y about once):
+//   * the CSR rows are cut into "segments" of at most E nonzeros (E = 4, 8,
+//     12 or 16, chosen per matrix so that almost every row is one segment);
+//   * one lane per segment: its (col, w) entries are loaded once into
+//     registers and reused while the lane streams through a chunk of fields,
+//     FU fields per step (FU * E independent gathers in flight).  A lane whose
+//     segment is a whole row writes Y directly; consecutive lanes hold
+//     consecutive rows, so Y for one field is written in full cache lines and
+//     the gathers of neighbouring rows share lines of X;
+//   * segments of a long row (e.g. a polar cell with ~1600 nonzeros) write
+//     partial sums to a workspace; a second kernel adds each long row's
+//     partials in segment order (deterministic) and writes Y;
+//   * work items are (segment block, field chunk); the blockIdx -> item map
+//     gives each XCD a contiguous run of segment blocks per chunk, so
+//     neighbouring rows, which share source columns, share that XCD's L2.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -9,18 +25,21 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 
 #include "axis_hip.hpp"
 
 namespace axis_hip::apply_detail {
 
-constexpr int kLongRow = 64;  // rows with more nonzeros go wave-per-row
-constexpr int kWave = 64;
+constexpr int kBlock = 256;  // segments per workgroup
+constexpr int kMaxE = 16;    // largest register segment
 
-/// Launch shape: T fields per lane, R rows per lane (rows-per-wavefront = 64 R).
+/// Launch shape chosen by the autotuner: FU fields per step and the number of
+/// field chunks the k fields are split into (more chunks = more parallelism,
+/// more re-reads of A).
 struct Launch {
-    int tile = 8;
-    int rows_per_lane = 1;
+    int fu = 4;
+    int chunks = 1;
 };
 
 __device__ inline float ld_f32(const float *p) { return *p; }
@@ -44,167 +63,197 @@ struct Acc32 {
         s = t;
         c += fmaf(l, x, err + pe);
     }
-    __device__ inline void merge(float s2, float c2) {
-        float t = s + s2;
-        float bp = t - s;
-        float err = (s - (t - bp)) + (s2 - bp);
-        s = t;
-        c = c + (c2 + err);
-    }
     __device__ inline double value() const { return static_cast<double>(s) + static_cast<double>(c); }
 };
 
-/// Matrix operands of one apply.
+/// Operands of one apply.
 struct Mat {
-    const index_t *row_ptr;
     const std::int32_t *col;
     const double *val;
     const float *hi, *lo;
     index_t n_src, n_dst;
+    // segments, in processing order
+    const std::int32_t *seg_row;   // destination row
+    const index_t *seg_begin;      // first nonzero
+    const std::int32_t *seg_len;   // <= E
+    const std::int32_t *seg_slot;  // -1: whole row; else partial-sum slot
+    index_t n_seg;
+    // long rows: partial slots [long_slot[j], long_slot[j + 1]) of row long_row[j]
+    const std::int32_t *long_row, *long_slot;
+    int n_long, n_slots;
 };
 
-/// Rows [r0, r0 + 64 R) of one wave (lane owns rows r0 + lane + 64 q), fields
-/// [f0, f0 + nf), nf <= T.  Rows longer than kLongRow are skipped.
-template <int T, bool MIXED, class XT, class YT>
-__device__ inline void rows_tile(const Mat &A, const XT *X, YT *Y, index_t r0, int R, index_t f0, int nf, int lane) {
-    for (int q = 0; q < R; ++q) {
-        const index_t r = r0 + lane + static_cast<index_t>(q) * kWave;
-        if (r >= A.n_dst) return;
-        const index_t b = A.row_ptr[r], e = A.row_ptr[r + 1];
-        if (e - b > kLongRow) continue;
-        if constexpr (MIXED) {
-            Acc32 acc[T];
-            for (index_t idx = b; idx < e; ++idx) {
-                const index_t c = A.col[idx];
-                const float h = A.hi[idx], l = A.lo[idx];
-#pragma unroll
-                for (int t = 0; t < T; ++t)
-                    if (t < nf) acc[t].add(h, l, ld_f32(X + c + (f0 + t) * A.n_src));
-            }
-#pragma unroll
-            for (int t = 0; t < T; ++t)
-                if (t < nf) st(Y + r + (f0 + t) * A.n_dst, acc[t].value());
-        } else {
-            double acc[T];
-#pragma unroll
-            for (int t = 0; t < T; ++t) acc[t] = 0.0;
-            for (index_t idx = b; idx < e; ++idx) {
-                const index_t c = A.col[idx];
-                const double w = A.val[idx];
-#pragma unroll
-                for (int t = 0; t < T; ++t)
-                    if (t < nf) acc[t] = fma(w, reinterpret_cast<const double *>(X)[c + (f0 + t) * A.n_src], acc[t]);
-            }
-#pragma unroll
-            for (int t = 0; t < T; ++t)
-                if (t < nf) reinterpret_cast<double *>(Y)[r + (f0 + t) * A.n_dst] = acc[t];
-        }
-    }
-}
-
-/// One long row by a whole wave: lanes stride over the row, partial sums are
-/// combined by a butterfly of cross-lane shuffles (DPP / ds_swizzle).
-template <int T, bool MIXED, class XT, class YT>
-__device__ inline void long_row(const Mat &A, const XT *X, YT *Y, index_t r, index_t f0, int nf, int lane) {
-    const index_t b = A.row_ptr[r], e = A.row_ptr[r + 1];
-    if constexpr (MIXED) {
-        Acc32 acc[T];
-        for (index_t idx = b + lane; idx < e; idx += kWave) {
-            const index_t c = A.col[idx];
-            const float h = A.hi[idx], l = A.lo[idx];
-#pragma unroll
-            for (int t = 0; t < T; ++t)
-                if (t < nf) acc[t].add(h, l, ld_f32(X + c + (f0 + t) * A.n_src));
-        }
-#pragma unroll
-        for (int t = 0; t < T; ++t) {
-            for (int off = kWave / 2; off >= 1; off >>= 1) {
-                float s2 = __shfl_xor(acc[t].s, off, kWave);
-                float c2 = __shfl_xor(acc[t].c, off, kWave);
-                acc[t].merge(s2, c2);
-            }
-            if (lane == t && t < nf) st(Y + r + (f0 + t) * A.n_dst, acc[t].value());
-        }
-    } else {
-        double acc[T];
-#pragma unroll
-        for (int t = 0; t < T; ++t) acc[t] = 0.0;
-        for (index_t idx = b + lane; idx < e; idx += kWave) {
-            const index_t c = A.col[idx];
-            const double w = A.val[idx];
-#pragma unroll
-            for (int t = 0; t < T; ++t)
-                if (t < nf) acc[t] = fma(w, reinterpret_cast<const double *>(X)[c + (f0 + t) * A.n_src], acc[t]);
-        }
-#pragma unroll
-        for (int t = 0; t < T; ++t) {
-            for (int off = kWave / 2; off >= 1; off >>= 1) acc[t] += __shfl_xor(acc[t], off, kWave);
-            if (lane == t && t < nf) reinterpret_cast<double *>(Y)[r + (f0 + t) * A.n_dst] = acc[t];
-        }
-    }
-}
-
 /// Hardware places workgroup b on XCD b % 8; give XCD x a contiguous range of
-/// work items so consecutive row blocks of one field tile share its L2.
+/// work items.
 __device__ inline unsigned xcd_remap(unsigned bid, unsigned n) {
     const unsigned xcd = bid & 7u, idx = bid >> 3;
     const unsigned q = n >> 3, rem = n & 7u;
     return xcd * q + (xcd < rem ? xcd : rem) + idx;
 }
 
-template <int T, bool MIXED, class XT, class YT>
-__global__ void __launch_bounds__(kWave) spmm_rows_kernel(Mat A, const XT *X, YT *Y, index_t k, int R, unsigned n_row_blocks) {
-    const unsigned w = xcd_remap(blockIdx.x, gridDim.x);
-    const unsigned rb = w % n_row_blocks, tile = w / n_row_blocks;
-    const index_t f0 = static_cast<index_t>(tile) * T;
-    const int nf = static_cast<int>(k - f0 < T ? k - f0 : T);
-    rows_tile<T, MIXED>(A, X, Y, static_cast<index_t>(rb) * kWave * R, R, f0, nf, threadIdx.x);
+template <class XT>
+__device__ inline float ldv(const XT *p) {
+    return ld_f32(p);
+}
+__device__ inline double ldv(const double *p) { return *p; }
+
+/// Segment i of the processing order, fields [f0, f1) in steps of FU.
+/// Partial sums of long-row segments go to P[f * n_slots + slot] (FP64).
+template <int E, int FU, bool MIXED, class XT, class YT>
+__device__ inline void segment_apply(const Mat &A, const XT *X, YT *Y, double *P, index_t i, index_t f0, index_t f1) {
+    using VT = typename std::conditional<MIXED, float, double>::type;
+    const index_t r = A.seg_row[i];
+    const index_t b = A.seg_begin[i];
+    const int n = A.seg_len[i];
+    const int slot = A.seg_slot[i];
+    std::int32_t c[E];
+    VT h[E], l[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const bool v = e < n;
+        c[e] = v ? A.col[b + e] : 0;
+        if constexpr (MIXED) {
+            h[e] = v ? A.hi[b + e] : 0.0f;
+            l[e] = v ? A.lo[b + e] : 0.0f;
+        } else {
+            h[e] = v ? A.val[b + e] : 0.0;
+            l[e] = 0.0;
+        }
+    }
+    for (index_t f = f0; f < f1; f += FU) {
+        VT x[FU][E];
+#pragma unroll
+        for (int u = 0; u < FU; ++u) {
+            const index_t fu = (f + u < f1) ? f + u : f1 - 1;
+            const XT *xf = X + fu * A.n_src;
+#pragma unroll
+            for (int e = 0; e < E; ++e) x[u][e] = (e < n) ? ldv(xf + c[e]) : VT(0);
+        }
+#pragma unroll
+        for (int u = 0; u < FU; ++u) {
+            double v;
+            if constexpr (MIXED) {
+                Acc32 acc;
+#pragma unroll
+                for (int e = 0; e < E; ++e)
+                    if (e < n) acc.add(h[e], l[e], x[u][e]);
+                v = acc.value();
+            } else {
+                double acc = 0.0;
+#pragma unroll
+                for (int e = 0; e < E; ++e)
+                    if (e < n) acc = fma(h[e], x[u][e], acc);
+                v = acc;
+            }
+            if (f + u < f1) {
+                if (slot < 0)
+                    st(Y + (f + u) * A.n_dst + r, v);
+                else
+                    P[(f + u) * A.n_slots + slot] = v;
+            }
+        }
+    }
 }
 
-template <int T, bool MIXED, class XT, class YT>
-__global__ void __launch_bounds__(kWave) spmm_long_kernel(Mat A, const XT *X, YT *Y, index_t k, const std::int32_t *rows, int n_rows) {
-    const unsigned w = blockIdx.x;
-    const int li = static_cast<int>(w % static_cast<unsigned>(n_rows));
-    const index_t f0 = static_cast<index_t>(w / static_cast<unsigned>(n_rows)) * T;
-    const int nf = static_cast<int>(k - f0 < T ? k - f0 : T);
-    long_row<T, MIXED>(A, X, Y, rows[li], f0, nf, threadIdx.x);
+template <int E, int FU, bool MIXED, class XT, class YT>
+__global__ void __launch_bounds__(kBlock) spmm_seg_kernel(Mat A, const XT *X, YT *Y, double *P, index_t k, unsigned n_blocks, int chunks) {
+    const unsigned w = xcd_remap(blockIdx.x, gridDim.x);
+    const unsigned sb = w % n_blocks, ch = w / n_blocks;
+    const index_t i = static_cast<index_t>(sb) * kBlock + threadIdx.x;
+    if (i >= A.n_seg) return;
+    const index_t per = (k + chunks - 1) / chunks;
+    const index_t f0 = static_cast<index_t>(ch) * per;
+    const index_t f1 = f0 + per < k ? f0 + per : k;
+    if (f0 >= f1) return;
+    segment_apply<E, FU, MIXED>(A, X, Y, P, i, f0, f1);
+}
+
+/// Y of the long rows: the partials of each row summed in segment order.
+template <class YT>
+__device__ inline void long_reduce(const Mat &A, const double *P, YT *Y, index_t k, index_t t) {
+    const int j = static_cast<int>(t % A.n_long);
+    const index_t f = t / A.n_long;
+    double s = 0.0;
+    for (int q = A.long_slot[j]; q < A.long_slot[j + 1]; ++q) s += P[f * A.n_slots + q];
+    st(Y + f * A.n_dst + A.long_row[j], s);
+}
+
+template <class YT>
+__global__ void long_reduce_kernel(Mat A, const double *P, YT *Y, index_t k) {
+    const index_t t = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<index_t>(A.n_long) * k) return;
+    long_reduce(A, P, Y, k, t);
 }
 
 inline Mat mat_of(const InterpolationMatrix &A) {
-    return Mat{A.row_ptr.get(), A.col32.get(), A.val.get(), A.val_hi.get(), A.val_lo.get(), A.n_src, A.n_dst};
+    Mat M{};
+    M.col = A.col32.get();
+    M.val = A.val.get();
+    M.hi = A.val_hi.get();
+    M.lo = A.val_lo.get();
+    M.n_src = A.n_src;
+    M.n_dst = A.n_dst;
+    M.seg_row = A.seg_row.get();
+    M.seg_begin = A.seg_begin.get();
+    M.seg_len = A.seg_len.get();
+    M.seg_slot = A.seg_slot.get();
+    M.n_seg = A.n_seg;
+    M.long_row = A.long_rows.get();
+    M.long_slot = A.long_slot.get();
+    M.n_long = A.n_long_rows;
+    M.n_slots = A.n_slots;
+    return M;
 }
 
-template <int T, bool MIXED, class XT, class YT>
-void launch_t(const InterpolationMatrix &A, const void *x, void *y, index_t k, int R, hipStream_t s) {
+/// FP64 partial-sum workspace for k fields, owned by the matrix (applies of
+/// one matrix that overlap on different streams share it and must not overlap).
+inline double *partials(InterpolationMatrix &A, index_t k) {
+    if (A.n_slots == 0) return nullptr;
+    const std::size_t need = static_cast<std::size_t>(A.n_slots) * static_cast<std::size_t>(k);
+    if (A.partial_ws.size() < need) A.partial_ws.resize(need);
+    return A.partial_ws.get();
+}
+
+template <int E, int FU, bool MIXED, class XT, class YT>
+void launch_e(InterpolationMatrix &A, const void *x, void *y, index_t k, int chunks, hipStream_t s) {
     const Mat M = mat_of(A);
-    const index_t rows_per_block = static_cast<index_t>(kWave) * R;
-    const unsigned nrb = static_cast<unsigned>((A.n_dst + rows_per_block - 1) / rows_per_block);
-    const unsigned ntiles = static_cast<unsigned>((k + T - 1) / T);
-    const std::uint64_t nwork = static_cast<std::uint64_t>(nrb) * ntiles;
+    double *P = partials(A, k);
+    const unsigned nb = static_cast<unsigned>(std::max<index_t>((A.n_seg + kBlock - 1) / kBlock, 1));
+    const std::uint64_t nwork = static_cast<std::uint64_t>(nb) * chunks;
     if (nwork >= (1ull << 31)) throw InvalidArgument("apply: launch too large");
-    spmm_rows_kernel<T, MIXED, XT, YT><<<static_cast<unsigned>(nwork), kWave, 0, s>>>(M, static_cast<const XT *>(x), static_cast<YT *>(y), k, R, nrb);
-    AXIS_LAUNCH_CHECK();
+    if (A.n_seg > 0) {
+        spmm_seg_kernel<E, FU, MIXED, XT, YT><<<static_cast<unsigned>(nwork), kBlock, 0, s>>>(M, static_cast<const XT *>(x), static_cast<YT *>(y), P,
+                                                                                             k, nb, chunks);
+        AXIS_LAUNCH_CHECK();
+    }
     if (A.n_long_rows > 0) {
-        const std::uint64_t nl = static_cast<std::uint64_t>(A.n_long_rows) * ntiles;
-        spmm_long_kernel<T, MIXED, XT, YT><<<static_cast<unsigned>(nl), kWave, 0, s>>>(M, static_cast<const XT *>(x), static_cast<YT *>(y), k,
-                                                                                       A.long_rows.get(), A.n_long_rows);
+        const index_t n = static_cast<index_t>(A.n_long_rows) * k;
+        long_reduce_kernel<YT><<<grid_for(n, 256), 256, 0, s>>>(M, P, static_cast<YT *>(y), k);
         AXIS_LAUNCH_CHECK();
     }
 }
 
-template <bool MIXED, class XT, class YT>
-void launch_xy(const InterpolationMatrix &A, const void *x, void *y, index_t k, const Launch &L, hipStream_t s) {
-    switch (L.tile) {
-        case 1: return launch_t<1, MIXED, XT, YT>(A, x, y, k, L.rows_per_lane, s);
-        case 2: return launch_t<2, MIXED, XT, YT>(A, x, y, k, L.rows_per_lane, s);
-        case 4: return launch_t<4, MIXED, XT, YT>(A, x, y, k, L.rows_per_lane, s);
-        case 8: return launch_t<8, MIXED, XT, YT>(A, x, y, k, L.rows_per_lane, s);
-        case 16: return launch_t<16, MIXED, XT, YT>(A, x, y, k, L.rows_per_lane, s);
-        default: throw InvalidArgument("apply: unsupported field tile");
+template <int FU, bool MIXED, class XT, class YT>
+void launch_fu(InterpolationMatrix &A, const void *x, void *y, index_t k, int chunks, hipStream_t s) {
+    switch (A.seg_cap) {
+        case 4: return launch_e<4, FU, MIXED, XT, YT>(A, x, y, k, chunks, s);
+        case 8: return launch_e<8, FU, MIXED, XT, YT>(A, x, y, k, chunks, s);
+        case 12: return launch_e<12, FU, MIXED, XT, YT>(A, x, y, k, chunks, s);
+        default: return launch_e<16, FU, MIXED, XT, YT>(A, x, y, k, chunks, s);
     }
 }
 
-inline void launch(const InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t k, ApplyMode mode, const Launch &L,
+template <bool MIXED, class XT, class YT>
+void launch_xy(InterpolationMatrix &A, const void *x, void *y, index_t k, const Launch &L, hipStream_t s) {
+    switch (L.fu) {
+        case 1: return launch_fu<1, MIXED, XT, YT>(A, x, y, k, L.chunks, s);
+        case 2: return launch_fu<2, MIXED, XT, YT>(A, x, y, k, L.chunks, s);
+        case 4: return launch_fu<4, MIXED, XT, YT>(A, x, y, k, L.chunks, s);
+        default: throw InvalidArgument("apply: unsupported fields per step");
+    }
+}
+
+inline void launch(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t k, ApplyMode mode, const Launch &L,
                    hipStream_t s) {
     if (mode == ApplyMode::Plain) return launch_xy<false, double, double>(A, x, y, k, L, s);
     if (xt == FieldType::F32 && yt == FieldType::F32) return launch_xy<true, float, float>(A, x, y, k, L, s);
@@ -226,10 +275,15 @@ inline std::mutex &tune_mutex() {
     return m;
 }
 
-inline Launch heuristic(index_t k) {
+/// Without a tuned entry: enough field chunks to give every CU a few
+/// workgroups, 4 fields per step (2 for FP64, whose gathers are twice as wide).
+inline Launch heuristic(const InterpolationMatrix &A, index_t k, ApplyMode mode) {
     Launch L;
-    L.tile = k >= 16 ? 16 : (k >= 8 ? 8 : (k >= 4 ? 4 : (k >= 2 ? 2 : 1)));
-    L.rows_per_lane = 1;
+    L.fu = (mode == ApplyMode::Plain) ? 2 : 4;
+    if (k < L.fu) L.fu = k >= 2 ? 2 : 1;
+    const index_t nb = (A.n_seg + kBlock - 1) / kBlock;
+    index_t want = (2048 + nb - 1) / std::max<index_t>(nb, 1);  // ~8 workgroups per CU
+    L.chunks = static_cast<int>(std::max<index_t>(1, std::min<index_t>(want, (k + L.fu - 1) / L.fu)));
     return L;
 }
 
@@ -237,7 +291,7 @@ inline Launch choose(const InterpolationMatrix &A, index_t k, FieldType xt, Fiel
     std::lock_guard<std::mutex> g(tune_mutex());
     auto it = tune_cache().find(TuneKey{A.n_dst, A.nnz, k, static_cast<int>(xt), static_cast<int>(yt), static_cast<int>(mode)});
     if (it != tune_cache().end()) return it->second;
-    return heuristic(k);
+    return heuristic(A, k, mode);
 }
 
 }  // namespace axis_hip::apply_detail

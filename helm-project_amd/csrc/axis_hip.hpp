@@ -85,8 +85,17 @@ class InterpolationMatrix {
     DeviceBuffer<double> frac_a, area_a;  // [n_src]
     DeviceBuffer<double> frac_b, area_b;  // [n_dst]
     int max_row_nnz = 0;
-    int n_long_rows = 0;                 // rows the apply hands to the wave-per-row kernel
-    DeviceBuffer<std::int32_t> long_rows;
+    // Apply-side layout, built once by prepare_apply(): rows cut into
+    // segments of at most seg_cap (E) nonzeros, in processing order.  A row
+    // longer than E (a "long row") owns slots long_slot[j] .. long_slot[j+1]
+    // of the FP64 partial-sum workspace, one per segment.
+    int seg_cap = 16;
+    index_t n_seg = 0;
+    DeviceBuffer<std::int32_t> seg_row, seg_len, seg_slot;
+    DeviceBuffer<index_t> seg_begin;
+    int n_long_rows = 0, n_slots = 0;
+    DeviceBuffer<std::int32_t> long_rows, long_slot;
+    DeviceBuffer<double> partial_ws;
     // diagnostics of the generation that produced it
     index_t n_candidates = 0, n_host_arbitrated = 0, n_cap_borderline = 0;
     double gen_ms = 0.0;
@@ -119,9 +128,9 @@ void batch_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, F
 
 ConservationReport check_conservation(const InterpolationMatrix &A, const double *d_src, const double *d_dst, NormType norm, hipStream_t s);
 
-/// Times the candidate launch shapes (fields per lane x rows per lane) of
+/// Times the candidate launch shapes (fields per step x field chunks) of
 /// batch_apply on the given device fields and caches the fastest for this
-/// (n_dst, nnz, k, types, mode).  Returns the chosen field tile and rows per lane.
+/// (n_dst, nnz, k, types, mode).  Returns {fields per step, field chunks}.
 std::pair<int, int> autotune_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t k, ApplyMode mode,
                                    hipStream_t s);
 

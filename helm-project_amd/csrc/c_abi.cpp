@@ -318,12 +318,12 @@ int axis_batch_apply_c(int matrix_handle, const void *src, int src_dtype, void *
 }
 
 int axis_autotune_apply_c(int matrix_handle, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n_fields, int mode, void *stream,
-                          int *fields_per_tile, int *rows_per_lane) {
+                          int *fields_per_step, int *field_chunks) {
     return guarded([&] {
         auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
         auto r = autotune_apply(*M, src, ftype(src_dtype), dst, ftype(dst_dtype), n_fields, amode(mode), static_cast<hipStream_t>(stream));
-        if (fields_per_tile) *fields_per_tile = r.first;
-        if (rows_per_lane) *rows_per_lane = r.second;
+        if (fields_per_step) *fields_per_step = r.first;
+        if (field_chunks) *field_chunks = r.second;
     });
 }
 

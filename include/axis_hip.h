@@ -125,11 +125,12 @@ int axis_batch_apply_c(int matrix_handle, const void *src, int src_dtype, void *
 int axis_batch_apply_async_c(int matrix_handle, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n_src, int64_t n_dst,
                              int64_t n_fields, int mode, void *stream);
 
-/* Autotuner (no reference counterpart): times the launch shapes of
+/* Autotuner (no reference counterpart): times the launch shapes (fields per
+ * step x field chunks) of
  * axis_batch_apply_c on these device fields, caches the fastest for this
  * (matrix size, n_fields, types, mode) and reports it. */
 int axis_autotune_apply_c(int matrix_handle, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n_fields, int mode, void *stream,
-                          int *fields_per_tile, int *rows_per_lane);
+                          int *fields_per_step, int *field_chunks);
 
 /* check_conservation (conservation.hpp:95-97, conservation.cpp:82-103) on
  * FP64 fields (host or device): out4 = {src_integral, dst_integral,
