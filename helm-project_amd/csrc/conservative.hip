@@ -143,6 +143,17 @@ __global__ void fill_rows_kernel(const index_t *cand_off, index_t n_dst, std::in
     for (index_t e = cand_off[j]; e < cand_off[j + 1]; ++e) pair_row[e] = static_cast<std::int32_t>(j);
 }
 
+// (pair, dst row, src cell) of the pairs handed to the host, in one buffer
+__global__ void gather_arbitrated_kernel(const std::int32_t *arb, int n, const std::int32_t *pair_row, const std::int32_t *cand_src,
+                                         std::int32_t *out) {
+    int k = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const std::int32_t p = arb[k];
+    out[3 * k] = p;
+    out[3 * k + 1] = pair_row[p];
+    out[3 * k + 2] = cand_src[p];
+}
+
 __global__ void scatter_overlaps_kernel(const std::int32_t *idx, const double *vals, int n, double *overlap) {
     int k = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k < n) overlap[idx[k]] = vals[k];
@@ -368,6 +379,7 @@ void check_longitude_ranges(const Mesh &src, const Mesh &dst, bool structured_fa
 
 std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, hipStream_t s) {
     const auto t0 = std::chrono::steady_clock::now();
+    PhaseTrace tr("conservative", s);
     const index_t n_src = src.n_cells(), n_dst = dst.n_cells();
     if (n_src >= (index_t(1) << 31) || n_dst >= (index_t(1) << 31)) throw InvalidArgument("generate_conservative: more than 2^31 cells");
     MeshGeom sg, dg;
@@ -380,12 +392,16 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
         throw InvalidArgument("generate_conservative: both meshes are rectilinear; the reference routes this pair to its analytic "
                               "rectilinear path, which this library does not provide");
     if (cfg.line_type != LineType::GreatCircle) throw InvalidArgument("generate_conservative: only LineType::GreatCircle is supported");
+    tr.mark("detect");
     prepare_geometry(src, sg, s);
+    tr.mark("geometry_src");
     prepare_geometry(dst, dg, s);
+    tr.mark("geometry_dst");
 
     DeviceBuffer<index_t> cand_off;
     DeviceBuffer<std::int32_t> cand_src;
     lbvh_candidates(sg.d_box.get(), n_src, dg.d_box.get(), n_dst, cand_off, cand_src, s);
+    tr.mark("lbvh");
     index_t n_pairs = 0;
     AXIS_HIP_CHECK(hipMemcpyAsync(&n_pairs, cand_off.get() + n_dst, sizeof(index_t), hipMemcpyDeviceToHost, s));
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
@@ -435,6 +451,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
                                                       counters.get() + 1);
         AXIS_LAUNCH_CHECK();
     }
+    tr.mark("overlap");
     int h_cnt[2] = {0, 0};
     AXIS_HIP_CHECK(hipMemcpyAsync(h_cnt, counters.get(), sizeof h_cnt, hipMemcpyDeviceToHost, s));
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
@@ -442,20 +459,20 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     M->n_cap_borderline = h_cnt[1];
     if (h_cnt[0] > 0) {
         const int na = h_cnt[0];
-        std::vector<std::int32_t> pairs(na), rows(na), srcs(na);
-        arb.download(pairs.data(), na, s);
+        DeviceBuffer<std::int32_t> d_trip(3 * static_cast<std::size_t>(na));
+        gather_arbitrated_kernel<<<grid_for(na, 256), 256, 0, s>>>(arb.get(), na, pair_row.get(), cand_src.get(), d_trip.get());
+        AXIS_LAUNCH_CHECK();
+        std::vector<std::int32_t> trip(3 * static_cast<std::size_t>(na)), pairs(na);
+        d_trip.download(trip.data(), 3 * static_cast<std::size_t>(na), s);
         AXIS_HIP_CHECK(hipStreamSynchronize(s));
-        std::sort(pairs.begin(), pairs.end());
         std::vector<double> vals(na);
-        for (int k = 0; k < na; ++k) {
-            AXIS_HIP_CHECK(hipMemcpyAsync(&rows[k], pair_row.get() + pairs[k], sizeof(std::int32_t), hipMemcpyDeviceToHost, s));
-            AXIS_HIP_CHECK(hipMemcpyAsync(&srcs[k], cand_src.get() + pairs[k], sizeof(std::int32_t), hipMemcpyDeviceToHost, s));
-        }
-        AXIS_HIP_CHECK(hipStreamSynchronize(s));
         HostSide HS{&src, &sg}, HD{&dst, &dg};
         parallel_range(
             na, [&](index_t b, index_t e) {
-                for (index_t k = b; k < e; ++k) vals[k] = host_pair_overlap(HS, HD, rows[k], srcs[k]);
+                for (index_t k = b; k < e; ++k) {
+                    pairs[k] = trip[3 * k];
+                    vals[k] = host_pair_overlap(HS, HD, trip[3 * k + 1], trip[3 * k + 2]);
+                }
             },
             64);
         DeviceBuffer<std::int32_t> d_idx(na);
@@ -467,6 +484,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
         AXIS_HIP_CHECK(hipStreamSynchronize(s));
     }
 
+    tr.mark("arbitration");
     // ---- CSR assembly
     DeviceBuffer<index_t> nzf(std::max<index_t>(n_pairs, 1)), pos(n_pairs + 1);
     if (n_pairs > 0) {
@@ -527,6 +545,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     if (n_dst > 0)
         AXIS_HIP_CHECK(hipMemcpyAsync(M->area_b.get(), dg.d_area.get(), sizeof(double) * n_dst, hipMemcpyDeviceToDevice, s));
     AXIS_HIP_CHECK(hipMemcpyAsync(&M->max_row_nnz, max_row.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    tr.mark("csr");
     if (cfg.unmapped == UnmappedAction::Error && n_dst > 0) {
         DeviceBuffer<unsigned long long> first(1);
         unsigned long long init = ~0ull, h_first = 0;

@@ -3,8 +3,12 @@
 // reference's bits).
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdint>
 #include <cstdlib>
 #include <thread>
@@ -45,6 +49,29 @@ void parallel_range(std::int64_t n, Fn fn, std::int64_t min_chunk = 4096) {
     }
     for (auto &t : th) t.join();
 }
+
+/// Phase timer for AXIS_TRACE=1: each mark() waits for the stream and prints
+/// the wall time since the previous mark to stderr (the role Kokkos Tools
+/// region hooks play for the reference).  Disabled, it costs one branch.
+class PhaseTrace {
+   public:
+    PhaseTrace(const char *scope, hipStream_t s) : scope_(scope), s_(s), on_(std::getenv("AXIS_TRACE") != nullptr) {
+        if (on_) last_ = std::chrono::steady_clock::now();
+    }
+    void mark(const char *what) {
+        if (!on_) return;
+        (void)hipStreamSynchronize(s_);
+        auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "axis[trace] %s.%s %.3f ms\n", scope_, what, std::chrono::duration<double, std::milli>(now - last_).count());
+        last_ = now;
+    }
+
+   private:
+    const char *scope_;
+    hipStream_t s_;
+    bool on_;
+    std::chrono::steady_clock::time_point last_;
+};
 
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kDeg2Rad = kPi / 180.0;  // folded once, as spherical_geometry.hpp:36 / spherical_cap_filter.hpp:40

@@ -18,6 +18,7 @@
 #include <cstdint>
 
 #include "axis_hip.hpp"
+#include "host_util.hpp"
 
 namespace axis_hip::detail {
 
@@ -79,7 +80,7 @@ __device__ inline int delta(const std::uint32_t *codes, int n, int i, int j) {
 }
 
 // internal node i (0..n-2): children and parents
-__global__ void karras_kernel(const std::uint32_t *codes, int n, std::uint32_t *left, std::uint32_t *right, int *parent) {
+__global__ void karras_kernel(const std::uint32_t *codes, int n, std::uint32_t *left, std::uint32_t *right, int *parent, int *rlo, int *rhi) {
     int i = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     int d = (delta(codes, n, i, i + 1) - delta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -99,6 +100,8 @@ __global__ void karras_kernel(const std::uint32_t *codes, int n, std::uint32_t *
     } while (t > 1);
     int gamma = i + s * d + (d < 0 ? d : 0);
     int lo = i < j ? i : j, hi = i < j ? j : i;
+    rlo[i] = lo;
+    rhi[i] = hi;
     std::uint32_t lc = (lo == gamma) ? (kLeaf | static_cast<std::uint32_t>(gamma)) : static_cast<std::uint32_t>(gamma);
     std::uint32_t rc = (hi == gamma + 1) ? (kLeaf | static_cast<std::uint32_t>(gamma + 1)) : static_cast<std::uint32_t>(gamma + 1);
     left[i] = lc;
@@ -108,49 +111,86 @@ __global__ void karras_kernel(const std::uint32_t *codes, int n, std::uint32_t *
     parent[(rc & kLeaf) ? (n - 1 + (rc & ~kLeaf)) : rc] = i;
 }
 
-// leaf boxes -> internal boxes, bottom-up; the second child to arrive at a
-// node merges the two child boxes and climbs on
-__global__ void refit_kernel(const float *boxes, const std::uint32_t *ids, int n, const std::uint32_t *left, const std::uint32_t *right,
-                             const int *parent, float *node_box, int *arrive) {
-    int k = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    int p = parent[n - 1 + k];
-    while (p >= 0) {
-        __threadfence();
-        if (atomicAdd(&arrive[p], 1) == 0) return;
-        __threadfence();
-        FBox b;
-        std::uint32_t c[2] = {left[p], right[p]};
+// Bottom-up box refit in two phases, so that no arrival counter is shared
+// between workgroups (an agent-scope fence per tree level costs an L2
+// write-back on this GPU):
+//   1. each workgroup owns a chunk of kRefitChunk sorted leaves and climbs
+//      from its leaves through the internal nodes whose leaf range lies inside
+//      the chunk, with workgroup-scope arrival counters (the second child to
+//      arrive merges the two child boxes and climbs on);
+//   2. one workgroup finishes the nodes whose range crosses a chunk boundary,
+//      starting from their children that phase 1 completed.
+constexpr int kRefitChunk = 1024;
+
+__device__ inline void merge_node(const float *boxes, const std::uint32_t *ids, const std::uint32_t *left, const std::uint32_t *right,
+                                  float *node_box, int p) {
+    FBox b;
+    std::uint32_t c[2] = {left[p], right[p]};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        b.lo[a] = INFINITY;
+        b.hi[a] = -INFINITY;
+    }
+    for (int h = 0; h < 2; ++h) {
+        FBox x = (c[h] & kLeaf) ? load_box(boxes, ids[c[h] & ~kLeaf]) : load_box(node_box, c[h]);
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            b.lo[a] = INFINITY;
-            b.hi[a] = -INFINITY;
+            b.lo[a] = fminf(b.lo[a], x.lo[a]);
+            b.hi[a] = fmaxf(b.hi[a], x.hi[a]);
         }
-        for (int h = 0; h < 2; ++h) {
-            FBox x;
-            if (c[h] & kLeaf) {
-                x = load_box(boxes, ids[c[h] & ~kLeaf]);
-            } else {
-                // written by another wave, possibly on another XCD: coherent loads
-                float *nb = node_box + 6 * static_cast<std::int64_t>(c[h]);
+    }
 #pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    x.lo[a] = __hip_atomic_load(nb + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    x.hi[a] = __hip_atomic_load(nb + 3 + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-            }
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                b.lo[a] = fminf(b.lo[a], x.lo[a]);
-                b.hi[a] = fmaxf(b.hi[a], x.hi[a]);
-            }
+    for (int a = 0; a < 3; ++a) {
+        node_box[6 * static_cast<std::int64_t>(p) + a] = b.lo[a];
+        node_box[6 * static_cast<std::int64_t>(p) + 3 + a] = b.hi[a];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) refit_chunk_kernel(const float *boxes, const std::uint32_t *ids, int n, const std::uint32_t *left,
+                                                             const std::uint32_t *right, const int *parent, const int *rlo, const int *rhi,
+                                                             float *node_box, int *arrive) {
+    const int c0 = static_cast<int>(blockIdx.x) * kRefitChunk;
+    const int c1 = min(c0 + kRefitChunk, n);
+    for (int k = c0 + static_cast<int>(threadIdx.x); k < c1; k += kBlock) {
+        int p = parent[n - 1 + k];
+        while (p >= 0 && rlo[p] >= c0 && rhi[p] < c1) {
+            if (__hip_atomic_fetch_add(&arrive[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0) break;
+            merge_node(boxes, ids, left, right, node_box, p);
+            p = parent[p];
         }
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            node_box[6 * static_cast<std::int64_t>(p) + a] = b.lo[a];
-            node_box[6 * static_cast<std::int64_t>(p) + 3 + a] = b.hi[a];
+    }
+}
+
+__device__ inline bool crosses_chunk(const int *rlo, const int *rhi, int p) { return rlo[p] / kRefitChunk != rhi[p] / kRefitChunk; }
+
+// children of chunk-crossing nodes that phase 1 completed (leaves included)
+__global__ void refit_frontier_kernel(int n, const std::uint32_t *left, const std::uint32_t *right, const int *rlo, const int *rhi, int *list,
+                                      int *count) {
+    const int p = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n - 1 || !crosses_chunk(rlo, rhi, p)) return;
+    const std::uint32_t c[2] = {left[p], right[p]};
+    for (int h = 0; h < 2; ++h) {
+        const bool leaf = (c[h] & kLeaf) != 0;
+        if (leaf || !crosses_chunk(rlo, rhi, static_cast<int>(c[h]))) {
+            // encode leaves as n - 1 + leaf index, internal nodes as themselves
+            list[atomicAdd(count, 1)] = leaf ? n - 1 + static_cast<int>(c[h] & ~kLeaf) : static_cast<int>(c[h]);
         }
-        p = parent[p];
+    }
+}
+
+constexpr int kTopBlock = 1024;
+
+__global__ void __launch_bounds__(kTopBlock) refit_top_kernel(const float *boxes, const std::uint32_t *ids, int n, const std::uint32_t *left,
+                                                              const std::uint32_t *right, const int *parent, float *node_box, int *arrive,
+                                                              const int *list, const int *count) {
+    const int m = *count;
+    for (int t = static_cast<int>(threadIdx.x); t < m; t += kTopBlock) {
+        int p = parent[list[t]];
+        while (p >= 0) {
+            if (__hip_atomic_fetch_add(&arrive[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0) break;
+            merge_node(boxes, ids, left, right, node_box, p);
+            p = parent[p];
+        }
     }
 }
 
@@ -204,10 +244,12 @@ void lbvh_candidates(const float *src_boxes, index_t n_src, const float *dst_box
                      DeviceBuffer<std::int32_t> &cand_src, hipStream_t s) {
     if (n_src >= (index_t(1) << 31) - 1) throw InvalidArgument("LBVH: more than 2^31 source cells");
     const int n = static_cast<int>(n_src);
+    PhaseTrace tr("lbvh", s);
     cand_off.resize(n_dst + 1);
     DeviceBuffer<std::uint32_t> codes(std::max(n, 1)), ids(std::max(n, 1));
     DeviceBuffer<std::uint32_t> left(std::max(n - 1, 1)), right(std::max(n - 1, 1));
     DeviceBuffer<int> parent(std::max(2 * n - 1, 1)), arrive(std::max(n - 1, 1)), err(1);
+    DeviceBuffer<int> rlo(std::max(n - 1, 1)), rhi(std::max(n - 1, 1)), frontier(std::max(2 * n, 2)), n_frontier(1);
     DeviceBuffer<float> node_box(6 * std::max(n - 1, 1));
     AXIS_HIP_CHECK(hipMemsetAsync(err.get(), 0, sizeof(int), s));
     if (n > 0) {
@@ -215,15 +257,24 @@ void lbvh_candidates(const float *src_boxes, index_t n_src, const float *dst_box
         AXIS_LAUNCH_CHECK();
         radix_sort_pairs_u32(codes.get(), ids.get(), n, 30, s);
     }
+    tr.mark("morton_sort");
     if (n > 1) {
         AXIS_HIP_CHECK(hipMemsetAsync(parent.get(), 0xFF, sizeof(int) * (2 * n - 1), s));  // root's parent = -1
         AXIS_HIP_CHECK(hipMemsetAsync(arrive.get(), 0, sizeof(int) * (n - 1), s));
-        karras_kernel<<<grid_for(n - 1, kBlock), kBlock, 0, s>>>(codes.get(), n, left.get(), right.get(), parent.get());
+        karras_kernel<<<grid_for(n - 1, kBlock), kBlock, 0, s>>>(codes.get(), n, left.get(), right.get(), parent.get(), rlo.get(), rhi.get());
         AXIS_LAUNCH_CHECK();
-        refit_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(src_boxes, ids.get(), n, left.get(), right.get(), parent.get(), node_box.get(),
-                                                            arrive.get());
+        refit_chunk_kernel<<<static_cast<unsigned>((n + kRefitChunk - 1) / kRefitChunk), kBlock, 0, s>>>(
+            src_boxes, ids.get(), n, left.get(), right.get(), parent.get(), rlo.get(), rhi.get(), node_box.get(), arrive.get());
+        AXIS_LAUNCH_CHECK();
+        AXIS_HIP_CHECK(hipMemsetAsync(n_frontier.get(), 0, sizeof(int), s));
+        refit_frontier_kernel<<<grid_for(n - 1, kBlock), kBlock, 0, s>>>(n, left.get(), right.get(), rlo.get(), rhi.get(), frontier.get(),
+                                                                         n_frontier.get());
+        AXIS_LAUNCH_CHECK();
+        refit_top_kernel<<<1, kTopBlock, 0, s>>>(src_boxes, ids.get(), n, left.get(), right.get(), parent.get(), node_box.get(), arrive.get(),
+                                                 frontier.get(), n_frontier.get());
         AXIS_LAUNCH_CHECK();
     }
+    tr.mark("build_refit");
     DeviceBuffer<index_t> count(std::max<index_t>(n_dst, 1));
     if (n_dst > 0) {
         query_kernel<false><<<grid_for(n_dst, kBlock), kBlock, 0, s>>>(src_boxes, ids.get(), n, left.get(), right.get(), node_box.get(), dst_boxes,
@@ -236,6 +287,7 @@ void lbvh_candidates(const float *src_boxes, index_t n_src, const float *dst_box
     AXIS_HIP_CHECK(hipMemcpyAsync(&total, cand_off.get() + n_dst, sizeof(index_t), hipMemcpyDeviceToHost, s));
     AXIS_HIP_CHECK(hipMemcpyAsync(&h_err, err.get(), sizeof(int), hipMemcpyDeviceToHost, s));
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    tr.mark("query_count");
     if (h_err) throw RuntimeError("LBVH traversal stack overflow");
     // the reference stores candidate offsets as int (weight_generator.cpp:247-254)
     if (total >= (index_t(1) << 31)) throw RuntimeError("LBVH: candidate pair count exceeds 2^31");
@@ -246,6 +298,7 @@ void lbvh_candidates(const float *src_boxes, index_t n_src, const float *dst_box
         AXIS_LAUNCH_CHECK();
     }
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    tr.mark("query_fill");
 }
 
 }  // namespace axis_hip::detail

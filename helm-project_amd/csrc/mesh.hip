@@ -452,6 +452,7 @@ void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s) {
     g.n_nodes = n_nodes;
     g.n_cells = n_cells;
     const bool deg = m.coord_system() == CoordinateSystem::SphericalDeg;
+    PhaseTrace tr("geometry", s);
     // node xyz with host libm (extract_cell_polygon_spherical, weight_generator.cpp:462-470)
     g.h_xyz.resize(3 * static_cast<std::size_t>(n_nodes));
     std::vector<double> h_xyz_deg;
@@ -471,6 +472,7 @@ void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s) {
             }
         }
     });
+    tr.mark("host_sincos");
     g.d_xyz.resize(std::max<std::size_t>(g.h_xyz.size(), 3));
     g.d_xyz.upload(g.h_xyz.data(), g.h_xyz.size(), s);
     DeviceBuffer<double> d_xyz_deg;
@@ -508,6 +510,7 @@ void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s) {
             g.d_box.get(), recheck.get(), n_recheck.get());
         AXIS_LAUNCH_CHECK();
     }
+    tr.mark("upload_trig_cellprep");
     int h_n = 0;
     AXIS_HIP_CHECK(hipMemcpyAsync(&h_n, n_recheck.get(), sizeof(int), hipMemcpyDeviceToHost, s));
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
