@@ -21,7 +21,7 @@ _HERE = pathlib.Path(__file__).resolve().parent
 LIB_PATH = pathlib.Path(os.environ.get("AXIS_HIP_LIBRARY", _HERE / "lib" / "libaxis_hip.so"))
 
 F64, F32, BF16 = 0, 1, 2
-PLAIN, MIXED = 0, 1
+PLAIN, MIXED, MIXED_ACC64 = 0, 1, 2
 BILINEAR, NEAREST, CONSERVATIVE = 0, 1, 2
 # InterpolationMethod declaration order (regrid_config.hpp:24-33)
 M_BILINEAR, M_NEAREST, M_BICUBIC, M_PATCH, M_CONSERVATIVE1, M_CONSERVATIVE2 = range(6)
@@ -73,7 +73,7 @@ def lib():
         "axis_batch_apply_async_c": [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int64, C.c_int64, C.c_int64,
                                      C.c_int, C.c_void_p],
         "axis_autotune_apply_c": [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int64, C.c_int, C.c_void_p,
-                                  C.POINTER(C.c_int), C.POINTER(C.c_int)],
+                                  C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
         "axis_check_conservation_c": [C.c_int, C.c_void_p, C.c_void_p, C.c_int, _D],
         "axis_destroy_mesh_c": [C.c_int],
         "axis_destroy_matrix_c": [C.c_int],
@@ -243,10 +243,11 @@ class Matrix:
         return dst
 
     def autotune(self, src, dst, n_fields, src_dtype, dst_dtype, mode, stream=None):
-        t, r = C.c_int(0), C.c_int(0)
+        """Times the launch shapes on these fields; returns (fields per step, field chunks, staged)."""
+        t, r, st = C.c_int(0), C.c_int(0), C.c_int(0)
         _check(lib().axis_autotune_apply_c(self.handle, _ptr(src), src_dtype, _ptr(dst), dst_dtype, n_fields, mode,
-                                           _stream(stream), C.byref(t), C.byref(r)))
-        return t.value, r.value
+                                           _stream(stream), C.byref(t), C.byref(r), C.byref(st)))
+        return t.value, r.value, bool(st.value)
 
     def check_conservation(self, src, dst, norm_type=0):
         out = np.zeros(4)

@@ -38,11 +38,211 @@ __global__ void split_kernel(const double *v, const index_t *col, index_t nnz, f
     c32[e] = static_cast<std::int32_t>(col[e]);
 }
 
+// histogram of row lengths (0 .. kMaxE, and "longer"): per-block LDS counts,
+// then one global atomic per bin and block
+__global__ void len_hist_kernel(const index_t *rp, index_t n_dst, index_t *hist) {
+    constexpr int kH = apply_detail::kMaxE + 2;
+    __shared__ unsigned cnt[kH];
+    if (threadIdx.x < kH) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    index_t r = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r < n_dst) {
+        const index_t len = rp[r + 1] - rp[r];
+        atomicAdd(&cnt[len > apply_detail::kMaxE ? apply_detail::kMaxE + 1 : len], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < kH && cnt[threadIdx.x])
+        atomicAdd(reinterpret_cast<unsigned long long *>(hist + threadIdx.x), static_cast<unsigned long long>(cnt[threadIdx.x]));
+}
+
+// sort key of row r: (coarse spatial cell << 32) | r; rows without a key keep index order
+__global__ void order_key_kernel(const std::uint32_t *row_key, int shift, index_t n_dst, std::uint64_t *key) {
+    index_t r = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= n_dst) return;
+    const std::uint64_t coarse = row_key ? static_cast<std::uint64_t>(row_key[r] >> shift) : 0ull;
+    key[r] = (coarse << 32) | static_cast<std::uint64_t>(r);
+}
+
+__global__ void seg_count_kernel(const std::uint64_t *key, const index_t *rp, index_t n_dst, int E, index_t *n_seg, index_t *n_slot,
+                                 index_t *n_long) {
+    index_t p = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n_dst) return;
+    const index_t r = static_cast<index_t>(key[p] & 0xFFFFFFFFull);
+    const index_t len = rp[r + 1] - rp[r];
+    const bool lng = len > E;
+    const index_t ns = lng ? (len + E - 1) / E : 1;
+    n_seg[p] = ns;
+    n_slot[p] = lng ? ns : 0;
+    n_long[p] = lng ? 1 : 0;
+}
+
+__global__ void seg_fill_kernel(const std::uint64_t *key, const index_t *rp, index_t n_dst, int E, const index_t *seg_off, const index_t *slot_off,
+                                const index_t *long_off, std::int32_t *seg_row, index_t *seg_begin, std::int32_t *seg_len, std::int32_t *seg_slot,
+                                std::int32_t *long_rows, std::int32_t *long_slot) {
+    index_t p = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n_dst) return;
+    const index_t r = static_cast<index_t>(key[p] & 0xFFFFFFFFull);
+    const index_t b = rp[r], len = rp[r + 1] - b;
+    const bool lng = len > E;
+    const index_t s0 = seg_off[p];
+    if (!lng) {
+        seg_row[s0] = static_cast<std::int32_t>(r);
+        seg_begin[s0] = b;
+        seg_len[s0] = static_cast<std::int32_t>(len);
+        seg_slot[s0] = -1;
+        return;
+    }
+    const index_t ns = (len + E - 1) / E;
+    for (index_t q = 0; q < ns; ++q) {
+        seg_row[s0 + q] = static_cast<std::int32_t>(r);
+        seg_begin[s0 + q] = b + q * E;
+        seg_len[s0 + q] = static_cast<std::int32_t>(len - q * E < E ? len - q * E : E);
+        seg_slot[s0 + q] = static_cast<std::int32_t>(slot_off[p] + q);
+    }
+    long_rows[long_off[p]] = static_cast<std::int32_t>(r);
+    long_slot[long_off[p]] = static_cast<std::int32_t>(slot_off[p]);
+}
+
+// Sorts the 256 segments of each tile by decreasing length (stable), so that
+// the lanes of one wave run nearly the same number of terms.
+__global__ void __launch_bounds__(apply_detail::kBlock) tile_sort_kernel(std::int32_t *seg_row, index_t *seg_begin, std::int32_t *seg_len,
+                                                                         std::int32_t *seg_slot, index_t n_seg) {
+    constexpr int kB = apply_detail::kBlock;
+    __shared__ int len[kB];
+    const index_t i = static_cast<index_t>(blockIdx.x) * kB + threadIdx.x;
+    const bool valid = i < n_seg;
+    const int me = valid ? seg_len[i] : -1;
+    std::int32_t row = 0, slot = 0;
+    index_t beg = 0;
+    if (valid) {
+        row = seg_row[i];
+        beg = seg_begin[i];
+        slot = seg_slot[i];
+    }
+    len[threadIdx.x] = me;
+    __syncthreads();
+    int rank = 0;
+    for (int j = 0; j < kB; ++j) {
+        const int o = len[j];
+        rank += (o > me) || (o == me && j < static_cast<int>(threadIdx.x));
+    }
+    __syncthreads();
+    if (valid) {
+        const index_t d = static_cast<index_t>(blockIdx.x) * kB + rank;  // invalid lanes (len -1) rank last
+        seg_row[d] = row;
+        seg_begin[d] = beg;
+        seg_len[d] = me;
+        seg_slot[d] = slot;
+    }
+}
+
+// exclusive scan of one int per thread over a 256-thread workgroup
+__device__ inline int block_exclusive_scan(int v, int *wsum, int &total) {
+    const int lane = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    int before = 0;
+    total = 0;
+    for (int w = 0; w < apply_detail::kBlock / 64; ++w) {
+        if (w < wave) before += wsum[w];
+        total += wsum[w];
+    }
+    __syncthreads();
+    return before + x - v;
+}
+
+constexpr int pow2_at_least(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// Tile t = segments [256 t, 256 t + 256): sorts the tile's column indices in
+// LDS (bitonic), keeps the distinct ones, and records for each nonzero the
+// position of its column among them.
+template <int E>
+__global__ void __launch_bounds__(apply_detail::kBlock) tile_build_kernel(const index_t *seg_begin, const std::int32_t *seg_len, index_t n_seg,
+                                                                          const std::int32_t *col32, std::int32_t *tile_nu, std::int32_t *tile_col,
+                                                                          std::uint16_t *lcol) {
+    constexpr int kB = apply_detail::kBlock;
+    constexpr int SZ = pow2_at_least(kB * E);
+    constexpr int C = SZ / kB;
+    constexpr int kEmpty = 0x7FFFFFFF;
+    __shared__ int key[SZ];
+    __shared__ int uniq[apply_detail::kTileCols];
+    __shared__ int wsum[kB / 64];
+    const index_t t = blockIdx.x;
+    const int tid = static_cast<int>(threadIdx.x);
+    const index_t i = t * kB + tid;
+    const int n = i < n_seg ? seg_len[i] : 0;
+    const index_t b = i < n_seg ? seg_begin[i] : 0;
+    for (int q = tid; q < SZ; q += kB) key[q] = kEmpty;
+    __syncthreads();
+    for (int e = 0; e < n; ++e) key[tid * E + e] = col32[b + e];
+    __syncthreads();
+    for (int kk = 2; kk <= SZ; kk <<= 1) {
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            for (int q = tid; q < SZ; q += kB) {
+                const int p = q ^ j;
+                if (p > q) {
+                    const int x = key[q], y = key[p];
+                    const bool up = (q & kk) == 0;
+                    if ((x > y) == up) {
+                        key[q] = y;
+                        key[p] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    int cnt = 0;
+    for (int q = tid * C; q < tid * C + C; ++q) {
+        const int v = key[q];
+        if (v != kEmpty && (q == 0 || key[q - 1] != v)) ++cnt;
+    }
+    int total = 0;
+    int pos = block_exclusive_scan(cnt, wsum, total);
+    if (total > apply_detail::kTileCols) {
+        if (tid == 0) tile_nu[t] = -1;
+        return;  // whole workgroup
+    }
+    for (int q = tid * C; q < tid * C + C; ++q) {
+        const int v = key[q];
+        if (v != kEmpty && (q == 0 || key[q - 1] != v)) {
+            uniq[pos] = v;
+            tile_col[t * apply_detail::kTileCols + pos] = v;
+            ++pos;
+        }
+    }
+    __syncthreads();
+    for (int e = 0; e < n; ++e) {
+        const int c = col32[b + e];
+        int lo = 0, hi = total;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (uniq[mid] < c)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        lcol[b + e] = static_cast<std::uint16_t>(lo);
+    }
+    if (tid == 0) tile_nu[t] = total;
+}
+
 }  // namespace
 
 void InterpolationMatrix::prepare_apply(hipStream_t s) {
-    if (col32.size() >= static_cast<std::size_t>(std::max<index_t>(nnz, 1)) && val_hi.size() > 0) return;
-    if (n_src >= (index_t(1) << 31)) throw InvalidArgument("apply: more than 2^31 source cells");
+    if (apply_ready) return;
+    if (n_src >= (index_t(1) << 31) - 1) throw InvalidArgument("apply: more than 2^31 - 2 source cells");
+    if (n_dst >= (index_t(1) << 31)) throw InvalidArgument("apply: more than 2^31 destination cells");
     const index_t n1 = std::max<index_t>(nnz, 1);
     col32.resize(n1);
     val_hi.resize(n1);
@@ -53,80 +253,93 @@ void InterpolationMatrix::prepare_apply(hipStream_t s) {
     }
     // E = the smallest register segment (4, 8, 12, 16) that leaves at most
     // 0.1% of the rows longer than one segment
-    std::vector<index_t> rp(n_dst + 1);
-    row_ptr.download(rp.data(), n_dst + 1, s);
-    AXIS_HIP_CHECK(hipStreamSynchronize(s));
-    index_t hist[apply_detail::kMaxE + 2] = {0};
-    for (index_t r = 0; r < n_dst; ++r) {
-        index_t len = rp[r + 1] - rp[r];
-        hist[len > apply_detail::kMaxE ? apply_detail::kMaxE + 1 : len]++;
+    constexpr int kH = apply_detail::kMaxE + 2;
+    DeviceBuffer<index_t> d_hist(kH);
+    AXIS_HIP_CHECK(hipMemsetAsync(d_hist.get(), 0, sizeof(index_t) * kH, s));
+    if (n_dst > 0) {
+        len_hist_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(row_ptr.get(), n_dst, d_hist.get());
+        AXIS_LAUNCH_CHECK();
     }
+    index_t hist[kH] = {0};
+    d_hist.download(hist, kH, s);
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
     seg_cap = apply_detail::kMaxE;
     for (int cap : {4, 8, 12, 16}) {
         index_t above = 0;
-        for (int l = cap + 1; l <= apply_detail::kMaxE + 1; ++l) above += hist[l];
+        for (int l = cap + 1; l < kH; ++l) above += hist[l];
         if (above * 1000 <= n_dst) {
             seg_cap = cap;
             break;
         }
     }
-    // rows are processed in index order: a destination mesh numbered along
-    // rows of cells (cubed-sphere faces, lat-lon rows) gives each wave
-    // neighbouring cells, whose source columns share cache lines, and full-line
-    // stores of Y; regrouping rows by source column was measured slower
-    std::vector<std::int32_t> order(n_dst);
-    for (index_t r = 0; r < n_dst; ++r) order[r] = static_cast<std::int32_t>(r);
-    std::vector<std::int32_t> h_row, h_len, h_slot, h_lrow, h_lslot;
-    std::vector<index_t> h_begin;
-    h_row.reserve(n_dst);
-    h_len.reserve(n_dst);
-    h_slot.reserve(n_dst);
-    h_begin.reserve(n_dst);
-    int slots = 0;
-    for (std::int32_t r : order) {
-        const index_t b = rp[r], e = rp[r + 1];
-        if (e == b) continue;  // empty rows are appended after the others
-        if (e - b <= seg_cap) {
-            h_row.push_back(r);
-            h_begin.push_back(b);
-            h_len.push_back(static_cast<std::int32_t>(e - b));
-            h_slot.push_back(-1);
-        } else {
-            h_lrow.push_back(r);
-            h_lslot.push_back(slots);
-            for (index_t q = b; q < e; q += seg_cap) {
-                h_row.push_back(r);
-                h_begin.push_back(q);
-                h_len.push_back(static_cast<std::int32_t>(std::min<index_t>(seg_cap, e - q)));
-                h_slot.push_back(slots++);
-            }
-        }
+    // Processing order: rows grouped by a coarse spatial cell of their
+    // centroid (about 256 destination cells per cell), index order inside a
+    // cell.  A tile of 256 segments then covers a compact patch, whose source
+    // columns are few and shared, and writes Y in runs of neighbouring indices.
+    const index_t nd1 = std::max<index_t>(n_dst, 1);
+    DeviceBuffer<std::uint64_t> key(nd1);
+    int coarse_bits = 0;
+    if (row_key.size() >= static_cast<std::size_t>(nd1) && n_dst > 0) {
+        // cells of side 2.2 / 2^b on the [-1.1, 1.1]^3 Morton grid holding ~256 unit-sphere cells
+        const double side = std::sqrt(4.0 * kPi * 256.0 / static_cast<double>(n_dst));
+        coarse_bits = static_cast<int>(std::lround(std::log2(2.2 / side)));
+        coarse_bits = std::clamp(coarse_bits, 1, 10);
     }
-    // empty rows: a zero-length segment writes y = 0 (the reference's spmv
-    // overwrites every destination entry)
-    for (std::int32_t r : order)
-        if (rp[r + 1] == rp[r]) {
-            h_row.push_back(r);
-            h_begin.push_back(rp[r]);
-            h_len.push_back(0);
-            h_slot.push_back(-1);
-        }
-    h_lslot.push_back(slots);
-    n_seg = static_cast<index_t>(h_row.size());
-    n_long_rows = static_cast<int>(h_lrow.size());
-    n_slots = slots;
-    auto put32 = [&](DeviceBuffer<std::int32_t> &d, const std::vector<std::int32_t> &h) {
-        d.resize(std::max<std::size_t>(h.size(), 1));
-        if (!h.empty()) d.upload(h.data(), h.size(), s);
-    };
-    put32(seg_row, h_row);
-    put32(seg_len, h_len);
-    put32(seg_slot, h_slot);
-    put32(long_rows, h_lrow);
-    put32(long_slot, h_lslot);
-    seg_begin.resize(std::max<std::size_t>(h_begin.size(), 1));
-    if (!h_begin.empty()) seg_begin.upload(h_begin.data(), h_begin.size(), s);
+    if (n_dst > 0) {
+        order_key_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(coarse_bits ? row_key.get() : nullptr, 3 * (10 - coarse_bits), n_dst, key.get());
+        AXIS_LAUNCH_CHECK();
+        if (coarse_bits) detail::radix_sort_keys_u64(key.get(), static_cast<std::size_t>(n_dst), s, 32 + 3 * coarse_bits);
+    }
+    DeviceBuffer<index_t> c_seg(nd1), c_slot(nd1), c_long(nd1), o_seg(nd1 + 1), o_slot(nd1 + 1), o_long(nd1 + 1);
+    if (n_dst > 0) {
+        seg_count_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(key.get(), row_ptr.get(), n_dst, seg_cap, c_seg.get(), c_slot.get(), c_long.get());
+        AXIS_LAUNCH_CHECK();
+    }
+    detail::exclusive_scan_i64(c_seg.get(), o_seg.get(), n_dst, s);
+    detail::exclusive_scan_i64(c_slot.get(), o_slot.get(), n_dst, s);
+    detail::exclusive_scan_i64(c_long.get(), o_long.get(), n_dst, s);
+    index_t tot[3] = {0, 0, 0};
+    AXIS_HIP_CHECK(hipMemcpyAsync(&tot[0], o_seg.get() + n_dst, sizeof(index_t), hipMemcpyDeviceToHost, s));
+    AXIS_HIP_CHECK(hipMemcpyAsync(&tot[1], o_slot.get() + n_dst, sizeof(index_t), hipMemcpyDeviceToHost, s));
+    AXIS_HIP_CHECK(hipMemcpyAsync(&tot[2], o_long.get() + n_dst, sizeof(index_t), hipMemcpyDeviceToHost, s));
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    if (tot[1] >= (index_t(1) << 31)) throw InvalidArgument("apply: too many long-row segments");
+    n_seg = tot[0];
+    n_slots = static_cast<int>(tot[1]);
+    n_long_rows = static_cast<int>(tot[2]);
+    const std::size_t ns1 = static_cast<std::size_t>(std::max<index_t>(n_seg, 1));
+    seg_row.resize(ns1);
+    seg_begin.resize(ns1);
+    seg_len.resize(ns1);
+    seg_slot.resize(ns1);
+    long_rows.resize(static_cast<std::size_t>(n_long_rows) + 1);
+    long_slot.resize(static_cast<std::size_t>(n_long_rows) + 1);
+    if (n_dst > 0) {
+        seg_fill_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(key.get(), row_ptr.get(), n_dst, seg_cap, o_seg.get(), o_slot.get(), o_long.get(),
+                                                            seg_row.get(), seg_begin.get(), seg_len.get(), seg_slot.get(), long_rows.get(),
+                                                            long_slot.get());
+        AXIS_LAUNCH_CHECK();
+    }
+    const std::int32_t end_slot = n_slots;
+    AXIS_HIP_CHECK(hipMemcpyAsync(long_slot.get() + n_long_rows, &end_slot, sizeof(std::int32_t), hipMemcpyHostToDevice, s));
+    // tiles
+    n_tiles = (n_seg + apply_detail::kBlock - 1) / apply_detail::kBlock;
+    tile_nu.resize(static_cast<std::size_t>(std::max<index_t>(n_tiles, 1)));
+    tile_col.resize(static_cast<std::size_t>(std::max<index_t>(n_tiles, 1)) * apply_detail::kTileCols);
+    t_lcol.resize(static_cast<std::size_t>(n1));
+    if (n_tiles > 0) {
+        const unsigned g = static_cast<unsigned>(n_tiles);
+        const int B = apply_detail::kBlock;
+        switch (seg_cap) {
+            case 4: tile_build_kernel<4><<<g, B, 0, s>>>(seg_begin.get(), seg_len.get(), n_seg, col32.get(), tile_nu.get(), tile_col.get(), t_lcol.get()); break;
+            case 8: tile_build_kernel<8><<<g, B, 0, s>>>(seg_begin.get(), seg_len.get(), n_seg, col32.get(), tile_nu.get(), tile_col.get(), t_lcol.get()); break;
+            case 12: tile_build_kernel<12><<<g, B, 0, s>>>(seg_begin.get(), seg_len.get(), n_seg, col32.get(), tile_nu.get(), tile_col.get(), t_lcol.get()); break;
+            default: tile_build_kernel<16><<<g, B, 0, s>>>(seg_begin.get(), seg_len.get(), n_seg, col32.get(), tile_nu.get(), tile_col.get(), t_lcol.get()); break;
+        }
+        AXIS_LAUNCH_CHECK();
+    }
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    apply_ready = true;
 }
 
 void batch_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t n_src, index_t n_dst, index_t k, ApplyMode mode,
@@ -139,7 +352,7 @@ void batch_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, F
     if (k < 0) throw InvalidArgument("batch_apply: negative field count");
     if (mode == ApplyMode::Plain && (xt != FieldType::F64 || yt != FieldType::F64))
         throw InvalidArgument("batch_apply: the plain apply takes FP64 fields");
-    if (mode == ApplyMode::Mixed && (xt == FieldType::F64 || yt == FieldType::BF16))
+    if (mode != ApplyMode::Plain && (xt == FieldType::F64 || yt == FieldType::BF16))
         throw InvalidArgument("batch_apply: the mixed apply takes FP32/BF16 fields and writes FP32/FP64");
     if (k == 0 || n_dst == 0) return;
     A.prepare_apply(s);
@@ -147,8 +360,7 @@ void batch_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, F
     apply_detail::launch(A, x, xt, y, yt, k, mode, L, s);
 }
 
-std::pair<int, int> autotune_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t k, ApplyMode mode,
-                                   hipStream_t s) {
+ApplyShape autotune_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t k, ApplyMode mode, hipStream_t s) {
     if (k <= 0) throw InvalidArgument("autotune_apply: no fields");
     A.prepare_apply(s);
     hipEvent_t e0, e1;
@@ -156,6 +368,7 @@ std::pair<int, int> autotune_apply(InterpolationMatrix &A, const void *x, FieldT
     AXIS_HIP_CHECK(hipEventCreate(&e1));
     apply_detail::Launch best = apply_detail::heuristic(A, k, mode);
     float best_ms = 1e30f;
+    for (bool tiled : {false, true})
     for (int fu : {1, 2, 4}) {
         if (fu > 1 && fu / 2 >= k) continue;
         for (int chunks : {1, 2, 4, 8, 16, 32, 64}) {
@@ -163,6 +376,7 @@ std::pair<int, int> autotune_apply(InterpolationMatrix &A, const void *x, FieldT
             apply_detail::Launch L;
             L.fu = fu;
             L.chunks = chunks;
+            L.tiled = tiled;
             apply_detail::launch(A, x, xt, y, yt, k, mode, L, s);  // warm
             AXIS_HIP_CHECK(hipEventRecord(e0, s));
             const int reps = 3;
@@ -187,7 +401,7 @@ std::pair<int, int> autotune_apply(InterpolationMatrix &A, const void *x, FieldT
     // leave the output of the chosen shape in y
     apply_detail::launch(A, x, xt, y, yt, k, mode, best, s);
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
-    return {best.fu, best.chunks};
+    return ApplyShape{best.fu, best.chunks, best.tiled};
 }
 
 // ------------------------------------------------------ conservation check

@@ -4,19 +4,33 @@
 //
 // Design (HBM-bound: touch every byte of A, X and Y about once):
 //   * the CSR rows are cut into "segments" of at most E nonzeros (E = 4, 8,
-//     12 or 16, chosen per matrix so that almost every row is one segment);
+//     12 or 16, chosen per matrix so that almost every row is one segment)
+//     and ordered spatially (destination cells grouped by a coarse Morton
+//     cell of their centroid);
 //   * one lane per segment: its (col, w) entries are loaded once into
 //     registers and reused while the lane streams through a chunk of fields,
-//     FU fields per step (FU * E independent gathers in flight).  A lane whose
-//     segment is a whole row writes Y directly; consecutive lanes hold
-//     consecutive rows, so Y for one field is written in full cache lines and
-//     the gathers of neighbouring rows share lines of X;
+//     FU fields per step.  A lane whose segment is a whole row writes Y
+//     directly;
 //   * segments of a long row (e.g. a polar cell with ~1600 nonzeros) write
 //     partial sums to a workspace; a second kernel adds each long row's
 //     partials in segment order (deterministic) and writes Y;
-//   * work items are (segment block, field chunk); the blockIdx -> item map
-//     gives each XCD a contiguous run of segment blocks per chunk, so
-//     neighbouring rows, which share source columns, share that XCD's L2.
+//   * staged tiles (spmm_tile_kernel): a workgroup's 256 segments form a
+//     tile whose distinct source columns (a compact patch, ~0.3 per nonzero)
+//     are read once per field, in sorted order, into LDS; the lanes then
+//     gather from LDS.  Inside a tile the segments are sorted by length, so
+//     the lanes of a wave run the same number of terms;
+//   * work items are (tile, field chunk); the blockIdx -> item map gives each
+//     XCD a contiguous run of tiles per chunk, so neighbouring tiles, which
+//     share source columns, share that XCD's L2.
+//
+// Numerics (template parameter NUM):
+//   kPlain   FP64 fields, FP64 weights, FP64 FMA chain (the plain method);
+//   kComp32  FP32/BF16 fields, weights split into FP32 hi + lo, FP32
+//            accumulation with TwoProduct/TwoSum compensation;
+//   kAcc64   FP32/BF16 fields, FP64 weights, FP64 FMA accumulation (on this
+//            GPU a FP64 FMA issues at the rate of an unpacked FP32 FMA, so
+//            this is both the cheapest and the most accurate reduction of
+//            narrow fields).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -31,8 +45,12 @@
 
 namespace axis_hip::apply_detail {
 
-constexpr int kBlock = 256;  // segments per workgroup
-constexpr int kMaxE = 16;    // largest register segment
+constexpr int kBlock = 256;       // segments per workgroup (and per tile)
+constexpr int kMaxE = 16;         // largest register segment
+constexpr int kTileCols = 1024;   // most distinct source columns a staged tile holds
+constexpr int kColsPerLane = kTileCols / kBlock;
+
+enum Num : int { kPlain = 0, kComp32 = 1, kAcc64 = 2 };
 
 /// Launch shape chosen by the autotuner: FU fields per step and the number of
 /// field chunks the k fields are split into (more chunks = more parallelism,
@@ -40,6 +58,7 @@ constexpr int kMaxE = 16;    // largest register segment
 struct Launch {
     int fu = 4;
     int chunks = 1;
+    bool tiled = false;  // stage each tile's source values in LDS
 };
 
 __device__ inline float ld_f32(const float *p) { return *p; }
@@ -81,6 +100,11 @@ struct Mat {
     // long rows: partial slots [long_slot[j], long_slot[j + 1]) of row long_row[j]
     const std::int32_t *long_row, *long_slot;
     int n_long, n_slots;
+    // tiles (segments [256 t, 256 t + 256)): distinct source columns and the
+    // position of each nonzero's column among them
+    const std::int32_t *tile_nu, *tile_col;
+    const std::uint16_t *lcol;
+    index_t n_tiles;
 };
 
 /// Hardware places workgroup b on XCD b % 8; give XCD x a contiguous range of
@@ -97,54 +121,90 @@ __device__ inline float ldv(const XT *p) {
 }
 __device__ inline double ldv(const double *p) { return *p; }
 
-/// Segment i of the processing order, fields [f0, f1) in steps of FU.
-/// Partial sums of long-row segments go to P[f * n_slots + slot] (FP64).
-template <int E, int FU, bool MIXED, class XT, class YT>
+template <int NUM>
+struct NumTraits {
+    using V = float;   // field value held in registers / LDS
+    using W = double;  // weight held in registers
+};
+template <>
+struct NumTraits<kPlain> {
+    using V = double;
+    using W = double;
+};
+template <>
+struct NumTraits<kComp32> {
+    using V = float;
+    using W = float;
+};
+
+/// Row-sum accumulator of one numerics mode.
+template <int NUM>
+struct RowAcc;
+template <>
+struct RowAcc<kPlain> {
+    double s = 0.0;
+    __device__ inline void add(double h, double, double x) { s = fma(h, x, s); }
+    __device__ inline double value() const { return s; }
+};
+template <>
+struct RowAcc<kAcc64> {
+    double s = 0.0;
+    __device__ inline void add(double h, double, float x) { s = fma(h, static_cast<double>(x), s); }
+    __device__ inline double value() const { return s; }
+};
+template <>
+struct RowAcc<kComp32> : Acc32 {};
+
+template <int NUM>
+__device__ inline void load_w(const Mat &A, index_t q, typename NumTraits<NUM>::W &h, typename NumTraits<NUM>::W &l) {
+    if constexpr (NUM == kComp32) {
+        h = A.hi[q];
+        l = A.lo[q];
+    } else {
+        h = A.val[q];
+        l = 0.0;
+    }
+}
+
+/// Segment i of the processing order, fields [f0, f1) in steps of FU, source
+/// values gathered straight from X.  Partial sums of long-row segments go to
+/// P[f * n_slots + slot] (FP64).
+template <int E, int FU, int NUM, class XT, class YT>
 __device__ inline void segment_apply(const Mat &A, const XT *X, YT *Y, double *P, index_t i, index_t f0, index_t f1) {
-    using VT = typename std::conditional<MIXED, float, double>::type;
+    using V = typename NumTraits<NUM>::V;
+    using W = typename NumTraits<NUM>::W;
     const index_t r = A.seg_row[i];
     const index_t b = A.seg_begin[i];
     const int n = A.seg_len[i];
     const int slot = A.seg_slot[i];
     std::int32_t c[E];
-    VT h[E], l[E];
+    W h[E], l[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const bool v = e < n;
-        c[e] = v ? A.col[b + e] : 0;
-        if constexpr (MIXED) {
-            h[e] = v ? A.hi[b + e] : 0.0f;
-            l[e] = v ? A.lo[b + e] : 0.0f;
-        } else {
-            h[e] = v ? A.val[b + e] : 0.0;
-            l[e] = 0.0;
+        c[e] = 0;
+        h[e] = W(0);
+        l[e] = W(0);
+        if (e < n) {
+            c[e] = A.col[b + e];
+            load_w<NUM>(A, b + e, h[e], l[e]);
         }
     }
     for (index_t f = f0; f < f1; f += FU) {
-        VT x[FU][E];
+        V x[FU][E];
 #pragma unroll
         for (int u = 0; u < FU; ++u) {
             const index_t fu = (f + u < f1) ? f + u : f1 - 1;
             const XT *xf = X + fu * A.n_src;
 #pragma unroll
-            for (int e = 0; e < E; ++e) x[u][e] = (e < n) ? ldv(xf + c[e]) : VT(0);
+            for (int e = 0; e < E; ++e) x[u][e] = (e < n) ? ldv(xf + c[e]) : V(0);
         }
 #pragma unroll
         for (int u = 0; u < FU; ++u) {
-            double v;
-            if constexpr (MIXED) {
-                Acc32 acc;
+            RowAcc<NUM> acc;
 #pragma unroll
-                for (int e = 0; e < E; ++e)
-                    if (e < n) acc.add(h[e], l[e], x[u][e]);
-                v = acc.value();
-            } else {
-                double acc = 0.0;
-#pragma unroll
-                for (int e = 0; e < E; ++e)
-                    if (e < n) acc = fma(h[e], x[u][e], acc);
-                v = acc;
-            }
+            for (int e = 0; e < E; ++e)
+                if (e < n) acc.add(h[e], l[e], x[u][e]);
+            const double v = acc.value();
             if (f + u < f1) {
                 if (slot < 0)
                     st(Y + (f + u) * A.n_dst + r, v);
@@ -155,7 +215,7 @@ __device__ inline void segment_apply(const Mat &A, const XT *X, YT *Y, double *P
     }
 }
 
-template <int E, int FU, bool MIXED, class XT, class YT>
+template <int E, int FU, int NUM, class XT, class YT>
 __global__ void __launch_bounds__(kBlock) spmm_seg_kernel(Mat A, const XT *X, YT *Y, double *P, index_t k, unsigned n_blocks, int chunks) {
     const unsigned w = xcd_remap(blockIdx.x, gridDim.x);
     const unsigned sb = w % n_blocks, ch = w / n_blocks;
@@ -165,7 +225,98 @@ __global__ void __launch_bounds__(kBlock) spmm_seg_kernel(Mat A, const XT *X, YT
     const index_t f0 = static_cast<index_t>(ch) * per;
     const index_t f1 = f0 + per < k ? f0 + per : k;
     if (f0 >= f1) return;
-    segment_apply<E, FU, MIXED>(A, X, Y, P, i, f0, f1);
+    segment_apply<E, FU, NUM>(A, X, Y, P, i, f0, f1);
+}
+
+/// Tile t of segments, fields [f0, f1): the tile's distinct source values of
+/// FU fields are loaded once into LDS by the whole workgroup (each source
+/// value read from HBM once per tile, in sorted column order, instead of once
+/// per nonzero), then every lane forms its segment's sums from LDS.  Tiles
+/// whose columns do not fit fall back to direct gathers.
+template <int E, int FU, int NUM, class XT, class YT>
+__global__ void __launch_bounds__(kBlock) spmm_tile_kernel(Mat A, const XT *X, YT *Y, double *P, index_t k, unsigned n_tiles, int chunks) {
+    // staged value type: the field's own precision widened to FP32 (FP64
+    // for the plain apply); staging FP64 copies of narrow fields was measured
+    // slower (twice the LDS per workgroup, lower occupancy)
+    using V = typename NumTraits<NUM>::V;
+    using W = typename NumTraits<NUM>::W;
+    __shared__ V xs[FU][kTileCols];
+    const unsigned w = xcd_remap(blockIdx.x, gridDim.x);
+    const unsigned t = w % n_tiles, ch = w / n_tiles;
+    const index_t per = (k + chunks - 1) / chunks;
+    const index_t f0 = static_cast<index_t>(ch) * per;
+    const index_t f1 = f0 + per < k ? f0 + per : k;
+    if (f0 >= f1) return;  // whole workgroup
+    const index_t i = static_cast<index_t>(t) * kBlock + threadIdx.x;
+    const bool valid = i < A.n_seg;
+    const int nu = A.tile_nu[t];
+    if (nu < 0) {  // whole workgroup; one field per step keeps this rare path's registers low
+        if (valid) segment_apply<E, 1, NUM>(A, X, Y, P, i, f0, f1);
+        return;
+    }
+    index_t r = 0, b = 0;
+    int n = 0, slot = -1;
+    if (valid) {
+        r = A.seg_row[i];
+        b = A.seg_begin[i];
+        n = A.seg_len[i];
+        slot = A.seg_slot[i];
+    }
+    // terms run by this wave: its longest segment
+    int n_max = n;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) n_max = max(n_max, __shfl_xor(n_max, d, 64));
+    const int n_wave = __builtin_amdgcn_readfirstlane(n_max);
+    unsigned lc[E];  // byte offsets into one field's staged values
+    W h[E], l[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        lc[e] = 0;
+        h[e] = W(0);
+        l[e] = W(0);
+        if (e < n) {
+            lc[e] = static_cast<unsigned>(A.lcol[b + e]) * static_cast<unsigned>(sizeof(V));
+            load_w<NUM>(A, b + e, h[e], l[e]);
+        }
+    }
+    int uc[kColsPerLane];
+#pragma unroll
+    for (int j = 0; j < kColsPerLane; ++j) {
+        const int u = static_cast<int>(threadIdx.x) + j * kBlock;
+        uc[j] = u < nu ? A.tile_col[static_cast<index_t>(t) * kTileCols + u] : -1;
+    }
+    for (index_t f = f0; f < f1; f += FU) {
+        V v[FU][kColsPerLane];
+#pragma unroll
+        for (int u = 0; u < FU; ++u) {
+            const index_t fu = (f + u < f1) ? f + u : f1 - 1;
+            const XT *xf = X + fu * A.n_src;
+#pragma unroll
+            for (int j = 0; j < kColsPerLane; ++j) v[u][j] = uc[j] >= 0 ? static_cast<V>(ldv(xf + uc[j])) : V(0);
+        }
+#pragma unroll
+        for (int u = 0; u < FU; ++u)
+#pragma unroll
+            for (int j = 0; j < kColsPerLane; ++j)
+                if (uc[j] >= 0) xs[u][threadIdx.x + j * kBlock] = v[u][j];
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < FU; ++u) {
+            RowAcc<NUM> acc;
+            const char *xb = reinterpret_cast<const char *>(&xs[u][0]);
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                if (e < n_wave && e < n) acc.add(h[e], l[e], *reinterpret_cast<const V *>(xb + lc[e]));  // e < n_wave: wave-uniform
+            const double val = acc.value();
+            if (valid && f + u < f1) {
+                if (slot < 0)
+                    st(Y + (f + u) * A.n_dst + r, val);
+                else
+                    P[(f + u) * A.n_slots + slot] = val;
+            }
+        }
+        __syncthreads();
+    }
 }
 
 /// Y of the long rows: the partials of each row summed in segment order.
@@ -202,6 +353,10 @@ inline Mat mat_of(const InterpolationMatrix &A) {
     M.long_slot = A.long_slot.get();
     M.n_long = A.n_long_rows;
     M.n_slots = A.n_slots;
+    M.tile_nu = A.tile_nu.get();
+    M.tile_col = A.tile_col.get();
+    M.lcol = A.t_lcol.get();
+    M.n_tiles = A.n_tiles;
     return M;
 }
 
@@ -214,16 +369,20 @@ inline double *partials(InterpolationMatrix &A, index_t k) {
     return A.partial_ws.get();
 }
 
-template <int E, int FU, bool MIXED, class XT, class YT>
-void launch_e(InterpolationMatrix &A, const void *x, void *y, index_t k, int chunks, hipStream_t s) {
+template <int E, int FU, int NUM, class XT, class YT>
+void launch_e(InterpolationMatrix &A, const void *x, void *y, index_t k, int chunks, bool tiled, hipStream_t s) {
     const Mat M = mat_of(A);
     double *P = partials(A, k);
     const unsigned nb = static_cast<unsigned>(std::max<index_t>((A.n_seg + kBlock - 1) / kBlock, 1));
     const std::uint64_t nwork = static_cast<std::uint64_t>(nb) * chunks;
     if (nwork >= (1ull << 31)) throw InvalidArgument("apply: launch too large");
     if (A.n_seg > 0) {
-        spmm_seg_kernel<E, FU, MIXED, XT, YT><<<static_cast<unsigned>(nwork), kBlock, 0, s>>>(M, static_cast<const XT *>(x), static_cast<YT *>(y), P,
-                                                                                             k, nb, chunks);
+        if (tiled)
+            spmm_tile_kernel<E, FU, NUM, XT, YT><<<static_cast<unsigned>(nwork), kBlock, 0, s>>>(M, static_cast<const XT *>(x),
+                                                                                                  static_cast<YT *>(y), P, k, nb, chunks);
+        else
+            spmm_seg_kernel<E, FU, NUM, XT, YT><<<static_cast<unsigned>(nwork), kBlock, 0, s>>>(M, static_cast<const XT *>(x),
+                                                                                                 static_cast<YT *>(y), P, k, nb, chunks);
         AXIS_LAUNCH_CHECK();
     }
     if (A.n_long_rows > 0) {
@@ -233,34 +392,40 @@ void launch_e(InterpolationMatrix &A, const void *x, void *y, index_t k, int chu
     }
 }
 
-template <int FU, bool MIXED, class XT, class YT>
-void launch_fu(InterpolationMatrix &A, const void *x, void *y, index_t k, int chunks, hipStream_t s) {
+template <int FU, int NUM, class XT, class YT>
+void launch_fu(InterpolationMatrix &A, const void *x, void *y, index_t k, int chunks, bool tiled, hipStream_t s) {
     switch (A.seg_cap) {
-        case 4: return launch_e<4, FU, MIXED, XT, YT>(A, x, y, k, chunks, s);
-        case 8: return launch_e<8, FU, MIXED, XT, YT>(A, x, y, k, chunks, s);
-        case 12: return launch_e<12, FU, MIXED, XT, YT>(A, x, y, k, chunks, s);
-        default: return launch_e<16, FU, MIXED, XT, YT>(A, x, y, k, chunks, s);
+        case 4: return launch_e<4, FU, NUM, XT, YT>(A, x, y, k, chunks, tiled, s);
+        case 8: return launch_e<8, FU, NUM, XT, YT>(A, x, y, k, chunks, tiled, s);
+        case 12: return launch_e<12, FU, NUM, XT, YT>(A, x, y, k, chunks, tiled, s);
+        default: return launch_e<16, FU, NUM, XT, YT>(A, x, y, k, chunks, tiled, s);
     }
 }
 
-template <bool MIXED, class XT, class YT>
+template <int NUM, class XT, class YT>
 void launch_xy(InterpolationMatrix &A, const void *x, void *y, index_t k, const Launch &L, hipStream_t s) {
     switch (L.fu) {
-        case 1: return launch_fu<1, MIXED, XT, YT>(A, x, y, k, L.chunks, s);
-        case 2: return launch_fu<2, MIXED, XT, YT>(A, x, y, k, L.chunks, s);
-        case 4: return launch_fu<4, MIXED, XT, YT>(A, x, y, k, L.chunks, s);
+        case 1: return launch_fu<1, NUM, XT, YT>(A, x, y, k, L.chunks, L.tiled, s);
+        case 2: return launch_fu<2, NUM, XT, YT>(A, x, y, k, L.chunks, L.tiled, s);
+        case 4: return launch_fu<4, NUM, XT, YT>(A, x, y, k, L.chunks, L.tiled, s);
         default: throw InvalidArgument("apply: unsupported fields per step");
     }
 }
 
+template <int NUM>
+void launch_num(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t k, const Launch &L, hipStream_t s) {
+    if (xt == FieldType::F32 && yt == FieldType::F32) return launch_xy<NUM, float, float>(A, x, y, k, L, s);
+    if (xt == FieldType::F32 && yt == FieldType::F64) return launch_xy<NUM, float, double>(A, x, y, k, L, s);
+    if (xt == FieldType::BF16 && yt == FieldType::F32) return launch_xy<NUM, std::uint16_t, float>(A, x, y, k, L, s);
+    if (xt == FieldType::BF16 && yt == FieldType::F64) return launch_xy<NUM, std::uint16_t, double>(A, x, y, k, L, s);
+    throw InvalidArgument("apply: unsupported field type combination");
+}
+
 inline void launch(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t k, ApplyMode mode, const Launch &L,
                    hipStream_t s) {
-    if (mode == ApplyMode::Plain) return launch_xy<false, double, double>(A, x, y, k, L, s);
-    if (xt == FieldType::F32 && yt == FieldType::F32) return launch_xy<true, float, float>(A, x, y, k, L, s);
-    if (xt == FieldType::F32 && yt == FieldType::F64) return launch_xy<true, float, double>(A, x, y, k, L, s);
-    if (xt == FieldType::BF16 && yt == FieldType::F32) return launch_xy<true, std::uint16_t, float>(A, x, y, k, L, s);
-    if (xt == FieldType::BF16 && yt == FieldType::F64) return launch_xy<true, std::uint16_t, double>(A, x, y, k, L, s);
-    throw InvalidArgument("apply: unsupported field type combination");
+    if (mode == ApplyMode::Plain) return launch_xy<kPlain, double, double>(A, x, y, k, L, s);
+    if (mode == ApplyMode::Mixed) return launch_num<kComp32>(A, x, xt, y, yt, k, L, s);
+    return launch_num<kAcc64>(A, x, xt, y, yt, k, L, s);
 }
 
 // ------------------------------------------------------------- autotuner
@@ -279,7 +444,8 @@ inline std::mutex &tune_mutex() {
 /// workgroups, 4 fields per step (2 for FP64, whose gathers are twice as wide).
 inline Launch heuristic(const InterpolationMatrix &A, index_t k, ApplyMode mode) {
     Launch L;
-    L.fu = (mode == ApplyMode::Plain) ? 2 : 4;
+    L.tiled = k >= 8;
+    L.fu = (mode == ApplyMode::Mixed) ? 4 : 2;
     if (k < L.fu) L.fu = k >= 2 ? 2 : 1;
     const index_t nb = (A.n_seg + kBlock - 1) / kBlock;
     index_t want = (2048 + nb - 1) / std::max<index_t>(nb, 1);  // ~8 workgroups per CU

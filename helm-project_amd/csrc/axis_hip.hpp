@@ -96,11 +96,26 @@ class InterpolationMatrix {
     int n_long_rows = 0, n_slots = 0;
     DeviceBuffer<std::int32_t> long_rows, long_slot;
     DeviceBuffer<double> partial_ws;
+    // Spatial processing order: 30-bit Morton code of each destination cell's
+    // centroid (unit-sphere xyz), set by the weight generators; empty for
+    // matrices built from triplets (rows then keep index order).
+    DeviceBuffer<std::uint32_t> row_key;
+    // Tiles: tile t = segments [256 t, 256 t + 256).  A tile whose nonzeros
+    // touch at most 1024 distinct source columns lists them (sorted) in
+    // tile_col[t * 1024 ...], and t_lcol[e] (aligned with the CSR arrays) holds
+    // the 16-bit position of nonzero e's column in its tile's list; the tiled
+    // apply stages those source values in LDS once per field.  tile_nu[t] = -1
+    // marks a tile that gathers straight from HBM.
+    index_t n_tiles = 0;
+    DeviceBuffer<std::int32_t> tile_nu, tile_col;
+    DeviceBuffer<std::uint16_t> t_lcol;
+    bool apply_ready = false;
     // diagnostics of the generation that produced it
     index_t n_candidates = 0, n_host_arbitrated = 0, n_cap_borderline = 0;
     double gen_ms = 0.0;
 
-    /// Builds the FP32 hi/lo split and int32 columns used by the mixed-precision apply.
+    /// Builds the apply layout once: FP32 hi/lo split, int32 columns, the
+    /// segment list in spatial order and the tiles.
     void prepare_apply(hipStream_t s);
 };
 
@@ -109,7 +124,10 @@ struct ConservationReport {
 };
 
 enum class FieldType : int { F64 = 0, F32 = 1, BF16 = 2 };
-enum class ApplyMode : int { Plain = 0, Mixed = 1 };
+/// Plain: FP64 fields, weights and accumulation.  Mixed: FP32/BF16 fields,
+/// FP32 hi + lo weights, TwoSum-compensated FP32 accumulation.  MixedAcc64:
+/// FP32/BF16 fields, FP64 weights and FP64 accumulation.
+enum class ApplyMode : int { Plain = 0, Mixed = 1, MixedAcc64 = 2 };
 
 class WeightGenerator {
    public:
@@ -128,11 +146,17 @@ void batch_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, F
 
 ConservationReport check_conservation(const InterpolationMatrix &A, const double *d_src, const double *d_dst, NormType norm, hipStream_t s);
 
-/// Times the candidate launch shapes (fields per step x field chunks) of
-/// batch_apply on the given device fields and caches the fastest for this
-/// (n_dst, nnz, k, types, mode).  Returns {fields per step, field chunks}.
-std::pair<int, int> autotune_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t k, ApplyMode mode,
-                                   hipStream_t s);
+/// Launch shape of the apply: fields per step, field chunks (more chunks =
+/// more workgroups, more re-reads of the matrix) and whether tiles stage
+/// their source values in LDS.
+struct ApplyShape {
+    int fields_per_step = 4, field_chunks = 1;
+    bool staged = false;
+};
+
+/// Times the candidate launch shapes of batch_apply on the given device
+/// fields and caches the fastest for this (n_dst, nnz, k, types, mode).
+ApplyShape autotune_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t k, ApplyMode mode, hipStream_t s);
 
 // ---------------------------------------------------------------- internal
 namespace detail {
@@ -165,10 +189,14 @@ bool rectilinear_from_axes(const Mesh &m, const GridAxes &ax);
 
 void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s);
 
+/// 30-bit Morton code of every cell's centroid (normalised mean of its vertex
+/// xyz), used to order the rows of a generated matrix for the apply.
+void cell_morton_keys(const Mesh &m, DeviceBuffer<std::uint32_t> &key, hipStream_t s);
+
 // scan / sort primitives (scan.hip, sort.hip)
 void exclusive_scan_i64(const index_t *in, index_t *out, index_t n, hipStream_t s);  // out[n] = total
 void radix_sort_pairs_u32(std::uint32_t *keys, std::uint32_t *vals, std::size_t n, int bits, hipStream_t s);
-void radix_sort_keys_u64(std::uint64_t *keys, std::size_t n, hipStream_t s);
+void radix_sort_keys_u64(std::uint64_t *keys, std::size_t n, hipStream_t s, int bits = 64);  // sorts the low `bits` bits
 void radix_sort_pairs_u64(std::uint64_t *keys, std::uint32_t *vals, std::size_t n, int bits, hipStream_t s);
 
 // LBVH broad phase (lbvh.hip): candidate CSR of src cells whose float boxes

@@ -111,6 +111,7 @@ std::unique_ptr<InterpolationMatrix> generate_bilinear_rect(const Mesh &src, con
     M->area_a.resize(std::max<index_t>(n_src, 1));
     M->area_b.resize(std::max<index_t>(n_dst, 1));
     M->max_row_nnz = n_dst > 0 ? 4 : 0;
+    cell_morton_keys(dst, M->row_key, s);
     bilinear_kernel<<<grid_for(n_dst + 1, 256), 256, 0, s>>>(dst.d_offsets.get(), dst.d_indices.get(), dst.d_coords.get(), dst.n_nodes(), n_dst,
                                                              d_clon.get(), ni, d_clat.get(), nj, M->row_ptr.get(), M->col.get(), M->val.get());
     AXIS_LAUNCH_CHECK();

@@ -109,6 +109,7 @@ FieldType ftype(int code) {
 ApplyMode amode(int code) {
     if (code == AXIS_APPLY_PLAIN) return ApplyMode::Plain;
     if (code == AXIS_APPLY_MIXED) return ApplyMode::Mixed;
+    if (code == AXIS_APPLY_MIXED_ACC64) return ApplyMode::MixedAcc64;
     throw InvalidArgument("unknown apply mode " + std::to_string(code));
 }
 
@@ -318,12 +319,13 @@ int axis_batch_apply_c(int matrix_handle, const void *src, int src_dtype, void *
 }
 
 int axis_autotune_apply_c(int matrix_handle, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n_fields, int mode, void *stream,
-                          int *fields_per_step, int *field_chunks) {
+                          int *fields_per_step, int *field_chunks, int *staged) {
     return guarded([&] {
         auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
         auto r = autotune_apply(*M, src, ftype(src_dtype), dst, ftype(dst_dtype), n_fields, amode(mode), static_cast<hipStream_t>(stream));
-        if (fields_per_step) *fields_per_step = r.first;
-        if (field_chunks) *field_chunks = r.second;
+        if (fields_per_step) *fields_per_step = r.fields_per_step;
+        if (field_chunks) *field_chunks = r.field_chunks;
+        if (staged) *staged = r.staged ? 1 : 0;
     });
 }
 

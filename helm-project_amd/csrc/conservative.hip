@@ -192,19 +192,23 @@ __global__ void unpack_sorted_kernel(const std::uint64_t *key, const std::uint32
 __global__ void row_weights_kernel(const index_t *row_ptr, index_t n_dst, const double *area_dst, const double *ov, double *w, double *frac_b,
                                    int frac_area, int *max_row) {
     index_t r = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r >= n_dst) return;
-    const index_t b = row_ptr[r], e = row_ptr[r + 1];
-    const double a = area_dst[r];
-    double fb = 0.0;
-    for (index_t k = b; k < e; ++k) {
-        double x = ov[k] / a;
-        w[k] = fmax(x, 0.0);
-        fb += x;
+    int len = 0;
+    if (r < n_dst) {
+        const index_t b = row_ptr[r], e = row_ptr[r + 1];
+        const double a = area_dst[r];
+        double fb = 0.0;
+        for (index_t k = b; k < e; ++k) {
+            double x = ov[k] / a;
+            w[k] = fmax(x, 0.0);
+            fb += x;
+        }
+        if (frac_area && fb > 0.0)
+            for (index_t k = b; k < e; ++k) w[k] /= fb;
+        frac_b[r] = fmin(fb, 1.0);
+        len = static_cast<int>(e - b);
     }
-    if (frac_area && fb > 0.0)
-        for (index_t k = b; k < e; ++k) w[k] /= fb;
-    frac_b[r] = fmin(fb, 1.0);
-    atomicMax(max_row, static_cast<int>(e - b));
+    for (int d = 32; d >= 1; d >>= 1) len = max(len, __shfl_xor(len, d, 64));  // one atomic per wave
+    if ((threadIdx.x & 63) == 0 && len > 0) atomicMax(max_row, len);
 }
 
 __global__ void transpose_keys_kernel(const index_t *col, const std::int32_t *row32, index_t nnz, int row_bits, std::uint64_t *key,
@@ -314,19 +318,31 @@ double host_pair_overlap(const HostSide &S, const HostSide &D, index_t j, index_
 }
 
 // WeightGenerator::generate's longitude-disjoint check, weight_generator.cpp:1898-1992
+// min / max of the cells' mean vertex longitude as order-preserving integer
+// keys: wave reduction, then one atomic per wave
 __global__ void lon_range_kernel(const index_t *off, const index_t *ind, const double *lon, index_t n_cells, unsigned long long *mn,
                                  unsigned long long *mx) {
     index_t c = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (c >= n_cells) return;
-    index_t b = off[c], e = off[c + 1];
-    if (e == b) return;
-    double s = 0.0;
-    for (index_t k = b; k < e; ++k) s += lon[ind[k]];
-    double cx = s / static_cast<double>(e - b);
-    std::uint64_t bits = static_cast<std::uint64_t>(__double_as_longlong(cx));
-    std::uint64_t key = (bits >> 63) ? ~bits : (bits | 0x8000000000000000ull);
-    atomicMin(mn, static_cast<unsigned long long>(key));
-    atomicMax(mx, static_cast<unsigned long long>(key));
+    unsigned long long kmin = ~0ull, kmax = 0ull;
+    if (c < n_cells) {
+        index_t b = off[c], e = off[c + 1];
+        if (e > b) {
+            double s = 0.0;
+            for (index_t k = b; k < e; ++k) s += lon[ind[k]];
+            double cx = s / static_cast<double>(e - b);
+            std::uint64_t bits = static_cast<std::uint64_t>(__double_as_longlong(cx));
+            kmin = kmax = (bits >> 63) ? ~bits : (bits | 0x8000000000000000ull);
+        }
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        const unsigned long long a = __shfl_xor(kmin, d, 64), z = __shfl_xor(kmax, d, 64);
+        kmin = a < kmin ? a : kmin;
+        kmax = z > kmax ? z : kmax;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (kmin != ~0ull) atomicMin(mn, kmin);
+        if (kmax != 0ull) atomicMax(mx, kmax);
+    }
 }
 
 double key_to_double(unsigned long long k) {
@@ -410,6 +426,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     M->n_src = n_src;
     M->n_dst = n_dst;
     M->n_candidates = n_pairs;
+    cell_morton_keys(dst, M->row_key, s);
 
     DeviceBuffer<std::int32_t> pair_row(std::max<index_t>(n_pairs, 1));
     DeviceBuffer<double> overlap(std::max<index_t>(n_pairs, 1));

@@ -19,6 +19,7 @@
 
 #include "axis_hip.hpp"
 #include "host_util.hpp"
+#include "morton.hpp"
 
 namespace axis_hip::detail {
 
@@ -46,28 +47,11 @@ __device__ inline bool overlap(const FBox &a, const FBox &b) {
     return !(a.hi[0] < b.lo[0] || b.hi[0] < a.lo[0] || a.hi[1] < b.lo[1] || b.hi[1] < a.lo[1] || a.hi[2] < b.lo[2] || b.hi[2] < a.lo[2]);
 }
 
-__device__ inline std::uint32_t spread10(std::uint32_t v) {
-    v &= 0x3FFu;
-    v = (v | (v << 16)) & 0x030000FFu;
-    v = (v | (v << 8)) & 0x0300F00Fu;
-    v = (v | (v << 4)) & 0x030C30C3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
-}
-
 __global__ void morton_kernel(const float *boxes, std::int64_t n, std::uint32_t *codes, std::uint32_t *ids) {
     std::int64_t i = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     FBox b = load_box(boxes, i);
-    std::uint32_t q[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        float c = 0.5f * (b.lo[a] + b.hi[a]);
-        float t = (c + 1.1f) * (1.0f / 2.2f);
-        t = fminf(fmaxf(t, 0.0f), 1.0f);
-        q[a] = static_cast<std::uint32_t>(fminf(t * 1024.0f, 1023.0f));
-    }
-    codes[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+    codes[i] = morton30(0.5f * (b.lo[0] + b.hi[0]), 0.5f * (b.lo[1] + b.hi[1]), 0.5f * (b.lo[2] + b.hi[2]));
     ids[i] = static_cast<std::uint32_t>(i);
 }
 

@@ -23,6 +23,7 @@
 #include "axis_hip.hpp"
 #include "geometry.hpp"
 #include "host_util.hpp"
+#include "morton.hpp"
 
 namespace axis_hip {
 
@@ -549,6 +550,36 @@ void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s) {
         }
         AXIS_HIP_CHECK(hipStreamSynchronize(s));
     }
+}
+
+namespace {
+__global__ void cell_key_kernel(index_t n_cells, const index_t *off, const index_t *ind, const double *coords, index_t n_nodes, double to_rad,
+                                std::uint32_t *key) {
+    const index_t c = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= n_cells) return;
+    double x = 0.0, y = 0.0, z = 0.0;
+    for (index_t q = off[c]; q < off[c + 1]; ++q) {
+        const index_t v = ind[q];
+        double so, co, sa, ca;
+        sincos(coords[v] * to_rad, &so, &co);
+        sincos(coords[n_nodes + v] * to_rad, &sa, &ca);
+        x += ca * co;
+        y += ca * so;
+        z += sa;
+    }
+    const double len = sqrt(x * x + y * y + z * z);
+    const double inv = len > 0.0 ? 1.0 / len : 0.0;
+    key[c] = morton30(static_cast<float>(x * inv), static_cast<float>(y * inv), static_cast<float>(z * inv));
+}
+}  // namespace
+
+void cell_morton_keys(const Mesh &m, DeviceBuffer<std::uint32_t> &key, hipStream_t s) {
+    key.resize(std::max<index_t>(m.n_cells(), 1));
+    if (m.n_cells() == 0) return;
+    const double to_rad = m.coord_system() == CoordinateSystem::SphericalDeg ? kDeg2Rad : 1.0;
+    cell_key_kernel<<<grid_for(m.n_cells(), 256), 256, 0, s>>>(m.n_cells(), m.d_offsets.get(), m.d_indices.get(), m.d_coords.get(), m.n_nodes(),
+                                                              to_rad, key.get());
+    AXIS_LAUNCH_CHECK();
 }
 
 }  // namespace detail

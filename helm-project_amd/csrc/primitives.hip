@@ -199,7 +199,9 @@ void radix_sort_impl(K *keys, std::uint32_t *vals, std::size_t n, int bits, hipS
 void radix_sort_pairs_u32(std::uint32_t *keys, std::uint32_t *vals, std::size_t n, int bits, hipStream_t s) {
     radix_sort_impl<std::uint32_t, true>(keys, vals, n, bits, s);
 }
-void radix_sort_keys_u64(std::uint64_t *keys, std::size_t n, hipStream_t s) { radix_sort_impl<std::uint64_t, false>(keys, nullptr, n, 64, s); }
+void radix_sort_keys_u64(std::uint64_t *keys, std::size_t n, hipStream_t s, int bits) {
+    radix_sort_impl<std::uint64_t, false>(keys, nullptr, n, bits, s);
+}
 void radix_sort_pairs_u64(std::uint64_t *keys, std::uint32_t *vals, std::size_t n, int bits, hipStream_t s) {
     radix_sort_impl<std::uint64_t, true>(keys, vals, n, bits, s);
 }

@@ -41,6 +41,7 @@ extern "C" {
 #define AXIS_BF16 2
 #define AXIS_APPLY_PLAIN 0 /* FP64 fields, FP64 weights and accumulation */
 #define AXIS_APPLY_MIXED 1 /* FP32/BF16 fields, FP32 hi+lo weights, compensated FP32 accumulation */
+#define AXIS_APPLY_MIXED_ACC64 2 /* FP32/BF16 fields, FP64 weights, FP64 accumulation */
 
 /* RegridConfig (libs/axis/include/axis/solver/regrid_config.hpp:80-89);
  * field values are the enums' declaration order. */
@@ -115,8 +116,8 @@ int axis_apply_c(int matrix_handle, const double *src, double *dst, int n_src, i
 /* batch_apply (apply.hpp:275-343, :336) on device-resident layout_left fields
  * src [n_src, n_fields], dst [n_dst, n_fields] of the given storage types,
  * enqueued on `stream` (a hipStream_t, NULL = default stream) and waited for
- * before returning.  mode AXIS_APPLY_PLAIN needs F64/F64; AXIS_APPLY_MIXED
- * takes F32 or BF16 src and F32 or F64 dst. */
+ * before returning.  mode AXIS_APPLY_PLAIN needs F64/F64; the mixed modes
+ * take F32 or BF16 src and F32 or F64 dst. */
 int axis_batch_apply_c(int matrix_handle, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n_src, int64_t n_dst,
                        int64_t n_fields, int mode, void *stream);
 
@@ -125,12 +126,13 @@ int axis_batch_apply_c(int matrix_handle, const void *src, int src_dtype, void *
 int axis_batch_apply_async_c(int matrix_handle, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n_src, int64_t n_dst,
                              int64_t n_fields, int mode, void *stream);
 
-/* Autotuner (no reference counterpart): times the launch shapes (fields per
- * step x field chunks) of
- * axis_batch_apply_c on these device fields, caches the fastest for this
- * (matrix size, n_fields, types, mode) and reports it. */
+/* Autotuner (no reference counterpart): times the launch shapes of
+ * axis_batch_apply_c (fields per step x field chunks x LDS-staged tiles or
+ * direct gathers) on these device fields, caches the fastest for this
+ * (matrix size, n_fields, types, mode) and reports it; output pointers may be
+ * NULL. */
 int axis_autotune_apply_c(int matrix_handle, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n_fields, int mode, void *stream,
-                          int *fields_per_step, int *field_chunks);
+                          int *fields_per_step, int *field_chunks, int *staged);
 
 /* check_conservation (conservation.hpp:95-97, conservation.cpp:82-103) on
  * FP64 fields (host or device): out4 = {src_integral, dst_integral,
