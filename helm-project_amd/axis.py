@@ -77,6 +77,12 @@ def lib():
         "axis_check_conservation_c": [C.c_int, C.c_void_p, C.c_void_p, C.c_int, _D],
         "axis_destroy_mesh_c": [C.c_int],
         "axis_destroy_matrix_c": [C.c_int],
+        "axis_batch_plan_create_c": [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                     C.POINTER(C.c_int64), C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                     C.POINTER(C.c_int)],
+        "axis_batch_plan_run_c": [C.c_int, C.c_void_p, C.c_int],
+        "axis_batch_plan_info_c": [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _I64, C.POINTER(C.c_int)],
+        "axis_destroy_batch_plan_c": [C.c_int],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -94,7 +100,8 @@ def exported_symbols():
             "axis_generate_weights_c", "axis_generate_weights_config_c", "axis_matrix_from_coo_c",
             "axis_matrix_to_csr_c", "axis_matrix_info_c", "axis_matrix_csr_c", "axis_matrix_fractions_c",
             "axis_matrix_stats_c", "axis_apply_c", "axis_batch_apply_c", "axis_batch_apply_async_c",
-            "axis_autotune_apply_c", "axis_check_conservation_c", "axis_destroy_mesh_c", "axis_destroy_matrix_c"]
+            "axis_autotune_apply_c", "axis_check_conservation_c", "axis_destroy_mesh_c", "axis_destroy_matrix_c",
+            "axis_batch_plan_create_c", "axis_batch_plan_run_c", "axis_batch_plan_info_c", "axis_destroy_batch_plan_c"]
 
 
 def _check(rc: int):
@@ -298,3 +305,42 @@ def matrix_from_coo(rows, cols, vals, n_src, n_dst, frac_a=None, frac_b=None, ar
                                         *(None if a is None else a.ctypes.data_as(_D) for a in (fa, fb, aa, ab)),
                                         C.byref(h)))
     return Matrix(h.value)
+
+
+class BatchPlan:
+    """Several (matrix, src, dst, k) regrid pairs applied by one persistent
+    work-queue launch recorded in a HIP graph (axis_batch_plan_*_c).  The
+    field tensors are bound at construction and must stay alive; run()
+    replays the graph."""
+
+    def __init__(self, pairs, src_dtype, dst_dtype, mode, autotune=False, stream=None):
+        n = len(pairs)
+        hs = (C.c_int * n)(*[m.handle for m, _, _, _ in pairs])
+        xs = (C.c_void_p * n)(*[_ptr(x) for _, x, _, _ in pairs])
+        ys = (C.c_void_p * n)(*[_ptr(y) for _, _, y, _ in pairs])
+        ks = (C.c_int64 * n)(*[int(k) for _, _, _, k in pairs])
+        self._keep = list(pairs)
+        h = C.c_int(0)
+        _check(lib().axis_batch_plan_create_c(n, hs, xs, ys, ks, src_dtype, dst_dtype, mode, 1 if autotune else 0,
+                                              _stream(stream), C.byref(h)))
+        self.handle = h.value
+
+    def run(self, stream=None, sync=True):
+        _check(lib().axis_batch_plan_run_c(self.handle, _stream(stream), 1 if sync else 0))
+
+    def info(self):
+        fs, fi, nw = C.c_int(0), C.c_int(0), C.c_int(0)
+        ni = C.c_int64(0)
+        _check(lib().axis_batch_plan_info_c(self.handle, C.byref(fs), C.byref(fi), C.byref(ni), C.byref(nw)))
+        return {"fields_per_step": fs.value, "fields_per_item": fi.value, "items": ni.value, "workgroups": nw.value}
+
+    def close(self):
+        if getattr(self, "handle", 0):
+            lib().axis_destroy_batch_plan_c(self.handle)
+            self.handle = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

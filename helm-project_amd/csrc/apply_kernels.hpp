@@ -228,26 +228,19 @@ __global__ void __launch_bounds__(kBlock) spmm_seg_kernel(Mat A, const XT *X, YT
     segment_apply<E, FU, NUM>(A, X, Y, P, i, f0, f1);
 }
 
-/// Tile t of segments, fields [f0, f1): the tile's distinct source values of
-/// FU fields are loaded once into LDS by the whole workgroup (each source
-/// value read from HBM once per tile, in sorted column order, instead of once
-/// per nonzero), then every lane forms its segment's sums from LDS.  Tiles
-/// whose columns do not fit fall back to direct gathers.
+/// Tile t of segments, fields [f0, f1) (f0 < f1): the tile's distinct source
+/// values of FU fields are loaded once into LDS (xs) by the whole workgroup
+/// (each source value read from HBM once per tile, in sorted column order,
+/// instead of once per nonzero), then every lane forms its segment's sums
+/// from LDS.  Tiles whose columns do not fit fall back to direct gathers.
+/// Called by every thread of the workgroup; returns after a barrier, so xs
+/// may be reused at once.
 template <int E, int FU, int NUM, class XT, class YT>
-__global__ void __launch_bounds__(kBlock) spmm_tile_kernel(Mat A, const XT *X, YT *Y, double *P, index_t k, unsigned n_tiles, int chunks) {
-    // staged value type: the field's own precision widened to FP32 (FP64
-    // for the plain apply); staging FP64 copies of narrow fields was measured
-    // slower (twice the LDS per workgroup, lower occupancy)
+__device__ inline void tile_apply(const Mat &A, const XT *X, YT *Y, double *P, index_t t, index_t f0, index_t f1,
+                                  typename NumTraits<NUM>::V (*xs)[kTileCols]) {
     using V = typename NumTraits<NUM>::V;
     using W = typename NumTraits<NUM>::W;
-    __shared__ V xs[FU][kTileCols];
-    const unsigned w = xcd_remap(blockIdx.x, gridDim.x);
-    const unsigned t = w % n_tiles, ch = w / n_tiles;
-    const index_t per = (k + chunks - 1) / chunks;
-    const index_t f0 = static_cast<index_t>(ch) * per;
-    const index_t f1 = f0 + per < k ? f0 + per : k;
-    if (f0 >= f1) return;  // whole workgroup
-    const index_t i = static_cast<index_t>(t) * kBlock + threadIdx.x;
+    const index_t i = t * kBlock + threadIdx.x;
     const bool valid = i < A.n_seg;
     const int nu = A.tile_nu[t];
     if (nu < 0) {  // whole workgroup; one field per step keeps this rare path's registers low
@@ -283,7 +276,7 @@ __global__ void __launch_bounds__(kBlock) spmm_tile_kernel(Mat A, const XT *X, Y
 #pragma unroll
     for (int j = 0; j < kColsPerLane; ++j) {
         const int u = static_cast<int>(threadIdx.x) + j * kBlock;
-        uc[j] = u < nu ? A.tile_col[static_cast<index_t>(t) * kTileCols + u] : -1;
+        uc[j] = u < nu ? A.tile_col[t * kTileCols + u] : -1;
     }
     for (index_t f = f0; f < f1; f += FU) {
         V v[FU][kColsPerLane];
@@ -317,6 +310,18 @@ __global__ void __launch_bounds__(kBlock) spmm_tile_kernel(Mat A, const XT *X, Y
         }
         __syncthreads();
     }
+}
+
+template <int E, int FU, int NUM, class XT, class YT>
+__global__ void __launch_bounds__(kBlock) spmm_tile_kernel(Mat A, const XT *X, YT *Y, double *P, index_t k, unsigned n_tiles, int chunks) {
+    __shared__ typename NumTraits<NUM>::V xs[FU][kTileCols];
+    const unsigned w = xcd_remap(blockIdx.x, gridDim.x);
+    const unsigned t = w % n_tiles, ch = w / n_tiles;
+    const index_t per = (k + chunks - 1) / chunks;
+    const index_t f0 = static_cast<index_t>(ch) * per;
+    const index_t f1 = f0 + per < k ? f0 + per : k;
+    if (f0 >= f1) return;  // whole workgroup
+    tile_apply<E, FU, NUM, XT, YT>(A, X, Y, P, t, f0, f1, xs);
 }
 
 /// Y of the long rows: the partials of each row summed in segment order.

@@ -158,6 +158,33 @@ struct ApplyShape {
 /// fields and caches the fastest for this (n_dst, nnz, k, types, mode).
 ApplyShape autotune_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t k, ApplyMode mode, hipStream_t s);
 
+/// Launch shape of a persistent multi-pair batch: fields per step (staged
+/// together in LDS) and fields per work item.
+struct BatchShape {
+    int fields_per_step = 4, fields_per_item = 16;
+};
+
+/// Several regrid pairs applied in one persistent work-queue launch recorded
+/// in a HIP graph.  Pair p computes dst[p] = A[p] src[p] on device-resident
+/// layout_left fields [n_src, k[p]] -> [n_dst, k[p]]; all pairs share field
+/// types and mode.  The field pointers are bound at construction (the graph
+/// records them); run() replays the graph on any stream.  Replays of one
+/// plan must not overlap.
+class BatchPlan {
+   public:
+    BatchPlan(std::vector<std::shared_ptr<InterpolationMatrix>> mats, std::vector<const void *> src, std::vector<void *> dst,
+              std::vector<index_t> k, FieldType xt, FieldType yt, ApplyMode mode, bool autotune, hipStream_t s);
+    ~BatchPlan();
+    void run(hipStream_t s, bool sync);
+    BatchShape shape() const;
+    index_t n_items() const;
+    int n_workgroups() const;
+
+   private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+};
+
 // ---------------------------------------------------------------- internal
 namespace detail {
 

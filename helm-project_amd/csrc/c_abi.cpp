@@ -13,6 +13,7 @@
 #include <string>
 #include <unordered_map>
 #include <variant>
+#include <vector>
 
 #include "../../include/axis_hip.h"
 #include "axis_hip.hpp"
@@ -26,7 +27,7 @@ thread_local std::string g_last_error;
 struct Registry {
     std::mutex mu;
     int next = 1;
-    std::unordered_map<int, std::variant<std::shared_ptr<Mesh>, std::shared_ptr<InterpolationMatrix>>> items;
+    std::unordered_map<int, std::variant<std::shared_ptr<Mesh>, std::shared_ptr<InterpolationMatrix>, std::shared_ptr<BatchPlan>>> items;
     static Registry &get() {
         static Registry r;
         return r;
@@ -350,6 +351,48 @@ int axis_destroy_mesh_c(int handle) {
 
 int axis_destroy_matrix_c(int handle) {
     return guarded([&] { Registry::get().release<InterpolationMatrix>(handle); });
+}
+
+int axis_batch_plan_create_c(int n_pairs, const int *matrix_handles, const void *const *src, void *const *dst, const int64_t *n_fields,
+                             int src_dtype, int dst_dtype, int mode, int autotune, void *stream, int *plan_handle) {
+    return guarded([&] {
+        if (!plan_handle) throw InvalidArgument("null plan_handle");
+        if (n_pairs <= 0 || !matrix_handles || !src || !dst || !n_fields) throw InvalidArgument("axis_batch_plan_create_c: empty or null pair list");
+        std::vector<std::shared_ptr<InterpolationMatrix>> mats;
+        std::vector<const void *> xs;
+        std::vector<void *> ys;
+        std::vector<index_t> ks;
+        for (int p = 0; p < n_pairs; ++p) {
+            mats.push_back(Registry::get().find<InterpolationMatrix>(matrix_handles[p]));
+            xs.push_back(src[p]);
+            ys.push_back(dst[p]);
+            ks.push_back(n_fields[p]);
+            if (n_fields[p] > 0 && (!is_device_ptr(src[p]) || !is_device_ptr(dst[p])))
+                throw InvalidArgument("axis_batch_plan_create_c: fields of pair " + std::to_string(p) + " are not device memory");
+        }
+        auto plan = std::make_shared<BatchPlan>(std::move(mats), std::move(xs), std::move(ys), std::move(ks), ftype(src_dtype), ftype(dst_dtype),
+                                                amode(mode), autotune != 0, static_cast<hipStream_t>(stream));
+        *plan_handle = Registry::get().add(std::move(plan));
+    });
+}
+
+int axis_batch_plan_run_c(int plan_handle, void *stream, int sync) {
+    return guarded([&] { Registry::get().find<BatchPlan>(plan_handle)->run(static_cast<hipStream_t>(stream), sync != 0); });
+}
+
+int axis_batch_plan_info_c(int plan_handle, int *fields_per_step, int *fields_per_item, int64_t *n_items, int *n_workgroups) {
+    return guarded([&] {
+        auto P = Registry::get().find<BatchPlan>(plan_handle);
+        const BatchShape sh = P->shape();
+        if (fields_per_step) *fields_per_step = sh.fields_per_step;
+        if (fields_per_item) *fields_per_item = sh.fields_per_item;
+        if (n_items) *n_items = P->n_items();
+        if (n_workgroups) *n_workgroups = P->n_workgroups();
+    });
+}
+
+int axis_destroy_batch_plan_c(int handle) {
+    return guarded([&] { Registry::get().release<BatchPlan>(handle); });
 }
 
 }  // extern "C"

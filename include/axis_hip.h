@@ -139,6 +139,27 @@ int axis_autotune_apply_c(int matrix_handle, const void *src, int src_dtype, voi
  * absolute_error, relative_error}. */
 int axis_check_conservation_c(int matrix_handle, const double *src, const double *dst, int norm_type, double *out4);
 
+/* Multi-pair persistent batch (no reference counterpart; the reference
+ * applies one matrix per batch_apply call, apply.hpp:275-343).  Binds
+ * n_pairs (matrix, device src [n_src, n_fields[p]], device dst
+ * [n_dst, n_fields[p]]) triples of one field type and mode, builds the work
+ * queues and records one HIP graph (persistent work-queue kernel + long-row
+ * reductions); autotune != 0 times the launch shapes first.  The matrices
+ * stay alive while the plan exists. */
+int axis_batch_plan_create_c(int n_pairs, const int *matrix_handles, const void *const *src, void *const *dst, const int64_t *n_fields,
+                             int src_dtype, int dst_dtype, int mode, int autotune, void *stream, int *plan_handle);
+
+/* Replays the plan's graph on `stream` (NULL = default stream); sync != 0
+ * waits for it.  Replays of one plan must not overlap. */
+int axis_batch_plan_run_c(int plan_handle, void *stream, int sync);
+
+/* Chosen shape (fields per step, fields per work item), work items and
+ * persistent workgroups of a plan; pointers may be NULL. */
+int axis_batch_plan_info_c(int plan_handle, int *fields_per_step, int *fields_per_item, int64_t *n_items, int *n_workgroups);
+
+/* Releases a plan handle (the graph and the plan's workspaces). */
+int axis_destroy_batch_plan_c(int handle);
+
 /* axis_destroy_mesh_c / axis_destroy_matrix_c (axis_c_interop.cpp:241-250). */
 int axis_destroy_mesh_c(int handle);
 int axis_destroy_matrix_c(int handle);

@@ -301,3 +301,64 @@ def test_c384_properties_and_sampled_rows():
     brp, bcol, bval = bil.csr()
     assert bil.nnz == 4 * grids.n_cells(dst)
     assert np.max(np.abs(np.add.reduceat(bval, brp[:-1]) - 1.0)) < 1e-12
+
+
+@pytest.fixture(scope="module")
+def batch_pairs():
+    """Three uneven pairs (sizes and field counts differ by ~10x)."""
+    specs = [(grids.latlon_deg(2.0), grids.cubed_sphere(24), 1),
+             (grids.latlon_deg(1.0), grids.cubed_sphere(48), 16),
+             (grids.latlon_deg(2.0), grids.cubed_sphere(12), 37)]
+    out = []
+    for i, (s, d, k) in enumerate(specs):
+        m = axis.generate_weights(axis.Mesh(s), axis.Mesh(d), axis.CONSERVATIVE)
+        n_src, n_dst, _ = m.info()
+        scales = torch.tensor(fields.level_scales(1004 + i, k), dtype=torch.float64, device="cuda")
+        base = torch.tensor(np.cos(np.arange(n_src) * 0.001 * (i + 1)), dtype=torch.float64, device="cuda")
+        out.append((m, (scales[:, None] * base[None, :]).contiguous(), n_src, n_dst, k))
+    return out
+
+
+@pytest.mark.parametrize("mode,xdt,autotune", [(axis.MIXED, axis.F32, False), (axis.MIXED_ACC64, axis.BF16, True),
+                                               (axis.PLAIN, axis.F64, False)])
+def test_batch_plan_matches_single_applies(batch_pairs, mode, xdt, autotune):
+    """The persistent multi-pair graph gives, bit for bit, what one
+    batch_apply per pair gives (every row is summed in the same order), on
+    every replay, with eager work in between."""
+    tdt = {axis.F64: torch.float64, axis.F32: torch.float32, axis.BF16: torch.bfloat16}
+    ydt = axis.F64 if mode == axis.PLAIN else axis.F32
+    pairs, refs = [], []
+    for m, X64, n_src, n_dst, k in batch_pairs:
+        X = X64.to(tdt[xdt]).contiguous()
+        Y = torch.full((k, n_dst), 7.0, dtype=tdt[ydt], device="cuda")
+        R = torch.empty_like(Y)
+        m.batch_apply(X, R, k, xdt, ydt, mode)
+        pairs.append((m, X, Y, k))
+        refs.append(R)
+    plan = axis.BatchPlan(pairs, xdt, ydt, mode, autotune=autotune)
+    say("plan", plan.info())
+    for rep in range(3):
+        for (_, _, Y, _) in pairs:
+            Y.fill_(-1.0)  # eager work between replays
+        torch.cuda.synchronize()
+        plan.run()
+        for (_, _, Y, _), R in zip(pairs, refs):
+            assert torch.equal(Y, R), rep
+    plan.close()
+
+
+def test_batch_plan_keeps_matrices_and_rejects_bad_input(batch_pairs):
+    m0, X64, n_src, n_dst, k = batch_pairs[0]
+    m = axis.generate_weights(axis.Mesh(grids.latlon_deg(2.0)), axis.Mesh(grids.cubed_sphere(24)), axis.CONSERVATIVE)
+    Y = torch.empty((k, n_dst), dtype=torch.float64, device="cuda")
+    plan = axis.BatchPlan([(m, X64, Y, k)], axis.F64, axis.F64, axis.PLAIN)
+    m.close()  # the plan holds the matrix
+    plan.run()
+    R = torch.empty_like(Y)
+    m0.batch_apply(X64, R, k, axis.F64, axis.F64, axis.PLAIN)
+    assert torch.equal(Y, R)
+    plan.close()
+    with pytest.raises(axis.AxisError, match="FP64"):
+        axis.BatchPlan([(m0, X64.float(), Y, k)], axis.F32, axis.F64, axis.PLAIN)
+    with pytest.raises(axis.AxisError, match="device memory"):
+        axis.BatchPlan([(m0, np.zeros(n_src), np.zeros(n_dst), 1)], axis.F64, axis.F64, axis.PLAIN)
