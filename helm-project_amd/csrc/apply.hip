@@ -55,11 +55,14 @@ __global__ void len_hist_kernel(const index_t *rp, index_t n_dst, index_t *hist)
         atomicAdd(reinterpret_cast<unsigned long long *>(hist + threadIdx.x), static_cast<unsigned long long>(cnt[threadIdx.x]));
 }
 
-// sort key of row r: (coarse spatial cell << 32) | r; rows without a key keep index order
+// sort key of row r: (coarse spatial cell << 32) | r, the cell taken from
+// the first row of r's aligned block of 32 rows, so that blocks stay whole
+// and Y is written in full 128-byte lines (FP32); rows without a key keep
+// index order
 __global__ void order_key_kernel(const std::uint32_t *row_key, int shift, index_t n_dst, std::uint64_t *key) {
     index_t r = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (r >= n_dst) return;
-    const std::uint64_t coarse = row_key ? static_cast<std::uint64_t>(row_key[r] >> shift) : 0ull;
+    const std::uint64_t coarse = row_key ? static_cast<std::uint64_t>(row_key[r & ~index_t(31)] >> shift) : 0ull;
     key[r] = (coarse << 32) | static_cast<std::uint64_t>(r);
 }
 
@@ -103,40 +106,7 @@ __global__ void seg_fill_kernel(const std::uint64_t *key, const index_t *rp, ind
     long_slot[long_off[p]] = static_cast<std::int32_t>(slot_off[p]);
 }
 
-// Sorts the 256 segments of each tile by decreasing length (stable), so that
-// the lanes of one wave run nearly the same number of terms.
-__global__ void __launch_bounds__(apply_detail::kBlock) tile_sort_kernel(std::int32_t *seg_row, index_t *seg_begin, std::int32_t *seg_len,
-                                                                         std::int32_t *seg_slot, index_t n_seg) {
-    constexpr int kB = apply_detail::kBlock;
-    __shared__ int len[kB];
-    const index_t i = static_cast<index_t>(blockIdx.x) * kB + threadIdx.x;
-    const bool valid = i < n_seg;
-    const int me = valid ? seg_len[i] : -1;
-    std::int32_t row = 0, slot = 0;
-    index_t beg = 0;
-    if (valid) {
-        row = seg_row[i];
-        beg = seg_begin[i];
-        slot = seg_slot[i];
-    }
-    len[threadIdx.x] = me;
-    __syncthreads();
-    int rank = 0;
-    for (int j = 0; j < kB; ++j) {
-        const int o = len[j];
-        rank += (o > me) || (o == me && j < static_cast<int>(threadIdx.x));
-    }
-    __syncthreads();
-    if (valid) {
-        const index_t d = static_cast<index_t>(blockIdx.x) * kB + rank;  // invalid lanes (len -1) rank last
-        seg_row[d] = row;
-        seg_begin[d] = beg;
-        seg_len[d] = me;
-        seg_slot[d] = slot;
-    }
-}
-
-// exclusive scan of one int per thread over a 256-thread workgroup
+// exclusive scan of one int per thread over a tile workgroup
 __device__ inline int block_exclusive_scan(int v, int *wsum, int &total) {
     const int lane = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
     int x = v;
@@ -149,7 +119,7 @@ __device__ inline int block_exclusive_scan(int v, int *wsum, int &total) {
     __syncthreads();
     int before = 0;
     total = 0;
-    for (int w = 0; w < apply_detail::kBlock / 64; ++w) {
+    for (int w = 0; w < apply_detail::kTileSeg / 64; ++w) {
         if (w < wave) before += wsum[w];
         total += wsum[w];
     }
@@ -167,10 +137,10 @@ constexpr int pow2_at_least(int v) {
 // LDS (bitonic), keeps the distinct ones, and records for each nonzero the
 // position of its column among them.
 template <int E>
-__global__ void __launch_bounds__(apply_detail::kBlock) tile_build_kernel(const index_t *seg_begin, const std::int32_t *seg_len, index_t n_seg,
+__global__ void __launch_bounds__(apply_detail::kTileSeg) tile_build_kernel(const index_t *seg_begin, const std::int32_t *seg_len, index_t n_seg,
                                                                           const std::int32_t *col32, std::int32_t *tile_nu, std::int32_t *tile_col,
                                                                           std::uint16_t *lcol) {
-    constexpr int kB = apply_detail::kBlock;
+    constexpr int kB = apply_detail::kTileSeg;
     constexpr int SZ = pow2_at_least(kB * E);
     constexpr int C = SZ / kB;
     constexpr int kEmpty = 0x7FFFFFFF;
@@ -323,13 +293,13 @@ void InterpolationMatrix::prepare_apply(hipStream_t s) {
     const std::int32_t end_slot = n_slots;
     AXIS_HIP_CHECK(hipMemcpyAsync(long_slot.get() + n_long_rows, &end_slot, sizeof(std::int32_t), hipMemcpyHostToDevice, s));
     // tiles
-    n_tiles = (n_seg + apply_detail::kBlock - 1) / apply_detail::kBlock;
+    n_tiles = (n_seg + apply_detail::kTileSeg - 1) / apply_detail::kTileSeg;
     tile_nu.resize(static_cast<std::size_t>(std::max<index_t>(n_tiles, 1)));
     tile_col.resize(static_cast<std::size_t>(std::max<index_t>(n_tiles, 1)) * apply_detail::kTileCols);
     t_lcol.resize(static_cast<std::size_t>(n1));
     if (n_tiles > 0) {
         const unsigned g = static_cast<unsigned>(n_tiles);
-        const int B = apply_detail::kBlock;
+        const int B = apply_detail::kTileSeg;
         switch (seg_cap) {
             case 4: tile_build_kernel<4><<<g, B, 0, s>>>(seg_begin.get(), seg_len.get(), n_seg, col32.get(), tile_nu.get(), tile_col.get(), t_lcol.get()); break;
             case 8: tile_build_kernel<8><<<g, B, 0, s>>>(seg_begin.get(), seg_len.get(), n_seg, col32.get(), tile_nu.get(), tile_col.get(), t_lcol.get()); break;

@@ -45,10 +45,11 @@
 
 namespace axis_hip::apply_detail {
 
-constexpr int kBlock = 256;       // segments per workgroup (and per tile)
-constexpr int kMaxE = 16;         // largest register segment
-constexpr int kTileCols = 1024;   // most distinct source columns a staged tile holds
-constexpr int kColsPerLane = kTileCols / kBlock;
+constexpr int kBlock = 256;        // segments per workgroup of the direct-gather kernel
+constexpr int kTileSeg = 256;      // segments per staged tile (= threads of its workgroup; 512 measured slower)
+constexpr int kMaxE = 16;          // largest register segment
+constexpr int kTileCols = 1024;    // most distinct source columns a staged tile holds
+constexpr int kColsPerLane = kTileCols / kTileSeg;
 
 enum Num : int { kPlain = 0, kComp32 = 1, kAcc64 = 2 };
 
@@ -240,7 +241,7 @@ __device__ inline void tile_apply(const Mat &A, const XT *X, YT *Y, double *P, i
                                   typename NumTraits<NUM>::V (*xs)[kTileCols]) {
     using V = typename NumTraits<NUM>::V;
     using W = typename NumTraits<NUM>::W;
-    const index_t i = t * kBlock + threadIdx.x;
+    const index_t i = t * kTileSeg + threadIdx.x;
     const bool valid = i < A.n_seg;
     const int nu = A.tile_nu[t];
     if (nu < 0) {  // whole workgroup; one field per step keeps this rare path's registers low
@@ -275,11 +276,13 @@ __device__ inline void tile_apply(const Mat &A, const XT *X, YT *Y, double *P, i
     int uc[kColsPerLane];
 #pragma unroll
     for (int j = 0; j < kColsPerLane; ++j) {
-        const int u = static_cast<int>(threadIdx.x) + j * kBlock;
+        const int u = static_cast<int>(threadIdx.x) + j * kTileSeg;
         uc[j] = u < nu ? A.tile_col[t * kTileCols + u] : -1;
     }
-    for (index_t f = f0; f < f1; f += FU) {
-        V v[FU][kColsPerLane];
+    // the source values of the next FU fields are loaded while this step's
+    // sums are formed (one step of software pipelining)
+    V v[FU][kColsPerLane];
+    auto fetch = [&](index_t f) {
 #pragma unroll
         for (int u = 0; u < FU; ++u) {
             const index_t fu = (f + u < f1) ? f + u : f1 - 1;
@@ -287,12 +290,16 @@ __device__ inline void tile_apply(const Mat &A, const XT *X, YT *Y, double *P, i
 #pragma unroll
             for (int j = 0; j < kColsPerLane; ++j) v[u][j] = uc[j] >= 0 ? static_cast<V>(ldv(xf + uc[j])) : V(0);
         }
+    };
+    fetch(f0);
+    for (index_t f = f0; f < f1; f += FU) {
 #pragma unroll
         for (int u = 0; u < FU; ++u)
 #pragma unroll
             for (int j = 0; j < kColsPerLane; ++j)
-                if (uc[j] >= 0) xs[u][threadIdx.x + j * kBlock] = v[u][j];
+                if (uc[j] >= 0) xs[u][threadIdx.x + j * kTileSeg] = v[u][j];
         __syncthreads();
+        if (f + FU < f1) fetch(f + FU);
 #pragma unroll
         for (int u = 0; u < FU; ++u) {
             RowAcc<NUM> acc;
@@ -313,7 +320,7 @@ __device__ inline void tile_apply(const Mat &A, const XT *X, YT *Y, double *P, i
 }
 
 template <int E, int FU, int NUM, class XT, class YT>
-__global__ void __launch_bounds__(kBlock) spmm_tile_kernel(Mat A, const XT *X, YT *Y, double *P, index_t k, unsigned n_tiles, int chunks) {
+__global__ void __launch_bounds__(kTileSeg) spmm_tile_kernel(Mat A, const XT *X, YT *Y, double *P, index_t k, unsigned n_tiles, int chunks) {
     __shared__ typename NumTraits<NUM>::V xs[FU][kTileCols];
     const unsigned w = xcd_remap(blockIdx.x, gridDim.x);
     const unsigned t = w % n_tiles, ch = w / n_tiles;
@@ -378,13 +385,13 @@ template <int E, int FU, int NUM, class XT, class YT>
 void launch_e(InterpolationMatrix &A, const void *x, void *y, index_t k, int chunks, bool tiled, hipStream_t s) {
     const Mat M = mat_of(A);
     double *P = partials(A, k);
-    const unsigned nb = static_cast<unsigned>(std::max<index_t>((A.n_seg + kBlock - 1) / kBlock, 1));
+    const unsigned nb = static_cast<unsigned>(std::max<index_t>(tiled ? A.n_tiles : (A.n_seg + kBlock - 1) / kBlock, 1));
     const std::uint64_t nwork = static_cast<std::uint64_t>(nb) * chunks;
     if (nwork >= (1ull << 31)) throw InvalidArgument("apply: launch too large");
     if (A.n_seg > 0) {
         if (tiled)
-            spmm_tile_kernel<E, FU, NUM, XT, YT><<<static_cast<unsigned>(nwork), kBlock, 0, s>>>(M, static_cast<const XT *>(x),
-                                                                                                  static_cast<YT *>(y), P, k, nb, chunks);
+            spmm_tile_kernel<E, FU, NUM, XT, YT><<<static_cast<unsigned>(nwork), kTileSeg, 0, s>>>(M, static_cast<const XT *>(x),
+                                                                                                    static_cast<YT *>(y), P, k, nb, chunks);
         else
             spmm_seg_kernel<E, FU, NUM, XT, YT><<<static_cast<unsigned>(nwork), kBlock, 0, s>>>(M, static_cast<const XT *>(x),
                                                                                                  static_cast<YT *>(y), P, k, nb, chunks);

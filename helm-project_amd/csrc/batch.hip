@@ -29,7 +29,7 @@ namespace axis_hip {
 
 namespace {
 
-using apply_detail::kBlock;
+using apply_detail::kTileSeg;
 using apply_detail::kTileCols;
 using apply_detail::Mat;
 using apply_detail::NumTraits;
@@ -52,7 +52,7 @@ __global__ void clear_counters_kernel(unsigned *ctr) {
 }
 
 template <int E, int FU, int NUM, class XT, class YT>
-__global__ void __launch_bounds__(kBlock) persistent_apply_kernel(const PairDesc *pairs, const Item *items, const int *qbeg, unsigned *ctr) {
+__global__ void __launch_bounds__(kTileSeg) persistent_apply_kernel(const PairDesc *pairs, const Item *items, const int *qbeg, unsigned *ctr) {
     __shared__ typename NumTraits<NUM>::V xs[FU][kTileCols];
     __shared__ int s_item;
     const int home = static_cast<int>(blockIdx.x & (kQueues - 1));  // workgroups are dealt to XCDs round-robin
@@ -172,13 +172,13 @@ struct BatchPlan::Impl {
         AXIS_HIP_CHECK(hipGetDevice(&dev));
         hipDeviceProp_t prop{};
         AXIS_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
-        AXIS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0));
+        AXIS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kTileSeg, 0));
         n_wg = std::max(1, per_cu) * prop.multiProcessorCount;
         n_wg = (n_wg + kQueues - 1) / kQueues * kQueues;
         (void)s;
         AXIS_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
         clear_counters_kernel<<<1, 64, 0, cap>>>(d_ctr.get());
-        if (n_items > 0) kern<<<static_cast<unsigned>(n_wg), kBlock, 0, cap>>>(d_pairs.get(), d_items.get(), d_qbeg.get(), d_ctr.get());
+        if (n_items > 0) kern<<<static_cast<unsigned>(n_wg), kTileSeg, 0, cap>>>(d_pairs.get(), d_items.get(), d_qbeg.get(), d_ctr.get());
         for (std::size_t p = 0; p < mats.size(); ++p) {
             const index_t n = static_cast<index_t>(mats[p]->n_long_rows) * k[p];
             if (n > 0) pair_long_reduce_kernel<YT><<<grid_for(n, 256), 256, 0, cap>>>(d_pairs.get(), static_cast<int>(p), k[p]);
