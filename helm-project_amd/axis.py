@@ -83,6 +83,15 @@ def lib():
         "axis_batch_plan_run_c": [C.c_int, C.c_void_p, C.c_int],
         "axis_batch_plan_info_c": [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _I64, C.POINTER(C.c_int)],
         "axis_destroy_batch_plan_c": [C.c_int],
+        "axis_mlp_create_c": [C.POINTER(C.c_float), C.c_uint64, C.c_float, C.POINTER(C.c_int)],
+        "axis_mlp_params_c": [C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)],
+        "axis_mlp_set_params_c": [C.c_int, C.POINTER(C.c_float)],
+        "axis_mlp_forward_c": [C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p],
+        "axis_mlp_gradient_c": [C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_float, C.c_float, C.c_void_p,
+                                C.POINTER(C.c_float), C.c_void_p],
+        "axis_mlp_train_c": [C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_float, C.c_float, C.c_float,
+                             C.POINTER(C.c_float), C.c_void_p],
+        "axis_destroy_mlp_c": [C.c_int],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -101,7 +110,9 @@ def exported_symbols():
             "axis_matrix_to_csr_c", "axis_matrix_info_c", "axis_matrix_csr_c", "axis_matrix_fractions_c",
             "axis_matrix_stats_c", "axis_apply_c", "axis_batch_apply_c", "axis_batch_apply_async_c",
             "axis_autotune_apply_c", "axis_check_conservation_c", "axis_destroy_mesh_c", "axis_destroy_matrix_c",
-            "axis_batch_plan_create_c", "axis_batch_plan_run_c", "axis_batch_plan_info_c", "axis_destroy_batch_plan_c"]
+            "axis_batch_plan_create_c", "axis_batch_plan_run_c", "axis_batch_plan_info_c", "axis_destroy_batch_plan_c",
+            "axis_mlp_create_c", "axis_mlp_params_c", "axis_mlp_set_params_c", "axis_mlp_forward_c",
+            "axis_mlp_gradient_c", "axis_mlp_train_c", "axis_destroy_mlp_c"]
 
 
 def _check(rc: int):
@@ -337,6 +348,65 @@ class BatchPlan:
     def close(self):
         if getattr(self, "handle", 0):
             lib().axis_destroy_batch_plan_c(self.handle)
+            self.handle = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+N_MLP_PARAMS = 17924
+
+
+class MLP:
+    """The candidate-proposer network on the GPU (axis_mlp_*_c): 6 -> 128 ->
+    128 -> 4, BF16 MFMA with FP32 accumulation, Adam training."""
+
+    def __init__(self, params=None, seed=0, feature_scale=1.0):
+        h = C.c_int(0)
+        p = None
+        if params is not None:
+            self._init = np.ascontiguousarray(params, dtype=np.float32)
+            p = self._init.ctypes.data_as(C.POINTER(C.c_float))
+        _check(lib().axis_mlp_create_c(p, C.c_uint64(seed), C.c_float(feature_scale), C.byref(h)))
+        self.handle = h.value
+
+    def params(self):
+        out = np.zeros(N_MLP_PARAMS, dtype=np.float32)
+        fs = C.c_float(0)
+        _check(lib().axis_mlp_params_c(self.handle, out.ctypes.data_as(C.POINTER(C.c_float)), C.byref(fs)))
+        return out
+
+    def feature_scale(self):
+        fs = C.c_float(0)
+        _check(lib().axis_mlp_params_c(self.handle, None, C.byref(fs)))
+        return fs.value
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().axis_mlp_set_params_c(self.handle, p.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def forward(self, x, out, stream=None):
+        _check(lib().axis_mlp_forward_c(self.handle, _ptr(x), _ptr(out), int(x.shape[0]), _stream(stream)))
+        return out
+
+    def gradient(self, x, t, grad, wc=1.0, tau=0.5, stream=None):
+        loss = C.c_float(0)
+        _check(lib().axis_mlp_gradient_c(self.handle, _ptr(x), _ptr(t), int(x.shape[0]), C.c_float(wc), C.c_float(tau),
+                                         _ptr(grad), C.byref(loss), _stream(stream)))
+        return loss.value
+
+    def train(self, x, t, batch, lr, wc=1.0, tau=0.5, stream=None):
+        loss = C.c_float(0)
+        _check(lib().axis_mlp_train_c(self.handle, _ptr(x), _ptr(t), int(x.shape[0]), int(batch), C.c_float(lr),
+                                      C.c_float(wc), C.c_float(tau), C.byref(loss), _stream(stream)))
+        return loss.value
+
+    def close(self):
+        if getattr(self, "handle", 0):
+            lib().axis_destroy_mlp_c(self.handle)
             self.handle = 0
 
     def __del__(self):

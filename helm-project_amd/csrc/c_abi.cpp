@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstring>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -17,6 +18,7 @@
 
 #include "../../include/axis_hip.h"
 #include "axis_hip.hpp"
+#include "mlp.hpp"
 
 namespace {
 
@@ -27,7 +29,9 @@ thread_local std::string g_last_error;
 struct Registry {
     std::mutex mu;
     int next = 1;
-    std::unordered_map<int, std::variant<std::shared_ptr<Mesh>, std::shared_ptr<InterpolationMatrix>, std::shared_ptr<BatchPlan>>> items;
+    std::unordered_map<int, std::variant<std::shared_ptr<Mesh>, std::shared_ptr<InterpolationMatrix>, std::shared_ptr<BatchPlan>,
+                                         std::shared_ptr<mlp::Model>>>
+        items;
     static Registry &get() {
         static Registry r;
         return r;
@@ -393,6 +397,66 @@ int axis_batch_plan_info_c(int plan_handle, int *fields_per_step, int *fields_pe
 
 int axis_destroy_batch_plan_c(int handle) {
     return guarded([&] { Registry::get().release<BatchPlan>(handle); });
+}
+
+int axis_mlp_create_c(const float *params, uint64_t seed, float feature_scale, int *mlp_handle) {
+    return guarded([&] {
+        if (!mlp_handle) throw InvalidArgument("null mlp_handle");
+        std::vector<float> p(mlp::kParams);
+        if (params)
+            std::memcpy(p.data(), params, sizeof(float) * mlp::kParams);
+        else
+            mlp::init_params(seed, p.data());
+        *mlp_handle = Registry::get().add(std::make_shared<mlp::Model>(p.data(), feature_scale));
+    });
+}
+
+int axis_mlp_params_c(int mlp_handle, float *params, float *feature_scale) {
+    return guarded([&] {
+        auto M = Registry::get().find<mlp::Model>(mlp_handle);
+        if (params) M->get_params(params, nullptr);
+        if (feature_scale) *feature_scale = M->feature_scale();
+    });
+}
+
+int axis_mlp_set_params_c(int mlp_handle, const float *params) {
+    return guarded([&] {
+        if (!params) throw InvalidArgument("null params");
+        Registry::get().find<mlp::Model>(mlp_handle)->set_params(params, nullptr);
+    });
+}
+
+int axis_mlp_forward_c(int mlp_handle, const float *x, float *out, int64_t n, void *stream) {
+    return guarded([&] {
+        if (n < 0) throw InvalidArgument("negative sample count");
+        if (n > 0 && (!is_device_ptr(x) || !is_device_ptr(out))) throw InvalidArgument("axis_mlp_forward_c: x and out must be device memory");
+        auto M = Registry::get().find<mlp::Model>(mlp_handle);
+        M->forward(x, n, out, static_cast<hipStream_t>(stream));
+        AXIS_HIP_CHECK(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    });
+}
+
+int axis_mlp_gradient_c(int mlp_handle, const float *x, const float *t, int64_t n, float wc, float tau, float *grad, float *loss, void *stream) {
+    return guarded([&] {
+        if (!is_device_ptr(x) || !is_device_ptr(t) || !is_device_ptr(grad)) throw InvalidArgument("axis_mlp_gradient_c: device pointers expected");
+        auto M = Registry::get().find<mlp::Model>(mlp_handle);
+        const float l = M->gradient(x, t, n, wc, tau, grad, static_cast<hipStream_t>(stream));
+        if (loss) *loss = l;
+    });
+}
+
+int axis_mlp_train_c(int mlp_handle, const float *x, const float *t, int64_t n, int64_t batch, float lr, float wc, float tau, float *mean_loss,
+                     void *stream) {
+    return guarded([&] {
+        if (!is_device_ptr(x) || !is_device_ptr(t)) throw InvalidArgument("axis_mlp_train_c: device pointers expected");
+        auto M = Registry::get().find<mlp::Model>(mlp_handle);
+        const float l = M->train(x, t, n, batch, lr, wc, tau, static_cast<hipStream_t>(stream));
+        if (mean_loss) *mean_loss = l;
+    });
+}
+
+int axis_destroy_mlp_c(int handle) {
+    return guarded([&] { Registry::get().release<mlp::Model>(handle); });
 }
 
 }  // extern "C"

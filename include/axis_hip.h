@@ -160,6 +160,27 @@ int axis_batch_plan_info_c(int plan_handle, int *fields_per_step, int *fields_pe
 /* Releases a plan handle (the graph and the plan's workspaces). */
 int axis_destroy_batch_plan_c(int handle);
 
+/* Candidate-proposer network (no reference counterpart; the reference's
+ * broad phase is an ArborX BVH only).  A 6 -> 128 -> 128 -> 4 MLP with BF16
+ * weights on the MFMA units and FP32 accumulation: inputs are a destination
+ * cell's centroid xyz and its box extents / feature_scale, outputs the
+ * proposed search-cap centre offset (units of feature_scale) and log of its
+ * radius (units of feature_scale).  params is NULL (seeded initialisation)
+ * or 17,924 FP32 values (W1 [128][6], b1, W2 [128][128], b2, W3 [4][128], b3). */
+int axis_mlp_create_c(const float *params, uint64_t seed, float feature_scale, int *mlp_handle);
+/* Copies the FP32 parameters (17,924 values) and/or the feature scale out; either pointer may be NULL. */
+int axis_mlp_params_c(int mlp_handle, float *params, float *feature_scale);
+int axis_mlp_set_params_c(int mlp_handle, const float *params);
+/* out [n][4] = network(x [n][6]); device pointers. */
+int axis_mlp_forward_c(int mlp_handle, const float *x, float *out, int64_t n, void *stream);
+/* Mean loss over (x [n][6], t [n][4]) and its gradient into grad [17,924]
+ * (device pointers): wc * |o[0:3] - t[0:3]|^2 + pinball_tau(t[3] - o[3]). */
+int axis_mlp_gradient_c(int mlp_handle, const float *x, const float *t, int64_t n, float wc, float tau, float *grad, float *loss, void *stream);
+/* One Adam pass over the samples in order, mini-batches of `batch`; *mean_loss = mean batch loss. */
+int axis_mlp_train_c(int mlp_handle, const float *x, const float *t, int64_t n, int64_t batch, float lr, float wc, float tau, float *mean_loss,
+                     void *stream);
+int axis_destroy_mlp_c(int handle);
+
 /* axis_destroy_mesh_c / axis_destroy_matrix_c (axis_c_interop.cpp:241-250). */
 int axis_destroy_mesh_c(int handle);
 int axis_destroy_matrix_c(int handle);
