@@ -92,6 +92,8 @@ def lib():
         "axis_mlp_train_c": [C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_float, C.c_float, C.c_float,
                              C.POINTER(C.c_float), C.c_void_p],
         "axis_destroy_mlp_c": [C.c_int],
+        "axis_generate_weights_proposed_c": [C.c_int, C.c_int, C.POINTER(RegridConfig), C.c_int, C.POINTER(C.c_int), _I64],
+        "axis_proposer_training_data_c": [C.c_int, C.c_int, C.c_float, C.c_void_p, C.c_void_p, _I64],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -112,7 +114,8 @@ def exported_symbols():
             "axis_autotune_apply_c", "axis_check_conservation_c", "axis_destroy_mesh_c", "axis_destroy_matrix_c",
             "axis_batch_plan_create_c", "axis_batch_plan_run_c", "axis_batch_plan_info_c", "axis_destroy_batch_plan_c",
             "axis_mlp_create_c", "axis_mlp_params_c", "axis_mlp_set_params_c", "axis_mlp_forward_c",
-            "axis_mlp_gradient_c", "axis_mlp_train_c", "axis_destroy_mlp_c"]
+            "axis_mlp_gradient_c", "axis_mlp_train_c", "axis_destroy_mlp_c", "axis_generate_weights_proposed_c",
+            "axis_proposer_training_data_c"]
 
 
 def _check(rc: int):
@@ -414,3 +417,24 @@ class MLP:
             self.close()
         except Exception:
             pass
+
+
+def generate_weights_proposed(src: "Mesh", dst: "Mesh", net: MLP, config: "RegridConfig" = None):
+    """Conservative weights whose broad phase uses the proposer network
+    (axis_generate_weights_proposed_c); returns (Matrix, stats dict)."""
+    h = C.c_int(0)
+    st = np.zeros(5, dtype=np.int64)
+    cfg = C.byref(config) if config is not None else None
+    _check(lib().axis_generate_weights_proposed_c(src.handle, dst.handle, cfg, net.handle, C.byref(h),
+                                                  st.ctypes.data_as(C.POINTER(C.c_int64))))
+    keys = ("cells", "guard_rejected", "area_failed", "pairs_proposed", "pairs_fallback")
+    return Matrix(h.value), dict(zip(keys, (int(v) for v in st)))
+
+
+def proposer_training_data(src: "Mesh", dst: "Mesh", feature_scale: float, x, t):
+    """Fills device tensors x [n_dst, 6], t [n_dst, 4] with the proposer's
+    features and labels for this mesh pair; returns the unlabelled count."""
+    bad = C.c_int64(0)
+    _check(lib().axis_proposer_training_data_c(src.handle, dst.handle, C.c_float(feature_scale), _ptr(x), _ptr(t),
+                                               C.byref(bad)))
+    return bad.value

@@ -22,6 +22,11 @@
 
 namespace axis_hip {
 
+namespace mlp {
+class Model;
+}
+struct ProposerStats;
+
 // regrid_config.hpp:24-89
 enum class InterpolationMethod : std::uint8_t { Bilinear, NearestNeighbor, Bicubic, Patch, Conservative1stOrder, Conservative2ndOrder };
 enum class NormType : std::uint8_t { DstArea, FracArea };
@@ -132,6 +137,11 @@ enum class ApplyMode : int { Plain = 0, Mixed = 1, MixedAcc64 = 2 };
 class WeightGenerator {
    public:
     static std::unique_ptr<InterpolationMatrix> generate(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, hipStream_t s = nullptr);
+    /// generate() whose conservative broad phase takes the proposer
+    /// network's guarded search caps (LBVH for the cells they cannot serve);
+    /// the weights equal generate()'s.
+    static std::unique_ptr<InterpolationMatrix> generate_proposed(const Mesh &src, const Mesh &dst, const RegridConfig &cfg,
+                                                                  const mlp::Model *proposer, ProposerStats *stats, hipStream_t s = nullptr);
 };
 
 /// Builds a matrix from COO triplets (the reference's InterpolationMatrix
@@ -231,7 +241,10 @@ void radix_sort_pairs_u64(std::uint64_t *keys, std::uint32_t *vals, std::size_t 
 void lbvh_candidates(const float *src_boxes, index_t n_src, const float *dst_boxes, index_t n_dst, DeviceBuffer<index_t> &cand_off,
                      DeviceBuffer<std::int32_t> &cand_src, hipStream_t s);
 
-std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, hipStream_t s);
+/// First-order conservative weights; with a proposer network the broad
+/// phase is the network's guarded search caps, with LBVH fallback rows.
+std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, hipStream_t s,
+                                                           const mlp::Model *model = nullptr, ProposerStats *pst = nullptr);
 std::unique_ptr<InterpolationMatrix> generate_bilinear_rect(const Mesh &src, const Mesh &dst, const GridAxes &src_axes, hipStream_t s);
 void check_longitude_ranges(const Mesh &src, const Mesh &dst, bool structured_fast_path, hipStream_t s);
 

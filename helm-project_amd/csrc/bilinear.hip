@@ -130,6 +130,13 @@ std::unique_ptr<InterpolationMatrix> generate_bilinear_rect(const Mesh &src, con
 // check, longitude-disjoint check, method dispatch.  The coastal post-pass
 // (:1736-1743) returns its input when the source has no mask.
 std::unique_ptr<InterpolationMatrix> WeightGenerator::generate(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, hipStream_t s) {
+    return generate_proposed(src, dst, cfg, nullptr, nullptr, s);
+}
+
+std::unique_ptr<InterpolationMatrix> WeightGenerator::generate_proposed(const Mesh &src, const Mesh &dst, const RegridConfig &cfg,
+                                                                        const mlp::Model *proposer, ProposerStats *stats, hipStream_t s) {
+    if (proposer && cfg.method != InterpolationMethod::Conservative1stOrder)
+        throw InvalidArgument("WeightGenerator::generate: the learned proposer serves Conservative1stOrder only");
     if (src.coord_system() != dst.coord_system())
         throw InvalidArgument("WeightGenerator::generate: source and destination meshes have different CoordinateSystem values (src=" +
                               std::to_string(static_cast<int>(src.coord_system())) + ", dst=" + std::to_string(static_cast<int>(dst.coord_system())) +
@@ -140,7 +147,7 @@ std::unique_ptr<InterpolationMatrix> WeightGenerator::generate(const Mesh &src, 
     switch (cfg.method) {
         case InterpolationMethod::Conservative1stOrder: {
             detail::check_longitude_ranges(src, dst, false, s);
-            return detail::generate_conservative(src, dst, cfg, s);
+            return detail::generate_conservative(src, dst, cfg, s, proposer, stats);
         }
         case InterpolationMethod::Bilinear: {
             detail::GridAxes sax = detail::unique_axes(src, s);

@@ -19,6 +19,7 @@
 #include "../../include/axis_hip.h"
 #include "axis_hip.hpp"
 #include "mlp.hpp"
+#include "proposer.hpp"
 
 namespace {
 
@@ -132,6 +133,20 @@ void check_device() {
     }
 }
 
+RegridConfig config_of(const axis_regrid_config *config) {
+    RegridConfig cfg;
+    if (config->method < 0 || config->method > 5 || config->norm_type < 0 || config->norm_type > 1 || config->line_type < 0 ||
+        config->line_type > 1 || config->unmapped < 0 || config->unmapped > 1 || config->extrap_method < 0 || config->extrap_method > 1)
+        throw InvalidArgument("RegridConfig field out of range");
+    cfg.method = static_cast<InterpolationMethod>(config->method);
+    cfg.norm_type = static_cast<NormType>(config->norm_type);
+    cfg.line_type = static_cast<LineType>(config->line_type);
+    cfg.unmapped = static_cast<UnmappedAction>(config->unmapped);
+    cfg.use_limiter = config->use_limiter != 0;
+    cfg.extrap_method = static_cast<ExtrapolationAction>(config->extrap_method);
+    return cfg;
+}
+
 }  // namespace
 
 extern "C" {
@@ -199,18 +214,45 @@ int axis_generate_weights_config_c(int src_handle, int dst_handle, const axis_re
         if (!config || !matrix_handle) throw InvalidArgument("null argument");
         auto src = Registry::get().find<Mesh>(src_handle);
         auto dst = Registry::get().find<Mesh>(dst_handle);
-        RegridConfig cfg;
-        if (config->method < 0 || config->method > 5 || config->norm_type < 0 || config->norm_type > 1 || config->line_type < 0 ||
-            config->line_type > 1 || config->unmapped < 0 || config->unmapped > 1 || config->extrap_method < 0 || config->extrap_method > 1)
-            throw InvalidArgument("RegridConfig field out of range");
-        cfg.method = static_cast<InterpolationMethod>(config->method);
-        cfg.norm_type = static_cast<NormType>(config->norm_type);
-        cfg.line_type = static_cast<LineType>(config->line_type);
-        cfg.unmapped = static_cast<UnmappedAction>(config->unmapped);
-        cfg.use_limiter = config->use_limiter != 0;
-        cfg.extrap_method = static_cast<ExtrapolationAction>(config->extrap_method);
+        const RegridConfig cfg = config_of(config);
         std::shared_ptr<InterpolationMatrix> M = WeightGenerator::generate(*src, *dst, cfg, nullptr);
         *matrix_handle = Registry::get().add(std::move(M));
+    });
+}
+
+int axis_generate_weights_proposed_c(int src_handle, int dst_handle, const axis_regrid_config *config, int mlp_handle, int *matrix_handle,
+                                     int64_t *stats5) {
+    return guarded([&] {
+        if (!matrix_handle) throw InvalidArgument("null argument");
+        auto src = Registry::get().find<Mesh>(src_handle);
+        auto dst = Registry::get().find<Mesh>(dst_handle);
+        auto net = Registry::get().find<mlp::Model>(mlp_handle);
+        RegridConfig cfg;
+        cfg.method = InterpolationMethod::Conservative1stOrder;
+        if (config) cfg = config_of(config);
+        ProposerStats st;
+        std::shared_ptr<InterpolationMatrix> M = WeightGenerator::generate_proposed(*src, *dst, cfg, net.get(), &st, nullptr);
+        if (stats5) {
+            stats5[0] = st.rows;
+            stats5[1] = st.guard_rejected;
+            stats5[2] = st.area_failed;
+            stats5[3] = st.pairs_proposed;
+            stats5[4] = st.pairs_fallback;
+        }
+        *matrix_handle = Registry::get().add(std::move(M));
+    });
+}
+
+int axis_proposer_training_data_c(int src_handle, int dst_handle, float feature_scale, float *x, float *t, int64_t *n_unlabelled) {
+    return guarded([&] {
+        auto src = Registry::get().find<Mesh>(src_handle);
+        auto dst = Registry::get().find<Mesh>(dst_handle);
+        if (!(feature_scale > 0.0f)) throw InvalidArgument("feature_scale must be positive");
+        if (dst->n_cells() > 0 && (!is_device_ptr(x) || !is_device_ptr(t)))
+            throw InvalidArgument("axis_proposer_training_data_c: x and t must be device memory");
+        index_t bad = 0;
+        detail::proposer_training_data(*src, *dst, feature_scale, x, t, nullptr, &bad);
+        if (n_unlabelled) *n_unlabelled = bad;
     });
 }
 

@@ -33,6 +33,7 @@
 #include "axis_hip.hpp"
 #include "geometry.hpp"
 #include "host_util.hpp"
+#include "proposer.hpp"
 
 namespace axis_hip::detail {
 
@@ -87,12 +88,12 @@ __device__ inline int cell_nverts(const Side &m, index_t c) {
 }
 
 template <int CAP>
-__global__ void __launch_bounds__(kWave) overlap_kernel(index_t n_pairs, const std::int32_t *pair_row, const std::int32_t *pair_src, Side src,
-                                                        Side dst, double *overlap, std::int32_t *arb, int *n_arb, int *n_tie) {
+__global__ void __launch_bounds__(kWave) overlap_kernel(index_t p0, index_t p1, const std::int32_t *pair_row, const std::int32_t *pair_src,
+                                                        Side src, Side dst, double *overlap, std::int32_t *arb, int *n_arb, int *n_tie) {
     __shared__ double buf[2][3][CAP][kWave];
     const int lane = threadIdx.x;
-    const index_t p = static_cast<index_t>(blockIdx.x) * kWave + lane;
-    if (p >= n_pairs) return;
+    const index_t p = p0 + static_cast<index_t>(blockIdx.x) * kWave + lane;
+    if (p >= p1) return;
     const index_t j = pair_row[p];
     const index_t i = pair_src[p];
     double ov = 0.0;
@@ -135,6 +136,53 @@ __global__ void __launch_bounds__(kWave) overlap_kernel(index_t n_pairs, const s
         int k = atomicAdd(n_arb, 1);
         arb[k] = static_cast<std::int32_t>(p);
     }
+}
+
+// Rows the proposer's round must hand to the LBVH: rejected by the guard, or
+// accepted but with overlaps (candidate order) not summing to the cell area
+// within 1e-12 relative.  Cells excluded from weights never need a redo.
+__global__ void redo_rows_kernel(const index_t *cand_off, index_t n_dst, const double *overlap, const double *area, const std::int32_t *flags,
+                                 const std::int32_t *accepted, index_t *redo, unsigned long long *n_guard, unsigned long long *n_area) {
+    index_t j = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n_dst) return;
+    index_t r = 0;
+    if ((flags[j] & 7) == 0) {
+        if (!accepted[j]) {
+            r = 1;
+            atomicAdd(n_guard, 1ull);
+        } else {
+            double sum = 0.0;
+            for (index_t e = cand_off[j]; e < cand_off[j + 1]; ++e) sum += overlap[e];
+            if (!(fabs(sum - area[j]) <= 1e-12 * area[j])) {
+                r = 1;
+                atomicAdd(n_area, 1ull);
+            }
+        }
+    }
+    redo[j] = r;
+}
+
+__global__ void redo_list_kernel(const index_t *redo, const index_t *pos, index_t n_dst, const float *box, std::int32_t *rows, float *rbox) {
+    index_t j = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n_dst || !redo[j]) return;
+    const index_t q = pos[j];
+    rows[q] = static_cast<std::int32_t>(j);
+    for (int a = 0; a < 6; ++a) rbox[6 * q + a] = box[6 * j + a];
+}
+
+// round-1 pairs of redone rows drop out (overlap 0 -> no entry)
+__global__ void zero_rows_kernel(const std::int32_t *rows, index_t n_rows, const index_t *cand_off, double *overlap) {
+    index_t q = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= n_rows) return;
+    const index_t j = rows[q];
+    for (index_t e = cand_off[j]; e < cand_off[j + 1]; ++e) overlap[e] = 0.0;
+}
+
+// round-2 pairs: dst row of each LBVH candidate of redone row q
+__global__ void fill_rows_mapped_kernel(const index_t *off, index_t n_rows, const std::int32_t *rows, std::int32_t *pair_row) {
+    index_t q = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= n_rows) return;
+    for (index_t e = off[q]; e < off[q + 1]; ++e) pair_row[e] = rows[q];
 }
 
 __global__ void fill_rows_kernel(const index_t *cand_off, index_t n_dst, std::int32_t *pair_row) {
@@ -393,7 +441,8 @@ void check_longitude_ranges(const Mesh &src, const Mesh &dst, bool structured_fa
                               "] are disjoint but would overlap after a shift of " + std::to_string(shift));
 }
 
-std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, hipStream_t s) {
+std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, hipStream_t s,
+                                                           const mlp::Model *model, ProposerStats *pst) {
     const auto t0 = std::chrono::steady_clock::now();
     PhaseTrace tr("conservative", s);
     const index_t n_src = src.n_cells(), n_dst = dst.n_cells();
@@ -415,9 +464,12 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     tr.mark("geometry_dst");
 
     DeviceBuffer<index_t> cand_off;
-    DeviceBuffer<std::int32_t> cand_src;
-    lbvh_candidates(sg.d_box.get(), n_src, dg.d_box.get(), n_dst, cand_off, cand_src, s);
-    tr.mark("lbvh");
+    DeviceBuffer<std::int32_t> cand_src, accepted;
+    if (model)
+        proposer_candidates(sg, dg, *model, cand_off, cand_src, accepted, s);
+    else
+        lbvh_candidates(sg.d_box.get(), n_src, dg.d_box.get(), n_dst, cand_off, cand_src, s);
+    tr.mark(model ? "propose" : "lbvh");
     index_t n_pairs = 0;
     AXIS_HIP_CHECK(hipMemcpyAsync(&n_pairs, cand_off.get() + n_dst, sizeof(index_t), hipMemcpyDeviceToHost, s));
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
@@ -425,7 +477,6 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     auto M = std::make_unique<InterpolationMatrix>();
     M->n_src = n_src;
     M->n_dst = n_dst;
-    M->n_candidates = n_pairs;
     cell_morton_keys(dst, M->row_key, s);
 
     DeviceBuffer<std::int32_t> pair_row(std::max<index_t>(n_pairs, 1));
@@ -452,32 +503,36 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
         return x;
     };
     const Side S = side(src, sg), D = side(dst, dg);
-    if (n_pairs > 0) {
-        const int nv_s = sg.regular ? 4 : std::min(src.max_cell_vertices(), 32);
-        const int nv_d = dg.regular ? 4 : std::min(dst.max_cell_vertices(), 32);
-        const int need = nv_s + nv_d;  // bound for convex cells; larger results go to the host
-        const unsigned grid = static_cast<unsigned>((n_pairs + kWave - 1) / kWave);
+    const int nv_s = sg.regular ? 4 : std::min(src.max_cell_vertices(), 32);
+    const int nv_d = dg.regular ? 4 : std::min(dst.max_cell_vertices(), 32);
+    const int need = nv_s + nv_d;  // bound for convex cells; larger results go to the host
+    // overlaps of pairs [p0, p1); pairs the device cannot decide are queued in arb
+    auto run_overlaps = [&](index_t p0, index_t p1) {
+        if (p1 <= p0) return;
+        const unsigned grid = static_cast<unsigned>((p1 - p0 + kWave - 1) / kWave);
         if (need <= 8)
-            overlap_kernel<8><<<grid, kWave, 0, s>>>(n_pairs, pair_row.get(), cand_src.get(), S, D, overlap.get(), arb.get(), counters.get(),
+            overlap_kernel<8><<<grid, kWave, 0, s>>>(p0, p1, pair_row.get(), cand_src.get(), S, D, overlap.get(), arb.get(), counters.get(),
                                                      counters.get() + 1);
         else if (need <= 16)
-            overlap_kernel<16><<<grid, kWave, 0, s>>>(n_pairs, pair_row.get(), cand_src.get(), S, D, overlap.get(), arb.get(), counters.get(),
+            overlap_kernel<16><<<grid, kWave, 0, s>>>(p0, p1, pair_row.get(), cand_src.get(), S, D, overlap.get(), arb.get(), counters.get(),
                                                       counters.get() + 1);
         else
-            overlap_kernel<24><<<grid, kWave, 0, s>>>(n_pairs, pair_row.get(), cand_src.get(), S, D, overlap.get(), arb.get(), counters.get(),
+            overlap_kernel<24><<<grid, kWave, 0, s>>>(p0, p1, pair_row.get(), cand_src.get(), S, D, overlap.get(), arb.get(), counters.get(),
                                                       counters.get() + 1);
         AXIS_LAUNCH_CHECK();
-    }
-    tr.mark("overlap");
-    int h_cnt[2] = {0, 0};
-    AXIS_HIP_CHECK(hipMemcpyAsync(h_cnt, counters.get(), sizeof h_cnt, hipMemcpyDeviceToHost, s));
-    AXIS_HIP_CHECK(hipStreamSynchronize(s));
-    M->n_host_arbitrated = h_cnt[0];
-    M->n_cap_borderline = h_cnt[1];
-    if (h_cnt[0] > 0) {
-        const int na = h_cnt[0];
+    };
+    // host libm decides the queued pairs arb[a0 .. current count)
+    int arb_done = 0;
+    auto arbitrate = [&]() {
+        int h_cnt[2] = {0, 0};
+        AXIS_HIP_CHECK(hipMemcpyAsync(h_cnt, counters.get(), sizeof h_cnt, hipMemcpyDeviceToHost, s));
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+        M->n_host_arbitrated = h_cnt[0];
+        M->n_cap_borderline = h_cnt[1];
+        const int na = h_cnt[0] - arb_done;
+        if (na <= 0) return;
         DeviceBuffer<std::int32_t> d_trip(3 * static_cast<std::size_t>(na));
-        gather_arbitrated_kernel<<<grid_for(na, 256), 256, 0, s>>>(arb.get(), na, pair_row.get(), cand_src.get(), d_trip.get());
+        gather_arbitrated_kernel<<<grid_for(na, 256), 256, 0, s>>>(arb.get() + arb_done, na, pair_row.get(), cand_src.get(), d_trip.get());
         AXIS_LAUNCH_CHECK();
         std::vector<std::int32_t> trip(3 * static_cast<std::size_t>(na)), pairs(na);
         d_trip.download(trip.data(), 3 * static_cast<std::size_t>(na), s);
@@ -499,7 +554,79 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
         scatter_overlaps_kernel<<<grid_for(na, 256), 256, 0, s>>>(d_idx.get(), d_vals.get(), na, overlap.get());
         AXIS_LAUNCH_CHECK();
         AXIS_HIP_CHECK(hipStreamSynchronize(s));
+        arb_done = h_cnt[0];
+    };
+    run_overlaps(0, n_pairs);
+    tr.mark("overlap");
+    arbitrate();
+    index_t n_cand = n_pairs;
+    if (model) {
+        // round 2: the LBVH redoes the rows whose proposal was rejected or
+        // whose overlaps do not account for the cell's area
+        const index_t nd1 = std::max<index_t>(n_dst, 1);
+        DeviceBuffer<index_t> redo(nd1), pos(nd1 + 1);
+        DeviceBuffer<unsigned long long> nfail(2);
+        AXIS_HIP_CHECK(hipMemsetAsync(nfail.get(), 0, 2 * sizeof(unsigned long long), s));
+        if (n_dst > 0) {
+            redo_rows_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(cand_off.get(), n_dst, overlap.get(), dg.d_area.get(), dg.d_flags.get(),
+                                                                 accepted.get(), redo.get(), nfail.get(), nfail.get() + 1);
+            AXIS_LAUNCH_CHECK();
+        }
+        exclusive_scan_i64(redo.get(), pos.get(), n_dst, s);
+        index_t n_redo = 0;
+        unsigned long long h_fail[2] = {0, 0};
+        AXIS_HIP_CHECK(hipMemcpyAsync(&n_redo, pos.get() + n_dst, sizeof(index_t), hipMemcpyDeviceToHost, s));
+        AXIS_HIP_CHECK(hipMemcpyAsync(h_fail, nfail.get(), sizeof h_fail, hipMemcpyDeviceToHost, s));
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+        if (pst) {
+            pst->rows = n_dst;
+            pst->guard_rejected = static_cast<index_t>(h_fail[0]);
+            pst->area_failed = static_cast<index_t>(h_fail[1]);
+            pst->pairs_proposed = n_pairs;
+            pst->pairs_fallback = 0;
+        }
+        if (n_redo > 0) {
+            DeviceBuffer<std::int32_t> rows(n_redo);
+            DeviceBuffer<float> rbox(6 * n_redo);
+            redo_list_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(redo.get(), pos.get(), n_dst, dg.d_box.get(), rows.get(), rbox.get());
+            AXIS_LAUNCH_CHECK();
+            zero_rows_kernel<<<grid_for(n_redo, 256), 256, 0, s>>>(rows.get(), n_redo, cand_off.get(), overlap.get());
+            AXIS_LAUNCH_CHECK();
+            DeviceBuffer<index_t> off2;
+            DeviceBuffer<std::int32_t> src2;
+            lbvh_candidates(sg.d_box.get(), n_src, rbox.get(), n_redo, off2, src2, s);
+            index_t n2 = 0;
+            AXIS_HIP_CHECK(hipMemcpyAsync(&n2, off2.get() + n_redo, sizeof(index_t), hipMemcpyDeviceToHost, s));
+            AXIS_HIP_CHECK(hipStreamSynchronize(s));
+            if (n_pairs + n2 >= (index_t(1) << 31)) throw RuntimeError("generate_conservative: candidate pair count exceeds 2^31");
+            const index_t nt = n_pairs + n2;
+            DeviceBuffer<std::int32_t> pr(std::max<index_t>(nt, 1)), cs(std::max<index_t>(nt, 1)), ab(std::max<index_t>(nt, 1));
+            DeviceBuffer<double> ovb(std::max<index_t>(nt, 1));
+            if (n_pairs > 0) {
+                AXIS_HIP_CHECK(hipMemcpyAsync(pr.get(), pair_row.get(), sizeof(std::int32_t) * n_pairs, hipMemcpyDeviceToDevice, s));
+                AXIS_HIP_CHECK(hipMemcpyAsync(cs.get(), cand_src.get(), sizeof(std::int32_t) * n_pairs, hipMemcpyDeviceToDevice, s));
+                AXIS_HIP_CHECK(hipMemcpyAsync(ovb.get(), overlap.get(), sizeof(double) * n_pairs, hipMemcpyDeviceToDevice, s));
+            }
+            if (arb_done > 0) AXIS_HIP_CHECK(hipMemcpyAsync(ab.get(), arb.get(), sizeof(std::int32_t) * arb_done, hipMemcpyDeviceToDevice, s));
+            if (n2 > 0) {
+                AXIS_HIP_CHECK(hipMemcpyAsync(cs.get() + n_pairs, src2.get(), sizeof(std::int32_t) * n2, hipMemcpyDeviceToDevice, s));
+                fill_rows_mapped_kernel<<<grid_for(n_redo, 256), 256, 0, s>>>(off2.get(), n_redo, rows.get(), pr.get() + n_pairs);
+                AXIS_LAUNCH_CHECK();
+            }
+            AXIS_HIP_CHECK(hipStreamSynchronize(s));
+            pair_row = std::move(pr);
+            cand_src = std::move(cs);
+            overlap = std::move(ovb);
+            arb = std::move(ab);
+            run_overlaps(n_pairs, nt);
+            arbitrate();
+            n_pairs = nt;
+            n_cand = nt;
+            if (pst) pst->pairs_fallback = n2;
+        }
+        tr.mark("fallback");
     }
+    M->n_candidates = n_cand;
 
     tr.mark("arbitration");
     // ---- CSR assembly
