@@ -92,7 +92,8 @@ def lib():
         "axis_mlp_train_c": [C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_float, C.c_float, C.c_float,
                              C.POINTER(C.c_float), C.c_void_p],
         "axis_destroy_mlp_c": [C.c_int],
-        "axis_generate_weights_proposed_c": [C.c_int, C.c_int, C.POINTER(RegridConfig), C.c_int, C.POINTER(C.c_int), _I64],
+        "axis_generate_weights_proposed_c": [C.c_int, C.c_int, C.POINTER(RegridConfig), C.c_int, C.c_double,
+                                             C.POINTER(C.c_int), _I64],
         "axis_proposer_training_data_c": [C.c_int, C.c_int, C.c_float, C.c_void_p, C.c_void_p, _I64],
     }
     for name, args in sig.items():
@@ -419,13 +420,13 @@ class MLP:
             pass
 
 
-def generate_weights_proposed(src: "Mesh", dst: "Mesh", net: MLP, config: "RegridConfig" = None):
+def generate_weights_proposed(src: "Mesh", dst: "Mesh", net: MLP, config: "RegridConfig" = None, area_tol: float = 1e-12):
     """Conservative weights whose broad phase uses the proposer network
     (axis_generate_weights_proposed_c); returns (Matrix, stats dict)."""
     h = C.c_int(0)
     st = np.zeros(5, dtype=np.int64)
     cfg = C.byref(config) if config is not None else None
-    _check(lib().axis_generate_weights_proposed_c(src.handle, dst.handle, cfg, net.handle, C.byref(h),
+    _check(lib().axis_generate_weights_proposed_c(src.handle, dst.handle, cfg, net.handle, C.c_double(area_tol), C.byref(h),
                                                   st.ctypes.data_as(C.POINTER(C.c_int64))))
     keys = ("cells", "guard_rejected", "area_failed", "pairs_proposed", "pairs_fallback")
     return Matrix(h.value), dict(zip(keys, (int(v) for v in st)))

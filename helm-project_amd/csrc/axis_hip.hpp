@@ -141,7 +141,8 @@ class WeightGenerator {
     /// network's guarded search caps (LBVH for the cells they cannot serve);
     /// the weights equal generate()'s.
     static std::unique_ptr<InterpolationMatrix> generate_proposed(const Mesh &src, const Mesh &dst, const RegridConfig &cfg,
-                                                                  const mlp::Model *proposer, ProposerStats *stats, hipStream_t s = nullptr);
+                                                                  const mlp::Model *proposer, ProposerStats *stats, hipStream_t s = nullptr,
+                                                                  double area_tol = 1e-12);
 };
 
 /// Builds a matrix from COO triplets (the reference's InterpolationMatrix
@@ -244,7 +245,8 @@ void lbvh_candidates(const float *src_boxes, index_t n_src, const float *dst_box
 /// First-order conservative weights; with a proposer network the broad
 /// phase is the network's guarded search caps, with LBVH fallback rows.
 std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, hipStream_t s,
-                                                           const mlp::Model *model = nullptr, ProposerStats *pst = nullptr);
+                                                           const mlp::Model *model = nullptr, ProposerStats *pst = nullptr,
+                                                           double area_tol = 1e-12);
 std::unique_ptr<InterpolationMatrix> generate_bilinear_rect(const Mesh &src, const Mesh &dst, const GridAxes &src_axes, hipStream_t s);
 void check_longitude_ranges(const Mesh &src, const Mesh &dst, bool structured_fast_path, hipStream_t s);
 

@@ -134,7 +134,8 @@ std::unique_ptr<InterpolationMatrix> WeightGenerator::generate(const Mesh &src, 
 }
 
 std::unique_ptr<InterpolationMatrix> WeightGenerator::generate_proposed(const Mesh &src, const Mesh &dst, const RegridConfig &cfg,
-                                                                        const mlp::Model *proposer, ProposerStats *stats, hipStream_t s) {
+                                                                        const mlp::Model *proposer, ProposerStats *stats, hipStream_t s,
+                                                                        double area_tol) {
     if (proposer && cfg.method != InterpolationMethod::Conservative1stOrder)
         throw InvalidArgument("WeightGenerator::generate: the learned proposer serves Conservative1stOrder only");
     if (src.coord_system() != dst.coord_system())
@@ -147,7 +148,7 @@ std::unique_ptr<InterpolationMatrix> WeightGenerator::generate_proposed(const Me
     switch (cfg.method) {
         case InterpolationMethod::Conservative1stOrder: {
             detail::check_longitude_ranges(src, dst, false, s);
-            return detail::generate_conservative(src, dst, cfg, s, proposer, stats);
+            return detail::generate_conservative(src, dst, cfg, s, proposer, stats, area_tol);
         }
         case InterpolationMethod::Bilinear: {
             detail::GridAxes sax = detail::unique_axes(src, s);

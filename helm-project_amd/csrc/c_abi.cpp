@@ -220,8 +220,8 @@ int axis_generate_weights_config_c(int src_handle, int dst_handle, const axis_re
     });
 }
 
-int axis_generate_weights_proposed_c(int src_handle, int dst_handle, const axis_regrid_config *config, int mlp_handle, int *matrix_handle,
-                                     int64_t *stats5) {
+int axis_generate_weights_proposed_c(int src_handle, int dst_handle, const axis_regrid_config *config, int mlp_handle, double area_tol,
+                                     int *matrix_handle, int64_t *stats5) {
     return guarded([&] {
         if (!matrix_handle) throw InvalidArgument("null argument");
         auto src = Registry::get().find<Mesh>(src_handle);
@@ -231,7 +231,7 @@ int axis_generate_weights_proposed_c(int src_handle, int dst_handle, const axis_
         cfg.method = InterpolationMethod::Conservative1stOrder;
         if (config) cfg = config_of(config);
         ProposerStats st;
-        std::shared_ptr<InterpolationMatrix> M = WeightGenerator::generate_proposed(*src, *dst, cfg, net.get(), &st, nullptr);
+        std::shared_ptr<InterpolationMatrix> M = WeightGenerator::generate_proposed(*src, *dst, cfg, net.get(), &st, nullptr, area_tol);
         if (stats5) {
             stats5[0] = st.rows;
             stats5[1] = st.guard_rejected;

@@ -140,9 +140,9 @@ __global__ void __launch_bounds__(kWave) overlap_kernel(index_t p0, index_t p1, 
 
 // Rows the proposer's round must hand to the LBVH: rejected by the guard, or
 // accepted but with overlaps (candidate order) not summing to the cell area
-// within 1e-12 relative.  Cells excluded from weights never need a redo.
+// within `tol` relative (1e-12 unless the caller sets it).  Cells excluded from weights never need a redo.
 __global__ void redo_rows_kernel(const index_t *cand_off, index_t n_dst, const double *overlap, const double *area, const std::int32_t *flags,
-                                 const std::int32_t *accepted, index_t *redo, unsigned long long *n_guard, unsigned long long *n_area) {
+                                 const std::int32_t *accepted, double tol, index_t *redo, unsigned long long *n_guard, unsigned long long *n_area) {
     index_t j = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= n_dst) return;
     index_t r = 0;
@@ -153,7 +153,7 @@ __global__ void redo_rows_kernel(const index_t *cand_off, index_t n_dst, const d
         } else {
             double sum = 0.0;
             for (index_t e = cand_off[j]; e < cand_off[j + 1]; ++e) sum += overlap[e];
-            if (!(fabs(sum - area[j]) <= 1e-12 * area[j])) {
+            if (!(fabs(sum - area[j]) <= tol * area[j])) {
                 r = 1;
                 atomicAdd(n_area, 1ull);
             }
@@ -442,7 +442,7 @@ void check_longitude_ranges(const Mesh &src, const Mesh &dst, bool structured_fa
 }
 
 std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, hipStream_t s,
-                                                           const mlp::Model *model, ProposerStats *pst) {
+                                                           const mlp::Model *model, ProposerStats *pst, double area_tol) {
     const auto t0 = std::chrono::steady_clock::now();
     PhaseTrace tr("conservative", s);
     const index_t n_src = src.n_cells(), n_dst = dst.n_cells();
@@ -479,9 +479,17 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     M->n_dst = n_dst;
     cell_morton_keys(dst, M->row_key, s);
 
-    DeviceBuffer<std::int32_t> pair_row(std::max<index_t>(n_pairs, 1));
-    DeviceBuffer<double> overlap(std::max<index_t>(n_pairs, 1));
-    DeviceBuffer<std::int32_t> arb(std::max<index_t>(n_pairs, 1));
+    // with a proposer, room for the LBVH pairs of redone rows (~10%) so that
+    // round 2 usually appends in place
+    const index_t cap_pairs = model ? n_pairs + n_pairs / 8 + 4096 : std::max<index_t>(n_pairs, 1);
+    DeviceBuffer<std::int32_t> pair_row(cap_pairs);
+    DeviceBuffer<double> overlap(cap_pairs);
+    DeviceBuffer<std::int32_t> arb(cap_pairs);
+    if (model && cand_src.size() < static_cast<std::size_t>(cap_pairs)) {
+        DeviceBuffer<std::int32_t> cs(cap_pairs);
+        if (n_pairs > 0) AXIS_HIP_CHECK(hipMemcpyAsync(cs.get(), cand_src.get(), sizeof(std::int32_t) * n_pairs, hipMemcpyDeviceToDevice, s));
+        cand_src = std::move(cs);
+    }
     DeviceBuffer<int> counters(2);
     AXIS_HIP_CHECK(hipMemsetAsync(counters.get(), 0, 2 * sizeof(int), s));
     if (n_dst > 0) {
@@ -569,7 +577,8 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
         AXIS_HIP_CHECK(hipMemsetAsync(nfail.get(), 0, 2 * sizeof(unsigned long long), s));
         if (n_dst > 0) {
             redo_rows_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(cand_off.get(), n_dst, overlap.get(), dg.d_area.get(), dg.d_flags.get(),
-                                                                 accepted.get(), redo.get(), nfail.get(), nfail.get() + 1);
+                                                                 accepted.get(), area_tol > 0.0 ? area_tol : 1e-12, redo.get(), nfail.get(),
+                                                                 nfail.get() + 1);
             AXIS_LAUNCH_CHECK();
         }
         exclusive_scan_i64(redo.get(), pos.get(), n_dst, s);
@@ -600,24 +609,26 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
             AXIS_HIP_CHECK(hipStreamSynchronize(s));
             if (n_pairs + n2 >= (index_t(1) << 31)) throw RuntimeError("generate_conservative: candidate pair count exceeds 2^31");
             const index_t nt = n_pairs + n2;
-            DeviceBuffer<std::int32_t> pr(std::max<index_t>(nt, 1)), cs(std::max<index_t>(nt, 1)), ab(std::max<index_t>(nt, 1));
-            DeviceBuffer<double> ovb(std::max<index_t>(nt, 1));
-            if (n_pairs > 0) {
-                AXIS_HIP_CHECK(hipMemcpyAsync(pr.get(), pair_row.get(), sizeof(std::int32_t) * n_pairs, hipMemcpyDeviceToDevice, s));
-                AXIS_HIP_CHECK(hipMemcpyAsync(cs.get(), cand_src.get(), sizeof(std::int32_t) * n_pairs, hipMemcpyDeviceToDevice, s));
-                AXIS_HIP_CHECK(hipMemcpyAsync(ovb.get(), overlap.get(), sizeof(double) * n_pairs, hipMemcpyDeviceToDevice, s));
+            if (nt > cap_pairs) {  // grow the pair arrays, keeping round 1
+                DeviceBuffer<std::int32_t> pr(nt), cs(nt), ab(nt);
+                DeviceBuffer<double> ovb(nt);
+                if (n_pairs > 0) {
+                    AXIS_HIP_CHECK(hipMemcpyAsync(pr.get(), pair_row.get(), sizeof(std::int32_t) * n_pairs, hipMemcpyDeviceToDevice, s));
+                    AXIS_HIP_CHECK(hipMemcpyAsync(cs.get(), cand_src.get(), sizeof(std::int32_t) * n_pairs, hipMemcpyDeviceToDevice, s));
+                    AXIS_HIP_CHECK(hipMemcpyAsync(ovb.get(), overlap.get(), sizeof(double) * n_pairs, hipMemcpyDeviceToDevice, s));
+                }
+                if (arb_done > 0) AXIS_HIP_CHECK(hipMemcpyAsync(ab.get(), arb.get(), sizeof(std::int32_t) * arb_done, hipMemcpyDeviceToDevice, s));
+                AXIS_HIP_CHECK(hipStreamSynchronize(s));
+                pair_row = std::move(pr);
+                cand_src = std::move(cs);
+                overlap = std::move(ovb);
+                arb = std::move(ab);
             }
-            if (arb_done > 0) AXIS_HIP_CHECK(hipMemcpyAsync(ab.get(), arb.get(), sizeof(std::int32_t) * arb_done, hipMemcpyDeviceToDevice, s));
             if (n2 > 0) {
-                AXIS_HIP_CHECK(hipMemcpyAsync(cs.get() + n_pairs, src2.get(), sizeof(std::int32_t) * n2, hipMemcpyDeviceToDevice, s));
-                fill_rows_mapped_kernel<<<grid_for(n_redo, 256), 256, 0, s>>>(off2.get(), n_redo, rows.get(), pr.get() + n_pairs);
+                AXIS_HIP_CHECK(hipMemcpyAsync(cand_src.get() + n_pairs, src2.get(), sizeof(std::int32_t) * n2, hipMemcpyDeviceToDevice, s));
+                fill_rows_mapped_kernel<<<grid_for(n_redo, 256), 256, 0, s>>>(off2.get(), n_redo, rows.get(), pair_row.get() + n_pairs);
                 AXIS_LAUNCH_CHECK();
             }
-            AXIS_HIP_CHECK(hipStreamSynchronize(s));
-            pair_row = std::move(pr);
-            cand_src = std::move(cs);
-            overlap = std::move(ovb);
-            arb = std::move(ab);
             run_overlaps(n_pairs, nt);
             arbitrate();
             n_pairs = nt;

@@ -280,8 +280,10 @@ void build_grid(const MeshGeom &sg, BucketGrid &B, hipStream_t s) {
 void proposer_candidates(const MeshGeom &sg, const MeshGeom &dg, const mlp::Model &model, DeviceBuffer<index_t> &cand_off,
                          DeviceBuffer<std::int32_t> &cand_src, DeviceBuffer<std::int32_t> &accepted, hipStream_t s) {
     const index_t n = dg.n_cells;
+    PhaseTrace tr("propose", s);
     BucketGrid B;
     build_grid(sg, B, s);
+    tr.mark("bucket_grid");
     const float scale = model.feature_scale();
     DeviceBuffer<float> x(6 * std::max<index_t>(n, 1)), out(4 * std::max<index_t>(n, 1));
     accepted.resize(std::max<index_t>(n, 1));
@@ -291,6 +293,7 @@ void proposer_candidates(const MeshGeom &sg, const MeshGeom &dg, const mlp::Mode
         feature_kernel<<<grid_for(n, 256), 256, 0, s>>>(dg.d_cap.get(), dg.d_box.get(), n, 1.0f / scale, x.get());
         AXIS_LAUNCH_CHECK();
         model.forward(x.get(), n, out.get(), s);
+        tr.mark("features_mlp");
         propose_kernel<false><<<grid_for(n, 256), 256, 0, s>>>(dg.d_cap.get(), dg.d_flags.get(), dg.d_box.get(), out.get(), n, scale,
                                                               sg.d_cap.get(), sg.d_box.get(), B.g, accepted.get(), count.get(), nullptr, nullptr);
         AXIS_LAUNCH_CHECK();
@@ -299,6 +302,7 @@ void proposer_candidates(const MeshGeom &sg, const MeshGeom &dg, const mlp::Mode
     index_t total = 0;
     AXIS_HIP_CHECK(hipMemcpyAsync(&total, cand_off.get() + n, sizeof(index_t), hipMemcpyDeviceToHost, s));
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    tr.mark("guard_count");
     if (total >= (index_t(1) << 31)) throw RuntimeError("proposer: candidate pair count exceeds 2^31");
     cand_src.resize(std::max<index_t>(total, 1));
     if (n > 0 && total > 0) {
@@ -307,6 +311,7 @@ void proposer_candidates(const MeshGeom &sg, const MeshGeom &dg, const mlp::Mode
         AXIS_LAUNCH_CHECK();
     }
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    tr.mark("fill");
 }
 
 void proposer_training_data(const Mesh &src, const Mesh &dst, float feature_scale, float *x, float *t, hipStream_t s, index_t *n_unlabelled) {

@@ -186,12 +186,12 @@ int axis_destroy_mlp_c(int handle);
  * proposer network's search caps (no reference counterpart).  A proposal is
  * used only when it provably contains every source cell the plain method
  * could clip against the cell and the clipped overlaps sum to the cell's area
- * within 1e-12; other cells go through the LBVH, so the matrix equals
- * axis_generate_weights_config_c's.  stats5 (may be NULL) = {cells, guard
- * rejections, area-check failures, proposed candidate pairs, LBVH candidate
- * pairs of the redone cells}. */
-int axis_generate_weights_proposed_c(int src_handle, int dst_handle, const axis_regrid_config *config, int mlp_handle, int *matrix_handle,
-                                     int64_t *stats5);
+ * within area_tol relative (<= 0: 1e-12); other cells go through the LBVH,
+ * so the matrix equals axis_generate_weights_config_c's.  stats5 (may be
+ * NULL) = {cells, guard rejections, area-check failures, proposed candidate
+ * pairs, LBVH candidate pairs of the redone cells}. */
+int axis_generate_weights_proposed_c(int src_handle, int dst_handle, const axis_regrid_config *config, int mlp_handle, double area_tol,
+                                     int *matrix_handle, int64_t *stats5);
 
 /* Training pairs for the proposer from one (source, destination) mesh pair:
  * x [n_dst][6] features, t [n_dst][4] labels (zero centre offset, log of the
