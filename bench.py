@@ -45,6 +45,159 @@ def profiled_traffic(name):
     return None
 
 
+def _timed(torch, stream, fn, reps=5):
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def bench_config3(axis, grids, fields, torch, stream, m384, f384, n_src384, n_dst384):
+    """Four uneven pairs (1deg->C48 k=1, 0.5deg->C96 k=16, 0.25deg->C192 k=64,
+    0.25deg->C384 k=1016), FP32 fields with per-level scales 10^U(-3,3): one
+    persistent graph launch against one autotuned batch_apply per pair."""
+    dev = torch.device("cuda")
+    specs = [(1.0, 48, 1, 1004), (0.5, 96, 16, 1005), (0.25, 192, 64, 1006)]
+    pairs, mats, tot_bytes, tot_k = [], [], 0, 0
+    for res, n, k, seed in specs:
+        s_m, d_m = grids.latlon_deg(res), grids.cubed_sphere(n)
+        m = axis.generate_weights(axis.Mesh(s_m), axis.Mesh(d_m), axis.CONSERVATIVE)
+        mats.append(m)
+        ns, nd, nnz = m.info()
+        cen = torch.tensor(grids.cell_centroids_xyz(s_m), device=dev)
+        base = fields.harmonic_sum_torch(cen, fields.harmonic_coeffs(seed)).float()
+        sc = torch.tensor(fields.level_scales(seed, k), device=dev).float()
+        X = (sc[:, None] * base[None, :]).contiguous()
+        pairs.append((m, X, torch.empty((k, nd), dtype=torch.float32, device=dev), k))
+        tot_bytes += ns * k * 4 + nd * k * 4 + nnz * 12 + (nd + 1) * 8
+        tot_k += k
+    k = 1016
+    _, _, nnz = m384.info()
+    sc = torch.tensor(fields.level_scales(1003, k), device=dev).float()
+    X = (sc[:, None] * f384.float()[None, :]).contiguous()
+    pairs.append((m384, X, torch.empty((k, n_dst384), dtype=torch.float32, device=dev), k))
+    tot_bytes += n_src384 * k * 4 + n_dst384 * k * 4 + nnz * 12 + (n_dst384 + 1) * 8
+    tot_k += k
+    res = {"pairs": "1deg->C48 k=1, 0.5deg->C96 k=16, 0.25deg->C192 k=64, 0.25deg->C384 k=1016", "fields": tot_k,
+           "algorithmic_bytes": tot_bytes}
+    for name, mode in (("mixed_comp32", axis.MIXED), ("mixed_acc64", axis.MIXED_ACC64)):
+        for m, Xp, Yp, kp in pairs:
+            m.autotune(Xp, Yp, kp, axis.F32, axis.F32, mode, stream=stream.cuda_stream)
+
+        def seq():
+            for m, Xp, Yp, kp in pairs:
+                m.batch_apply(Xp, Yp, kp, axis.F32, axis.F32, mode, stream=stream.cuda_stream, sync=False)
+
+        t_seq = _timed(torch, stream, seq)
+        refs = [Yp.clone() for _, _, Yp, _ in pairs]
+        plan = axis.BatchPlan(pairs, axis.F32, axis.F32, mode, autotune=True, stream=stream.cuda_stream)
+        t_plan = _timed(torch, stream, lambda: plan.run(stream=stream.cuda_stream, sync=False))
+        same = all(torch.equal(Yp, R) for (_, _, Yp, _), R in zip(pairs, refs))
+        res[name] = {"persistent_graph_ms": t_plan, "per_pair_applies_ms": t_seq, "plan": plan.info(),
+                     "GBs": tot_bytes / (t_plan * 1e-3) / 1e9, "frac_of_8TBs": tot_bytes / (t_plan * 1e-3) / 8e12,
+                     "fields_per_s": tot_k / (t_plan * 1e-3), "bitwise_equal_to_per_pair": same}
+        plan.close()
+    for m in mats:
+        m.close()
+    return res
+
+
+def bench_config4(axis, grids, fields, torch, stream):
+    """0.1deg (3600x1800) -> C768 conservative weights; 64 fields (32 fronts
+    tanh(lat/0.02)*cos(lon), 32 degree-32 harmonics) stored as BF16 and as
+    FP32, applied with FP32/compensated accumulation; error per field against
+    the FP64 apply of the same decoded inputs, and global-integral drift."""
+    import math
+    dev = torch.device("cuda")
+    s_m, d_m = grids.latlon_deg(0.1), grids.cubed_sphere(768)
+    S, D = axis.Mesh(s_m), axis.Mesh(d_m)
+    t = time.perf_counter()
+    m = axis.generate_weights(S, D, axis.CONSERVATIVE)
+    torch.cuda.synchronize()
+    t_gen = (time.perf_counter() - t) * 1e3
+    ns, nd, nnz = m.info()
+    cen = torch.tensor(grids.cell_centroids_xyz(s_m), device=dev)
+    lat = torch.asin(cen[:, 2].clamp(-1, 1))
+    lon = torch.atan2(cen[:, 1], cen[:, 0])
+    k = 64
+    X64 = torch.empty((k, ns), dtype=torch.float64, device=dev)
+    for i in range(32):
+        X64[i] = torch.tanh((lat - 0.03 * (i - 16)) / 0.02) * torch.cos(lon + 0.2 * i)
+    X64[32:] = fields.harmonic_sums_torch(cen, [fields.harmonic_coeffs(1007 + i, lmax=32) for i in range(32)])
+    res = {"workload": "0.1deg (3600x1800) -> C768 (6x768x768), 64 fields", "n_src": ns, "n_dst": nd, "nnz": nnz,
+           "conservative_gen_ms": t_gen}
+    Yref = torch.empty((k, nd), dtype=torch.float64, device=dev)
+    for sname, sdt, xt in (("bf16", torch.bfloat16, axis.BF16), ("f32", torch.float32, axis.F32)):
+        Xs = X64.to(sdt).contiguous()
+        m.batch_apply(Xs.double().contiguous(), Yref, k, axis.F64, axis.F64, axis.PLAIN)
+        for oname, odt, yt in (("f32_out", torch.float32, axis.F32), ("f64_out", torch.float64, axis.F64)):
+            for mname, mode in (("comp32", axis.MIXED), ("acc64", axis.MIXED_ACC64)):
+                Y = torch.empty((k, nd), dtype=odt, device=dev)
+                m.autotune(Xs, Y, k, xt, yt, mode, stream=stream.cuda_stream)
+                ms = _timed(torch, stream, lambda: m.batch_apply(Xs, Y, k, xt, yt, mode, stream=stream.cuda_stream, sync=False))
+                m.batch_apply(Xs, Y, k, xt, yt, mode)
+                err = ((Y.double() - Yref).abs().amax(dim=1) / Yref.abs().amax(dim=1)).max().item()
+                # global-integral drift |dst integral - src integral|, relative to the
+                # src integral of |f| (fronts integrate to ~0, which makes the
+                # reference's |d| / max(|src|, |dst|) meaningless for them), over
+                # two fronts and two harmonics
+                drift = 0.0
+                for fi in (0, 16, 32, 48):
+                    xf = Xs[fi].double().contiguous()
+                    rep = m.check_conservation(xf, Y[fi].double().contiguous())
+                    l1 = m.check_conservation(xf.abs(), Y[fi].double().abs().contiguous())["src_integral"]
+                    drift = max(drift, rep["absolute_error"] / l1)
+                b = ns * k * Xs.element_size() + nd * k * Y.element_size() + nnz * 12 + (nd + 1) * 8
+                res[f"{sname}_in_{oname}_{mname}"] = {"ms": ms, "GBs": b / (ms * 1e-3) / 1e9, "frac_of_8TBs": b / (ms * 1e-3) / 8e12,
+                                                      "max_rel_err_vs_fp64": err, "conservation_drift_max": drift}
+        del Xs
+    ms_plain = _timed(torch, stream, lambda: m.batch_apply(X64, Yref, k, axis.F64, axis.F64, axis.PLAIN,
+                                                           stream=stream.cuda_stream, sync=False))
+    res["plain_f64_ms"] = ms_plain
+    m.close()
+    return res
+
+
+def bench_config5(axis, grids, torch):
+    """Learned proposer: spherical Voronoi source (jittered level-8
+    icosahedral generators, 655,362 cells, seed 5020, not in the training set)
+    -> C384; plain LBVH generation against the proposer (trained network in
+    helm-project_amd/data), same matrix required."""
+    from helm_project_amd import proposer
+    src = grids.jittered_voronoi(8, 5020)
+    dst = grids.cubed_sphere(384)
+    net = proposer.load(str(ROOT / "helm-project_amd" / "data" / "proposer_voronoi8_c384.npz"))
+    S, D = axis.Mesh(src), axis.Mesh(dst)
+    res = {"workload": "Voronoi level 8 (655,362 cells, seed 5020) -> C384"}
+    for rep in range(2):
+        t = time.perf_counter()
+        plain = axis.generate_weights(S, D, axis.CONSERVATIVE)
+        torch.cuda.synchronize()
+        t_plain = (time.perf_counter() - t) * 1e3
+        res["plain_ms"] = t_plain
+        res["plain_candidates"] = plain.stats()["candidates"]
+        for tol in (1e-12, 1e-10):
+            t = time.perf_counter()
+            prop, st = axis.generate_weights_proposed(S, D, net, area_tol=tol)
+            torch.cuda.synchronize()
+            t_prop = (time.perf_counter() - t) * 1e3
+            ra, ca, va = plain.csr()
+            rb, cb, vb = prop.csr()
+            same = bool((ra == rb).all() and (ca == cb).all() and (va == vb).all())
+            res[f"proposed_area_tol_{tol:g}"] = dict(st, ms=t_prop, identical_to_plain=same,
+                                                     served_fraction=1 - (st["guard_rejected"] + st["area_failed"]) / st["cells"],
+                                                     candidate_pairs=st["pairs_proposed"] + st["pairs_fallback"])
+            prop.close()
+        plain.close()
+    net.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -52,6 +205,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--apply-fields", type=int, default=64, help="fields of the chosen-vs-plain apply comparison")
+    ap.add_argument("--sections", default="c3,c4,c5",
+                    help="comma list of the extra measurements after the headline: c3 (multi-pair batch), "
+                         "c4 (0.1deg -> C768 mixed apply), c5 (learned proposer); 'none' for the headline only")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (the apply and the weight generation "
@@ -197,6 +353,19 @@ def main():
             cpu["max_weight_rel_err"] = float(np.max(np.abs(val - ref["val"]) / np.maximum(ref["val"], 1e-13)))
             cpu["speedup_conservative_gen"] = t_cpu * 1000.0 / t_cons
 
+    extra = {}
+    sections = set(args.sections.split(",")) - {"none", ""}
+    del X64, X32, Y64, Y32
+    if "c3" in sections:
+        log("config 3: multi-pair persistent batch")
+        extra["config3_batched_apply"] = bench_config3(axis, grids, fields, torch, stream, mc, f, n_src, n_dst)
+    if "c4" in sections:
+        log("config 4: 0.1deg -> C768 mixed apply")
+        extra["config4_mixed_apply"] = bench_config4(axis, grids, fields, torch, stream)
+    if "c5" in sections:
+        log("config 5: learned proposer")
+        extra["config5_learned_proposer"] = bench_config5(axis, grids, torch)
+
     out = {
         "metric": "fields/s & HBM GB/s (apply), cells/s (weight gen): chosen vs plain @<=1e-6 err",
         "value": n_dst / (ms_per_step * 1e-3),
@@ -229,6 +398,7 @@ def main():
                      "frac_of_copy_ceiling": gbs_mixed / copy_gbs},
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out), flush=True)
 
 

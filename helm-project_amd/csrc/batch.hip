@@ -52,7 +52,9 @@ __global__ void clear_counters_kernel(unsigned *ctr) {
 }
 
 template <int E, int FU, int NUM, class XT, class YT>
-__global__ void __launch_bounds__(kTileSeg) persistent_apply_kernel(const PairDesc *pairs, const Item *items, const int *qbeg, unsigned *ctr) {
+// at least 5 waves per SIMD: every workgroup stays resident for the whole
+// launch, so occupancy is fixed at launch time
+__global__ void __launch_bounds__(kTileSeg, 5) persistent_apply_kernel(const PairDesc *pairs, const Item *items, const int *qbeg, unsigned *ctr) {
     __shared__ typename NumTraits<NUM>::V xs[FU][kTileCols];
     __shared__ int s_item;
     const int home = static_cast<int>(blockIdx.x & (kQueues - 1));  // workgroups are dealt to XCDs round-robin

@@ -129,6 +129,40 @@ def harmonic_sum_torch(xyz, coeffs: np.ndarray):
     return out
 
 
+def harmonic_sums_torch(xyz, coeff_list):
+    """[F, n] float64: harmonic_sum_torch for F coefficient sets at once (one
+    Legendre recurrence, a rank-1 update per (l, m) term)."""
+    import torch
+
+    C = np.stack(coeff_list)  # [F, L+1, 2L+1]
+    lmax = C.shape[1] - 1
+    Ct = torch.tensor(C, dtype=torch.float64, device=xyz.device)
+    z = torch.clamp(xyz[:, 2], -1.0, 1.0)
+    phi = torch.atan2(xyz[:, 1], xyz[:, 0])
+    s = torch.sqrt(torch.clamp(1.0 - z * z, min=0.0))
+    out = torch.zeros((C.shape[0], z.shape[0]), dtype=torch.float64, device=xyz.device)
+    pmm = torch.full_like(z, math.sqrt(1.0 / (4.0 * math.pi)))
+    for m in range(lmax + 1):
+        if m > 0:
+            pmm = -math.sqrt((2.0 * m + 1.0) / (2.0 * m)) * s * pmm
+        fac = math.sqrt(2.0) if m > 0 else 1.0
+        cm, sm = fac * torch.cos(m * phi), fac * torch.sin(m * phi)
+        p_lm2, p_lm1 = None, pmm
+        for l in range(m, lmax + 1):
+            if l > m:
+                a = math.sqrt((4.0 * l * l - 1.0) / (l * l - m * m))
+                if p_lm2 is None:
+                    p_l = a * z * p_lm1
+                else:
+                    b = math.sqrt(((l - 1.0) ** 2 - m * m) / (4.0 * (l - 1.0) ** 2 - 1.0))
+                    p_l = a * (z * p_lm1 - b * p_lm2)
+                p_lm2, p_lm1 = p_lm1, p_l
+            out.addr_(Ct[:, l, lmax + m], p_lm1 * cm)
+            if m > 0:
+                out.addr_(Ct[:, l, lmax - m], p_lm1 * sm)
+    return out
+
+
 def level_scales(seed: int, n: int) -> np.ndarray:
     """Per-level scale 10^U(-3, 3)."""
     return 10.0 ** (6.0 * uniform(seed, n) - 3.0)

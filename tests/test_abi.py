@@ -86,3 +86,18 @@ def test_harmonics_orthonormal():
     B = np.array(basis)
     G = (B * w) @ B.T
     assert np.allclose(G, np.eye(B.shape[0]), atol=2e-3)
+
+
+def test_batched_harmonics_equal_single():
+    """fields.harmonic_sums_torch (all coefficient sets in one recurrence)
+    equals harmonic_sum per set."""
+    torch = pytest.importorskip("torch")
+    from helm_project_amd import fields
+
+    rng = np.random.default_rng(4)
+    p = rng.normal(size=(300, 3))
+    p /= np.linalg.norm(p, axis=1)[:, None]
+    cs = [fields.harmonic_coeffs(s, lmax=8) for s in (1, 2, 3)]
+    got = fields.harmonic_sums_torch(torch.tensor(p), cs).numpy()
+    for i, c in enumerate(cs):
+        assert np.allclose(got[i], fields.harmonic_sum(p, c), rtol=1e-12, atol=1e-12)
