@@ -324,3 +324,18 @@ def test_generate_error_paths():
     far["coords"] = np.asfortranarray(dst["coords"] + np.array([720.0, 0.0]))
     with pytest.raises(O.OracleError, match="longitude"):
         O.generate(src, far, 2)
+
+
+def test_golden_vectors_of_reference_geometry():
+    """The oracle against vectors written by the reference's own
+    spherical_geometry.hpp (tests/golden/make_ref_vectors.py): vertex
+    conversion bit for bit, polygon areas within 1e-12."""
+    import pathlib
+
+    z = np.load(pathlib.Path(__file__).parent / "golden" / "ref_geometry_vectors.npz")
+    for (lo, la), want in zip(z["lonlat"], z["xyz"]):
+        assert np.array_equal(O.lonlat_to_xyz(lo, la), want)
+    off = np.concatenate([[0], np.cumsum(z["poly_counts"])])
+    for k, want in enumerate(z["poly_area"]):
+        p = z["poly_xyz"][off[k]:off[k + 1]]
+        assert O.polygon_area(p) == pytest.approx(want, rel=1e-12)
