@@ -133,9 +133,10 @@ constexpr int pow2_at_least(int v) {
     return p;
 }
 
-// Tile t = segments [256 t, 256 t + 256): sorts the tile's column indices in
-// LDS (bitonic), keeps the distinct ones, and records for each nonzero the
-// position of its column among them.
+// Tile t = segments [256 t, 256 t + 256): sorts the aligned 4-column chunks
+// its nonzeros touch in LDS (bitonic), keeps the distinct ones, and records
+// for each nonzero the position of its column in the staged values
+// (4 * chunk position + column % 4).
 template <int E>
 __global__ void __launch_bounds__(apply_detail::kTileSeg) tile_build_kernel(const index_t *seg_begin, const std::int32_t *seg_len, index_t n_seg,
                                                                           const std::int32_t *col32, std::int32_t *tile_nu, std::int32_t *tile_col,
@@ -145,7 +146,7 @@ __global__ void __launch_bounds__(apply_detail::kTileSeg) tile_build_kernel(cons
     constexpr int C = SZ / kB;
     constexpr int kEmpty = 0x7FFFFFFF;
     __shared__ int key[SZ];
-    __shared__ int uniq[apply_detail::kTileCols];
+    __shared__ int uniq[apply_detail::kTileChunks];
     __shared__ int wsum[kB / 64];
     const index_t t = blockIdx.x;
     const int tid = static_cast<int>(threadIdx.x);
@@ -154,7 +155,7 @@ __global__ void __launch_bounds__(apply_detail::kTileSeg) tile_build_kernel(cons
     const index_t b = i < n_seg ? seg_begin[i] : 0;
     for (int q = tid; q < SZ; q += kB) key[q] = kEmpty;
     __syncthreads();
-    for (int e = 0; e < n; ++e) key[tid * E + e] = col32[b + e];
+    for (int e = 0; e < n; ++e) key[tid * E + e] = col32[b + e] / apply_detail::kChunk;
     __syncthreads();
     for (int kk = 2; kk <= SZ; kk <<= 1) {
         for (int j = kk >> 1; j > 0; j >>= 1) {
@@ -179,7 +180,7 @@ __global__ void __launch_bounds__(apply_detail::kTileSeg) tile_build_kernel(cons
     }
     int total = 0;
     int pos = block_exclusive_scan(cnt, wsum, total);
-    if (total > apply_detail::kTileCols) {
+    if (total > apply_detail::kTileChunks) {
         if (tid == 0) tile_nu[t] = -1;
         return;  // whole workgroup
     }
@@ -187,13 +188,14 @@ __global__ void __launch_bounds__(apply_detail::kTileSeg) tile_build_kernel(cons
         const int v = key[q];
         if (v != kEmpty && (q == 0 || key[q - 1] != v)) {
             uniq[pos] = v;
-            tile_col[t * apply_detail::kTileCols + pos] = v;
+            tile_col[t * apply_detail::kTileChunks + pos] = v * apply_detail::kChunk;
             ++pos;
         }
     }
     __syncthreads();
     for (int e = 0; e < n; ++e) {
-        const int c = col32[b + e];
+        const int col = col32[b + e];
+        const int c = col / apply_detail::kChunk;
         int lo = 0, hi = total;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
@@ -202,7 +204,7 @@ __global__ void __launch_bounds__(apply_detail::kTileSeg) tile_build_kernel(cons
             else
                 hi = mid;
         }
-        lcol[b + e] = static_cast<std::uint16_t>(lo);
+        lcol[b + e] = static_cast<std::uint16_t>(lo * apply_detail::kChunk + col % apply_detail::kChunk);
     }
     if (tid == 0) tile_nu[t] = total;
 }
@@ -295,7 +297,7 @@ void InterpolationMatrix::prepare_apply(hipStream_t s) {
     // tiles
     n_tiles = (n_seg + apply_detail::kTileSeg - 1) / apply_detail::kTileSeg;
     tile_nu.resize(static_cast<std::size_t>(std::max<index_t>(n_tiles, 1)));
-    tile_col.resize(static_cast<std::size_t>(std::max<index_t>(n_tiles, 1)) * apply_detail::kTileCols);
+    tile_col.resize(static_cast<std::size_t>(std::max<index_t>(n_tiles, 1)) * apply_detail::kTileChunks);
     t_lcol.resize(static_cast<std::size_t>(n1));
     if (n_tiles > 0) {
         const unsigned g = static_cast<unsigned>(n_tiles);

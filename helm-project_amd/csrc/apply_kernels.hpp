@@ -48,8 +48,9 @@ namespace axis_hip::apply_detail {
 constexpr int kBlock = 256;        // segments per workgroup of the direct-gather kernel
 constexpr int kTileSeg = 256;      // segments per staged tile (= threads of its workgroup; 512 measured slower)
 constexpr int kMaxE = 16;          // largest register segment
-constexpr int kTileCols = 1024;    // most distinct source columns a staged tile holds
-constexpr int kColsPerLane = kTileCols / kTileSeg;
+constexpr int kChunk = 4;          // source columns staged per aligned chunk (one 16-byte FP32 load)
+constexpr int kTileChunks = kTileSeg;             // chunks a staged tile holds: one per lane
+constexpr int kTileCols = kTileChunks * kChunk;   // staged values per field
 
 enum Num : int { kPlain = 0, kComp32 = 1, kAcc64 = 2 };
 
@@ -101,8 +102,10 @@ struct Mat {
     // long rows: partial slots [long_slot[j], long_slot[j + 1]) of row long_row[j]
     const std::int32_t *long_row, *long_slot;
     int n_long, n_slots;
-    // tiles (segments [256 t, 256 t + 256)): distinct source columns and the
-    // position of each nonzero's column among them
+    // tiles (segments [256 t, 256 t + 256)): number of distinct aligned
+    // 4-column chunks their nonzeros touch (-1: too many, gather directly),
+    // the chunks' first columns, and per nonzero the position of its column
+    // in the tile's staged values (4 * chunk + column % 4)
     const std::int32_t *tile_nu, *tile_col;
     const std::uint16_t *lcol;
     index_t n_tiles;
@@ -229,6 +232,42 @@ __global__ void __launch_bounds__(kBlock) spmm_seg_kernel(Mat A, const XT *X, YT
     segment_apply<E, FU, NUM>(A, X, Y, P, i, f0, f1);
 }
 
+/// The 4 source values of an aligned chunk starting at column c0 of one
+/// field: one 16-byte (FP32), 8-byte (BF16) or two 16-byte (FP64) loads when
+/// the field is suitably aligned and the chunk lies inside it, else one load
+/// per column (columns past n_src read as 0).
+template <class V>
+__device__ inline void load_chunk(const float *xf, index_t c0, index_t n_src, bool vec, V out[kChunk]) {
+    if (vec && c0 + kChunk <= n_src) {
+        const float4 v = *reinterpret_cast<const float4 *>(xf + c0);
+        out[0] = v.x, out[1] = v.y, out[2] = v.z, out[3] = v.w;
+    } else {
+#pragma unroll
+        for (int q = 0; q < kChunk; ++q) out[q] = c0 + q < n_src ? xf[c0 + q] : 0.0f;
+    }
+}
+template <class V>
+__device__ inline void load_chunk(const std::uint16_t *xf, index_t c0, index_t n_src, bool vec, V out[kChunk]) {
+    if (vec && c0 + kChunk <= n_src) {
+        const uint2 v = *reinterpret_cast<const uint2 *>(xf + c0);
+        out[0] = __uint_as_float(v.x << 16), out[1] = __uint_as_float(v.x & 0xFFFF0000u);
+        out[2] = __uint_as_float(v.y << 16), out[3] = __uint_as_float(v.y & 0xFFFF0000u);
+    } else {
+#pragma unroll
+        for (int q = 0; q < kChunk; ++q) out[q] = c0 + q < n_src ? ld_f32(xf + c0 + q) : 0.0f;
+    }
+}
+template <class V>
+__device__ inline void load_chunk(const double *xf, index_t c0, index_t n_src, bool vec, V out[kChunk]) {
+    if (vec && c0 + kChunk <= n_src) {
+        const double2 a = *reinterpret_cast<const double2 *>(xf + c0), b = *reinterpret_cast<const double2 *>(xf + c0 + 2);
+        out[0] = a.x, out[1] = a.y, out[2] = b.x, out[3] = b.y;
+    } else {
+#pragma unroll
+        for (int q = 0; q < kChunk; ++q) out[q] = c0 + q < n_src ? xf[c0 + q] : 0.0;
+    }
+}
+
 /// Tile t of segments, fields [f0, f1) (f0 < f1): the tile's distinct source
 /// values of FU fields are loaded once into LDS (xs) by the whole workgroup
 /// (each source value read from HBM once per tile, in sorted column order,
@@ -236,9 +275,17 @@ __global__ void __launch_bounds__(kBlock) spmm_seg_kernel(Mat A, const XT *X, YT
 /// from LDS.  Tiles whose columns do not fit fall back to direct gathers.
 /// Called by every thread of the workgroup; returns after a barrier, so xs
 /// may be reused at once.
+/// LDS staging buffer of one tile workgroup: FU fields of the tile's chunks.
+/// (A second buffer, one barrier per step, measured slower: the extra LDS
+/// lowers occupancy more than the barrier costs.)
+template <int FU, int NUM>
+struct Stage {
+    using V = typename NumTraits<NUM>::V;
+    V xs[FU][kTileCols];
+};
+
 template <int E, int FU, int NUM, class XT, class YT>
-__device__ inline void tile_apply(const Mat &A, const XT *X, YT *Y, double *P, index_t t, index_t f0, index_t f1,
-                                  typename NumTraits<NUM>::V (*xs)[kTileCols]) {
+__device__ inline void tile_apply(const Mat &A, const XT *X, YT *Y, double *P, index_t t, index_t f0, index_t f1, Stage<FU, NUM> &S) {
     using V = typename NumTraits<NUM>::V;
     using W = typename NumTraits<NUM>::W;
     const index_t i = t * kTileSeg + threadIdx.x;
@@ -273,37 +320,36 @@ __device__ inline void tile_apply(const Mat &A, const XT *X, YT *Y, double *P, i
             load_w<NUM>(A, b + e, h[e], l[e]);
         }
     }
-    int uc[kColsPerLane];
-#pragma unroll
-    for (int j = 0; j < kColsPerLane; ++j) {
-        const int u = static_cast<int>(threadIdx.x) + j * kTileSeg;
-        uc[j] = u < nu ? A.tile_col[t * kTileCols + u] : -1;
-    }
+    // this lane's chunk: columns c0 .. c0 + 3 of every field
+    const int lane_chunk = static_cast<int>(threadIdx.x);
+    const index_t c0 = lane_chunk < nu ? static_cast<index_t>(A.tile_col[t * kTileChunks + lane_chunk]) : -1;
+    // vector loads need the field start aligned like the chunk (16 bytes for FP32)
+    constexpr int kAlignCols = 16 / static_cast<int>(sizeof(XT)) < kChunk ? 16 / static_cast<int>(sizeof(XT)) : kChunk;
+    const bool vec = (A.n_src % kAlignCols) == 0 && (reinterpret_cast<std::uintptr_t>(X) % 16) == 0;
     // the source values of the next FU fields are loaded while this step's
     // sums are formed (one step of software pipelining)
-    V v[FU][kColsPerLane];
+    V v[FU][kChunk];
     auto fetch = [&](index_t f) {
 #pragma unroll
         for (int u = 0; u < FU; ++u) {
             const index_t fu = (f + u < f1) ? f + u : f1 - 1;
-            const XT *xf = X + fu * A.n_src;
-#pragma unroll
-            for (int j = 0; j < kColsPerLane; ++j) v[u][j] = uc[j] >= 0 ? static_cast<V>(ldv(xf + uc[j])) : V(0);
+            if (c0 >= 0) load_chunk<V>(X + fu * A.n_src, c0, A.n_src, vec, v[u]);
         }
     };
     fetch(f0);
     for (index_t f = f0; f < f1; f += FU) {
+        if (c0 >= 0) {
 #pragma unroll
-        for (int u = 0; u < FU; ++u)
+            for (int u = 0; u < FU; ++u)
 #pragma unroll
-            for (int j = 0; j < kColsPerLane; ++j)
-                if (uc[j] >= 0) xs[u][threadIdx.x + j * kTileSeg] = v[u][j];
+                for (int q = 0; q < kChunk; ++q) S.xs[u][lane_chunk * kChunk + q] = v[u][q];
+        }
         __syncthreads();
         if (f + FU < f1) fetch(f + FU);
 #pragma unroll
         for (int u = 0; u < FU; ++u) {
             RowAcc<NUM> acc;
-            const char *xb = reinterpret_cast<const char *>(&xs[u][0]);
+            const char *xb = reinterpret_cast<const char *>(&S.xs[u][0]);
 #pragma unroll
             for (int e = 0; e < E; ++e)
                 if (e < n_wave && e < n) acc.add(h[e], l[e], *reinterpret_cast<const V *>(xb + lc[e]));  // e < n_wave: wave-uniform
@@ -321,14 +367,14 @@ __device__ inline void tile_apply(const Mat &A, const XT *X, YT *Y, double *P, i
 
 template <int E, int FU, int NUM, class XT, class YT>
 __global__ void __launch_bounds__(kTileSeg) spmm_tile_kernel(Mat A, const XT *X, YT *Y, double *P, index_t k, unsigned n_tiles, int chunks) {
-    __shared__ typename NumTraits<NUM>::V xs[FU][kTileCols];
+    __shared__ Stage<FU, NUM> S;
     const unsigned w = xcd_remap(blockIdx.x, gridDim.x);
     const unsigned t = w % n_tiles, ch = w / n_tiles;
     const index_t per = (k + chunks - 1) / chunks;
     const index_t f0 = static_cast<index_t>(ch) * per;
     const index_t f1 = f0 + per < k ? f0 + per : k;
     if (f0 >= f1) return;  // whole workgroup
-    tile_apply<E, FU, NUM, XT, YT>(A, X, Y, P, t, f0, f1, xs);
+    tile_apply<E, FU, NUM, XT, YT>(A, X, Y, P, t, f0, f1, S);
 }
 
 /// Y of the long rows: the partials of each row summed in segment order.

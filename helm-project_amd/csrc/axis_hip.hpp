@@ -106,11 +106,12 @@ class InterpolationMatrix {
     // matrices built from triplets (rows then keep index order).
     DeviceBuffer<std::uint32_t> row_key;
     // Tiles: tile t = segments [256 t, 256 t + 256).  A tile whose nonzeros
-    // touch at most 1024 distinct source columns lists them (sorted) in
-    // tile_col[t * 1024 ...], and t_lcol[e] (aligned with the CSR arrays) holds
-    // the 16-bit position of nonzero e's column in its tile's list; the tiled
-    // apply stages those source values in LDS once per field.  tile_nu[t] = -1
-    // marks a tile that gathers straight from HBM.
+    // touch at most 256 distinct aligned 4-column chunks lists their first
+    // columns (sorted) in tile_col[t * 256 ...], and t_lcol[e] (aligned with
+    // the CSR arrays) holds nonzero e's position in the staged values
+    // (4 * chunk position + column % 4); the tiled apply stages those chunks
+    // in LDS once per field with 16-byte loads.  tile_nu[t] = -1 marks a tile
+    // that gathers straight from HBM.
     index_t n_tiles = 0;
     DeviceBuffer<std::int32_t> tile_nu, tile_col;
     DeviceBuffer<std::uint16_t> t_lcol;

@@ -55,7 +55,7 @@ template <int E, int FU, int NUM, class XT, class YT>
 // at least 5 waves per SIMD: every workgroup stays resident for the whole
 // launch, so occupancy is fixed at launch time
 __global__ void __launch_bounds__(kTileSeg, 5) persistent_apply_kernel(const PairDesc *pairs, const Item *items, const int *qbeg, unsigned *ctr) {
-    __shared__ typename NumTraits<NUM>::V xs[FU][kTileCols];
+    __shared__ apply_detail::Stage<FU, NUM> S;
     __shared__ int s_item;
     const int home = static_cast<int>(blockIdx.x & (kQueues - 1));  // workgroups are dealt to XCDs round-robin
     for (int qq = 0; qq < kQueues; ++qq) {
@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(kTileSeg, 5) persistent_apply_kernel(const Pai
             const Item I = items[it];
             const PairDesc &D = pairs[I.pair];
             apply_detail::tile_apply<E, FU, NUM, XT, YT>(D.A, static_cast<const XT *>(D.X), static_cast<YT *>(D.Y), D.P, I.tile, I.f0, I.f1,
-                                                         xs);
+                                                         S);
         }
     }
 }
