@@ -213,6 +213,7 @@ __global__ void __launch_bounds__(apply_detail::kTileSeg) tile_build_kernel(cons
 
 void InterpolationMatrix::prepare_apply(hipStream_t s) {
     if (apply_ready) return;
+    PhaseTrace tr("prepare_apply", s);
     if (n_src >= (index_t(1) << 31) - 1) throw InvalidArgument("apply: more than 2^31 - 2 source cells");
     if (n_dst >= (index_t(1) << 31)) throw InvalidArgument("apply: more than 2^31 destination cells");
     const index_t n1 = std::max<index_t>(nnz, 1);
@@ -235,6 +236,7 @@ void InterpolationMatrix::prepare_apply(hipStream_t s) {
     index_t hist[kH] = {0};
     d_hist.download(hist, kH, s);
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    tr.mark("split_hist");
     seg_cap = apply_detail::kMaxE;
     for (int cap : {4, 8, 12, 16}) {
         index_t above = 0;
@@ -262,6 +264,7 @@ void InterpolationMatrix::prepare_apply(hipStream_t s) {
         AXIS_LAUNCH_CHECK();
         if (coarse_bits) detail::radix_sort_keys_u64(key.get(), static_cast<std::size_t>(n_dst), s, 32 + 3 * coarse_bits);
     }
+    tr.mark("order_sort");
     DeviceBuffer<index_t> c_seg(nd1), c_slot(nd1), c_long(nd1), o_seg(nd1 + 1), o_slot(nd1 + 1), o_long(nd1 + 1);
     if (n_dst > 0) {
         seg_count_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(key.get(), row_ptr.get(), n_dst, seg_cap, c_seg.get(), c_slot.get(), c_long.get());
@@ -294,6 +297,7 @@ void InterpolationMatrix::prepare_apply(hipStream_t s) {
     }
     const std::int32_t end_slot = n_slots;
     AXIS_HIP_CHECK(hipMemcpyAsync(long_slot.get() + n_long_rows, &end_slot, sizeof(std::int32_t), hipMemcpyHostToDevice, s));
+    tr.mark("segments");
     // tiles
     n_tiles = (n_seg + apply_detail::kTileSeg - 1) / apply_detail::kTileSeg;
     tile_nu.resize(static_cast<std::size_t>(std::max<index_t>(n_tiles, 1)));
@@ -311,6 +315,7 @@ void InterpolationMatrix::prepare_apply(hipStream_t s) {
         AXIS_LAUNCH_CHECK();
     }
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    tr.mark("tiles");
     apply_ready = true;
 }
 

@@ -35,7 +35,32 @@ struct RuntimeError : std::runtime_error {
 
 #define AXIS_LAUNCH_CHECK() AXIS_HIP_CHECK(hipGetLastError())
 
-/// Owning device allocation (hipMalloc / hipFree), movable, not copyable.
+/// Process-wide cache of device blocks.  hipFree unmaps memory and waits for
+/// the device, which costs ~0.4 ms per block; a generation or a matrix's
+/// lifetime creates dozens.  Released blocks are parked as pending; the first
+/// allocation that needs one waits for the device once and recycles all of
+/// them, so a block is never reused while a kernel queued before its release
+/// may still touch it.  Blocks are sized in steps (powers of two up to 2 MiB,
+/// then multiples of 2 MiB) and reused for requests at least half their size.
+class BlockCache {
+   public:
+    static BlockCache &get();
+    void *alloc(std::size_t bytes);
+    void release(void *p, std::size_t bytes);
+    /// Returns every cached block to the runtime (waits for the device).
+    void trim();
+    static std::size_t rounded(std::size_t bytes) {
+        if (bytes <= (std::size_t(2) << 20)) {
+            std::size_t r = 256;
+            while (r < bytes) r <<= 1;
+            return r;
+        }
+        const std::size_t step = std::size_t(2) << 20;
+        return (bytes + step - 1) / step * step;
+    }
+};
+
+/// Owning device allocation from the block cache, movable, not copyable.
 template <class T>
 class DeviceBuffer {
    public:
@@ -55,11 +80,11 @@ class DeviceBuffer {
     }
     void resize(std::size_t n) {
         reset();
-        if (n) AXIS_HIP_CHECK(hipMalloc(reinterpret_cast<void **>(&p_), n * sizeof(T)));
+        if (n) p_ = static_cast<T *>(BlockCache::get().alloc(n * sizeof(T)));
         n_ = n;
     }
     void reset() {
-        if (p_) (void)hipFree(p_);
+        if (p_) BlockCache::get().release(p_, n_ * sizeof(T));
         p_ = nullptr;
         n_ = 0;
     }

@@ -4,8 +4,105 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
+#include <mutex>
+#include <map>
 
 #include "axis_hip.hpp"
+
+namespace axis_hip {
+
+// ------------------------------------------------------------- block cache
+namespace {
+struct Block {
+    int device;
+    std::size_t bytes;
+    void *p;
+};
+struct CacheState {
+    std::mutex mu;
+    std::multimap<std::pair<int, std::size_t>, void *> free;  // (device, rounded size) -> block
+    std::vector<Block> pending;                               // released, possibly still in use by queued work
+    void recycle() {
+        for (const Block &b : pending) free.emplace(std::make_pair(b.device, b.bytes), b.p);
+        pending.clear();
+    }
+};
+int current_device() {
+    int d = 0;
+    AXIS_HIP_CHECK(hipGetDevice(&d));
+    return d;
+}
+CacheState &cache_state() {
+    static CacheState *c = new CacheState;  // never destroyed: blocks live until the process ends
+    return *c;
+}
+}  // namespace
+
+BlockCache &BlockCache::get() {
+    static BlockCache b;
+    return b;
+}
+
+void *BlockCache::alloc(std::size_t bytes) {
+    const std::size_t r = rounded(bytes);
+    const int dev = current_device();
+    CacheState &c = cache_state();
+    std::lock_guard<std::mutex> g(c.mu);
+    for (int pass = 0; pass < 2; ++pass) {
+        auto it = c.free.lower_bound(std::make_pair(dev, r));
+        if (it != c.free.end() && it->first.first == dev && it->first.second <= 2 * r) {
+            void *p = it->second;
+            c.free.erase(it);
+            return p;
+        }
+        if (pass == 0 && !c.pending.empty()) {
+            AXIS_HIP_CHECK(hipDeviceSynchronize());  // work queued before the releases has finished
+            c.recycle();
+            continue;
+        }
+        break;
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, r);
+    if (e != hipSuccess) {  // out of memory: give this device's cached blocks back and retry once
+        (void)hipGetLastError();
+        AXIS_HIP_CHECK(hipDeviceSynchronize());
+        c.recycle();
+        for (auto it = c.free.begin(); it != c.free.end();) {
+            if (it->first.first == dev) {
+                (void)hipFree(it->second);
+                it = c.free.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        AXIS_HIP_CHECK(hipMalloc(&p, r));
+    }
+    return p;
+}
+
+void BlockCache::release(void *p, std::size_t bytes) {
+    int dev = 0;
+    if (hipPointerAttribute_t a{}; hipPointerGetAttributes(&a, p) == hipSuccess)
+        dev = a.device;
+    else
+        (void)hipGetLastError();
+    CacheState &c = cache_state();
+    std::lock_guard<std::mutex> g(c.mu);
+    c.pending.push_back(Block{dev, rounded(bytes), p});
+}
+
+void BlockCache::trim() {
+    CacheState &c = cache_state();
+    std::lock_guard<std::mutex> g(c.mu);
+    AXIS_HIP_CHECK(hipDeviceSynchronize());
+    c.recycle();
+    for (auto &b : c.free) (void)hipFree(b.second);
+    c.free.clear();
+}
+
+}  // namespace axis_hip
 
 namespace axis_hip::detail {
 
