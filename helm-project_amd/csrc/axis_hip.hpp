@@ -203,7 +203,7 @@ namespace detail {
 /// Per-mesh data derived inside one generate() call (never cached across calls).
 struct MeshGeom {
     index_t n_cells = 0, n_nodes = 0;
-    std::vector<double> h_xyz;          // node xyz [3 * n_nodes] (libm, reference bits)
+    HostDoubles h_xyz;                  // node xyz [3 * n_nodes] (libm, reference bits)
     DeviceBuffer<double> d_xyz;         // same on the device
     DeviceBuffer<double> d_area;        // [n_cells]
     DeviceBuffer<std::int32_t> d_flags; // bit0 degenerate, bit1 masked, bit2 area<=0

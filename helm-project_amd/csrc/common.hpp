@@ -5,13 +5,38 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <memory>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <vector>
 
 namespace axis_hip {
 
 using index_t = std::int64_t;  // reference libs/axis/include/axis/types.hpp:90
+
+/// Allocator whose resize() leaves new elements uninitialised, so that large
+/// host arrays are first touched (page-faulted) by the threads that fill them
+/// rather than zero-filled serially.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <class U>
+    NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+    template <class U, class... A>
+    void construct(U *p, A &&...a) {
+        if constexpr (sizeof...(A) == 0)
+            ::new (static_cast<void *>(p)) U;
+        else
+            ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
+    }
+};
+using HostDoubles = std::vector<double, NoInitAlloc<double>>;
 
 /// Error raised for invalid arguments (maps to std::invalid_argument in the reference).
 struct InvalidArgument : std::invalid_argument {

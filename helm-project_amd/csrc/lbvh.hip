@@ -12,7 +12,9 @@
 //      construction of BVHs, octrees and k-d trees"), duplicate codes broken
 //      by sorted position,
 //   4. bottom-up box refit with one atomic arrival counter per internal node,
-//   5. stack traversal per query: count pass, exclusive scan, fill pass.
+//   5. one stack traversal per query that keeps up to 32 hits; the queries
+//      with more (rows around the poles of a lat-lon source meet thousands)
+//      are counted and filled by one wave each, breadth-first.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -26,6 +28,7 @@ namespace axis_hip::detail {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kWave = 64;
 constexpr int kStack = 64;
 constexpr std::uint32_t kLeaf = 0x80000000u;
 
@@ -178,36 +181,48 @@ __global__ void __launch_bounds__(kTopBlock) refit_top_kernel(const float *boxes
     }
 }
 
-// Traversal; kFill == false counts, kFill == true writes.  Stack overflow
-// (never expected: depth <= 62 with 30-bit codes + position tie-break) sets
-// *err so the host reports it instead of returning a short list.
-template <bool kFill>
+// Per-lane stack traversal of query j (rows[qi] when a row list is given).
+//   kMode 0: count all hits;  kMode 1: write all hits at offsets[j];
+//   kMode 2: first pass over every query -- keep up to kCap hits in
+//            tmp[j * kCap ...]; a query that meets more stops there, is
+//            appended to `over` and finished by the wave kernel below.
+// Stack overflow (never expected: depth <= 62 with 30-bit codes + position
+// tie-break) sets *err so the host reports it instead of returning a short list.
+constexpr int kCap = 32;
+
+template <int kMode>
 __global__ void __launch_bounds__(kBlock) query_kernel(const float *src_boxes, const std::uint32_t *ids, int n_src, const std::uint32_t *left,
-                                                       const std::uint32_t *right, const float *node_box, const float *dst_boxes,
-                                                       std::int64_t n_dst, std::int64_t *count, const std::int64_t *offsets, std::int32_t *out,
-                                                       int *err) {
-    std::int64_t j = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (j >= n_dst) return;
-    FBox q = load_box(dst_boxes, j);
+                                                       const std::uint32_t *right, const float *node_box, const float *dst_boxes, std::int64_t n_q,
+                                                       const std::int32_t *rows, std::int64_t *count, const std::int64_t *offsets, std::int32_t *out,
+                                                       std::int32_t *over, int *n_over, int *err) {
+    const std::int64_t qi = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (qi >= n_q) return;
+    const std::int64_t j = rows ? rows[qi] : qi;
+    const FBox q = load_box(dst_boxes, j);
     std::int64_t cnt = 0;
-    std::int64_t w = kFill ? offsets[j] : 0;
+    const std::int64_t w = kMode == 1 ? offsets[j] : (kMode == 2 ? j * kCap : 0);
+    bool full = false;
     if (n_src == 1) {
         if (overlap(q, load_box(src_boxes, ids[0]))) {
-            if (kFill) out[w] = static_cast<std::int32_t>(ids[0]);
+            if (kMode != 0) out[w] = static_cast<std::int32_t>(ids[0]);
             ++cnt;
         }
     } else if (n_src > 1) {
         std::uint32_t stack[kStack];
         int sp = 0;
         stack[sp++] = 0;
-        while (sp > 0) {
+        while (sp > 0 && !full) {
             std::uint32_t nd = stack[--sp];
             std::uint32_t c[2] = {left[nd], right[nd]};
             for (int h = 0; h < 2; ++h) {
                 if (c[h] & kLeaf) {
                     std::uint32_t id = ids[c[h] & ~kLeaf];
                     if (overlap(q, load_box(src_boxes, id))) {
-                        if (kFill) out[w + cnt] = static_cast<std::int32_t>(id);
+                        if (kMode == 2 && cnt == kCap) {
+                            full = true;
+                            break;
+                        }
+                        if (kMode != 0) out[w + cnt] = static_cast<std::int32_t>(id);
                         ++cnt;
                     }
                 } else if (overlap(q, load_box(node_box, c[h]))) {
@@ -219,7 +234,90 @@ __global__ void __launch_bounds__(kBlock) query_kernel(const float *src_boxes, c
             }
         }
     }
-    if (!kFill) count[j] = cnt;
+    if (kMode == 2 && full) {
+        count[j] = kCap + 1;  // replaced by the exact count before the scan
+        over[atomicAdd(n_over, 1)] = static_cast<std::int32_t>(j);
+        return;
+    }
+    if (kMode != 1) count[j] = cnt;
+}
+
+// One wave per query with more than kCap hits: breadth-first over the tree,
+// each lane taking one frontier node per step, hits and next-level nodes
+// placed by ballot prefix counts (so the hit order is fixed and the count and
+// fill passes agree).  A level wider than kFront marks the query slow: it is
+// then counted and filled by the per-lane kernel instead.
+constexpr int kFront = 4096;
+
+template <bool kFill>
+__global__ void __launch_bounds__(kWave) query_wave_kernel(const float *src_boxes, const std::uint32_t *ids, const std::uint32_t *left,
+                                                           const std::uint32_t *right, const float *node_box, const float *dst_boxes,
+                                                           const std::int32_t *rows, std::int64_t *count, const std::int64_t *offsets,
+                                                           std::int32_t *out, std::uint8_t *slow, std::int32_t *slow_rows, int *n_slow) {
+    __shared__ std::uint32_t front[2][kFront];
+    const int lane = static_cast<int>(threadIdx.x);
+    const std::int64_t j = rows[blockIdx.x];
+    if (kFill && slow[j]) return;
+    const FBox q = load_box(dst_boxes, j);
+    const std::uint64_t below = (std::uint64_t(1) << lane) - 1;
+    const std::int64_t w = kFill ? offsets[j] : 0;
+    std::int64_t cnt = 0;
+    if (lane == 0) front[0][0] = 0;
+    __syncthreads();
+    int cur = 0, m = 1;
+    bool wide = false;
+    while (m > 0) {
+        int nm = 0;
+        for (int i0 = 0; i0 < m; i0 += kWave) {
+            const bool act = i0 + lane < m;
+            const std::uint32_t nd = act ? front[cur][i0 + lane] : 0u;
+            const std::uint32_t c[2] = {act ? left[nd] : 0u, act ? right[nd] : 0u};
+            for (int h = 0; h < 2; ++h) {
+                bool hit = false, push = false;
+                std::uint32_t id = 0;
+                if (act) {
+                    if (c[h] & kLeaf) {
+                        id = ids[c[h] & ~kLeaf];
+                        hit = overlap(q, load_box(src_boxes, id));
+                    } else {
+                        push = overlap(q, load_box(node_box, c[h]));
+                    }
+                }
+                const std::uint64_t mh = __ballot(hit);
+                if (kFill && hit) out[w + cnt + __popcll(mh & below)] = static_cast<std::int32_t>(id);
+                cnt += __popcll(mh);
+                const std::uint64_t mp = __ballot(push);
+                const int at = nm + __popcll(mp & below);
+                if (push && at < kFront) front[cur ^ 1][at] = c[h];
+                nm += __popcll(mp);
+            }
+        }
+        __syncthreads();
+        if (nm > kFront) {
+            wide = true;
+            break;
+        }
+        cur ^= 1;
+        m = nm;
+    }
+    if (kFill || lane != 0) return;
+    if (wide) {
+        slow[j] = 1;
+        slow_rows[atomicAdd(n_slow, 1)] = static_cast<std::int32_t>(j);
+    } else {
+        count[j] = cnt;
+    }
+}
+
+// first-pass hits of the queries with at most kCap of them -> candidate CSR
+__global__ void compact_hits_kernel(const std::int32_t *tmp, const std::int64_t *count, const std::int64_t *offsets, std::int64_t n,
+                                    std::int32_t *out) {
+    const std::int64_t j = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const std::int64_t c = count[j];
+    if (c > kCap) return;
+    const std::int64_t o = offsets[j];
+    for (std::int64_t e = 0; e < c; ++e) out[o + e] = tmp[j * kCap + e];
 }
 
 }  // namespace
@@ -259,15 +357,41 @@ void lbvh_candidates(const float *src_boxes, index_t n_src, const float *dst_box
         AXIS_LAUNCH_CHECK();
     }
     tr.mark("build_refit");
-    DeviceBuffer<index_t> count(std::max<index_t>(n_dst, 1));
+    const index_t nd1 = std::max<index_t>(n_dst, 1);
+    DeviceBuffer<index_t> count(nd1);
+    DeviceBuffer<std::int32_t> tmp(static_cast<std::size_t>(nd1) * kCap), over(nd1), slow_rows(nd1);
+    DeviceBuffer<std::uint8_t> slow(nd1);
+    DeviceBuffer<int> n_list(2);  // queries over kCap, slow queries
+    AXIS_HIP_CHECK(hipMemsetAsync(n_list.get(), 0, 2 * sizeof(int), s));
     if (n_dst > 0) {
-        query_kernel<false><<<grid_for(n_dst, kBlock), kBlock, 0, s>>>(src_boxes, ids.get(), n, left.get(), right.get(), node_box.get(), dst_boxes,
-                                                                      n_dst, count.get(), nullptr, nullptr, err.get());
+        query_kernel<2><<<grid_for(n_dst, kBlock), kBlock, 0, s>>>(src_boxes, ids.get(), n, left.get(), right.get(), node_box.get(), dst_boxes,
+                                                                  n_dst, nullptr, count.get(), nullptr, tmp.get(), over.get(), n_list.get(), err.get());
         AXIS_LAUNCH_CHECK();
     }
+    int h_list[2] = {0, 0}, h_err = 0;
+    AXIS_HIP_CHECK(hipMemcpyAsync(h_list, n_list.get(), sizeof h_list, hipMemcpyDeviceToHost, s));
+    AXIS_HIP_CHECK(hipMemcpyAsync(&h_err, err.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    if (h_err) throw RuntimeError("LBVH traversal stack overflow");
+    const int n_over = h_list[0];
+    if (n_over > 0) {
+        AXIS_HIP_CHECK(hipMemsetAsync(slow.get(), 0, static_cast<std::size_t>(nd1), s));
+        query_wave_kernel<false><<<static_cast<unsigned>(n_over), kWave, 0, s>>>(src_boxes, ids.get(), left.get(), right.get(), node_box.get(),
+                                                                                 dst_boxes, over.get(), count.get(), nullptr, nullptr, slow.get(),
+                                                                                 slow_rows.get(), n_list.get() + 1);
+        AXIS_LAUNCH_CHECK();
+        AXIS_HIP_CHECK(hipMemcpyAsync(h_list, n_list.get(), sizeof h_list, hipMemcpyDeviceToHost, s));
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+        if (h_list[1] > 0) {
+            query_kernel<0><<<grid_for(h_list[1], kBlock), kBlock, 0, s>>>(src_boxes, ids.get(), n, left.get(), right.get(), node_box.get(),
+                                                                          dst_boxes, h_list[1], slow_rows.get(), count.get(), nullptr, nullptr,
+                                                                          nullptr, nullptr, err.get());
+            AXIS_LAUNCH_CHECK();
+        }
+    }
+    const int n_slow = h_list[1];
     exclusive_scan_i64(count.get(), cand_off.get(), n_dst, s);
     index_t total = 0;
-    int h_err = 0;
     AXIS_HIP_CHECK(hipMemcpyAsync(&total, cand_off.get() + n_dst, sizeof(index_t), hipMemcpyDeviceToHost, s));
     AXIS_HIP_CHECK(hipMemcpyAsync(&h_err, err.get(), sizeof(int), hipMemcpyDeviceToHost, s));
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
@@ -277,9 +401,20 @@ void lbvh_candidates(const float *src_boxes, index_t n_src, const float *dst_box
     if (total >= (index_t(1) << 31)) throw RuntimeError("LBVH: candidate pair count exceeds 2^31");
     cand_src.resize(std::max<index_t>(total, 1));
     if (n_dst > 0 && total > 0) {
-        query_kernel<true><<<grid_for(n_dst, kBlock), kBlock, 0, s>>>(src_boxes, ids.get(), n, left.get(), right.get(), node_box.get(), dst_boxes,
-                                                                     n_dst, nullptr, cand_off.get(), cand_src.get(), err.get());
+        compact_hits_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(tmp.get(), count.get(), cand_off.get(), n_dst, cand_src.get());
         AXIS_LAUNCH_CHECK();
+        if (n_over > 0) {
+            query_wave_kernel<true><<<static_cast<unsigned>(n_over), kWave, 0, s>>>(src_boxes, ids.get(), left.get(), right.get(), node_box.get(),
+                                                                                    dst_boxes, over.get(), nullptr, cand_off.get(), cand_src.get(),
+                                                                                    slow.get(), nullptr, nullptr);
+            AXIS_LAUNCH_CHECK();
+        }
+        if (n_slow > 0) {
+            query_kernel<1><<<grid_for(n_slow, kBlock), kBlock, 0, s>>>(src_boxes, ids.get(), n, left.get(), right.get(), node_box.get(), dst_boxes,
+                                                                       n_slow, slow_rows.get(), nullptr, cand_off.get(), cand_src.get(), nullptr,
+                                                                       nullptr, err.get());
+            AXIS_LAUNCH_CHECK();
+        }
     }
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
     tr.mark("query_fill");

@@ -185,6 +185,9 @@ GridAxes unique_axes(const Mesh &m, hipStream_t s) {
     std::vector<double> tmp;
     index_t nlat = unique_column(m.d_coords.get() + n, m.h_coords.data() + n, n, want_lat, nullptr, tmp, s);
     ctx.other = nlat;
+    // (nlon - 1) * (nlat - 1) == n_cells needs nlat - 1 to divide n_cells:
+    // otherwise no lon count can make the mesh rectilinear, skip that sort
+    if (nlat < 2 || n_cells % (nlat - 1) != 0) return ax;
     auto want_lon = [](index_t cnt, void *p) {
         auto *c = static_cast<Ctx *>(p);
         return cnt >= 2 && c->other >= 2 && (cnt - 1) * (c->other - 1) == c->n_cells;
@@ -456,7 +459,7 @@ void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s) {
     PhaseTrace tr("geometry", s);
     // node xyz with host libm (extract_cell_polygon_spherical, weight_generator.cpp:462-470)
     g.h_xyz.resize(3 * static_cast<std::size_t>(n_nodes));
-    std::vector<double> h_xyz_deg;
+    HostDoubles h_xyz_deg;
     if (!deg) h_xyz_deg.resize(3 * static_cast<std::size_t>(n_nodes));
     const double *lon = m.h_coords.data(), *lat = m.h_coords.data() + n_nodes;
     parallel_range(n_nodes, [&](index_t b, index_t e) {
