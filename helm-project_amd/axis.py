@@ -59,6 +59,7 @@ def lib():
         "axis_mesh_create_c": [_D, C.c_int64, C.c_int64, _I64, C.c_int64, _I64, C.c_int, _D, _I32, C.POINTER(C.c_int)],
         "axis_mesh_regular_latlon_c": [C.c_double] * 6 + [C.POINTER(C.c_int)],
         "axis_mesh_info_c": [C.c_int, _I64, _I64],
+        "axis_mesh_grid_info_c": [C.c_int, C.POINTER(C.c_int), _I64, _D],
         "axis_generate_weights_c": [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)],
         "axis_generate_weights_config_c": [C.c_int, C.c_int, C.POINTER(RegridConfig), C.POINTER(C.c_int)],
         "axis_matrix_from_coo_c": [_I64, _I64, _D, C.c_int64, C.c_int64, C.c_int64, _D, _D, _D, _D, C.POINTER(C.c_int)],
@@ -108,7 +109,7 @@ def lib():
 
 def exported_symbols():
     """Names of the C entry points the binding declares."""
-    return ["axis_init_c", "axis_last_error_c", "axis_mesh_create_c", "axis_mesh_regular_latlon_c", "axis_mesh_info_c",
+    return ["axis_init_c", "axis_last_error_c", "axis_mesh_create_c", "axis_mesh_regular_latlon_c", "axis_mesh_info_c", "axis_mesh_grid_info_c",
             "axis_generate_weights_c", "axis_generate_weights_config_c", "axis_matrix_from_coo_c",
             "axis_matrix_to_csr_c", "axis_matrix_info_c", "axis_matrix_csr_c", "axis_matrix_fractions_c",
             "axis_matrix_stats_c", "axis_apply_c", "axis_batch_apply_c", "axis_batch_apply_async_c",
@@ -180,6 +181,15 @@ class Mesh:
         nn, nc = C.c_int64(0), C.c_int64(0)
         _check(lib().axis_mesh_info_c(self.handle, C.byref(nn), C.byref(nc)))
         return nn.value, nc.value
+
+    def grid_info(self):
+        """(kind, (ni, nj), out6): kind 0 unstructured, 1 rectilinear, 2 regular;
+        out6 = (lon_min, lon_max, delta_lon, lat_min, lat_max, delta_lat) when regular."""
+        kind = C.c_int(0)
+        ninj = np.zeros(2, dtype=np.int64)
+        out6 = np.zeros(6)
+        _check(lib().axis_mesh_grid_info_c(self.handle, C.byref(kind), ninj.ctypes.data_as(_I64), out6.ctypes.data_as(_D)))
+        return kind.value, ninj, out6
 
     def close(self):
         if getattr(self, "handle", 0):

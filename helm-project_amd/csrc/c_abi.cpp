@@ -209,6 +209,26 @@ int axis_mesh_info_c(int mesh_handle, int64_t *n_nodes, int64_t *n_cells) {
     });
 }
 
+int axis_mesh_grid_info_c(int mesh_handle, int *kind, int64_t *ninj, double *out6) {
+    return guarded([&] {
+        auto m = Registry::get().find<Mesh>(mesh_handle);
+        check_device();
+        detail::GridAxes ax = detail::unique_axes(*m, nullptr);
+        detail::MeshGeom g;
+        const bool rect = detail::rectilinear_from_axes(*m, ax), reg = detail::regular_from_axes(*m, ax, g);
+        if (kind) *kind = reg ? 2 : (rect ? 1 : 0);
+        if (ninj) {
+            ninj[0] = rect ? static_cast<int64_t>(ax.ulon.size()) - 1 : 0;
+            ninj[1] = rect ? static_cast<int64_t>(ax.ulat.size()) - 1 : 0;
+        }
+        if (out6) {
+            const double v[6] = {reg ? g.lon_min : 0.0, reg ? ax.ulon.back() : 0.0, reg ? g.delta_lon : 0.0,
+                                 reg ? g.lat_min : 0.0, reg ? ax.ulat.back() : 0.0, reg ? g.delta_lat : 0.0};
+            for (int i = 0; i < 6; ++i) out6[i] = v[i];
+        }
+    });
+}
+
 int axis_generate_weights_config_c(int src_handle, int dst_handle, const axis_regrid_config *config, int *matrix_handle) {
     return guarded([&] {
         if (!config || !matrix_handle) throw InvalidArgument("null argument");

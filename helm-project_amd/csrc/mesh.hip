@@ -112,6 +112,144 @@ __global__ void span_check_kernel(const double *vals, const index_t *head, const
     if (fabs(vals[i] - u[cls]) > 1.0e-12) atomicOr(bad, 1);
 }
 
+// ---- quick pass over both coordinate columns
+// Before sorting every node coordinate, two cheap looks:
+//  * a lower bound on each column's tolerance-unique count: occupied bins of
+//    width >= 1e-11 over [min, max] among the first kSample nodes, halved
+//    (one tolerance class spans at most 1e-12, so it touches at most two
+//    bins; a subset of the nodes occupies no more bins than all of them);
+//    (nlon - 1) * (nlat - 1) grows with both counts, so a product of the
+//    bounds above n_cells rules the grid out;
+//  * each column's exact distinct values, gathered in a small hash set: the
+//    reference's sort + sequential tolerance filter keeps the same values
+//    when run on the distinct values alone (an exact duplicate is always
+//    dropped and never becomes the last kept value).
+constexpr int kSetBits = 15;            // hash slots per column
+constexpr int kSetSlots = 1 << kSetBits;
+constexpr int kSetMax = kSetSlots / 2;  // distinct values kept before giving up
+constexpr int kBinBits = 20;            // occupancy bins per column
+constexpr int kBinWords = (1 << kBinBits) / 32;
+constexpr index_t kSample = 1 << 16;
+constexpr int kQuickBlock = 256;
+
+struct QuickAxes {
+    unsigned long long mm[4];  // order keys: min lon, max lon, min lat, max lat
+    int nonfinite;
+    int n_distinct[2];
+    int overflow[2];
+    int occupied[2];
+};
+
+__global__ void __launch_bounds__(kQuickBlock) minmax_kernel(const double *lon, const double *lat, index_t n, QuickAxes *q) {
+    std::uint64_t r[4] = {~0ull, 0ull, ~0ull, 0ull};
+    bool bad = false;
+    for (index_t i = static_cast<index_t>(blockIdx.x) * kQuickBlock + threadIdx.x; i < n; i += static_cast<index_t>(gridDim.x) * kQuickBlock) {
+        const double a = lon[i], b = lat[i];
+        bad = bad || !isfinite(a) || !isfinite(b);
+        const std::uint64_t ka = order_key(a), kb = order_key(b);
+        r[0] = min(r[0], ka);
+        r[1] = max(r[1], ka);
+        r[2] = min(r[2], kb);
+        r[3] = max(r[3], kb);
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int k = 0; k < 4; ++k) {
+            const std::uint64_t v = static_cast<std::uint64_t>(__shfl_xor(static_cast<unsigned long long>(r[k]), o));
+            r[k] = (k & 1) ? max(r[k], v) : min(r[k], v);
+        }
+    __shared__ unsigned long long part[kQuickBlock / 64][4];
+    __shared__ int any_bad;
+    if (threadIdx.x == 0) any_bad = 0;
+    __syncthreads();
+    if (bad) any_bad = 1;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 4; ++k) part[threadIdx.x / 64][k] = r[k];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int k = threadIdx.x;
+        std::uint64_t v = part[0][k];
+        for (int w = 1; w < kQuickBlock / 64; ++w) v = (k & 1) ? max(v, static_cast<std::uint64_t>(part[w][k])) : min(v, static_cast<std::uint64_t>(part[w][k]));
+        if (k & 1)
+            atomicMax(&q->mm[k], static_cast<unsigned long long>(v));
+        else
+            atomicMin(&q->mm[k], static_cast<unsigned long long>(v));
+        if (k == 0 && any_bad) atomicOr(&q->nonfinite, 1);
+    }
+}
+
+__device__ inline std::uint32_t mix_key(std::uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return static_cast<std::uint32_t>(k);
+}
+
+// occupancy bits of the first n nodes (finite inputs)
+__global__ void __launch_bounds__(kQuickBlock) sample_bins_kernel(const double *lon, const double *lat, index_t n, const QuickAxes *q,
+                                                                  std::uint32_t *bins) {
+    const index_t i = static_cast<index_t>(blockIdx.x) * kQuickBlock + threadIdx.x;
+    if (i >= n) return;
+    for (int c = 0; c < 2; ++c) {
+        const double v = c == 0 ? lon[i] : lat[i];
+        const double mn = from_order_key(q->mm[2 * c]), mx = from_order_key(q->mm[2 * c + 1]);
+        const double range = mx - mn;
+        const double nb = fmax(1.0, fmin(static_cast<double>(1 << kBinBits), floor(range / 1.0e-11)));
+        const double x = range > 0.0 ? (v - mn) * (nb / range) : 0.0;
+        const int bin = static_cast<int>(fmin(nb - 1.0, fmax(0.0, floor(x))));
+        std::uint32_t *bw = bins + c * kBinWords + (bin >> 5);
+        const std::uint32_t bit = 1u << (bin & 31);
+        if (!(__hip_atomic_load(bw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit)) atomicOr(bw, bit);
+    }
+}
+
+// occupied-bin counts (one wave-reduced atomic per wave)
+__global__ void count_bins_kernel(const std::uint32_t *bins, QuickAxes *q) {
+    const int w = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;  // < 2 * kBinWords
+    int c = __popc(bins[w]);
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&q->occupied[w / kBinWords], c);
+}
+
+// distinct values of both columns (finite inputs: order keys are then never
+// 0, the empty-slot marker); a lane skips a value equal to its left
+// neighbour's, which removes most repeats of structured node orders
+__global__ void __launch_bounds__(kQuickBlock) distinct_kernel(const double *lon, const double *lat, index_t n, QuickAxes *q,
+                                                               unsigned long long *set) {
+    const index_t i = static_cast<index_t>(blockIdx.x) * kQuickBlock + threadIdx.x;
+    const bool act = i < n;
+    for (int c = 0; c < 2; ++c) {
+        const unsigned long long key = act ? order_key(c == 0 ? lon[i] : lat[i]) : 0ull;
+        const unsigned long long left = __shfl_up(key, 1);
+        if (!act || ((threadIdx.x & 63) != 0 && left == key)) continue;
+        if (__hip_atomic_load(&q->overflow[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
+        unsigned long long *tab = set + c * kSetSlots;
+        std::uint32_t h = mix_key(key) & (kSetSlots - 1);
+        for (int probe = 0; probe < kSetSlots; ++probe) {
+            unsigned long long cur = __hip_atomic_load(&tab[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (cur == 0) {
+                cur = atomicCAS(&tab[h], 0ull, key);
+                if (cur == 0) {
+                    if (atomicAdd(&q->n_distinct[c], 1) + 1 > kSetMax) atomicOr(&q->overflow[c], 1);
+                    break;
+                }
+            }
+            if (cur == key) break;
+            if (__hip_atomic_load(&q->overflow[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+            h = (h + 1) & (kSetSlots - 1);
+        }
+    }
+}
+
+__global__ void gather_set_kernel(const unsigned long long *set, double *out, int *pos) {
+    const int t = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;  // < 2 * kSetSlots
+    const unsigned long long k = set[t];
+    if (k == 0) return;
+    const int p = atomicAdd(&pos[t / kSetSlots], 1);
+    if (p < kSetMax) out[(t / kSetSlots) * kSetMax + p] = from_order_key(k);  // an overflowed set holds more
+}
+
 std::vector<double> host_unique(std::vector<double> s) {
     std::sort(s.begin(), s.end());
     std::vector<double> u;
@@ -177,6 +315,59 @@ GridAxes unique_axes(const Mesh &m, hipStream_t s) {
         }
     if (!ax.all_quads) return ax;
     const index_t n = m.n_nodes();
+    PhaseTrace tr("axes", s);
+    if (n > 0) {
+        // quick pass: refute with the lower bounds, or take both columns'
+        // distinct values from the hash sets
+        DeviceBuffer<QuickAxes> dq(1);
+        DeviceBuffer<std::uint32_t> bins(2 * kBinWords);
+        QuickAxes hq{};
+        hq.mm[0] = hq.mm[2] = ~0ull;
+        const double *lon = m.d_coords.get(), *lat = m.d_coords.get() + n;
+        AXIS_HIP_CHECK(hipMemcpyAsync(dq.get(), &hq, sizeof hq, hipMemcpyHostToDevice, s));
+        AXIS_HIP_CHECK(hipMemsetAsync(bins.get(), 0, sizeof(std::uint32_t) * 2 * kBinWords, s));
+        const unsigned mm_grid = static_cast<unsigned>(std::min<index_t>((n + kQuickBlock - 1) / kQuickBlock, 1024));
+        minmax_kernel<<<mm_grid, kQuickBlock, 0, s>>>(lon, lat, n, dq.get());
+        AXIS_LAUNCH_CHECK();
+        const index_t ns = std::min(n, kSample);
+        sample_bins_kernel<<<grid_for(ns, kQuickBlock), kQuickBlock, 0, s>>>(lon, lat, ns, dq.get(), bins.get());
+        AXIS_LAUNCH_CHECK();
+        count_bins_kernel<<<2 * kBinWords / 256, 256, 0, s>>>(bins.get(), dq.get());
+        AXIS_LAUNCH_CHECK();
+        AXIS_HIP_CHECK(hipMemcpyAsync(&hq, dq.get(), sizeof hq, hipMemcpyDeviceToHost, s));
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+        tr.mark("bound");
+        if (!hq.nonfinite) {
+            // (the bins above are meaningless with non-finite coordinates)
+            const index_t lb_lon = (hq.occupied[0] + 1) / 2, lb_lat = (hq.occupied[1] + 1) / 2;
+            if (lb_lon >= 2 && lb_lat >= 2 && (lb_lon - 1) > n_cells / (lb_lat - 1)) return ax;
+            DeviceBuffer<unsigned long long> set(2 * kSetSlots);
+            DeviceBuffer<double> vals(2 * kSetMax);
+            DeviceBuffer<int> pos(2);
+            AXIS_HIP_CHECK(hipMemsetAsync(set.get(), 0, sizeof(unsigned long long) * 2 * kSetSlots, s));
+            AXIS_HIP_CHECK(hipMemsetAsync(pos.get(), 0, 2 * sizeof(int), s));
+            distinct_kernel<<<grid_for(n, kQuickBlock), kQuickBlock, 0, s>>>(lon, lat, n, dq.get(), set.get());
+            AXIS_LAUNCH_CHECK();
+            gather_set_kernel<<<2 * kSetSlots / 256, 256, 0, s>>>(set.get(), vals.get(), pos.get());
+            AXIS_LAUNCH_CHECK();
+            AXIS_HIP_CHECK(hipMemcpyAsync(&hq, dq.get(), sizeof hq, hipMemcpyDeviceToHost, s));
+            AXIS_HIP_CHECK(hipStreamSynchronize(s));
+            if (!hq.overflow[0] && !hq.overflow[1]) {
+                std::vector<double> h(2 * static_cast<std::size_t>(kSetMax));
+                vals.download(h.data(), h.size(), s);
+                AXIS_HIP_CHECK(hipStreamSynchronize(s));
+                std::vector<double> ulon = host_unique(std::vector<double>(h.begin(), h.begin() + hq.n_distinct[0]));
+                std::vector<double> ulat = host_unique(std::vector<double>(h.begin() + kSetMax, h.begin() + kSetMax + hq.n_distinct[1]));
+                const index_t nlon = static_cast<index_t>(ulon.size()), nlat = static_cast<index_t>(ulat.size());
+                if (nlon >= 2 && nlat >= 2 && (nlon - 1) * (nlat - 1) == n_cells) {
+                    ax.ulon = std::move(ulon);
+                    ax.ulat = std::move(ulat);
+                }
+                tr.mark("distinct");
+                return ax;
+            }
+        }
+    }
     // the lat count is needed to decide whether the lon values are worth downloading
     struct Ctx {
         index_t n_cells, other;

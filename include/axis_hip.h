@@ -76,6 +76,13 @@ int axis_mesh_regular_latlon_c(double min_x, double max_x, double min_y, double 
 /* Sizes of a mesh (UnstructuredMesh::n_nodes / n_cells, unstructured_mesh.hpp:86-97). */
 int axis_mesh_info_c(int mesh_handle, int64_t *n_nodes, int64_t *n_cells);
 
+/* Grid structure as the weight generators see it: *kind = 0 unstructured,
+ * 1 rectilinear (not regular), 2 regular lat-lon; ninj[2] = (ni, nj) for the
+ * last two; out6 = (lon_min, lon_max, delta_lon, lat_min, lat_max, delta_lat)
+ * for a regular grid.  Replaces detect_regular_grid / detect_rectilinear_grid
+ * (detail/regular_grid_detector.hpp:64-144, 180-299). */
+int axis_mesh_grid_info_c(int mesh_handle, int *kind, int64_t *ninj, double *out6);
+
 /* axis_generate_weights_c (axis_c_interop.cpp:117-145): method 0 Bilinear,
  * 2 Conservative1stOrder with the default RegridConfig otherwise. */
 int axis_generate_weights_c(int src_handle, int dst_handle, int method, int *matrix_handle);
