@@ -204,6 +204,8 @@ namespace detail {
 struct MeshGeom {
     index_t n_cells = 0, n_nodes = 0;
     HostDoubles h_xyz;                  // node xyz [3 * n_nodes] (libm, reference bits)
+    HostDoubles h_xyz_deg;              // radian meshes: xyz of the coordinates read as degrees
+    bool xyz_ready = false;             // h_xyz (and h_xyz_deg) filled by node_xyz_host
     DeviceBuffer<double> d_xyz;         // same on the device
     DeviceBuffer<double> d_area;        // [n_cells]
     DeviceBuffer<std::int32_t> d_flags; // bit0 degenerate, bit1 masked, bit2 area<=0
@@ -226,6 +228,10 @@ GridAxes unique_axes(const Mesh &m, hipStream_t s);
 bool regular_from_axes(const Mesh &m, const GridAxes &ax, MeshGeom &g);
 bool rectilinear_from_axes(const Mesh &m, const GridAxes &ax);
 
+/// Node xyz (and, for radian meshes, the degree reading) with host libm;
+/// touches only g.h_xyz, g.h_xyz_deg and g.xyz_ready, so it may run on a host
+/// thread while the GPU works on other members.
+void node_xyz_host(const Mesh &m, MeshGeom &g);
 void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s);
 
 /// 30-bit Morton code of every cell's centroid (normalised mean of its vertex

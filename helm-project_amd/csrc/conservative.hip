@@ -23,6 +23,8 @@
 // arbitration hands it to host libm.
 #include <hip/hip_runtime.h>
 
+#include <future>
+
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -448,6 +450,10 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     const index_t n_src = src.n_cells(), n_dst = dst.n_cells();
     if (n_src >= (index_t(1) << 31) || n_dst >= (index_t(1) << 31)) throw InvalidArgument("generate_conservative: more than 2^31 cells");
     MeshGeom sg, dg;
+    // host libm node xyz of the source runs beside the GPU grid detection,
+    // that of the destination beside the source's GPU cell pass
+    auto xyz_src = std::async(std::launch::async, [&] { node_xyz_host(src, sg); });
+    std::future<void> xyz_dst;
     GridAxes sax = unique_axes(src, s), dax = unique_axes(dst, s);
     const bool s_reg = regular_from_axes(src, sax, sg), d_reg = regular_from_axes(dst, dax, dg);
     if (s_reg && d_reg)
@@ -458,8 +464,11 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
                               "rectilinear path, which this library does not provide");
     if (cfg.line_type != LineType::GreatCircle) throw InvalidArgument("generate_conservative: only LineType::GreatCircle is supported");
     tr.mark("detect");
+    xyz_src.get();
+    xyz_dst = std::async(std::launch::async, [&] { node_xyz_host(dst, dg); });
     prepare_geometry(src, sg, s);
     tr.mark("geometry_src");
+    xyz_dst.get();
     prepare_geometry(dst, dg, s);
     tr.mark("geometry_dst");
 

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdint>
 #include <cstdlib>
+#include <cstring>
 #include <thread>
 #include <vector>
 
@@ -90,5 +91,47 @@ inline geo::V3 lonlat_to_xyz_host(double lon_rad, double lat_rad) {
     sin_cos(lon_rad, so, co);
     return geo::V3{cl * co, cl * so, sl};
 }
+
+/// Direct-mapped memo of sin_cos keyed by the argument's bits (the same bits
+/// in give the same bits out), for node coordinates that repeat: every row of
+/// a lat-lon grid shares one latitude and every column one longitude.
+class SinCosMemo {
+   public:
+    SinCosMemo() : e_(kSize) {
+        for (auto &x : e_) x.key = kEmpty;
+    }
+    void get(double x, double &s, double &c) {
+        std::uint64_t b;
+        std::memcpy(&b, &x, sizeof b);
+        if (b == kEmpty) {
+            sin_cos(x, s, c);
+            return;
+        }
+        Entry &e = e_[(b * 0x9E3779B97F4A7C15ull) >> (64 - kBits)];
+        if (e.key != b) {
+            sin_cos(x, e.s, e.c);
+            e.key = b;
+        }
+        s = e.s;
+        c = e.c;
+    }
+    /// lonlat_to_xyz_host through the memo
+    geo::V3 xyz(double lon_rad, double lat_rad) {
+        double sl, cl, so, co;
+        get(lat_rad, sl, cl);
+        get(lon_rad, so, co);
+        return geo::V3{cl * co, cl * so, sl};
+    }
+
+   private:
+    static constexpr int kBits = 12;
+    static constexpr std::size_t kSize = std::size_t(1) << kBits;
+    static constexpr std::uint64_t kEmpty = ~std::uint64_t(0);  // a NaN pattern, passed straight through
+    struct Entry {
+        std::uint64_t key;
+        double s, c;
+    };
+    std::vector<Entry> e_;
+};
 
 }  // namespace axis_hip

@@ -642,6 +642,31 @@ int host_classify(const double *xyz, const index_t *idx, int nv, double &fan_are
 
 }  // namespace
 
+void node_xyz_host(const Mesh &m, MeshGeom &g) {
+    const index_t n_nodes = m.n_nodes();
+    const bool deg = m.coord_system() == CoordinateSystem::SphericalDeg;
+    g.h_xyz.resize(3 * static_cast<std::size_t>(n_nodes));
+    g.h_xyz_deg.clear();
+    if (!deg) g.h_xyz_deg.resize(3 * static_cast<std::size_t>(n_nodes));
+    const double *lon = m.h_coords.data(), *lat = m.h_coords.data() + n_nodes;
+    parallel_range(n_nodes, [&](index_t b, index_t e) {
+        SinCosMemo memo;
+        for (index_t i = b; i < e; ++i) {
+            geo::V3 p = deg ? memo.xyz(lon[i] * kDeg2Rad, lat[i] * kDeg2Rad) : memo.xyz(lon[i], lat[i]);
+            g.h_xyz[3 * i] = p.x;
+            g.h_xyz[3 * i + 1] = p.y;
+            g.h_xyz[3 * i + 2] = p.z;
+            if (!deg) {
+                geo::V3 q = memo.xyz(lon[i] * kDeg2Rad, lat[i] * kDeg2Rad);
+                g.h_xyz_deg[3 * i] = q.x;
+                g.h_xyz_deg[3 * i + 1] = q.y;
+                g.h_xyz_deg[3 * i + 2] = q.z;
+            }
+        }
+    });
+    g.xyz_ready = true;
+}
+
 void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s) {
     const index_t n_nodes = m.n_nodes(), n_cells = m.n_cells();
     g.n_nodes = n_nodes;
@@ -649,24 +674,8 @@ void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s) {
     const bool deg = m.coord_system() == CoordinateSystem::SphericalDeg;
     PhaseTrace tr("geometry", s);
     // node xyz with host libm (extract_cell_polygon_spherical, weight_generator.cpp:462-470)
-    g.h_xyz.resize(3 * static_cast<std::size_t>(n_nodes));
-    HostDoubles h_xyz_deg;
-    if (!deg) h_xyz_deg.resize(3 * static_cast<std::size_t>(n_nodes));
-    const double *lon = m.h_coords.data(), *lat = m.h_coords.data() + n_nodes;
-    parallel_range(n_nodes, [&](index_t b, index_t e) {
-        for (index_t i = b; i < e; ++i) {
-            geo::V3 p = deg ? lonlat_to_xyz_host(lon[i] * kDeg2Rad, lat[i] * kDeg2Rad) : lonlat_to_xyz_host(lon[i], lat[i]);
-            g.h_xyz[3 * i] = p.x;
-            g.h_xyz[3 * i + 1] = p.y;
-            g.h_xyz[3 * i + 2] = p.z;
-            if (!deg) {
-                geo::V3 q = lonlat_to_xyz_host(lon[i] * kDeg2Rad, lat[i] * kDeg2Rad);
-                h_xyz_deg[3 * i] = q.x;
-                h_xyz_deg[3 * i + 1] = q.y;
-                h_xyz_deg[3 * i + 2] = q.z;
-            }
-        }
-    });
+    if (!g.xyz_ready) node_xyz_host(m, g);
+    const HostDoubles &h_xyz_deg = g.h_xyz_deg;
     tr.mark("host_sincos");
     g.d_xyz.resize(std::max<std::size_t>(g.h_xyz.size(), 3));
     g.d_xyz.upload(g.h_xyz.data(), g.h_xyz.size(), s);
