@@ -225,8 +225,10 @@ __global__ void compact_kernel(const double *overlap, const index_t *pos, index_
     ov_out[q] = overlap[p];
 }
 
+// sorted (row, col) keys -> col, overlap, row; row_ptr from the row boundaries
+// (rows without entries get the offset of the next entry)
 __global__ void unpack_sorted_kernel(const std::uint64_t *key, const std::uint32_t *perm, const double *ov_in, index_t nnz, int col_bits,
-                                     index_t *col, double *ov, std::int32_t *row32, index_t *row_count) {
+                                     index_t n_dst, index_t *col, double *ov, std::int32_t *row32, index_t *row_ptr) {
     index_t e = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= nnz) return;
     std::uint64_t k = key[e];
@@ -234,16 +236,18 @@ __global__ void unpack_sorted_kernel(const std::uint64_t *key, const std::uint32
     col[e] = static_cast<index_t>(k & ((std::uint64_t(1) << col_bits) - 1));
     ov[e] = ov_in[perm[e]];
     row32[e] = static_cast<std::int32_t>(r);
-    atomicAdd(reinterpret_cast<unsigned long long *>(row_count + r), 1ull);
+    const index_t rp = e == 0 ? -1 : static_cast<index_t>(key[e - 1] >> col_bits);
+    for (index_t q = rp + 1; q <= r; ++q) row_ptr[q] = e;
+    if (e == nnz - 1)
+        for (index_t q = r + 1; q <= n_dst; ++q) row_ptr[q] = nnz;
 }
 
 // per row: w = ov / area_dst (>= 0), frac_b = Σ ov / area_dst in column order,
 // FracArea division by the unclamped frac_b, clamp (weight_generator.cpp:3682-3724)
-__global__ void row_weights_kernel(const index_t *row_ptr, index_t n_dst, const double *area_dst, const double *ov, double *w, double *frac_b,
-                                   int frac_area, int *max_row) {
-    index_t r = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) row_weights_kernel(const index_t *row_ptr, index_t n_dst, const double *area_dst, const double *ov, double *w,
+                                                          double *frac_b, int frac_area, int *max_row) {
     int len = 0;
-    if (r < n_dst) {
+    for (index_t r = static_cast<index_t>(blockIdx.x) * 256 + threadIdx.x; r < n_dst; r += static_cast<index_t>(gridDim.x) * 256) {
         const index_t b = row_ptr[r], e = row_ptr[r + 1];
         const double a = area_dst[r];
         double fb = 0.0;
@@ -255,10 +259,17 @@ __global__ void row_weights_kernel(const index_t *row_ptr, index_t n_dst, const 
         if (frac_area && fb > 0.0)
             for (index_t k = b; k < e; ++k) w[k] /= fb;
         frac_b[r] = fmin(fb, 1.0);
-        len = static_cast<int>(e - b);
+        len = max(len, static_cast<int>(e - b));
     }
-    for (int d = 32; d >= 1; d >>= 1) len = max(len, __shfl_xor(len, d, 64));  // one atomic per wave
-    if ((threadIdx.x & 63) == 0 && len > 0) atomicMax(max_row, len);
+    // longest row: wave, then workgroup reduction, one atomic per workgroup
+    for (int d = 32; d >= 1; d >>= 1) len = max(len, __shfl_xor(len, d, 64));
+    __shared__ int part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = len;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        len = max(max(part[0], part[1]), max(part[2], part[3]));
+        if (len > 0) atomicMax(max_row, len);
+    }
 }
 
 __global__ void transpose_keys_kernel(const index_t *col, const std::int32_t *row32, index_t nnz, int row_bits, std::uint64_t *key,
@@ -369,19 +380,24 @@ double host_pair_overlap(const HostSide &S, const HostSide &D, index_t j, index_
 
 // WeightGenerator::generate's longitude-disjoint check, weight_generator.cpp:1898-1992
 // min / max of the cells' mean vertex longitude as order-preserving integer
-// keys: wave reduction, then one atomic per wave
-__global__ void lon_range_kernel(const index_t *off, const index_t *ind, const double *lon, index_t n_cells, unsigned long long *mn,
-                                 unsigned long long *mx) {
-    index_t c = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+// keys: grid-stride over the cells, wave then workgroup reduction, one atomic
+// per workgroup (per-wave atomics on two addresses serialise in L2)
+constexpr int kRangeBlock = 256;
+
+__global__ void __launch_bounds__(kRangeBlock) lon_range_kernel(const index_t *off, const index_t *ind, const double *lon, index_t n_cells,
+                                                                unsigned long long *mn, unsigned long long *mx) {
     unsigned long long kmin = ~0ull, kmax = 0ull;
-    if (c < n_cells) {
+    for (index_t c = static_cast<index_t>(blockIdx.x) * kRangeBlock + threadIdx.x; c < n_cells;
+         c += static_cast<index_t>(gridDim.x) * kRangeBlock) {
         index_t b = off[c], e = off[c + 1];
         if (e > b) {
             double s = 0.0;
             for (index_t k = b; k < e; ++k) s += lon[ind[k]];
             double cx = s / static_cast<double>(e - b);
             std::uint64_t bits = static_cast<std::uint64_t>(__double_as_longlong(cx));
-            kmin = kmax = (bits >> 63) ? ~bits : (bits | 0x8000000000000000ull);
+            const unsigned long long key = (bits >> 63) ? ~bits : (bits | 0x8000000000000000ull);
+            kmin = key < kmin ? key : kmin;
+            kmax = key > kmax ? key : kmax;
         }
     }
     for (int d = 32; d >= 1; d >>= 1) {
@@ -389,7 +405,17 @@ __global__ void lon_range_kernel(const index_t *off, const index_t *ind, const d
         kmin = a < kmin ? a : kmin;
         kmax = z > kmax ? z : kmax;
     }
+    __shared__ unsigned long long part[2][kRangeBlock / 64];
     if ((threadIdx.x & 63) == 0) {
+        part[0][threadIdx.x / 64] = kmin;
+        part[1][threadIdx.x / 64] = kmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kRangeBlock / 64; ++w) {
+            kmin = part[0][w] < kmin ? part[0][w] : kmin;
+            kmax = part[1][w] > kmax ? part[1][w] : kmax;
+        }
         if (kmin != ~0ull) atomicMin(mn, kmin);
         if (kmax != 0ull) atomicMax(mx, kmax);
     }
@@ -407,8 +433,8 @@ bool lon_range(const Mesh &m, double &lo, double &hi, hipStream_t s) {
     unsigned long long init[2] = {~0ull, 0ull};
     AXIS_HIP_CHECK(hipMemcpyAsync(r.get(), init, sizeof init, hipMemcpyHostToDevice, s));
     if (m.n_cells() > 0) {
-        lon_range_kernel<<<grid_for(m.n_cells(), 256), 256, 0, s>>>(m.d_offsets.get(), m.d_indices.get(), m.d_coords.get(), m.n_cells(), r.get(),
-                                                                    r.get() + 1);
+        const unsigned grid = static_cast<unsigned>(std::min<index_t>((m.n_cells() + kRangeBlock - 1) / kRangeBlock, 1024));
+        lon_range_kernel<<<grid, kRangeBlock, 0, s>>>(m.d_offsets.get(), m.d_indices.get(), m.d_coords.get(), m.n_cells(), r.get(), r.get() + 1);
         AXIS_LAUNCH_CHECK();
     }
     unsigned long long h[2];
@@ -681,19 +707,19 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     M->area_b.resize(std::max<index_t>(n_dst, 1));
     DeviceBuffer<double> ov(nz1);
     DeviceBuffer<std::int32_t> row32(nz1);
-    DeviceBuffer<index_t> row_count(std::max<index_t>(n_dst, 1));
-    AXIS_HIP_CHECK(hipMemsetAsync(row_count.get(), 0, sizeof(index_t) * std::max<index_t>(n_dst, 1), s));
     if (nnz > 0) {
-        unpack_sorted_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(key.get(), perm.get(), ov_c.get(), nnz, col_bits, M->col.get(), ov.get(), row32.get(),
-                                                                row_count.get());
+        unpack_sorted_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(key.get(), perm.get(), ov_c.get(), nnz, col_bits, n_dst, M->col.get(), ov.get(),
+                                                                row32.get(), M->row_ptr.get());
         AXIS_LAUNCH_CHECK();
+    } else {
+        AXIS_HIP_CHECK(hipMemsetAsync(M->row_ptr.get(), 0, sizeof(index_t) * (n_dst + 1), s));
     }
-    exclusive_scan_i64(row_count.get(), M->row_ptr.get(), n_dst, s);
     DeviceBuffer<int> max_row(1);
     AXIS_HIP_CHECK(hipMemsetAsync(max_row.get(), 0, sizeof(int), s));
     if (n_dst > 0) {
-        row_weights_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(M->row_ptr.get(), n_dst, dg.d_area.get(), ov.get(), M->val.get(), M->frac_b.get(),
-                                                               cfg.norm_type == NormType::FracArea ? 1 : 0, max_row.get());
+        const unsigned grid = static_cast<unsigned>(std::min<index_t>((n_dst + 255) / 256, 2048));
+        row_weights_kernel<<<grid, 256, 0, s>>>(M->row_ptr.get(), n_dst, dg.d_area.get(), ov.get(), M->val.get(), M->frac_b.get(),
+                                                cfg.norm_type == NormType::FracArea ? 1 : 0, max_row.get());
         AXIS_LAUNCH_CHECK();
     }
     AXIS_HIP_CHECK(hipMemsetAsync(M->frac_a.get(), 0, sizeof(double) * std::max<index_t>(n_src, 1), s));

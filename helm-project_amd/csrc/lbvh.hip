@@ -14,7 +14,8 @@
 //   4. bottom-up box refit with one atomic arrival counter per internal node,
 //   5. one stack traversal per query that keeps up to 32 hits; the queries
 //      with more (rows around the poles of a lat-lon source meet thousands)
-//      are counted and filled by one wave each, breadth-first.
+//      are counted and filled by one wave (or workgroup) each,
+//      breadth-first; nodes are packed with both child boxes.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -181,18 +182,51 @@ __global__ void __launch_bounds__(kTopBlock) refit_top_kernel(const float *boxes
     }
 }
 
+// Traversal layout: one 64-byte record per internal node holding both child
+// boxes and child references (internal node index, or kLeaf | source cell),
+// so that visiting a node is one dependent load and testing a leaf none.
+struct __align__(16) Node {
+    float box[2][6];  // per child: lo xyz, hi xyz
+    std::uint32_t child[2];
+    std::uint32_t pad[2];
+};
+
+__global__ void pack_kernel(const float *boxes, const std::uint32_t *ids, int n, const std::uint32_t *left, const std::uint32_t *right,
+                            const float *node_box, Node *nodes) {
+    const int p = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n - 1) return;
+    Node x;
+    const std::uint32_t c[2] = {left[p], right[p]};
+    for (int h = 0; h < 2; ++h) {
+        const bool leaf = (c[h] & kLeaf) != 0;
+        const std::uint32_t id = leaf ? ids[c[h] & ~kLeaf] : c[h];
+        const FBox b = leaf ? load_box(boxes, id) : load_box(node_box, c[h]);
+        for (int a = 0; a < 3; ++a) {
+            x.box[h][a] = b.lo[a];
+            x.box[h][3 + a] = b.hi[a];
+        }
+        x.child[h] = leaf ? (kLeaf | id) : id;
+    }
+    x.pad[0] = x.pad[1] = 0;
+    nodes[p] = x;
+}
+
+__device__ inline bool overlap_child(const FBox &q, const Node &x, int h) {
+    const float *b = x.box[h];
+    return !(q.hi[0] < b[0] || b[3] < q.lo[0] || q.hi[1] < b[1] || b[4] < q.lo[1] || q.hi[2] < b[2] || b[5] < q.lo[2]);
+}
+
 // Per-lane stack traversal of query j (rows[qi] when a row list is given).
 //   kMode 0: count all hits;  kMode 1: write all hits at offsets[j];
 //   kMode 2: first pass over every query -- keep up to kCap hits in
 //            tmp[j * kCap ...]; a query that meets more stops there, is
-//            appended to `over` and finished by the wave kernel below.
+//            appended to `over` and finished by the workgroup kernel below.
 // Stack overflow (never expected: depth <= 62 with 30-bit codes + position
 // tie-break) sets *err so the host reports it instead of returning a short list.
 constexpr int kCap = 32;
 
 template <int kMode>
-__global__ void __launch_bounds__(kBlock) query_kernel(const float *src_boxes, const std::uint32_t *ids, int n_src, const std::uint32_t *left,
-                                                       const std::uint32_t *right, const float *node_box, const float *dst_boxes, std::int64_t n_q,
+__global__ void __launch_bounds__(kBlock) query_kernel(const Node *nodes, const float *src_boxes, int n_src, const float *dst_boxes, std::int64_t n_q,
                                                        const std::int32_t *rows, std::int64_t *count, const std::int64_t *offsets, std::int32_t *out,
                                                        std::int32_t *over, int *n_over, int *err) {
     const std::int64_t qi = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -203,34 +237,47 @@ __global__ void __launch_bounds__(kBlock) query_kernel(const float *src_boxes, c
     const std::int64_t w = kMode == 1 ? offsets[j] : (kMode == 2 ? j * kCap : 0);
     bool full = false;
     if (n_src == 1) {
-        if (overlap(q, load_box(src_boxes, ids[0]))) {
-            if (kMode != 0) out[w] = static_cast<std::int32_t>(ids[0]);
+        if (overlap(q, load_box(src_boxes, 0))) {
+            if (kMode != 0) out[w] = 0;
             ++cnt;
         }
     } else if (n_src > 1) {
+        // descend into an overlapping child directly, stacking the other one
+        // when both overlap; pop only at dead ends
         std::uint32_t stack[kStack];
         int sp = 0;
-        stack[sp++] = 0;
-        while (sp > 0 && !full) {
-            std::uint32_t nd = stack[--sp];
-            std::uint32_t c[2] = {left[nd], right[nd]};
+        std::uint32_t nd = 0;
+        while (true) {
+            const Node x = nodes[nd];
+            bool inner[2] = {false, false};
             for (int h = 0; h < 2; ++h) {
-                if (c[h] & kLeaf) {
-                    std::uint32_t id = ids[c[h] & ~kLeaf];
-                    if (overlap(q, load_box(src_boxes, id))) {
-                        if (kMode == 2 && cnt == kCap) {
-                            full = true;
-                            break;
-                        }
-                        if (kMode != 0) out[w + cnt] = static_cast<std::int32_t>(id);
-                        ++cnt;
-                    }
-                } else if (overlap(q, load_box(node_box, c[h]))) {
-                    if (sp < kStack)
-                        stack[sp++] = c[h];
-                    else
-                        atomicOr(err, 1);
+                if (!overlap_child(q, x, h)) continue;
+                const std::uint32_t c = x.child[h];
+                if (!(c & kLeaf)) {
+                    inner[h] = true;
+                    continue;
                 }
+                if (kMode == 2 && cnt == kCap) {
+                    full = true;
+                    break;
+                }
+                if (kMode != 0) out[w + cnt] = static_cast<std::int32_t>(c & ~kLeaf);
+                ++cnt;
+            }
+            if (full) break;
+            if (inner[0] && inner[1]) {
+                if (sp < kStack)
+                    stack[sp++] = x.child[1];
+                else
+                    atomicOr(err, 1);
+                nd = x.child[0];
+            } else if (inner[0]) {
+                nd = x.child[0];
+            } else if (inner[1]) {
+                nd = x.child[1];
+            } else {
+                if (sp == 0) break;
+                nd = stack[--sp];
             }
         }
     }
@@ -242,72 +289,85 @@ __global__ void __launch_bounds__(kBlock) query_kernel(const float *src_boxes, c
     if (kMode != 1) count[j] = cnt;
 }
 
-// One wave per query with more than kCap hits: breadth-first over the tree,
-// each lane taking one frontier node per step, hits and next-level nodes
-// placed by ballot prefix counts (so the hit order is fixed and the count and
-// fill passes agree).  A level wider than kFront marks the query slow: it is
-// then counted and filled by the per-lane kernel instead.
-constexpr int kFront = 4096;
-
-template <bool kFill>
-__global__ void __launch_bounds__(kWave) query_wave_kernel(const float *src_boxes, const std::uint32_t *ids, const std::uint32_t *left,
-                                                           const std::uint32_t *right, const float *node_box, const float *dst_boxes,
-                                                           const std::int32_t *rows, std::int64_t *count, const std::int64_t *offsets,
-                                                           std::int32_t *out, std::uint8_t *slow, std::int32_t *slow_rows, int *n_slow) {
-    __shared__ std::uint32_t front[2][kFront];
-    const int lane = static_cast<int>(threadIdx.x);
+// Queries with more than kCap hits: one workgroup of kThreads per query,
+// breadth-first over the tree, each thread taking one frontier node per step;
+// hits and next-level nodes are placed by ballot prefix counts within each
+// wave plus the counts of the lower waves, so the hit order is fixed and the
+// count and fill passes agree.  Rows start at level 0 (one wave, a 2048-node
+// frontier); a level wider than the frontier moves a row up one level (it is
+// counted and filled there instead): level 1 is a 256-thread workgroup with a
+// 6144-node frontier, level 2 the per-lane kernel.
+template <bool kFill, int kThreads, int kFrontCap, int kLevel>
+__global__ void __launch_bounds__(kThreads) query_long_kernel(const Node *nodes, const float *dst_boxes, const std::int32_t *rows,
+                                                              std::int64_t *count, const std::int64_t *offsets, std::int32_t *out,
+                                                              std::uint8_t *level, std::int32_t *next_rows, int *n_next) {
+    constexpr int kWaves = kThreads / kWave;
+    __shared__ std::uint32_t front[2][kFrontCap];
+    __shared__ int wsum[2][kWaves];
+    const int tid = static_cast<int>(threadIdx.x), lane = tid & (kWave - 1), wv = tid / kWave;
     const std::int64_t j = rows[blockIdx.x];
-    if (kFill && slow[j]) return;
+    if (kFill && level[j] != kLevel) return;
     const FBox q = load_box(dst_boxes, j);
     const std::uint64_t below = (std::uint64_t(1) << lane) - 1;
     const std::int64_t w = kFill ? offsets[j] : 0;
     std::int64_t cnt = 0;
-    if (lane == 0) front[0][0] = 0;
+    if (tid == 0) front[0][0] = 0;
     __syncthreads();
     int cur = 0, m = 1;
     bool wide = false;
     while (m > 0) {
         int nm = 0;
-        for (int i0 = 0; i0 < m; i0 += kWave) {
-            const bool act = i0 + lane < m;
-            const std::uint32_t nd = act ? front[cur][i0 + lane] : 0u;
-            const std::uint32_t c[2] = {act ? left[nd] : 0u, act ? right[nd] : 0u};
+        for (int i0 = 0; i0 < m; i0 += kThreads) {
+            const bool act = i0 + tid < m;
+            Node x;
+            if (act) x = nodes[front[cur][i0 + tid]];
             for (int h = 0; h < 2; ++h) {
-                bool hit = false, push = false;
-                std::uint32_t id = 0;
-                if (act) {
-                    if (c[h] & kLeaf) {
-                        id = ids[c[h] & ~kLeaf];
-                        hit = overlap(q, load_box(src_boxes, id));
-                    } else {
-                        push = overlap(q, load_box(node_box, c[h]));
+                const bool ov = act && overlap_child(q, x, h);
+                const std::uint32_t c = act ? x.child[h] : 0u;
+                const bool hit = ov && (c & kLeaf), push = ov && !(c & kLeaf);
+                const std::uint64_t mh = __ballot(hit), mp = __ballot(push);
+                int hb = 0, pb = 0, ht = __popcll(mh), pt = __popcll(mp);
+                if (kWaves > 1) {
+                    if (lane == 0) {
+                        wsum[0][wv] = ht;
+                        wsum[1][wv] = pt;
+                    }
+                    __syncthreads();
+                    ht = pt = 0;
+                    for (int v = 0; v < kWaves; ++v) {
+                        if (v < wv) {
+                            hb += wsum[0][v];
+                            pb += wsum[1][v];
+                        }
+                        ht += wsum[0][v];
+                        pt += wsum[1][v];
                     }
                 }
-                const std::uint64_t mh = __ballot(hit);
-                if (kFill && hit) out[w + cnt + __popcll(mh & below)] = static_cast<std::int32_t>(id);
-                cnt += __popcll(mh);
-                const std::uint64_t mp = __ballot(push);
-                const int at = nm + __popcll(mp & below);
-                if (push && at < kFront) front[cur ^ 1][at] = c[h];
-                nm += __popcll(mp);
+                if (kFill && hit) out[w + cnt + hb + __popcll(mh & below)] = static_cast<std::int32_t>(c & ~kLeaf);
+                const int at = nm + pb + __popcll(mp & below);
+                if (push && at < kFrontCap) front[cur ^ 1][at] = c;
+                cnt += ht;
+                nm += pt;
+                __syncthreads();
             }
         }
-        __syncthreads();
-        if (nm > kFront) {
+        if (nm > kFrontCap) {
             wide = true;
             break;
         }
         cur ^= 1;
         m = nm;
     }
-    if (kFill || lane != 0) return;
+    if (kFill || tid != 0) return;
     if (wide) {
-        slow[j] = 1;
-        slow_rows[atomicAdd(n_slow, 1)] = static_cast<std::int32_t>(j);
+        level[j] = kLevel + 1;
+        next_rows[atomicAdd(n_next, 1)] = static_cast<std::int32_t>(j);
     } else {
         count[j] = cnt;
     }
 }
+
+constexpr int kFront0 = 2048, kFront1 = 6144, kBlock1 = 256;
 
 // first-pass hits of the queries with at most kCap of them -> candidate CSR
 __global__ void compact_hits_kernel(const std::int32_t *tmp, const std::int64_t *count, const std::int64_t *offsets, std::int64_t n,
@@ -356,40 +416,48 @@ void lbvh_candidates(const float *src_boxes, index_t n_src, const float *dst_box
                                                  frontier.get(), n_frontier.get());
         AXIS_LAUNCH_CHECK();
     }
+    DeviceBuffer<Node> nodes(std::max(n - 1, 1));
+    if (n > 1) {
+        pack_kernel<<<grid_for(n - 1, kBlock), kBlock, 0, s>>>(src_boxes, ids.get(), n, left.get(), right.get(), node_box.get(), nodes.get());
+        AXIS_LAUNCH_CHECK();
+    }
     tr.mark("build_refit");
     const index_t nd1 = std::max<index_t>(n_dst, 1);
     DeviceBuffer<index_t> count(nd1);
-    DeviceBuffer<std::int32_t> tmp(static_cast<std::size_t>(nd1) * kCap), over(nd1), slow_rows(nd1);
-    DeviceBuffer<std::uint8_t> slow(nd1);
-    DeviceBuffer<int> n_list(2);  // queries over kCap, slow queries
-    AXIS_HIP_CHECK(hipMemsetAsync(n_list.get(), 0, 2 * sizeof(int), s));
+    DeviceBuffer<std::int32_t> tmp(static_cast<std::size_t>(nd1) * kCap), over(nd1), rows1(nd1), rows2(nd1);
+    DeviceBuffer<std::uint8_t> level(nd1);
+    DeviceBuffer<int> n_list(3);  // queries over kCap, moved to level 1, moved to level 2
+    AXIS_HIP_CHECK(hipMemsetAsync(n_list.get(), 0, 3 * sizeof(int), s));
     if (n_dst > 0) {
-        query_kernel<2><<<grid_for(n_dst, kBlock), kBlock, 0, s>>>(src_boxes, ids.get(), n, left.get(), right.get(), node_box.get(), dst_boxes,
-                                                                  n_dst, nullptr, count.get(), nullptr, tmp.get(), over.get(), n_list.get(), err.get());
+        query_kernel<2><<<grid_for(n_dst, kBlock), kBlock, 0, s>>>(nodes.get(), src_boxes, n, dst_boxes, n_dst, nullptr, count.get(), nullptr,
+                                                                  tmp.get(), over.get(), n_list.get(), err.get());
         AXIS_LAUNCH_CHECK();
     }
-    int h_list[2] = {0, 0}, h_err = 0;
+    int h_list[3] = {0, 0, 0}, h_err = 0;
     AXIS_HIP_CHECK(hipMemcpyAsync(h_list, n_list.get(), sizeof h_list, hipMemcpyDeviceToHost, s));
     AXIS_HIP_CHECK(hipMemcpyAsync(&h_err, err.get(), sizeof(int), hipMemcpyDeviceToHost, s));
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
     if (h_err) throw RuntimeError("LBVH traversal stack overflow");
-    const int n_over = h_list[0];
-    if (n_over > 0) {
-        AXIS_HIP_CHECK(hipMemsetAsync(slow.get(), 0, static_cast<std::size_t>(nd1), s));
-        query_wave_kernel<false><<<static_cast<unsigned>(n_over), kWave, 0, s>>>(src_boxes, ids.get(), left.get(), right.get(), node_box.get(),
-                                                                                 dst_boxes, over.get(), count.get(), nullptr, nullptr, slow.get(),
-                                                                                 slow_rows.get(), n_list.get() + 1);
+    if (h_list[0] > 0) {
+        AXIS_HIP_CHECK(hipMemsetAsync(level.get(), 0, static_cast<std::size_t>(nd1), s));
+        query_long_kernel<false, kWave, kFront0, 0><<<static_cast<unsigned>(h_list[0]), kWave, 0, s>>>(
+            nodes.get(), dst_boxes, over.get(), count.get(), nullptr, nullptr, level.get(), rows1.get(), n_list.get() + 1);
         AXIS_LAUNCH_CHECK();
         AXIS_HIP_CHECK(hipMemcpyAsync(h_list, n_list.get(), sizeof h_list, hipMemcpyDeviceToHost, s));
         AXIS_HIP_CHECK(hipStreamSynchronize(s));
         if (h_list[1] > 0) {
-            query_kernel<0><<<grid_for(h_list[1], kBlock), kBlock, 0, s>>>(src_boxes, ids.get(), n, left.get(), right.get(), node_box.get(),
-                                                                          dst_boxes, h_list[1], slow_rows.get(), count.get(), nullptr, nullptr,
-                                                                          nullptr, nullptr, err.get());
+            query_long_kernel<false, kBlock1, kFront1, 1><<<static_cast<unsigned>(h_list[1]), kBlock1, 0, s>>>(
+                nodes.get(), dst_boxes, rows1.get(), count.get(), nullptr, nullptr, level.get(), rows2.get(), n_list.get() + 2);
             AXIS_LAUNCH_CHECK();
+            AXIS_HIP_CHECK(hipMemcpyAsync(h_list, n_list.get(), sizeof h_list, hipMemcpyDeviceToHost, s));
+            AXIS_HIP_CHECK(hipStreamSynchronize(s));
+            if (h_list[2] > 0) {
+                query_kernel<0><<<grid_for(h_list[2], kBlock), kBlock, 0, s>>>(nodes.get(), src_boxes, n, dst_boxes, h_list[2], rows2.get(),
+                                                                              count.get(), nullptr, nullptr, nullptr, nullptr, err.get());
+                AXIS_LAUNCH_CHECK();
+            }
         }
     }
-    const int n_slow = h_list[1];
     exclusive_scan_i64(count.get(), cand_off.get(), n_dst, s);
     index_t total = 0;
     AXIS_HIP_CHECK(hipMemcpyAsync(&total, cand_off.get() + n_dst, sizeof(index_t), hipMemcpyDeviceToHost, s));
@@ -403,18 +471,16 @@ void lbvh_candidates(const float *src_boxes, index_t n_src, const float *dst_box
     if (n_dst > 0 && total > 0) {
         compact_hits_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(tmp.get(), count.get(), cand_off.get(), n_dst, cand_src.get());
         AXIS_LAUNCH_CHECK();
-        if (n_over > 0) {
-            query_wave_kernel<true><<<static_cast<unsigned>(n_over), kWave, 0, s>>>(src_boxes, ids.get(), left.get(), right.get(), node_box.get(),
-                                                                                    dst_boxes, over.get(), nullptr, cand_off.get(), cand_src.get(),
-                                                                                    slow.get(), nullptr, nullptr);
-            AXIS_LAUNCH_CHECK();
-        }
-        if (n_slow > 0) {
-            query_kernel<1><<<grid_for(n_slow, kBlock), kBlock, 0, s>>>(src_boxes, ids.get(), n, left.get(), right.get(), node_box.get(), dst_boxes,
-                                                                       n_slow, slow_rows.get(), nullptr, cand_off.get(), cand_src.get(), nullptr,
-                                                                       nullptr, err.get());
-            AXIS_LAUNCH_CHECK();
-        }
+        if (h_list[0] > 0)
+            query_long_kernel<true, kWave, kFront0, 0><<<static_cast<unsigned>(h_list[0]), kWave, 0, s>>>(
+                nodes.get(), dst_boxes, over.get(), nullptr, cand_off.get(), cand_src.get(), level.get(), nullptr, nullptr);
+        if (h_list[1] > 0)
+            query_long_kernel<true, kBlock1, kFront1, 1><<<static_cast<unsigned>(h_list[1]), kBlock1, 0, s>>>(
+                nodes.get(), dst_boxes, rows1.get(), nullptr, cand_off.get(), cand_src.get(), level.get(), nullptr, nullptr);
+        if (h_list[2] > 0)
+            query_kernel<1><<<grid_for(h_list[2], kBlock), kBlock, 0, s>>>(nodes.get(), src_boxes, n, dst_boxes, h_list[2], rows2.get(), nullptr,
+                                                                          cand_off.get(), cand_src.get(), nullptr, nullptr, err.get());
+        AXIS_LAUNCH_CHECK();
     }
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
     tr.mark("query_fill");
