@@ -92,7 +92,7 @@ __device__ inline int cell_nverts(const Side &m, index_t c) {
 template <int CAP>
 __global__ void __launch_bounds__(kWave) overlap_kernel(index_t p0, index_t p1, const std::int32_t *pair_row, const std::int32_t *pair_src,
                                                         Side src, Side dst, double *overlap, std::int32_t *arb, int *n_arb, int *n_tie) {
-    __shared__ double buf[2][3][CAP][kWave];
+    __shared__ double buf[3][CAP][kWave];  // one lane-private polygon per lane, clipped in place
     const int lane = threadIdx.x;
     const index_t p = p0 + static_cast<index_t>(blockIdx.x) * kWave + lane;
     if (p >= p1) return;
@@ -114,15 +114,14 @@ __global__ void __launch_bounds__(kWave) overlap_kernel(index_t p0, index_t p1, 
                 to_host = true;
                 atomicAdd(n_tie, 1);
             }
-            PolyRef A{&buf[0][0][0][lane], &buf[0][1][0][lane], &buf[0][2][0][lane], kWave};
-            PolyRef B{&buf[1][0][0][lane], &buf[1][1][0][lane], &buf[1][2][0][lane], kWave};
+            PolyRef A{&buf[0][0][lane], &buf[1][0][lane], &buf[2][0][lane], kWave};
             const int ns = cell_nverts(src, i);
             bool overflow = ns > CAP;
             if (!overflow) {
                 for (int k = 0; k < ns; ++k) A.set(k, cell_vertex(src, i, k));
                 const int nc = cell_nverts(dst, j);
                 auto clipv = [&](int k) { return cell_vertex(dst, j, k); };
-                int n = geo::clip_polygon(A, ns, B, clipv, nc, CAP, &overflow);
+                int n = geo::clip_polygon_inplace(A, ns, clipv, nc, CAP, &overflow);
                 if (!overflow && n >= 3) {
                     double abs_sum;
                     bool zero_num;

@@ -134,6 +134,53 @@ __host__ __device__ inline int clip_polygon(PolyRef a, int na, PolyRef b, ClipVe
     return na;
 }
 
+/// clip_polygon with one buffer: the walk over the subject's vertices reads
+/// one vertex ahead, and each step writes its (at most two) results at
+/// positions the walk has already read, which holds whenever the subject
+/// enters the clip half-space at most once per edge -- always for the convex
+/// cells a valid mesh has.  Anything else (a write that would land ahead of
+/// the walk, or more than `cap` vertices) sets *overflow, for the caller to
+/// decide the pair with clip_polygon on the host.  The arithmetic and the
+/// order of the results are those of clip_polygon.
+template <class ClipVertex>
+__host__ __device__ inline int clip_polygon_inplace(PolyRef a, int na, ClipVertex clip_vertex, int nc, int cap, bool *overflow) {
+    if (na < 3 || nc < 3) return 0;
+    for (int ci = 0; ci < nc; ++ci) {
+        const V3 ea = clip_vertex(ci);
+        const V3 eb = clip_vertex(ci + 1 == nc ? 0 : ci + 1);
+        V3 normal = cross(ea, eb);
+        if (length_sq(normal) < 1.0e-30) continue;
+        int nb = 0;
+        constexpr double eps = -1.0e-14;
+        V3 prev = a.get(na - 1);
+        double d_prev = dot(normal, prev);
+        V3 next = a.get(0);
+        for (int i = 0; i < na; ++i) {
+            const V3 cur = next;
+            if (i + 1 < na) next = a.get(i + 1);
+            const double d_cur = dot(normal, cur);
+            const bool cur_in = d_cur >= eps;
+            const bool prev_in = d_prev >= eps;
+            const int emit = cur_in ? (prev_in ? 1 : 2) : (prev_in ? 1 : 0);
+            if (emit > 0 && (nb + emit - 1 > i + 1 || nb + emit > cap)) {
+                *overflow = true;
+                return 0;
+            }
+            if (cur_in) {
+                if (!prev_in) a.set(nb++, gc_intersection(prev, cur, ea, eb));
+                a.set(nb++, cur);
+            } else if (prev_in) {
+                a.set(nb++, gc_intersection(prev, cur, ea, eb));
+            }
+            prev = cur;
+            d_prev = d_cur;
+        }
+        na = nb;
+        if (na < 3) return 0;
+    }
+    return na;
+}
+
 /// SphericalPolygon::area (spherical_clipper.hpp:112-137).  Also returns the
 /// sum of |terms| and whether every numerator was exactly zero, so a caller
 /// can detect cancellation that a different atan2 could resolve differently.
