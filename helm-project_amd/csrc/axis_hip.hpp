@@ -203,8 +203,8 @@ namespace detail {
 /// Per-mesh data derived inside one generate() call (never cached across calls).
 struct MeshGeom {
     index_t n_cells = 0, n_nodes = 0;
-    HostDoubles h_xyz;                  // node xyz [3 * n_nodes] (libm, reference bits)
-    HostDoubles h_xyz_deg;              // radian meshes: xyz of the coordinates read as degrees
+    PinnedArray<double> h_xyz;          // node xyz [3 * n_nodes] (libm, reference bits)
+    PinnedArray<double> h_xyz_deg;      // radian meshes: xyz of the coordinates read as degrees
     bool xyz_ready = false;             // h_xyz (and h_xyz_deg) filled by node_xyz_host
     DeviceBuffer<double> d_xyz;         // same on the device
     DeviceBuffer<double> d_area;        // [n_cells]
@@ -246,6 +246,15 @@ void radix_sort_pairs_u64(std::uint64_t *keys, std::uint32_t *vals, std::size_t 
 
 // LBVH broad phase (lbvh.hip): candidate CSR of src cells whose float boxes
 // intersect each dst cell's box.
+/// Source-side LBVH (packed internal nodes); the source boxes must outlive it.
+struct LbvhTree {
+    int n = 0;
+    const float *src_boxes = nullptr;
+    DeviceBuffer<std::uint8_t> nodes;
+};
+void lbvh_build(const float *src_boxes, index_t n_src, LbvhTree &t, hipStream_t s);
+void lbvh_query(const LbvhTree &t, const float *dst_boxes, index_t n_dst, DeviceBuffer<index_t> &cand_off, DeviceBuffer<std::int32_t> &cand_src,
+                hipStream_t s);
 void lbvh_candidates(const float *src_boxes, index_t n_src, const float *dst_boxes, index_t n_dst, DeviceBuffer<index_t> &cand_off,
                      DeviceBuffer<std::int32_t> &cand_src, hipStream_t s);
 

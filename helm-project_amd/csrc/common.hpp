@@ -86,6 +86,57 @@ class BlockCache {
 };
 
 /// Owning device allocation from the block cache, movable, not copyable.
+/// Process-wide cache of pinned host blocks (hipHostMalloc pins pages, which
+/// costs milliseconds for the tens of MB a mesh's node coordinates take).
+/// Same policy as BlockCache: released blocks wait for one device
+/// synchronisation before reuse, so an upload queued from a block has finished
+/// when the block is handed out again.
+class PinnedCache {
+   public:
+    static PinnedCache &get();
+    void *alloc(std::size_t bytes);
+    void release(void *p, std::size_t bytes);
+};
+
+/// Uninitialised pinned host array (resize discards the contents).
+template <class T>
+class PinnedArray {
+   public:
+    PinnedArray() = default;
+    ~PinnedArray() { clear(); }
+    PinnedArray(const PinnedArray &) = delete;
+    PinnedArray &operator=(const PinnedArray &) = delete;
+    PinnedArray(PinnedArray &&o) noexcept : p_(std::exchange(o.p_, nullptr)), n_(std::exchange(o.n_, 0)) {}
+    PinnedArray &operator=(PinnedArray &&o) noexcept {
+        if (this != &o) {
+            clear();
+            p_ = std::exchange(o.p_, nullptr);
+            n_ = std::exchange(o.n_, 0);
+        }
+        return *this;
+    }
+    void resize(std::size_t n) {
+        clear();
+        if (n) p_ = static_cast<T *>(PinnedCache::get().alloc(n * sizeof(T)));
+        n_ = n;
+    }
+    void clear() {
+        if (p_) PinnedCache::get().release(p_, n_ * sizeof(T));
+        p_ = nullptr;
+        n_ = 0;
+    }
+    T *data() { return p_; }
+    const T *data() const { return p_; }
+    T &operator[](std::size_t i) { return p_[i]; }
+    const T &operator[](std::size_t i) const { return p_[i]; }
+    std::size_t size() const { return n_; }
+    bool empty() const { return n_ == 0; }
+
+   private:
+    T *p_ = nullptr;
+    std::size_t n_ = 0;
+};
+
 template <class T>
 class DeviceBuffer {
    public:

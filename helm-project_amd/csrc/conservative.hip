@@ -475,10 +475,20 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     const index_t n_src = src.n_cells(), n_dst = dst.n_cells();
     if (n_src >= (index_t(1) << 31) || n_dst >= (index_t(1) << 31)) throw InvalidArgument("generate_conservative: more than 2^31 cells");
     MeshGeom sg, dg;
-    // host libm node xyz of the source runs beside the GPU grid detection,
-    // that of the destination beside the source's GPU cell pass
-    auto xyz_src = std::async(std::launch::async, [&] { node_xyz_host(src, sg); });
-    std::future<void> xyz_dst;
+    // host libm node xyz (source, then destination) on a host thread beside
+    // the GPU grid detection and the source's upload and cell pass
+    std::promise<void> src_done;
+    std::future<void> src_ready = src_done.get_future();
+    auto xyz_task = std::async(std::launch::async, [&] {
+        try {
+            node_xyz_host(src, sg);
+            src_done.set_value();
+        } catch (...) {
+            src_done.set_exception(std::current_exception());
+            return;
+        }
+        node_xyz_host(dst, dg);
+    });
     GridAxes sax = unique_axes(src, s), dax = unique_axes(dst, s);
     const bool s_reg = regular_from_axes(src, sax, sg), d_reg = regular_from_axes(dst, dax, dg);
     if (s_reg && d_reg)
@@ -489,11 +499,15 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
                               "rectilinear path, which this library does not provide");
     if (cfg.line_type != LineType::GreatCircle) throw InvalidArgument("generate_conservative: only LineType::GreatCircle is supported");
     tr.mark("detect");
-    xyz_src.get();
-    xyz_dst = std::async(std::launch::async, [&] { node_xyz_host(dst, dg); });
+    src_ready.get();
     prepare_geometry(src, sg, s);
     tr.mark("geometry_src");
-    xyz_dst.get();
+    // the source tree does not need the destination: build it while the host
+    // finishes the destination's node coordinates
+    LbvhTree tree;
+    lbvh_build(sg.d_box.get(), n_src, tree, s);
+    tr.mark("lbvh_build");
+    xyz_task.get();
     prepare_geometry(dst, dg, s);
     tr.mark("geometry_dst");
 
@@ -502,7 +516,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     if (model)
         proposer_candidates(sg, dg, *model, cand_off, cand_src, accepted, s);
     else
-        lbvh_candidates(sg.d_box.get(), n_src, dg.d_box.get(), n_dst, cand_off, cand_src, s);
+        lbvh_query(tree, dg.d_box.get(), n_dst, cand_off, cand_src, s);
     tr.mark(model ? "propose" : "lbvh");
     index_t n_pairs = 0;
     AXIS_HIP_CHECK(hipMemcpyAsync(&n_pairs, cand_off.get() + n_dst, sizeof(index_t), hipMemcpyDeviceToHost, s));
@@ -637,7 +651,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
             AXIS_LAUNCH_CHECK();
             DeviceBuffer<index_t> off2;
             DeviceBuffer<std::int32_t> src2;
-            lbvh_candidates(sg.d_box.get(), n_src, rbox.get(), n_redo, off2, src2, s);
+            lbvh_query(tree, rbox.get(), n_redo, off2, src2, s);
             index_t n2 = 0;
             AXIS_HIP_CHECK(hipMemcpyAsync(&n2, off2.get() + n_redo, sizeof(index_t), hipMemcpyDeviceToHost, s));
             AXIS_HIP_CHECK(hipStreamSynchronize(s));

@@ -382,18 +382,15 @@ __global__ void compact_hits_kernel(const std::int32_t *tmp, const std::int64_t 
 
 }  // namespace
 
-void lbvh_candidates(const float *src_boxes, index_t n_src, const float *dst_boxes, index_t n_dst, DeviceBuffer<index_t> &cand_off,
-                     DeviceBuffer<std::int32_t> &cand_src, hipStream_t s) {
+void lbvh_build(const float *src_boxes, index_t n_src, LbvhTree &t, hipStream_t s) {
     if (n_src >= (index_t(1) << 31) - 1) throw InvalidArgument("LBVH: more than 2^31 source cells");
     const int n = static_cast<int>(n_src);
     PhaseTrace tr("lbvh", s);
-    cand_off.resize(n_dst + 1);
     DeviceBuffer<std::uint32_t> codes(std::max(n, 1)), ids(std::max(n, 1));
     DeviceBuffer<std::uint32_t> left(std::max(n - 1, 1)), right(std::max(n - 1, 1));
-    DeviceBuffer<int> parent(std::max(2 * n - 1, 1)), arrive(std::max(n - 1, 1)), err(1);
+    DeviceBuffer<int> parent(std::max(2 * n - 1, 1)), arrive(std::max(n - 1, 1));
     DeviceBuffer<int> rlo(std::max(n - 1, 1)), rhi(std::max(n - 1, 1)), frontier(std::max(2 * n, 2)), n_frontier(1);
     DeviceBuffer<float> node_box(6 * std::max(n - 1, 1));
-    AXIS_HIP_CHECK(hipMemsetAsync(err.get(), 0, sizeof(int), s));
     if (n > 0) {
         morton_kernel<<<grid_for(n, kBlock), kBlock, 0, s>>>(src_boxes, n, codes.get(), ids.get());
         AXIS_LAUNCH_CHECK();
@@ -416,12 +413,26 @@ void lbvh_candidates(const float *src_boxes, index_t n_src, const float *dst_box
                                                  frontier.get(), n_frontier.get());
         AXIS_LAUNCH_CHECK();
     }
-    DeviceBuffer<Node> nodes(std::max(n - 1, 1));
+    t.n = n;
+    t.src_boxes = src_boxes;
+    t.nodes.resize(sizeof(Node) * static_cast<std::size_t>(std::max(n - 1, 1)));
+    Node *nodes = reinterpret_cast<Node *>(t.nodes.get());
     if (n > 1) {
-        pack_kernel<<<grid_for(n - 1, kBlock), kBlock, 0, s>>>(src_boxes, ids.get(), n, left.get(), right.get(), node_box.get(), nodes.get());
+        pack_kernel<<<grid_for(n - 1, kBlock), kBlock, 0, s>>>(src_boxes, ids.get(), n, left.get(), right.get(), node_box.get(), nodes);
         AXIS_LAUNCH_CHECK();
     }
     tr.mark("build_refit");
+}
+
+void lbvh_query(const LbvhTree &t, const float *dst_boxes, index_t n_dst, DeviceBuffer<index_t> &cand_off, DeviceBuffer<std::int32_t> &cand_src,
+                hipStream_t s) {
+    const int n = t.n;
+    const float *src_boxes = t.src_boxes;
+    const Node *nodes = reinterpret_cast<const Node *>(t.nodes.get());
+    PhaseTrace tr("lbvh", s);
+    cand_off.resize(n_dst + 1);
+    DeviceBuffer<int> err(1);
+    AXIS_HIP_CHECK(hipMemsetAsync(err.get(), 0, sizeof(int), s));
     const index_t nd1 = std::max<index_t>(n_dst, 1);
     DeviceBuffer<index_t> count(nd1);
     DeviceBuffer<std::int32_t> tmp(static_cast<std::size_t>(nd1) * kCap), over(nd1), rows1(nd1), rows2(nd1);
@@ -429,7 +440,7 @@ void lbvh_candidates(const float *src_boxes, index_t n_src, const float *dst_box
     DeviceBuffer<int> n_list(3);  // queries over kCap, moved to level 1, moved to level 2
     AXIS_HIP_CHECK(hipMemsetAsync(n_list.get(), 0, 3 * sizeof(int), s));
     if (n_dst > 0) {
-        query_kernel<2><<<grid_for(n_dst, kBlock), kBlock, 0, s>>>(nodes.get(), src_boxes, n, dst_boxes, n_dst, nullptr, count.get(), nullptr,
+        query_kernel<2><<<grid_for(n_dst, kBlock), kBlock, 0, s>>>(nodes, src_boxes, n, dst_boxes, n_dst, nullptr, count.get(), nullptr,
                                                                   tmp.get(), over.get(), n_list.get(), err.get());
         AXIS_LAUNCH_CHECK();
     }
@@ -441,18 +452,18 @@ void lbvh_candidates(const float *src_boxes, index_t n_src, const float *dst_box
     if (h_list[0] > 0) {
         AXIS_HIP_CHECK(hipMemsetAsync(level.get(), 0, static_cast<std::size_t>(nd1), s));
         query_long_kernel<false, kWave, kFront0, 0><<<static_cast<unsigned>(h_list[0]), kWave, 0, s>>>(
-            nodes.get(), dst_boxes, over.get(), count.get(), nullptr, nullptr, level.get(), rows1.get(), n_list.get() + 1);
+            nodes, dst_boxes, over.get(), count.get(), nullptr, nullptr, level.get(), rows1.get(), n_list.get() + 1);
         AXIS_LAUNCH_CHECK();
         AXIS_HIP_CHECK(hipMemcpyAsync(h_list, n_list.get(), sizeof h_list, hipMemcpyDeviceToHost, s));
         AXIS_HIP_CHECK(hipStreamSynchronize(s));
         if (h_list[1] > 0) {
             query_long_kernel<false, kBlock1, kFront1, 1><<<static_cast<unsigned>(h_list[1]), kBlock1, 0, s>>>(
-                nodes.get(), dst_boxes, rows1.get(), count.get(), nullptr, nullptr, level.get(), rows2.get(), n_list.get() + 2);
+                nodes, dst_boxes, rows1.get(), count.get(), nullptr, nullptr, level.get(), rows2.get(), n_list.get() + 2);
             AXIS_LAUNCH_CHECK();
             AXIS_HIP_CHECK(hipMemcpyAsync(h_list, n_list.get(), sizeof h_list, hipMemcpyDeviceToHost, s));
             AXIS_HIP_CHECK(hipStreamSynchronize(s));
             if (h_list[2] > 0) {
-                query_kernel<0><<<grid_for(h_list[2], kBlock), kBlock, 0, s>>>(nodes.get(), src_boxes, n, dst_boxes, h_list[2], rows2.get(),
+                query_kernel<0><<<grid_for(h_list[2], kBlock), kBlock, 0, s>>>(nodes, src_boxes, n, dst_boxes, h_list[2], rows2.get(),
                                                                               count.get(), nullptr, nullptr, nullptr, nullptr, err.get());
                 AXIS_LAUNCH_CHECK();
             }
@@ -473,17 +484,24 @@ void lbvh_candidates(const float *src_boxes, index_t n_src, const float *dst_box
         AXIS_LAUNCH_CHECK();
         if (h_list[0] > 0)
             query_long_kernel<true, kWave, kFront0, 0><<<static_cast<unsigned>(h_list[0]), kWave, 0, s>>>(
-                nodes.get(), dst_boxes, over.get(), nullptr, cand_off.get(), cand_src.get(), level.get(), nullptr, nullptr);
+                nodes, dst_boxes, over.get(), nullptr, cand_off.get(), cand_src.get(), level.get(), nullptr, nullptr);
         if (h_list[1] > 0)
             query_long_kernel<true, kBlock1, kFront1, 1><<<static_cast<unsigned>(h_list[1]), kBlock1, 0, s>>>(
-                nodes.get(), dst_boxes, rows1.get(), nullptr, cand_off.get(), cand_src.get(), level.get(), nullptr, nullptr);
+                nodes, dst_boxes, rows1.get(), nullptr, cand_off.get(), cand_src.get(), level.get(), nullptr, nullptr);
         if (h_list[2] > 0)
-            query_kernel<1><<<grid_for(h_list[2], kBlock), kBlock, 0, s>>>(nodes.get(), src_boxes, n, dst_boxes, h_list[2], rows2.get(), nullptr,
+            query_kernel<1><<<grid_for(h_list[2], kBlock), kBlock, 0, s>>>(nodes, src_boxes, n, dst_boxes, h_list[2], rows2.get(), nullptr,
                                                                           cand_off.get(), cand_src.get(), nullptr, nullptr, err.get());
         AXIS_LAUNCH_CHECK();
     }
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
     tr.mark("query_fill");
+}
+
+void lbvh_candidates(const float *src_boxes, index_t n_src, const float *dst_boxes, index_t n_dst, DeviceBuffer<index_t> &cand_off,
+                     DeviceBuffer<std::int32_t> &cand_src, hipStream_t s) {
+    LbvhTree t;
+    lbvh_build(src_boxes, n_src, t, s);
+    lbvh_query(t, dst_boxes, n_dst, cand_off, cand_src, s);
 }
 
 }  // namespace axis_hip::detail

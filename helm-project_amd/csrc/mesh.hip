@@ -675,7 +675,7 @@ void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s) {
     PhaseTrace tr("geometry", s);
     // node xyz with host libm (extract_cell_polygon_spherical, weight_generator.cpp:462-470)
     if (!g.xyz_ready) node_xyz_host(m, g);
-    const HostDoubles &h_xyz_deg = g.h_xyz_deg;
+    const PinnedArray<double> &h_xyz_deg = g.h_xyz_deg;
     tr.mark("host_sincos");
     g.d_xyz.resize(std::max<std::size_t>(g.h_xyz.size(), 3));
     g.d_xyz.upload(g.h_xyz.data(), g.h_xyz.size(), s);

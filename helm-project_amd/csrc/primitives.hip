@@ -7,6 +7,7 @@
 #include <vector>
 #include <mutex>
 #include <map>
+#include <unordered_map>
 
 #include "axis_hip.hpp"
 
@@ -23,6 +24,7 @@ struct CacheState {
     std::mutex mu;
     std::multimap<std::pair<int, std::size_t>, void *> free;  // (device, rounded size) -> block
     std::vector<Block> pending;                               // released, possibly still in use by queued work
+    std::unordered_map<void *, int> owner;                    // live block -> device
     void recycle() {
         for (const Block &b : pending) free.emplace(std::make_pair(b.device, b.bytes), b.p);
         pending.clear();
@@ -35,6 +37,10 @@ int current_device() {
 }
 CacheState &cache_state() {
     static CacheState *c = new CacheState;  // never destroyed: blocks live until the process ends
+    return *c;
+}
+CacheState &pinned_state() {
+    static CacheState *c = new CacheState;
     return *c;
 }
 }  // namespace
@@ -54,6 +60,7 @@ void *BlockCache::alloc(std::size_t bytes) {
         if (it != c.free.end() && it->first.first == dev && it->first.second <= 2 * r) {
             void *p = it->second;
             c.free.erase(it);
+            c.owner[p] = dev;
             return p;
         }
         if (pass == 0 && !c.pending.empty()) {
@@ -79,18 +86,53 @@ void *BlockCache::alloc(std::size_t bytes) {
         }
         AXIS_HIP_CHECK(hipMalloc(&p, r));
     }
+    c.owner[p] = dev;
     return p;
 }
 
 void BlockCache::release(void *p, std::size_t bytes) {
-    int dev = 0;
-    if (hipPointerAttribute_t a{}; hipPointerGetAttributes(&a, p) == hipSuccess)
-        dev = a.device;
-    else
-        (void)hipGetLastError();
     CacheState &c = cache_state();
     std::lock_guard<std::mutex> g(c.mu);
+    int dev = 0;
+    if (auto it = c.owner.find(p); it != c.owner.end()) {
+        dev = it->second;
+        c.owner.erase(it);
+    }
     c.pending.push_back(Block{dev, rounded(bytes), p});
+}
+
+PinnedCache &PinnedCache::get() {
+    static PinnedCache b;
+    return b;
+}
+
+void *PinnedCache::alloc(std::size_t bytes) {
+    const std::size_t r = BlockCache::rounded(bytes);
+    CacheState &c = pinned_state();
+    std::lock_guard<std::mutex> g(c.mu);
+    for (int pass = 0; pass < 2; ++pass) {
+        auto it = c.free.lower_bound(std::make_pair(0, r));
+        if (it != c.free.end() && it->first.second <= 2 * r) {
+            void *p = it->second;
+            c.free.erase(it);
+            return p;
+        }
+        if (pass == 0 && !c.pending.empty()) {
+            AXIS_HIP_CHECK(hipDeviceSynchronize());  // uploads queued from the released blocks have finished
+            c.recycle();
+            continue;
+        }
+        break;
+    }
+    void *p = nullptr;
+    AXIS_HIP_CHECK(hipHostMalloc(&p, r, hipHostMallocDefault));
+    return p;
+}
+
+void PinnedCache::release(void *p, std::size_t bytes) {
+    CacheState &c = pinned_state();
+    std::lock_guard<std::mutex> g(c.mu);
+    c.pending.push_back(Block{0, BlockCache::rounded(bytes), p});
 }
 
 void BlockCache::trim() {
