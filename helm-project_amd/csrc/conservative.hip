@@ -227,14 +227,13 @@ __global__ void compact_kernel(const double *overlap, const index_t *pos, index_
 // sorted (row, col) keys -> col, overlap, row; row_ptr from the row boundaries
 // (rows without entries get the offset of the next entry)
 __global__ void unpack_sorted_kernel(const std::uint64_t *key, const std::uint32_t *perm, const double *ov_in, index_t nnz, int col_bits,
-                                     index_t n_dst, index_t *col, double *ov, std::int32_t *row32, index_t *row_ptr) {
+                                     index_t n_dst, index_t *col, double *ov, index_t *row_ptr) {
     index_t e = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= nnz) return;
     std::uint64_t k = key[e];
     index_t r = static_cast<index_t>(k >> col_bits);
     col[e] = static_cast<index_t>(k & ((std::uint64_t(1) << col_bits) - 1));
     ov[e] = ov_in[perm[e]];
-    row32[e] = static_cast<std::int32_t>(r);
     const index_t rp = e == 0 ? -1 : static_cast<index_t>(key[e - 1] >> col_bits);
     for (index_t q = rp + 1; q <= r; ++q) row_ptr[q] = e;
     if (e == nnz - 1)
@@ -271,26 +270,26 @@ __global__ void __launch_bounds__(256) row_weights_kernel(const index_t *row_ptr
     }
 }
 
-__global__ void transpose_keys_kernel(const index_t *col, const std::int32_t *row32, index_t nnz, int row_bits, std::uint64_t *key,
-                                      std::uint32_t *perm) {
+__global__ void column_keys_kernel(const index_t *col, index_t nnz, std::uint32_t *key, std::uint32_t *perm) {
     index_t e = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= nnz) return;
-    key[e] = (static_cast<std::uint64_t>(col[e]) << row_bits) | static_cast<std::uint64_t>(static_cast<std::uint32_t>(row32[e]));
+    key[e] = static_cast<std::uint32_t>(col[e]);
     perm[e] = static_cast<std::uint32_t>(e);
 }
 
 // frac_a: Σ ov / area_src over each source column's entries in row order.
-// keys are sorted by (col, row); a thread owns the run that starts at its
-// position.
-__global__ void frac_a_kernel(const std::uint64_t *key, const std::uint32_t *perm, index_t nnz, int row_bits, const double *ov,
-                              const double *area_src, double *frac_a) {
+// The CSR entries are in row order and the column sort is stable, so each
+// column's run lists its rows in order; a thread owns the run that starts at
+// its position.
+__global__ void frac_a_kernel(const std::uint32_t *key, const std::uint32_t *perm, index_t nnz, const double *ov, const double *area_src,
+                              double *frac_a) {
     index_t e = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= nnz) return;
-    index_t c = static_cast<index_t>(key[e] >> row_bits);
-    if (e > 0 && static_cast<index_t>(key[e - 1] >> row_bits) == c) return;
+    const std::uint32_t c = key[e];
+    if (e > 0 && key[e - 1] == c) return;
     const double a = area_src[c];
     double s = 0.0;
-    for (index_t k = e; k < nnz && static_cast<index_t>(key[k] >> row_bits) == c; ++k) s += ov[perm[k]] / a;
+    for (index_t k = e; k < nnz && key[k] == c; ++k) s += ov[perm[k]] / a;
     frac_a[c] = fmin(s, 1.0);
 }
 
@@ -301,6 +300,131 @@ __global__ void unmapped_kernel(const index_t *row_ptr, index_t n_dst, const std
     int f = dflags[r];
     if (f & 3) return;  // masked or degenerate: skipped by the reference
     if ((f & 4) || row_ptr[r + 1] == row_ptr[r]) atomicMin(first, static_cast<unsigned long long>(r));
+}
+
+// ---- CSR assembly for pairs grouped by destination row (every path but a
+// proposer round 2 that appended pairs): after compaction the nonzero overlaps
+// of a row are contiguous, so the (row, col) order is a per-row column sort.
+// Weights, frac_b and the longest row come out of the same pass
+// (weight_generator.cpp:3682-3724: w = max(ov / area_b, 0), frac_b = Σ ov /
+// area_b in column order, FracArea division by the unclamped sum).
+constexpr int kShortRow = 16;     // sorted by one thread
+constexpr int kLongRowMax = 4096; // sorted by one workgroup in LDS
+constexpr int kLongRowBlock = 256;
+
+__global__ void compact_grouped_kernel(const double *overlap, const index_t *pos, index_t n, const std::int32_t *pair_src, std::int32_t *col_c,
+                                       double *ov_c) {
+    const index_t p = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n || !(overlap[p] > 0.0)) return;
+    const index_t q = pos[p];
+    col_c[q] = pair_src[p];
+    ov_c[q] = overlap[p];
+}
+
+__global__ void __launch_bounds__(256) short_rows_kernel(const index_t *cand_off, const index_t *pos, index_t n_dst, const std::int32_t *col_c,
+                                                         const double *ov_c, const double *area_dst, int frac_area, index_t *row_ptr,
+                                                         index_t *col, double *val, double *ov_s, double *frac_b,
+                                                         int *max_row, std::int32_t *long_rows, int *n_long) {
+    int longest = 0;
+    for (index_t r = static_cast<index_t>(blockIdx.x) * 256 + threadIdx.x; r < n_dst; r += static_cast<index_t>(gridDim.x) * 256) {
+        const index_t b = pos[cand_off[r]], e = pos[cand_off[r + 1]];
+        row_ptr[r] = b;
+        if (r == n_dst - 1) row_ptr[n_dst] = e;
+        const int len = static_cast<int>(min<index_t>(e - b, index_t(1) << 30));
+        longest = max(longest, len);
+        if (len > kShortRow) {
+            long_rows[atomicAdd(n_long, 1)] = static_cast<std::int32_t>(r);
+            continue;
+        }
+        std::int32_t cc[kShortRow];
+        double oo[kShortRow];
+        for (int k = 0; k < len; ++k) {
+            std::int32_t c = col_c[b + k];
+            double o = ov_c[b + k];
+            int m = k;
+            for (; m > 0 && cc[m - 1] > c; --m) {
+                cc[m] = cc[m - 1];
+                oo[m] = oo[m - 1];
+            }
+            cc[m] = c;
+            oo[m] = o;
+        }
+        const double a = area_dst[r];
+        double fb = 0.0;
+        for (int k = 0; k < len; ++k) fb += oo[k] / a;
+        const bool divide = frac_area && fb > 0.0;
+        for (int k = 0; k < len; ++k) {
+            double w = fmax(oo[k] / a, 0.0);
+            if (divide) w /= fb;
+            val[b + k] = w;
+            col[b + k] = cc[k];
+            ov_s[b + k] = oo[k];
+        }
+        frac_b[r] = fmin(fb, 1.0);
+    }
+    for (int d = 32; d >= 1; d >>= 1) longest = max(longest, __shfl_xor(longest, d, 64));
+    __shared__ int part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = longest;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        longest = max(max(part[0], part[1]), max(part[2], part[3]));
+        if (longest > 0) atomicMax(max_row, longest);
+    }
+}
+
+// rows of kShortRow < len <= kLongRowMax: bitonic column sort in LDS, the
+// ordered sum by one thread, the rest in parallel
+__global__ void __launch_bounds__(kLongRowBlock) long_rows_kernel(const std::int32_t *long_rows, const index_t *row_ptr, const std::int32_t *col_c,
+                                                                  const double *ov_c, const double *area_dst, int frac_area, index_t *col,
+                                                                  double *val, double *ov_s, double *frac_b) {
+    __shared__ std::int32_t sc[kLongRowMax];
+    __shared__ double so[kLongRowMax];
+    __shared__ double s_fb;
+    const int tid = static_cast<int>(threadIdx.x);
+    const index_t r = long_rows[blockIdx.x];
+    const index_t b = row_ptr[r];
+    const int len = static_cast<int>(row_ptr[r + 1] - b);  // <= kLongRowMax (checked by the host)
+    int P = 2;
+    while (P < len) P <<= 1;
+    for (int k = tid; k < P; k += kLongRowBlock) {
+        sc[k] = k < len ? col_c[b + k] : 0x7FFFFFFF;
+        so[k] = k < len ? ov_c[b + k] : 0.0;
+    }
+    __syncthreads();
+    for (int size = 2; size <= P; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = tid; i < P / 2; i += kLongRowBlock) {
+                const int lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
+                const bool asc = (lo & size) == 0;
+                if ((sc[lo] > sc[hi]) == asc) {
+                    const std::int32_t tc = sc[lo];
+                    sc[lo] = sc[hi];
+                    sc[hi] = tc;
+                    const double to = so[lo];
+                    so[lo] = so[hi];
+                    so[hi] = to;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const double a = area_dst[r];
+    if (tid == 0) {
+        double fb = 0.0;
+        for (int k = 0; k < len; ++k) fb += so[k] / a;
+        s_fb = fb;
+        frac_b[r] = fmin(fb, 1.0);
+    }
+    __syncthreads();
+    const double fb = s_fb;
+    const bool divide = frac_area && fb > 0.0;
+    for (int k = tid; k < len; k += kLongRowBlock) {
+        double w = fmax(so[k] / a, 0.0);
+        if (divide) w /= fb;
+        val[b + k] = w;
+        col[b + k] = sc[k];
+        ov_s[b + k] = so[k];
+    }
 }
 
 int bits_for(index_t n) {
@@ -616,6 +740,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     tr.mark("overlap");
     arbitrate();
     index_t n_cand = n_pairs;
+    bool appended = false;  // round-2 pairs sit after all of round 1, out of row order
     if (model) {
         // round 2: the LBVH redoes the rows whose proposal was rejected or
         // whose overlaps do not account for the cell's area
@@ -677,6 +802,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
                 fill_rows_mapped_kernel<<<grid_for(n_redo, 256), 256, 0, s>>>(off2.get(), n_redo, rows.get(), pair_row.get() + n_pairs);
                 AXIS_LAUNCH_CHECK();
             }
+            appended = n2 > 0;
             run_overlaps(n_pairs, nt);
             arbitrate();
             n_pairs = nt;
@@ -702,15 +828,6 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     const int col_bits = bits_for(std::max<index_t>(n_src, 2));
     const int row_bits = bits_for(std::max<index_t>(n_dst, 2));
     const index_t nz1 = std::max<index_t>(nnz, 1);
-    DeviceBuffer<std::uint64_t> key(nz1);
-    DeviceBuffer<std::uint32_t> perm(nz1);
-    DeviceBuffer<double> ov_c(nz1);
-    if (nnz > 0) {
-        compact_kernel<<<grid_for(n_pairs, 256), 256, 0, s>>>(overlap.get(), pos.get(), n_pairs, pair_row.get(), cand_src.get(), col_bits, key.get(),
-                                                             perm.get(), ov_c.get());
-        AXIS_LAUNCH_CHECK();
-        radix_sort_pairs_u64(key.get(), perm.get(), nnz, col_bits + row_bits, s);
-    }
     M->row_ptr.resize(n_dst + 1);
     M->col.resize(nz1);
     M->val.resize(nz1);
@@ -718,29 +835,71 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     M->frac_b.resize(std::max<index_t>(n_dst, 1));
     M->area_a.resize(std::max<index_t>(n_src, 1));
     M->area_b.resize(std::max<index_t>(n_dst, 1));
-    DeviceBuffer<double> ov(nz1);
-    DeviceBuffer<std::int32_t> row32(nz1);
-    if (nnz > 0) {
-        unpack_sorted_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(key.get(), perm.get(), ov_c.get(), nnz, col_bits, n_dst, M->col.get(), ov.get(),
-                                                                row32.get(), M->row_ptr.get());
-        AXIS_LAUNCH_CHECK();
-    } else {
-        AXIS_HIP_CHECK(hipMemsetAsync(M->row_ptr.get(), 0, sizeof(index_t) * (n_dst + 1), s));
-    }
+    DeviceBuffer<double> ov(nz1);  // overlaps in CSR order
+    DeviceBuffer<std::uint64_t> key(nz1);
+    DeviceBuffer<std::uint32_t> perm(nz1);
     DeviceBuffer<int> max_row(1);
     AXIS_HIP_CHECK(hipMemsetAsync(max_row.get(), 0, sizeof(int), s));
-    if (n_dst > 0) {
+    const int frac_area = cfg.norm_type == NormType::FracArea ? 1 : 0;
+    bool assembled = false;
+    if (!appended && n_dst > 0) {
+        DeviceBuffer<std::int32_t> col_c(nz1), long_rows(std::max<index_t>(n_dst, 1));
+        DeviceBuffer<double> ov_c(nz1);
+        DeviceBuffer<int> n_long(1);
+        AXIS_HIP_CHECK(hipMemsetAsync(n_long.get(), 0, sizeof(int), s));
+        if (nnz > 0) {
+            compact_grouped_kernel<<<grid_for(n_pairs, 256), 256, 0, s>>>(overlap.get(), pos.get(), n_pairs, cand_src.get(), col_c.get(), ov_c.get());
+            AXIS_LAUNCH_CHECK();
+        }
         const unsigned grid = static_cast<unsigned>(std::min<index_t>((n_dst + 255) / 256, 2048));
-        row_weights_kernel<<<grid, 256, 0, s>>>(M->row_ptr.get(), n_dst, dg.d_area.get(), ov.get(), M->val.get(), M->frac_b.get(),
-                                                cfg.norm_type == NormType::FracArea ? 1 : 0, max_row.get());
+        short_rows_kernel<<<grid, 256, 0, s>>>(cand_off.get(), pos.get(), n_dst, col_c.get(), ov_c.get(), dg.d_area.get(), frac_area,
+                                               M->row_ptr.get(), M->col.get(), M->val.get(), ov.get(), M->frac_b.get(), max_row.get(),
+                                               long_rows.get(), n_long.get());
         AXIS_LAUNCH_CHECK();
+        int h[2] = {0, 0};
+        AXIS_HIP_CHECK(hipMemcpyAsync(&h[0], max_row.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+        AXIS_HIP_CHECK(hipMemcpyAsync(&h[1], n_long.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+        if (h[0] <= kLongRowMax) {
+            if (h[1] > 0) {
+                long_rows_kernel<<<static_cast<unsigned>(h[1]), kLongRowBlock, 0, s>>>(long_rows.get(), M->row_ptr.get(), col_c.get(), ov_c.get(),
+                                                                                      dg.d_area.get(), frac_area, M->col.get(), M->val.get(),
+                                                                                      ov.get(), M->frac_b.get());
+                AXIS_LAUNCH_CHECK();
+            }
+            assembled = true;
+        }
+    }
+    if (!assembled) {
+        // pairs in any order (or rows beyond the LDS sort): one global (row, col) sort
+        DeviceBuffer<double> ov_c(nz1);
+        if (nnz > 0) {
+            compact_kernel<<<grid_for(n_pairs, 256), 256, 0, s>>>(overlap.get(), pos.get(), n_pairs, pair_row.get(), cand_src.get(), col_bits,
+                                                                 key.get(), perm.get(), ov_c.get());
+            AXIS_LAUNCH_CHECK();
+            radix_sort_pairs_u64(key.get(), perm.get(), nnz, col_bits + row_bits, s);
+            unpack_sorted_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(key.get(), perm.get(), ov_c.get(), nnz, col_bits, n_dst, M->col.get(), ov.get(),
+                                                                    M->row_ptr.get());
+            AXIS_LAUNCH_CHECK();
+        } else {
+            AXIS_HIP_CHECK(hipMemsetAsync(M->row_ptr.get(), 0, sizeof(index_t) * (n_dst + 1), s));
+        }
+        AXIS_HIP_CHECK(hipMemsetAsync(max_row.get(), 0, sizeof(int), s));
+        if (n_dst > 0) {
+            const unsigned grid = static_cast<unsigned>(std::min<index_t>((n_dst + 255) / 256, 2048));
+            row_weights_kernel<<<grid, 256, 0, s>>>(M->row_ptr.get(), n_dst, dg.d_area.get(), ov.get(), M->val.get(), M->frac_b.get(), frac_area,
+                                                    max_row.get());
+            AXIS_LAUNCH_CHECK();
+        }
     }
     AXIS_HIP_CHECK(hipMemsetAsync(M->frac_a.get(), 0, sizeof(double) * std::max<index_t>(n_src, 1), s));
     if (nnz > 0) {
-        transpose_keys_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(M->col.get(), row32.get(), nnz, row_bits, key.get(), perm.get());
+        // stable sort of the CSR entries by column alone (LSD radix)
+        DeviceBuffer<std::uint32_t> ckey(nz1);
+        column_keys_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(M->col.get(), nnz, ckey.get(), perm.get());
         AXIS_LAUNCH_CHECK();
-        radix_sort_pairs_u64(key.get(), perm.get(), nnz, col_bits + row_bits, s);
-        frac_a_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(key.get(), perm.get(), nnz, row_bits, ov.get(), sg.d_area.get(), M->frac_a.get());
+        radix_sort_pairs_u32(ckey.get(), perm.get(), nnz, col_bits, s);
+        frac_a_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(ckey.get(), perm.get(), nnz, ov.get(), sg.d_area.get(), M->frac_a.get());
         AXIS_LAUNCH_CHECK();
     }
     if (n_src > 0)
