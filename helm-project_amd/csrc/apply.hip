@@ -262,7 +262,8 @@ void InterpolationMatrix::prepare_apply(hipStream_t s) {
     if (n_dst > 0) {
         order_key_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(coarse_bits ? row_key.get() : nullptr, 3 * (10 - coarse_bits), n_dst, key.get());
         AXIS_LAUNCH_CHECK();
-        if (coarse_bits) detail::radix_sort_keys_u64(key.get(), static_cast<std::size_t>(n_dst), s, 32 + 3 * coarse_bits);
+        // the low 32 bits (the row) are already ascending: sort the cell bits only
+        if (coarse_bits) detail::radix_sort_keys_u64(key.get(), static_cast<std::size_t>(n_dst), s, 32 + 3 * coarse_bits, 32);
     }
     tr.mark("order_sort");
     DeviceBuffer<index_t> c_seg(nd1), c_slot(nd1), c_long(nd1), o_seg(nd1 + 1), o_slot(nd1 + 1), o_long(nd1 + 1);
@@ -298,8 +299,17 @@ void InterpolationMatrix::prepare_apply(hipStream_t s) {
     const std::int32_t end_slot = n_slots;
     AXIS_HIP_CHECK(hipMemcpyAsync(long_slot.get() + n_long_rows, &end_slot, sizeof(std::int32_t), hipMemcpyHostToDevice, s));
     tr.mark("segments");
-    // tiles
     n_tiles = (n_seg + apply_detail::kTileSeg - 1) / apply_detail::kTileSeg;
+    tiles_ready = false;
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    apply_ready = true;
+}
+
+void InterpolationMatrix::prepare_tiles(hipStream_t s) {
+    if (tiles_ready) return;
+    prepare_apply(s);
+    PhaseTrace tr("prepare_tiles", s);
+    const index_t n1 = std::max<index_t>(nnz, 1);
     tile_nu.resize(static_cast<std::size_t>(std::max<index_t>(n_tiles, 1)));
     tile_col.resize(static_cast<std::size_t>(std::max<index_t>(n_tiles, 1)) * apply_detail::kTileChunks);
     t_lcol.resize(static_cast<std::size_t>(n1));
@@ -316,7 +326,7 @@ void InterpolationMatrix::prepare_apply(hipStream_t s) {
     }
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
     tr.mark("tiles");
-    apply_ready = true;
+    tiles_ready = true;
 }
 
 void batch_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t n_src, index_t n_dst, index_t k, ApplyMode mode,

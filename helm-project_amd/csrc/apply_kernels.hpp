@@ -481,6 +481,7 @@ void launch_num(InterpolationMatrix &A, const void *x, FieldType xt, void *y, Fi
 
 inline void launch(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t k, ApplyMode mode, const Launch &L,
                    hipStream_t s) {
+    if (L.tiled) A.prepare_tiles(s);
     if (mode == ApplyMode::Plain) return launch_xy<kPlain, double, double>(A, x, y, k, L, s);
     if (mode == ApplyMode::Mixed) return launch_num<kComp32>(A, x, xt, y, yt, k, L, s);
     return launch_num<kAcc64>(A, x, xt, y, yt, k, L, s);

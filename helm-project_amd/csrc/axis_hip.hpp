@@ -115,14 +115,16 @@ class InterpolationMatrix {
     index_t n_tiles = 0;
     DeviceBuffer<std::int32_t> tile_nu, tile_col;
     DeviceBuffer<std::uint16_t> t_lcol;
-    bool apply_ready = false;
+    bool apply_ready = false, tiles_ready = false;
     // diagnostics of the generation that produced it
     index_t n_candidates = 0, n_host_arbitrated = 0, n_cap_borderline = 0;
     double gen_ms = 0.0;
 
-    /// Builds the apply layout once: FP32 hi/lo split, int32 columns, the
-    /// segment list in spatial order and the tiles.
+    /// Builds the apply layout once: FP32 hi/lo split, int32 columns and the
+    /// segment list in spatial order.
     void prepare_apply(hipStream_t s);
+    /// Builds the tiles once (on the first tiled launch).
+    void prepare_tiles(hipStream_t s);
 };
 
 struct ConservationReport {
@@ -241,7 +243,9 @@ void cell_morton_keys(const Mesh &m, DeviceBuffer<std::uint32_t> &key, hipStream
 // scan / sort primitives (scan.hip, sort.hip)
 void exclusive_scan_i64(const index_t *in, index_t *out, index_t n, hipStream_t s);  // out[n] = total
 void radix_sort_pairs_u32(std::uint32_t *keys, std::uint32_t *vals, std::size_t n, int bits, hipStream_t s);
-void radix_sort_keys_u64(std::uint64_t *keys, std::size_t n, hipStream_t s, int bits = 64);  // sorts the low `bits` bits
+/// Stable LSD sort on key bits [first_bit, bits) (first_bit a multiple of 8):
+/// keys whose lower bits are already ascending need only the upper passes.
+void radix_sort_keys_u64(std::uint64_t *keys, std::size_t n, hipStream_t s, int bits = 64, int first_bit = 0);
 void radix_sort_pairs_u64(std::uint64_t *keys, std::uint32_t *vals, std::size_t n, int bits, hipStream_t s);
 
 // LBVH broad phase (lbvh.hip): candidate CSR of src cells whose float boxes

@@ -263,7 +263,7 @@ BatchPlan::BatchPlan(std::vector<std::shared_ptr<InterpolationMatrix>> mats, std
     index_t k_max = 0;
     for (std::size_t p = 0; p < n; ++p) {
         auto &A = *I.mats[p];
-        A.prepare_apply(s);
+        A.prepare_tiles(s);
         I.max_e = std::max(I.max_e, A.seg_cap);
         pd[p].A = apply_detail::mat_of(A);
         pd[p].X = I.src[p];

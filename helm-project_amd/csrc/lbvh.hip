@@ -108,7 +108,8 @@ __global__ void karras_kernel(const std::uint32_t *codes, int n, std::uint32_t *
 //      arrive merges the two child boxes and climbs on);
 //   2. one workgroup finishes the nodes whose range crosses a chunk boundary,
 //      starting from their children that phase 1 completed.
-constexpr int kRefitChunk = 1024;
+constexpr int kRefitChunk = 4096;
+constexpr int kRefitBlock = 1024;
 
 __device__ inline void merge_node(const float *boxes, const std::uint32_t *ids, const std::uint32_t *left, const std::uint32_t *right,
                                   float *node_box, int p) {
@@ -134,12 +135,12 @@ __device__ inline void merge_node(const float *boxes, const std::uint32_t *ids, 
     }
 }
 
-__global__ void __launch_bounds__(kBlock) refit_chunk_kernel(const float *boxes, const std::uint32_t *ids, int n, const std::uint32_t *left,
+__global__ void __launch_bounds__(kRefitBlock) refit_chunk_kernel(const float *boxes, const std::uint32_t *ids, int n, const std::uint32_t *left,
                                                              const std::uint32_t *right, const int *parent, const int *rlo, const int *rhi,
                                                              float *node_box, int *arrive) {
     const int c0 = static_cast<int>(blockIdx.x) * kRefitChunk;
     const int c1 = min(c0 + kRefitChunk, n);
-    for (int k = c0 + static_cast<int>(threadIdx.x); k < c1; k += kBlock) {
+    for (int k = c0 + static_cast<int>(threadIdx.x); k < c1; k += kRefitBlock) {
         int p = parent[n - 1 + k];
         while (p >= 0 && rlo[p] >= c0 && rhi[p] < c1) {
             if (__hip_atomic_fetch_add(&arrive[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0) break;
@@ -370,14 +371,16 @@ __global__ void __launch_bounds__(kThreads) query_long_kernel(const Node *nodes,
 constexpr int kFront0 = 2048, kFront1 = 6144, kBlock1 = 256;
 
 // first-pass hits of the queries with at most kCap of them -> candidate CSR
+// (kCap consecutive lanes per query, so reads and writes are contiguous)
 __global__ void compact_hits_kernel(const std::int32_t *tmp, const std::int64_t *count, const std::int64_t *offsets, std::int64_t n,
                                     std::int32_t *out) {
-    const std::int64_t j = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const std::int64_t g = static_cast<std::int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const std::int64_t j = g / kCap;
+    const int e = static_cast<int>(g % kCap);
     if (j >= n) return;
     const std::int64_t c = count[j];
-    if (c > kCap) return;
-    const std::int64_t o = offsets[j];
-    for (std::int64_t e = 0; e < c; ++e) out[o + e] = tmp[j * kCap + e];
+    if (c > kCap || e >= c) return;
+    out[offsets[j] + e] = tmp[j * kCap + e];
 }
 
 }  // namespace
@@ -402,7 +405,7 @@ void lbvh_build(const float *src_boxes, index_t n_src, LbvhTree &t, hipStream_t 
         AXIS_HIP_CHECK(hipMemsetAsync(arrive.get(), 0, sizeof(int) * (n - 1), s));
         karras_kernel<<<grid_for(n - 1, kBlock), kBlock, 0, s>>>(codes.get(), n, left.get(), right.get(), parent.get(), rlo.get(), rhi.get());
         AXIS_LAUNCH_CHECK();
-        refit_chunk_kernel<<<static_cast<unsigned>((n + kRefitChunk - 1) / kRefitChunk), kBlock, 0, s>>>(
+        refit_chunk_kernel<<<static_cast<unsigned>((n + kRefitChunk - 1) / kRefitChunk), kRefitBlock, 0, s>>>(
             src_boxes, ids.get(), n, left.get(), right.get(), parent.get(), rlo.get(), rhi.get(), node_box.get(), arrive.get());
         AXIS_LAUNCH_CHECK();
         AXIS_HIP_CHECK(hipMemsetAsync(n_frontier.get(), 0, sizeof(int), s));
@@ -480,7 +483,8 @@ void lbvh_query(const LbvhTree &t, const float *dst_boxes, index_t n_dst, Device
     if (total >= (index_t(1) << 31)) throw RuntimeError("LBVH: candidate pair count exceeds 2^31");
     cand_src.resize(std::max<index_t>(total, 1));
     if (n_dst > 0 && total > 0) {
-        compact_hits_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(tmp.get(), count.get(), cand_off.get(), n_dst, cand_src.get());
+        compact_hits_kernel<<<grid_for(static_cast<std::size_t>(n_dst) * kCap, 256), 256, 0, s>>>(tmp.get(), count.get(), cand_off.get(), n_dst,
+                                                                                               cand_src.get());
         AXIS_LAUNCH_CHECK();
         if (h_list[0] > 0)
             query_long_kernel<true, kWave, kFront0, 0><<<static_cast<unsigned>(h_list[0]), kWave, 0, s>>>(

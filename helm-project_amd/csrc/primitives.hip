@@ -305,8 +305,10 @@ __global__ void __launch_bounds__(kSortBlock) radix_scatter_kernel(const K *keys
     }
 }
 
+// LSD passes over key bits [first_bit, bits); stable, so keys whose lower
+// bits are already in order need only the upper passes
 template <class K, bool kVals>
-void radix_sort_impl(K *keys, std::uint32_t *vals, std::size_t n, int bits, hipStream_t s) {
+void radix_sort_impl(K *keys, std::uint32_t *vals, std::size_t n, int bits, hipStream_t s, int first_bit = 0) {
     if (n <= 1) return;
     if (n >= (std::size_t(1) << 32)) throw InvalidArgument("radix sort: more than 2^32 keys");
     const unsigned nb = static_cast<unsigned>((n + kSortTile - 1) / kSortTile);
@@ -315,8 +317,9 @@ void radix_sort_impl(K *keys, std::uint32_t *vals, std::size_t n, int bits, hipS
     DeviceBuffer<index_t> hist(static_cast<std::size_t>(256) * nb), offs(static_cast<std::size_t>(256) * nb + 1);
     K *kin = keys, *kout = k2.get();
     std::uint32_t *vin = vals, *vout = v2.get();
-    int passes = (bits + 7) / 8;
-    for (int p = 0; p < passes; ++p) {
+    const int p0 = first_bit / 8;
+    const int passes = (bits + 7) / 8;
+    for (int p = p0; p < passes; ++p) {
         const int shift = 8 * p;
         radix_hist_kernel<K><<<nb, kSortBlock, 0, s>>>(kin, n, shift, hist.get(), nb);
         AXIS_LAUNCH_CHECK();
@@ -338,8 +341,8 @@ void radix_sort_impl(K *keys, std::uint32_t *vals, std::size_t n, int bits, hipS
 void radix_sort_pairs_u32(std::uint32_t *keys, std::uint32_t *vals, std::size_t n, int bits, hipStream_t s) {
     radix_sort_impl<std::uint32_t, true>(keys, vals, n, bits, s);
 }
-void radix_sort_keys_u64(std::uint64_t *keys, std::size_t n, hipStream_t s, int bits) {
-    radix_sort_impl<std::uint64_t, false>(keys, nullptr, n, bits, s);
+void radix_sort_keys_u64(std::uint64_t *keys, std::size_t n, hipStream_t s, int bits, int first_bit) {
+    radix_sort_impl<std::uint64_t, false>(keys, nullptr, n, bits, s, first_bit);
 }
 void radix_sort_pairs_u64(std::uint64_t *keys, std::uint32_t *vals, std::size_t n, int bits, hipStream_t s) {
     radix_sort_impl<std::uint64_t, true>(keys, vals, n, bits, s);
