@@ -278,11 +278,33 @@ __device__ inline void load_chunk(const double *xf, index_t c0, index_t n_src, b
 /// LDS staging buffer of one tile workgroup: FU fields of the tile's chunks.
 /// (A second buffer, one barrier per step, measured slower: the extra LDS
 /// lowers occupancy more than the barrier costs.)
+/// Slot kTileCols of each field holds 0: the padding terms of a lane whose
+/// segment is shorter than its wave's longest read it (with zero weights), so
+/// every lane runs the same terms without per-lane predication.
 template <int FU, int NUM>
 struct Stage {
     using V = typename NumTraits<NUM>::V;
-    V xs[FU][kTileCols];
+    V xs[FU][kTileCols + kChunk];
 };
+
+/// The 4 values of an aligned chunk, unconditionally vector-loaded (fields
+/// 16-byte aligned, n_src a multiple of 4, so the chunk lies inside the field).
+template <class V>
+__device__ inline void load_chunk_vec(const float *xf, V out[kChunk]) {
+    const float4 v = *reinterpret_cast<const float4 *>(xf);
+    out[0] = v.x, out[1] = v.y, out[2] = v.z, out[3] = v.w;
+}
+template <class V>
+__device__ inline void load_chunk_vec(const std::uint16_t *xf, V out[kChunk]) {
+    const uint2 v = *reinterpret_cast<const uint2 *>(xf);
+    out[0] = __uint_as_float(v.x << 16), out[1] = __uint_as_float(v.x & 0xFFFF0000u);
+    out[2] = __uint_as_float(v.y << 16), out[3] = __uint_as_float(v.y & 0xFFFF0000u);
+}
+template <class V>
+__device__ inline void load_chunk_vec(const double *xf, V out[kChunk]) {
+    const double2 a = *reinterpret_cast<const double2 *>(xf), b = *reinterpret_cast<const double2 *>(xf + 2);
+    out[0] = a.x, out[1] = a.y, out[2] = b.x, out[3] = b.y;
+}
 
 template <int E, int FU, int NUM, class XT, class YT>
 __device__ inline void tile_apply(const Mat &A, const XT *X, YT *Y, double *P, index_t t, index_t f0, index_t f1, Stage<FU, NUM> &S) {
@@ -308,11 +330,11 @@ __device__ inline void tile_apply(const Mat &A, const XT *X, YT *Y, double *P, i
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) n_max = max(n_max, __shfl_xor(n_max, d, 64));
     const int n_wave = __builtin_amdgcn_readfirstlane(n_max);
-    unsigned lc[E];  // byte offsets into one field's staged values
+    unsigned lc[E];  // byte offsets into one field's staged values (padding terms: the zero slot)
     W h[E], l[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        lc[e] = 0;
+        lc[e] = static_cast<unsigned>(kTileCols) * static_cast<unsigned>(sizeof(V));
         h[e] = W(0);
         l[e] = W(0);
         if (e < n) {
@@ -320,48 +342,95 @@ __device__ inline void tile_apply(const Mat &A, const XT *X, YT *Y, double *P, i
             load_w<NUM>(A, b + e, h[e], l[e]);
         }
     }
+    if (threadIdx.x < FU) S.xs[threadIdx.x][kTileCols] = V(0);
     // this lane's chunk: columns c0 .. c0 + 3 of every field
     const int lane_chunk = static_cast<int>(threadIdx.x);
-    const index_t c0 = lane_chunk < nu ? static_cast<index_t>(A.tile_col[t * kTileChunks + lane_chunk]) : -1;
-    // vector loads need the field start aligned like the chunk (16 bytes for FP32)
-    constexpr int kAlignCols = 16 / static_cast<int>(sizeof(XT)) < kChunk ? 16 / static_cast<int>(sizeof(XT)) : kChunk;
-    const bool vec = (A.n_src % kAlignCols) == 0 && (reinterpret_cast<std::uintptr_t>(X) % 16) == 0;
-    // the source values of the next FU fields are loaded while this step's
-    // sums are formed (one step of software pipelining)
-    V v[FU][kChunk];
-    auto fetch = [&](index_t f) {
+    const bool has = lane_chunk < nu;
+    const index_t c0 = has ? static_cast<index_t>(A.tile_col[t * kTileChunks + lane_chunk]) : 0;
+    // unconditional vector loads need 16-byte aligned fields holding whole chunks
+    const bool vec = (A.n_src % kChunk) == 0 && (reinterpret_cast<std::uintptr_t>(X) % 16) == 0;
+    auto sums = [&](int u, index_t f) {
+        RowAcc<NUM> acc;
+        const char *xb = reinterpret_cast<const char *>(&S.xs[u][0]);
+        // terms in groups of 4 (E is a multiple of 4): the group's LDS reads
+        // are issued together; the bound is wave-uniform
+#pragma unroll
+        for (int g = 0; g < E; g += 4) {
+            if (g < n_wave) {
+                V xv[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) xv[q] = *reinterpret_cast<const V *>(xb + lc[g + q]);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc.add(h[g + q], l[g + q], xv[q]);
+            }
+        }
+        const double val = acc.value();
+        if (valid && f < f1) {
+            if (slot < 0)
+                st(Y + f * A.n_dst + r, val);
+            else
+                P[f * A.n_slots + slot] = val;
+        }
+    };
+    if (!vec) {
+        // per-column loads, one step of prefetch
+        V v[FU][kChunk];
+        auto fetch = [&](index_t f) {
+#pragma unroll
+            for (int u = 0; u < FU; ++u) {
+                const index_t fu = (f + u < f1) ? f + u : f1 - 1;
+                if (has) load_chunk<V>(X + fu * A.n_src, c0, A.n_src, false, v[u]);
+            }
+        };
+        fetch(f0);
+        for (index_t f = f0; f < f1; f += FU) {
+            if (has) {
+#pragma unroll
+                for (int u = 0; u < FU; ++u)
+#pragma unroll
+                    for (int q = 0; q < kChunk; ++q) S.xs[u][lane_chunk * kChunk + q] = v[u][q];
+            }
+            __syncthreads();
+            if (f + FU < f1) fetch(f + FU);
+#pragma unroll
+            for (int u = 0; u < FU; ++u) sums(u, f + u);
+            __syncthreads();
+        }
+        return;
+    }
+    // software pipeline: the chunks of the next kDepth steps (FU fields each)
+    // are in flight while a step's sums are formed.  Every lane loads (lanes
+    // without a chunk load columns 0..3 and store them to slots no nonzero
+    // reads), so no load sits under a per-lane branch.
+    constexpr int kDepth = FU >= 4 ? 1 : 4 / FU;
+    constexpr index_t kAhead = static_cast<index_t>(kDepth) * FU;
+    V v[kDepth][FU][kChunk];
+    auto fetch = [&](index_t f, V (&dst)[FU][kChunk]) {
 #pragma unroll
         for (int u = 0; u < FU; ++u) {
             const index_t fu = (f + u < f1) ? f + u : f1 - 1;
-            if (c0 >= 0) load_chunk<V>(X + fu * A.n_src, c0, A.n_src, vec, v[u]);
+            load_chunk_vec<V>(X + fu * A.n_src + c0, dst[u]);
         }
     };
-    fetch(f0);
-    for (index_t f = f0; f < f1; f += FU) {
-        if (c0 >= 0) {
 #pragma unroll
-            for (int u = 0; u < FU; ++u)
+    for (int d = 0; d < kDepth; ++d)
+        if (f0 + d * FU < f1) fetch(f0 + d * FU, v[d]);
+    for (index_t fb = f0; fb < f1; fb += kAhead) {
 #pragma unroll
-                for (int q = 0; q < kChunk; ++q) S.xs[u][lane_chunk * kChunk + q] = v[u][q];
-        }
-        __syncthreads();
-        if (f + FU < f1) fetch(f + FU);
+        for (int d = 0; d < kDepth; ++d) {
+            const index_t f = fb + d * FU;
+            if (f < f1) {  // uniform over the workgroup
 #pragma unroll
-        for (int u = 0; u < FU; ++u) {
-            RowAcc<NUM> acc;
-            const char *xb = reinterpret_cast<const char *>(&S.xs[u][0]);
+                for (int u = 0; u < FU; ++u)
 #pragma unroll
-            for (int e = 0; e < E; ++e)
-                if (e < n_wave && e < n) acc.add(h[e], l[e], *reinterpret_cast<const V *>(xb + lc[e]));  // e < n_wave: wave-uniform
-            const double val = acc.value();
-            if (valid && f + u < f1) {
-                if (slot < 0)
-                    st(Y + (f + u) * A.n_dst + r, val);
-                else
-                    P[(f + u) * A.n_slots + slot] = val;
+                    for (int q = 0; q < kChunk; ++q) S.xs[u][lane_chunk * kChunk + q] = v[d][u][q];
+                __syncthreads();
+                if (f + kAhead < f1) fetch(f + kAhead, v[d]);
+#pragma unroll
+                for (int u = 0; u < FU; ++u) sums(u, f + u);
+                __syncthreads();
             }
         }
-        __syncthreads();
     }
 }
 
