@@ -64,6 +64,10 @@ def lib():
         "axis_generate_weights_config_c": [C.c_int, C.c_int, C.POINTER(RegridConfig), C.POINTER(C.c_int)],
         "axis_matrix_from_coo_c": [_I64, _I64, _D, C.c_int64, C.c_int64, C.c_int64, _D, _D, _D, _D, C.POINTER(C.c_int)],
         "axis_matrix_to_csr_c": [C.c_int],
+        "axis_matrix_serialize_c": [C.c_int, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)],
+        "axis_matrix_deserialize_c": [C.c_void_p, C.c_uint64, C.POINTER(C.c_int)],
+        "axis_matrix_write_esmf_c": [C.c_int, C.c_char_p],
+        "axis_matrix_read_esmf_c": [C.c_char_p, C.POINTER(C.c_int)],
         "axis_matrix_info_c": [C.c_int, _I64, _I64, _I64],
         "axis_matrix_csr_c": [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p],
         "axis_matrix_fractions_c": [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
@@ -117,7 +121,8 @@ def exported_symbols():
             "axis_batch_plan_create_c", "axis_batch_plan_run_c", "axis_batch_plan_info_c", "axis_destroy_batch_plan_c",
             "axis_mlp_create_c", "axis_mlp_params_c", "axis_mlp_set_params_c", "axis_mlp_forward_c",
             "axis_mlp_gradient_c", "axis_mlp_train_c", "axis_destroy_mlp_c", "axis_generate_weights_proposed_c",
-            "axis_proposer_training_data_c"]
+            "axis_proposer_training_data_c", "axis_matrix_serialize_c", "axis_matrix_deserialize_c",
+            "axis_matrix_write_esmf_c", "axis_matrix_read_esmf_c"]
 
 
 def _check(rc: int):
@@ -229,6 +234,21 @@ class Matrix:
     def to_csr(self):
         _check(lib().axis_matrix_to_csr_c(self.handle))
 
+    def to_bytes(self) -> bytes:
+        """The AXWC weight blob (WeightCache::serialize)."""
+        n = C.c_uint64(0)
+        _check(lib().axis_matrix_serialize_c(self.handle, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        _check(lib().axis_matrix_serialize_c(self.handle, buf, n.value, C.byref(n)))
+        return buf.raw[:n.value]
+
+    @classmethod
+    def from_bytes(cls, data: bytes) -> "Matrix":
+        """A matrix from an AXWC weight blob (WeightCache::deserialize)."""
+        h = C.c_int(0)
+        _check(lib().axis_matrix_deserialize_c(data, len(data), C.byref(h)))
+        return cls(h.value)
+
     def csr(self):
         n_src, n_dst, nnz = self.info()
         rp = np.zeros(n_dst + 1, dtype=np.int64)
@@ -312,6 +332,18 @@ def generate_weights(src: Mesh, dst: Mesh, method: int = CONSERVATIVE) -> Matrix
 def generate(src: Mesh, dst: Mesh, config: RegridConfig) -> Matrix:
     h = C.c_int(0)
     _check(lib().axis_generate_weights_config_c(src.handle, dst.handle, C.byref(config), C.byref(h)))
+    return Matrix(h.value)
+
+
+def write_esmf(path, matrix: Matrix):
+    """EsmfWeightIO::write_esmf: S, 1-based col_idx / row_idx, n_a / n_b (NetCDF classic file)."""
+    _check(lib().axis_matrix_write_esmf_c(matrix.handle, str(path).encode()))
+
+
+def read_esmf(path) -> Matrix:
+    """EsmfWeightIO::read_esmf (NetCDF classic files; fractions and areas come back as zeros)."""
+    h = C.c_int(0)
+    _check(lib().axis_matrix_read_esmf_c(str(path).encode(), C.byref(h)))
     return Matrix(h.value)
 
 

@@ -150,6 +150,22 @@ class WeightGenerator {
 
 /// Builds a matrix from COO triplets (the reference's InterpolationMatrix
 /// constructor followed by to_csr()); duplicates are kept.
+/// AXWC weight blob (weight_cache.hpp): size, write (buf == nullptr: size
+/// query), header validation alone, and read.
+std::size_t axwc_size(index_t n_src, index_t n_dst, index_t nnz);
+std::size_t serialize_axwc(const InterpolationMatrix &M, void *buf, std::size_t buf_size);
+void check_axwc(const void *buf, std::size_t buf_size, index_t &n_src, index_t &n_dst, index_t &nnz);
+std::unique_ptr<InterpolationMatrix> deserialize_axwc(const void *buf, std::size_t buf_size);
+/// ESMF weight file (esmf_weight_io.cpp), NetCDF classic format: write;
+/// read = parse on the host (0-based triplets, indices checked) + build.
+void write_esmf(const std::string &path, const InterpolationMatrix &M);
+struct EsmfTriplets {
+    index_t n_src = 0, n_dst = 0;
+    std::vector<index_t> rows, cols;
+    std::vector<double> val;
+};
+EsmfTriplets parse_esmf(const std::string &path);
+std::unique_ptr<InterpolationMatrix> matrix_from_esmf(const EsmfTriplets &t);
 std::unique_ptr<InterpolationMatrix> matrix_from_coo(const index_t *rows, const index_t *cols, const double *vals, index_t nnz, index_t n_src,
                                                      index_t n_dst, const double *frac_a, const double *frac_b, const double *area_a,
                                                      const double *area_b);

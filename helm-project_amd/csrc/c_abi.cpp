@@ -297,6 +297,45 @@ int axis_matrix_from_coo_c(const int64_t *rows, const int64_t *cols, const doubl
     });
 }
 
+int axis_matrix_serialize_c(int matrix_handle, void *buf, uint64_t buf_size, uint64_t *bytes) {
+    return guarded([&] {
+        auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
+        if (buf) check_device();
+        const std::size_t n = serialize_axwc(*M, buf, static_cast<std::size_t>(buf_size));
+        if (bytes) *bytes = n;
+    });
+}
+
+int axis_matrix_deserialize_c(const void *buf, uint64_t buf_size, int *matrix_handle) {
+    return guarded([&] {
+        if (!matrix_handle) throw InvalidArgument("null argument");
+        index_t n_src = 0, n_dst = 0, nnz = 0;
+        check_axwc(buf, static_cast<std::size_t>(buf_size), n_src, n_dst, nnz);  // before touching the device
+        check_device();
+        std::shared_ptr<InterpolationMatrix> M = deserialize_axwc(buf, static_cast<std::size_t>(buf_size));
+        *matrix_handle = Registry::get().add(std::move(M));
+    });
+}
+
+int axis_matrix_write_esmf_c(int matrix_handle, const char *path) {
+    return guarded([&] {
+        if (!path) throw InvalidArgument("null path");
+        auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
+        check_device();
+        write_esmf(path, *M);
+    });
+}
+
+int axis_matrix_read_esmf_c(const char *path, int *matrix_handle) {
+    return guarded([&] {
+        if (!path || !matrix_handle) throw InvalidArgument("null argument");
+        const EsmfTriplets t = parse_esmf(path);  // before touching the device
+        check_device();
+        std::shared_ptr<InterpolationMatrix> M = matrix_from_esmf(t);
+        *matrix_handle = Registry::get().add(std::move(M));
+    });
+}
+
 int axis_matrix_to_csr_c(int matrix_handle) {
     return guarded([&] { (void)Registry::get().find<InterpolationMatrix>(matrix_handle); });
 }

@@ -205,6 +205,21 @@ int axis_generate_weights_proposed_c(int src_handle, int dst_handle, const axis_
  * containment guard radius / feature_scale); device pointers. */
 int axis_proposer_training_data_c(int src_handle, int dst_handle, float feature_scale, float *x, float *t, int64_t *n_unlabelled);
 
+/* Weight blob in the AXWC format (WeightCache::serialize / deserialize,
+ * weight_cache.hpp:64-157, 286-296): 40-byte header, then factor_list,
+ * factor_row, factor_col, frac_a, frac_b, area_a, area_b.  buf == NULL asks
+ * for the size (*bytes); the caller owns the buffer.  Deserialize validates
+ * magic, version, endianness and size with the reference's messages. */
+int axis_matrix_serialize_c(int matrix_handle, void *buf, uint64_t buf_size, uint64_t *bytes);
+int axis_matrix_deserialize_c(const void *buf, uint64_t buf_size, int *matrix_handle);
+
+/* ESMF weight files (EsmfWeightIO::write_esmf / read_esmf,
+ * esmf_weight_io.cpp:68-173): num_wgts, S, 1-based col_idx / row_idx, global
+ * n_a / n_b.  Written in the NetCDF classic 64-bit-offset format; the
+ * classic formats are read, a NetCDF-4 (HDF5) file is refused. */
+int axis_matrix_write_esmf_c(int matrix_handle, const char *path);
+int axis_matrix_read_esmf_c(const char *path, int *matrix_handle);
+
 /* axis_destroy_mesh_c / axis_destroy_matrix_c (axis_c_interop.cpp:241-250). */
 int axis_destroy_mesh_c(int handle);
 int axis_destroy_matrix_c(int handle);
