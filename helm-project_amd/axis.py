@@ -65,6 +65,8 @@ def lib():
         "axis_matrix_from_coo_c": [_I64, _I64, _D, C.c_int64, C.c_int64, C.c_int64, _D, _D, _D, _D, C.POINTER(C.c_int)],
         "axis_matrix_to_csr_c": [C.c_int],
         "axis_matrix_serialize_c": [C.c_int, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)],
+        "axis_nan_batch_apply_c": [C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_void_p],
+        "axis_apply_masked_c": [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_void_p],
         "axis_matrix_deserialize_c": [C.c_void_p, C.c_uint64, C.POINTER(C.c_int)],
         "axis_matrix_write_esmf_c": [C.c_int, C.c_char_p],
         "axis_matrix_read_esmf_c": [C.c_char_p, C.POINTER(C.c_int)],
@@ -122,7 +124,7 @@ def exported_symbols():
             "axis_mlp_create_c", "axis_mlp_params_c", "axis_mlp_set_params_c", "axis_mlp_forward_c",
             "axis_mlp_gradient_c", "axis_mlp_train_c", "axis_destroy_mlp_c", "axis_generate_weights_proposed_c",
             "axis_proposer_training_data_c", "axis_matrix_serialize_c", "axis_matrix_deserialize_c",
-            "axis_matrix_write_esmf_c", "axis_matrix_read_esmf_c"]
+            "axis_matrix_write_esmf_c", "axis_matrix_read_esmf_c", "axis_nan_batch_apply_c", "axis_apply_masked_c"]
 
 
 def _check(rc: int):
@@ -292,6 +294,22 @@ class Matrix:
         _check(fn(self.handle, _ptr(src) if not isinstance(src, int) else src, src_dtype,
                   _ptr(dst) if not isinstance(dst, int) else dst, dst_dtype, n_src, n_dst, n_fields, mode,
                   _stream(stream)))
+        return dst
+
+    def nan_batch_apply(self, src, dst, n_fields, na_thres, stream=None):
+        """nan_batch_apply on FP64 device fields ((k, n) torch tensors): NaN sources
+        skipped, rows renormalised by their valid weight."""
+        n_src, n_dst, _ = self.info()
+        _check(lib().axis_nan_batch_apply_c(self.handle, _ptr(src), _ptr(dst), n_src, n_dst, n_fields, na_thres, _stream(stream)))
+        return dst
+
+    def apply_masked(self, src, dst, dst_mask, n_fields=1, stream=None):
+        """Apply to FP64 device fields leaving rows with dst_mask == 0 unchanged
+        (the mask applies when it has n_dst entries)."""
+        n_src, n_dst, _ = self.info()
+        n_mask = 0 if dst_mask is None else int(dst_mask.numel() if hasattr(dst_mask, "numel") else len(dst_mask))
+        _check(lib().axis_apply_masked_c(self.handle, _ptr(src), _ptr(dst), _ptr(dst_mask), n_mask, n_src, n_dst, n_fields,
+                                         _stream(stream)))
         return dst
 
     def autotune(self, src, dst, n_fields, src_dtype, dst_dtype, mode, stream=None):

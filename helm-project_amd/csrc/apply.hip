@@ -466,6 +466,51 @@ ConservationReport check_conservation(const InterpolationMatrix &A, const double
     return r;
 }
 
+// ------------------------------------------- NaN-aware and masked applies
+// Both run through the segment/tile machinery of the apply (spatial order,
+// LDS-staged source columns) with their own row accumulators; long rows are
+// finished whole by a per-(row, field) kernel, so every row is summed in CSR
+// order.
+namespace {
+apply_detail::Launch special_launch(InterpolationMatrix &A, index_t k, hipStream_t s) {
+    A.prepare_apply(s);
+    apply_detail::Launch L = apply_detail::heuristic(A, k, ApplyMode::Plain);
+    L.tiled = true;
+    A.prepare_tiles(s);
+    return L;
+}
+}  // namespace
+
+void nan_batch_apply(InterpolationMatrix &A, const double *x, double *y, index_t n_src, index_t n_dst, index_t k, double na_thres,
+                     hipStream_t s) {
+    // apply.hpp:362-370: validation before dst is touched
+    if (n_src != A.n_src) throw InvalidArgument("axis::solver::nan_batch_apply: src.extent(0) != matrix.n_src()");
+    if (n_dst != A.n_dst) throw InvalidArgument("axis::solver::nan_batch_apply: dst.extent(0) != matrix.n_dst()");
+    if (k < 0) throw InvalidArgument("axis::solver::nan_batch_apply: src.extent(1) != dst.extent(1)");
+    if (k == 0 || n_dst == 0) return;
+    const apply_detail::Launch L = special_launch(A, k, s);
+    apply_detail::SpecialArgs sa;
+    sa.thres = na_thres;
+    apply_detail::launch_xy<apply_detail::kNanAware, double, double>(A, x, y, k, L, s, sa);
+}
+
+void masked_apply(InterpolationMatrix &A, const double *x, double *y, const std::int32_t *dst_mask, index_t n_mask, index_t n_src,
+                  index_t n_dst, index_t k, hipStream_t s) {
+    // apply.hpp:453-461
+    if (n_src != A.n_src)
+        throw InvalidArgument("axis::solver::apply (masked): src.extent(0) (" + std::to_string(n_src) + ") != matrix.n_src() (" +
+                              std::to_string(A.n_src) + ")");
+    if (n_dst != A.n_dst)
+        throw InvalidArgument("axis::solver::apply (masked): dst.extent(0) (" + std::to_string(n_dst) + ") != matrix.n_dst() (" +
+                              std::to_string(A.n_dst) + ")");
+    if (k < 0) throw InvalidArgument("apply (masked): negative field count");
+    if (k == 0 || n_dst == 0) return;
+    const apply_detail::Launch L = special_launch(A, k, s);
+    apply_detail::SpecialArgs sa;
+    sa.mask = (dst_mask && n_mask == n_dst) ? dst_mask : nullptr;  // apply.hpp:464
+    apply_detail::launch_xy<apply_detail::kMasked, double, double>(A, x, y, k, L, s, sa);
+}
+
 // ------------------------------------------------------ COO constructor
 std::unique_ptr<InterpolationMatrix> matrix_from_coo(const index_t *rows, const index_t *cols, const double *vals, index_t nnz, index_t n_src,
                                                      index_t n_dst, const double *frac_a, const double *frac_b, const double *area_a,

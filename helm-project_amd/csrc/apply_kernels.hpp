@@ -52,7 +52,13 @@ constexpr int kChunk = 4;          // source columns staged per aligned chunk (o
 constexpr int kTileChunks = kTileSeg;             // chunks a staged tile holds: one per lane
 constexpr int kTileCols = kTileChunks * kChunk;   // staged values per field
 
-enum Num : int { kPlain = 0, kComp32 = 1, kAcc64 = 2 };
+enum Num : int { kPlain = 0, kComp32 = 1, kAcc64 = 2, kNanAware = 3, kMasked = 4 };
+
+/// kNanAware / kMasked: FP64 fields and weights, each whole row summed in CSR
+/// order with separate multiply and add (the reference's host arithmetic of
+/// nan_batch_apply and the masked apply, apply.hpp:359-484); long rows are
+/// finished by long_rows_special_kernel instead of partial sums.
+constexpr bool is_special(int num) { return num == kNanAware || num == kMasked; }
 
 /// Launch shape chosen by the autotuner: FU fields per step and the number of
 /// field chunks the k fields are split into (more chunks = more parallelism,
@@ -109,6 +115,10 @@ struct Mat {
     const std::int32_t *tile_nu, *tile_col;
     const std::uint16_t *lcol;
     index_t n_tiles;
+    // nan-aware / masked applies: the valid-fraction threshold, the
+    // destination mask (nullptr: none)
+    double thres;
+    const std::int32_t *mask;
 };
 
 /// Hardware places workgroup b on XCD b % 8; give XCD x a contiguous range of
@@ -140,6 +150,16 @@ struct NumTraits<kComp32> {
     using V = float;
     using W = float;
 };
+template <>
+struct NumTraits<kNanAware> {
+    using V = double;
+    using W = double;
+};
+template <>
+struct NumTraits<kMasked> {
+    using V = double;
+    using W = double;
+};
 
 /// Row-sum accumulator of one numerics mode.
 template <int NUM>
@@ -158,6 +178,44 @@ struct RowAcc<kAcc64> {
 };
 template <>
 struct RowAcc<kComp32> : Acc32 {};
+template <>
+struct RowAcc<kNanAware> {
+    double ws = 0.0, sv = 0.0, total = 0.0, thres = 0.0;
+    __device__ inline void add(double h, double, double x) {
+        if (!isnan(x)) {
+            ws += h * x;
+            sv += h;
+        }
+    }
+    __device__ inline double value() const {
+        if (sv == 0.0 || total == 0.0) return __builtin_nan("");
+        const double frac = sv / total;
+        return frac < (1.0 - thres - 1e-6) ? __builtin_nan("") : ws / sv;
+    }
+};
+template <>
+struct RowAcc<kMasked> {
+    double s = 0.0;
+    __device__ inline void add(double h, double, double x) { s += h * x; }
+    __device__ inline double value() const { return s; }
+};
+
+/// Start of a row's sums: the NaN-aware accumulator needs the row's total
+/// weight (CSR order) and the threshold.
+template <int NUM, int E, class W>
+__device__ inline void begin_row(RowAcc<NUM> &acc, const Mat &A, const W (&h)[E], int n) {
+    if constexpr (NUM == kNanAware) {
+        double t = 0.0;
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            if (e < n) t += h[e];
+        acc.total = t;
+        acc.thres = A.thres;
+    }
+}
+
+/// Whether the special modes write row r (whole rows only; masked rows stay).
+__device__ inline bool special_writes(const Mat &A, index_t r, int slot) { return slot < 0 && (A.mask == nullptr || A.mask[r] != 0); }
 
 template <int NUM>
 __device__ inline void load_w(const Mat &A, index_t q, typename NumTraits<NUM>::W &h, typename NumTraits<NUM>::W &l) {
@@ -205,15 +263,19 @@ __device__ inline void segment_apply(const Mat &A, const XT *X, YT *Y, double *P
 #pragma unroll
         for (int u = 0; u < FU; ++u) {
             RowAcc<NUM> acc;
+            begin_row(acc, A, h, n);
 #pragma unroll
             for (int e = 0; e < E; ++e)
                 if (e < n) acc.add(h[e], l[e], x[u][e]);
             const double v = acc.value();
             if (f + u < f1) {
-                if (slot < 0)
+                if constexpr (is_special(NUM)) {
+                    if (special_writes(A, r, slot)) st(Y + (f + u) * A.n_dst + r, v);
+                } else if (slot < 0) {
                     st(Y + (f + u) * A.n_dst + r, v);
-                else
+                } else {
                     P[(f + u) * A.n_slots + slot] = v;
+                }
             }
         }
     }
@@ -351,6 +413,7 @@ __device__ inline void tile_apply(const Mat &A, const XT *X, YT *Y, double *P, i
     const bool vec = (A.n_src % kChunk) == 0 && (reinterpret_cast<std::uintptr_t>(X) % 16) == 0;
     auto sums = [&](int u, index_t f) {
         RowAcc<NUM> acc;
+        begin_row(acc, A, h, n);
         const char *xb = reinterpret_cast<const char *>(&S.xs[u][0]);
         // terms in groups of 4 (E is a multiple of 4): the group's LDS reads
         // are issued together; the bound is wave-uniform
@@ -366,10 +429,13 @@ __device__ inline void tile_apply(const Mat &A, const XT *X, YT *Y, double *P, i
         }
         const double val = acc.value();
         if (valid && f < f1) {
-            if (slot < 0)
+            if constexpr (is_special(NUM)) {
+                if (special_writes(A, r, slot)) st(Y + f * A.n_dst + r, val);
+            } else if (slot < 0) {
                 st(Y + f * A.n_dst + r, val);
-            else
+            } else {
                 P[f * A.n_slots + slot] = val;
+            }
         }
     };
     if (!vec) {
@@ -463,6 +529,39 @@ __global__ void long_reduce_kernel(Mat A, const double *P, YT *Y, index_t k) {
     long_reduce(A, P, Y, k, t);
 }
 
+/// Long rows of the special modes: one lane per (long row, field), the whole
+/// row in CSR order (loads issued eight entries ahead of the sums).
+template <int NUM>
+__global__ void long_rows_special_kernel(Mat A, const index_t *row_ptr, const double *X, double *Y, index_t k) {
+    const index_t t = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<index_t>(A.n_long) * k) return;
+    const index_t j = t % A.n_long, f = t / A.n_long;
+    const index_t r = A.long_row[j];
+    if (!special_writes(A, r, -1)) return;
+    const index_t b = row_ptr[r], e = row_ptr[r + 1];
+    const double *xf = X + f * A.n_src;
+    RowAcc<NUM> acc;
+    if constexpr (NUM == kNanAware) {
+        double tot = 0.0;
+        for (index_t q = b; q < e; ++q) tot += A.val[q];
+        acc.total = tot;
+        acc.thres = A.thres;
+    }
+    index_t q = b;
+    for (; q + 8 <= e; q += 8) {
+        double w[8], x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            w[u] = A.val[q + u];
+            x[u] = xf[A.col[q + u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc.add(w[u], 0.0, x[u]);
+    }
+    for (; q < e; ++q) acc.add(A.val[q], 0.0, xf[A.col[q]]);
+    Y[f * A.n_dst + r] = acc.value();
+}
+
 inline Mat mat_of(const InterpolationMatrix &A) {
     Mat M{};
     M.col = A.col32.get();
@@ -484,6 +583,8 @@ inline Mat mat_of(const InterpolationMatrix &A) {
     M.tile_col = A.tile_col.get();
     M.lcol = A.t_lcol.get();
     M.n_tiles = A.n_tiles;
+    M.thres = 0.0;
+    M.mask = nullptr;
     return M;
 }
 
@@ -496,9 +597,17 @@ inline double *partials(InterpolationMatrix &A, index_t k) {
     return A.partial_ws.get();
 }
 
+/// Extra operands of the special modes.
+struct SpecialArgs {
+    double thres = 0.0;
+    const std::int32_t *mask = nullptr;
+};
+
 template <int E, int FU, int NUM, class XT, class YT>
-void launch_e(InterpolationMatrix &A, const void *x, void *y, index_t k, int chunks, bool tiled, hipStream_t s) {
-    const Mat M = mat_of(A);
+void launch_e(InterpolationMatrix &A, const void *x, void *y, index_t k, int chunks, bool tiled, hipStream_t s, const SpecialArgs &sa) {
+    Mat M = mat_of(A);
+    M.thres = sa.thres;
+    M.mask = sa.mask;
     double *P = partials(A, k);
     const unsigned nb = static_cast<unsigned>(std::max<index_t>(tiled ? A.n_tiles : (A.n_seg + kBlock - 1) / kBlock, 1));
     const std::uint64_t nwork = static_cast<std::uint64_t>(nb) * chunks;
@@ -514,27 +623,31 @@ void launch_e(InterpolationMatrix &A, const void *x, void *y, index_t k, int chu
     }
     if (A.n_long_rows > 0) {
         const index_t n = static_cast<index_t>(A.n_long_rows) * k;
-        long_reduce_kernel<YT><<<grid_for(n, 256), 256, 0, s>>>(M, P, static_cast<YT *>(y), k);
+        if constexpr (is_special(NUM))
+            long_rows_special_kernel<NUM><<<grid_for(n, 256), 256, 0, s>>>(M, A.row_ptr.get(), static_cast<const double *>(x),
+                                                                          static_cast<double *>(y), k);
+        else
+            long_reduce_kernel<YT><<<grid_for(n, 256), 256, 0, s>>>(M, P, static_cast<YT *>(y), k);
         AXIS_LAUNCH_CHECK();
     }
 }
 
 template <int FU, int NUM, class XT, class YT>
-void launch_fu(InterpolationMatrix &A, const void *x, void *y, index_t k, int chunks, bool tiled, hipStream_t s) {
+void launch_fu(InterpolationMatrix &A, const void *x, void *y, index_t k, int chunks, bool tiled, hipStream_t s, const SpecialArgs &sa) {
     switch (A.seg_cap) {
-        case 4: return launch_e<4, FU, NUM, XT, YT>(A, x, y, k, chunks, tiled, s);
-        case 8: return launch_e<8, FU, NUM, XT, YT>(A, x, y, k, chunks, tiled, s);
-        case 12: return launch_e<12, FU, NUM, XT, YT>(A, x, y, k, chunks, tiled, s);
-        default: return launch_e<16, FU, NUM, XT, YT>(A, x, y, k, chunks, tiled, s);
+        case 4: return launch_e<4, FU, NUM, XT, YT>(A, x, y, k, chunks, tiled, s, sa);
+        case 8: return launch_e<8, FU, NUM, XT, YT>(A, x, y, k, chunks, tiled, s, sa);
+        case 12: return launch_e<12, FU, NUM, XT, YT>(A, x, y, k, chunks, tiled, s, sa);
+        default: return launch_e<16, FU, NUM, XT, YT>(A, x, y, k, chunks, tiled, s, sa);
     }
 }
 
 template <int NUM, class XT, class YT>
-void launch_xy(InterpolationMatrix &A, const void *x, void *y, index_t k, const Launch &L, hipStream_t s) {
+void launch_xy(InterpolationMatrix &A, const void *x, void *y, index_t k, const Launch &L, hipStream_t s, const SpecialArgs &sa = SpecialArgs{}) {
     switch (L.fu) {
-        case 1: return launch_fu<1, NUM, XT, YT>(A, x, y, k, L.chunks, L.tiled, s);
-        case 2: return launch_fu<2, NUM, XT, YT>(A, x, y, k, L.chunks, L.tiled, s);
-        case 4: return launch_fu<4, NUM, XT, YT>(A, x, y, k, L.chunks, L.tiled, s);
+        case 1: return launch_fu<1, NUM, XT, YT>(A, x, y, k, L.chunks, L.tiled, s, sa);
+        case 2: return launch_fu<2, NUM, XT, YT>(A, x, y, k, L.chunks, L.tiled, s, sa);
+        case 4: return launch_fu<4, NUM, XT, YT>(A, x, y, k, L.chunks, L.tiled, s, sa);
         default: throw InvalidArgument("apply: unsupported fields per step");
     }
 }

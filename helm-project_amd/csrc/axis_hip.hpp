@@ -150,6 +150,15 @@ class WeightGenerator {
 
 /// Builds a matrix from COO triplets (the reference's InterpolationMatrix
 /// constructor followed by to_csr()); duplicates are kept.
+/// nan_batch_apply (apply.hpp:359-420) on FP64 layout_left fields (device
+/// pointers): NaN sources are left out and each row renormalised by its
+/// valid weight; NaN where the valid fraction is below 1 - na_thres - 1e-6.
+void nan_batch_apply(InterpolationMatrix &A, const double *x, double *y, index_t n_src, index_t n_dst, index_t k, double na_thres,
+                     hipStream_t s);
+/// Masked apply (apply.hpp:438-484): destination rows with mask 0 keep their
+/// values; the mask applies when it has n_dst entries (n_mask == n_dst).
+void masked_apply(InterpolationMatrix &A, const double *x, double *y, const std::int32_t *dst_mask, index_t n_mask, index_t n_src,
+                  index_t n_dst, index_t k, hipStream_t s);
 /// AXWC weight blob (weight_cache.hpp): size, write (buf == nullptr: size
 /// query), header validation alone, and read.
 std::size_t axwc_size(index_t n_src, index_t n_dst, index_t nnz);

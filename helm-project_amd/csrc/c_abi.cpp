@@ -424,6 +424,26 @@ int axis_batch_apply_c(int matrix_handle, const void *src, int src_dtype, void *
     });
 }
 
+int axis_nan_batch_apply_c(int matrix_handle, const double *src, double *dst, int64_t n_src, int64_t n_dst, int64_t n_fields, double na_thres,
+                           void *stream) {
+    return guarded([&] {
+        auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
+        auto s = static_cast<hipStream_t>(stream);
+        nan_batch_apply(*M, src, dst, n_src, n_dst, n_fields, na_thres, s);
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
+int axis_apply_masked_c(int matrix_handle, const double *src, double *dst, const int32_t *dst_mask, int64_t n_mask, int64_t n_src, int64_t n_dst,
+                        int64_t n_fields, void *stream) {
+    return guarded([&] {
+        auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
+        auto s = static_cast<hipStream_t>(stream);
+        masked_apply(*M, src, dst, dst_mask, n_mask, n_src, n_dst, n_fields, s);
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    });
+}
+
 int axis_autotune_apply_c(int matrix_handle, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n_fields, int mode, void *stream,
                           int *fields_per_step, int *field_chunks, int *staged) {
     return guarded([&] {

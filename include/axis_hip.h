@@ -205,6 +205,19 @@ int axis_generate_weights_proposed_c(int src_handle, int dst_handle, const axis_
  * containment guard radius / feature_scale); device pointers. */
 int axis_proposer_training_data_c(int src_handle, int dst_handle, float feature_scale, float *x, float *t, int64_t *n_unlabelled);
 
+/* nan_batch_apply (apply.hpp:359-420): FP64 layout_left fields [n, n_fields]
+ * as device pointers; NaN sources are skipped and each row renormalised by
+ * its valid weight; the output is NaN where the valid weight fraction is
+ * below 1 - na_thres - 1e-6 (or zero).  Synchronous on `stream`. */
+int axis_nan_batch_apply_c(int matrix_handle, const double *src, double *dst, int64_t n_src, int64_t n_dst, int64_t n_fields, double na_thres,
+                           void *stream);
+
+/* Masked apply (apply.hpp:438-484, the dst_mask overload of apply): rows with
+ * dst_mask == 0 keep their values; the mask applies when n_mask == n_dst.
+ * FP64 layout_left fields (device pointers), n_fields >= 1.  Synchronous. */
+int axis_apply_masked_c(int matrix_handle, const double *src, double *dst, const int32_t *dst_mask, int64_t n_mask, int64_t n_src, int64_t n_dst,
+                        int64_t n_fields, void *stream);
+
 /* Weight blob in the AXWC format (WeightCache::serialize / deserialize,
  * weight_cache.hpp:64-157, 286-296): 40-byte header, then factor_list,
  * factor_row, factor_col, frac_a, frac_b, area_a, area_b.  buf == NULL asks

@@ -22,6 +22,7 @@
 //   * generate_bilinear_rect_nonuniform              weight_generator_bilinear_rect_nonuniform.cpp:22-149
 //   * InterpolationMatrix::to_csr                    interpolation_matrix.hpp:180-255
 //   * apply / batch_apply (CSR, sequential rows)     apply.hpp:61-343
+//   * nan_batch_apply, masked apply                  apply.hpp:359-484
 //   * check_conservation                             conservation.cpp:23-103
 //
 // Broad phase: the reference queries an ArborX v2.1 BVH with float boxes
@@ -1033,6 +1034,54 @@ void om_batch_apply_csr(const int64_t *row_ptr, const int64_t *col, const double
     if (n_threads > 0) omp_set_num_threads(n_threads);
 #pragma omp parallel for schedule(static)
     for (i64 r = 0; r < n_rows; ++r) {
+        for (i64 f = 0; f < k; ++f) {
+            double s = 0.0;
+            for (i64 e = row_ptr[r]; e < row_ptr[r + 1]; ++e) s += val[e] * x[col[e] + f * n_src];
+            y[r + f * n_rows] = s;
+        }
+    }
+}
+
+// nan_batch_apply, apply.hpp:359-420: per row the total weight Σ w (CSR
+// order); per field the weighted sum and the valid-weight sum over non-NaN
+// sources (CSR order); NaN when no valid weight, a zero total, or a valid
+// fraction below 1 - na_thres - 1e-6; else weighted sum / valid weight.
+void om_nan_batch_apply(const int64_t *row_ptr, const int64_t *col, const double *val, int64_t n_rows, int64_t n_src, const double *x,
+                        double *y, int64_t k, double na_thres) {
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+#pragma omp parallel for schedule(static)
+    for (i64 r = 0; r < n_rows; ++r) {
+        double total = 0.0;
+        for (i64 e = row_ptr[r]; e < row_ptr[r + 1]; ++e) total += val[e];
+        for (i64 f = 0; f < k; ++f) {
+            double ws = 0.0, sv = 0.0;
+            for (i64 e = row_ptr[r]; e < row_ptr[r + 1]; ++e) {
+                const double v = x[col[e] + f * n_src];
+                if (!std::isnan(v)) {
+                    ws += val[e] * v;
+                    sv += val[e];
+                }
+            }
+            double out;
+            if (sv == 0.0 || total == 0.0) {
+                out = nan;
+            } else {
+                const double frac = sv / total;
+                out = frac < (1.0 - na_thres - 1e-6) ? nan : ws / sv;
+            }
+            y[r + f * n_rows] = out;
+        }
+    }
+}
+
+// masked apply, apply.hpp:438-484: rows whose mask is 0 keep their dst
+// value; the mask applies only when it has n_rows entries.  Sums in CSR order.
+void om_masked_apply(const int64_t *row_ptr, const int64_t *col, const double *val, int64_t n_rows, int64_t n_src, const double *x, double *y,
+                     int64_t k, const int32_t *mask, int64_t n_mask) {
+    const bool has_mask = mask != nullptr && n_mask == n_rows;
+#pragma omp parallel for schedule(static)
+    for (i64 r = 0; r < n_rows; ++r) {
+        if (has_mask && mask[r] == 0) continue;
         for (i64 f = 0; f < k; ++f) {
             double s = 0.0;
             for (i64 e = row_ptr[r]; e < row_ptr[r + 1]; ++e) s += val[e] * x[col[e] + f * n_src];

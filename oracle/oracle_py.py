@@ -63,6 +63,8 @@ def lib():
         L.om_result_free.argtypes = [C.c_void_p]
         L.om_to_csr.argtypes = [_I64, _I64, _D, C.c_int64, C.c_int64, _I64, _I64, _D]
         L.om_batch_apply_csr.argtypes = [_I64, _I64, _D, C.c_int64, C.c_int64, _D, _D, C.c_int64, C.c_int]
+        L.om_nan_batch_apply.argtypes = [_I64, _I64, _D, C.c_int64, C.c_int64, _D, _D, C.c_int64, C.c_double]
+        L.om_masked_apply.argtypes = [_I64, _I64, _D, C.c_int64, C.c_int64, _D, _D, C.c_int64, _I32, C.c_int64]
         L.om_check_conservation.argtypes = [_D, C.c_int64, _D, C.c_int64, _D, _D, _D, _D, C.c_int, _D]
         L.om_max_threads.restype = C.c_int
         _LIB = L
@@ -251,6 +253,37 @@ def batch_apply(row_ptr, col, val, n_src, x, n_threads: int = 0):
     y = np.zeros((n_rows, k), order="F")
     lib().om_batch_apply_csr(_p(row_ptr, _I64), _p(col, _I64), _p(val, _D), n_rows, n_src, xf.ctypes.data_as(_D),
                              y.ctypes.data_as(_D), k, n_threads)
+    return y[:, 0].copy() if x.ndim == 1 else y
+
+
+def nan_batch_apply(row_ptr, col, val, n_src, x, na_thres: float):
+    """nan_batch_apply (apply.hpp:359-420) for layout_left X [n_src, k] or 1-D."""
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int64)
+    col = np.ascontiguousarray(col, dtype=np.int64)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    x = np.asarray(x, dtype=np.float64)
+    k = 1 if x.ndim == 1 else x.shape[1]
+    xf = np.asfortranarray(x.reshape(n_src, k))
+    n_rows = row_ptr.shape[0] - 1
+    y = np.zeros((n_rows, k), order="F")
+    lib().om_nan_batch_apply(_p(row_ptr, _I64), _p(col, _I64), _p(val, _D), n_rows, n_src, xf.ctypes.data_as(_D),
+                             y.ctypes.data_as(_D), k, C.c_double(na_thres))
+    return y[:, 0].copy() if x.ndim == 1 else y
+
+
+def masked_apply(row_ptr, col, val, n_src, x, y0, mask):
+    """Masked apply (apply.hpp:438-484): rows with mask 0 keep y0's values."""
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int64)
+    col = np.ascontiguousarray(col, dtype=np.int64)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    x = np.asarray(x, dtype=np.float64)
+    k = 1 if x.ndim == 1 else x.shape[1]
+    xf = np.asfortranarray(x.reshape(n_src, k))
+    n_rows = row_ptr.shape[0] - 1
+    y = np.asfortranarray(np.array(y0, dtype=np.float64).reshape(n_rows, k))
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.int32)
+    lib().om_masked_apply(_p(row_ptr, _I64), _p(col, _I64), _p(val, _D), n_rows, n_src, xf.ctypes.data_as(_D), y.ctypes.data_as(_D), k,
+                          None if m is None else _p(m, _I32), 0 if m is None else m.shape[0])
     return y[:, 0].copy() if x.ndim == 1 else y
 
 

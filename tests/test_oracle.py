@@ -339,3 +339,59 @@ def test_golden_vectors_of_reference_geometry():
     for k, want in enumerate(z["poly_area"]):
         p = z["poly_xyz"][off[k]:off[k + 1]]
         assert O.polygon_area(p) == pytest.approx(want, rel=1e-12)
+
+
+# ---- NaN-aware and masked applies (apply.hpp:359-484): pinned by what the
+# definitions fix, on small hand-made matrices
+def _small_csr():
+    # 3 rows over 4 sources; row 1 has weights 0.3 / 0.7, row 2 is empty
+    rows = np.array([0, 0, 1, 1, 0], dtype=np.int64)
+    cols = np.array([0, 1, 2, 3, 3], dtype=np.int64)
+    vals = np.array([0.25, 0.5, 0.3, 0.7, 0.25])
+    return O.to_csr(rows, cols, vals, 3)
+
+
+def test_nan_apply_without_nans_is_weighted_mean():
+    rp, c, v = _small_csr()
+    x = np.array([1.0, 2.0, 3.0, 5.0])
+    plain = O.batch_apply(rp, c, v, 4, x)
+    out = O.nan_batch_apply(rp, c, v, 4, x, 0.0)
+    totals = np.array([v[rp[0]:rp[1]].sum(), v[rp[1]:rp[2]].sum()])
+    assert np.array_equal(out[:2], plain[:2] / totals)  # same sums, same order
+    assert math.isnan(out[2])  # empty row: no valid weight
+
+
+def test_nan_apply_threshold_and_renormalisation():
+    rp, c, v = _small_csr()
+    x = np.array([1.0, 2.0, np.nan, 5.0])  # row 1 keeps 0.7 of its weight
+    # valid fraction 0.7: NaN unless 0.7 >= 1 - na_thres - 1e-6
+    assert math.isnan(O.nan_batch_apply(rp, c, v, 4, x, 0.2)[1])
+    assert O.nan_batch_apply(rp, c, v, 4, x, 0.3)[1] == (0.7 * 5.0) / 0.7
+    assert O.nan_batch_apply(rp, c, v, 4, x, 0.5)[1] == (0.7 * 5.0) / 0.7
+    # all sources NaN: NaN whatever the threshold
+    assert math.isnan(O.nan_batch_apply(rp, c, v, 4, np.full(4, np.nan), 1.0)[0])
+    # row 0 with source 0 missing: (0.5*2 + 0.25*5) / 0.75
+    y = O.nan_batch_apply(rp, c, v, 4, np.array([np.nan, 2.0, 3.0, 5.0]), 0.5)
+    assert y[0] == (0.5 * 2.0 + 0.25 * 5.0) / (0.5 + 0.25)
+
+
+def test_nan_apply_fields_independent():
+    rp, c, v = _small_csr()
+    rng = np.random.default_rng(1)
+    X = rng.normal(size=(4, 3))
+    X[1, 0] = np.nan
+    Y = O.nan_batch_apply(rp, c, v, 4, X, 0.5)
+    for f in range(3):
+        assert np.array_equal(Y[:, f], O.nan_batch_apply(rp, c, v, 4, X[:, f], 0.5), equal_nan=True)
+
+
+def test_masked_apply_keeps_masked_rows():
+    rp, c, v = _small_csr()
+    x = np.array([1.0, 2.0, 3.0, 5.0])
+    y0 = np.array([-7.0, -8.0, -9.0])
+    plain = O.batch_apply(rp, c, v, 4, x)
+    y = O.masked_apply(rp, c, v, 4, x, y0, np.array([1, 0, 1]))
+    assert y[1] == -8.0 and y[0] == plain[0] and y[2] == 0.0
+    # a mask of the wrong length is ignored (apply.hpp:464)
+    assert np.array_equal(O.masked_apply(rp, c, v, 4, x, y0, np.array([0, 0])), plain)
+    assert np.array_equal(O.masked_apply(rp, c, v, 4, x, y0, None), plain)
