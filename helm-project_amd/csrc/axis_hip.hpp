@@ -294,6 +294,11 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
                                                            double area_tol = 1e-12);
 std::unique_ptr<InterpolationMatrix> generate_bilinear_rect(const Mesh &src, const Mesh &dst, const GridAxes &src_axes, hipStream_t s);
 void check_longitude_ranges(const Mesh &src, const Mesh &dst, bool structured_fast_path, hipStream_t s);
+/// coastal_renormalize_and_extrapolate (weight_generator.cpp:1736-1881):
+/// with a source mask, keeps the wet weights (renormalised where their row
+/// sum is in (0, 1)) and, for NearestWet, maps each row left empty to its
+/// nearest wet source cell with weight 1.  No-op without a source mask.
+void coastal_pass(const Mesh &src, const Mesh &dst, ExtrapolationAction extrap, InterpolationMatrix &M, hipStream_t s);
 
 }  // namespace detail
 }  // namespace axis_hip

@@ -127,8 +127,8 @@ std::unique_ptr<InterpolationMatrix> generate_bilinear_rect(const Mesh &src, con
 }  // namespace detail
 
 // WeightGenerator::generate, weight_generator.cpp:1887-2019: coordinate-system
-// check, longitude-disjoint check, method dispatch.  The coastal post-pass
-// (:1736-1743) returns its input when the source has no mask.
+// check, longitude-disjoint check, method dispatch, then the masked-source
+// post-pass (:2018), which returns its input when the source has no mask.
 std::unique_ptr<InterpolationMatrix> WeightGenerator::generate(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, hipStream_t s) {
     return generate_proposed(src, dst, cfg, nullptr, nullptr, s);
 }
@@ -142,13 +142,12 @@ std::unique_ptr<InterpolationMatrix> WeightGenerator::generate_proposed(const Me
         throw InvalidArgument("WeightGenerator::generate: source and destination meshes have different CoordinateSystem values (src=" +
                               std::to_string(static_cast<int>(src.coord_system())) + ", dst=" + std::to_string(static_cast<int>(dst.coord_system())) +
                               ")");
-    if (src.has_mask())
-        throw InvalidArgument("WeightGenerator::generate: a source mask triggers the coastal renormalisation/extrapolation pass, "
-                              "which this library does not provide");
+    std::unique_ptr<InterpolationMatrix> M;
     switch (cfg.method) {
         case InterpolationMethod::Conservative1stOrder: {
             detail::check_longitude_ranges(src, dst, false, s);
-            return detail::generate_conservative(src, dst, cfg, s, proposer, stats, area_tol);
+            M = detail::generate_conservative(src, dst, cfg, s, proposer, stats, area_tol);
+            break;
         }
         case InterpolationMethod::Bilinear: {
             detail::GridAxes sax = detail::unique_axes(src, s);
@@ -164,11 +163,16 @@ std::unique_ptr<InterpolationMatrix> WeightGenerator::generate_proposed(const Me
             if (!s_rect)
                 throw InvalidArgument("WeightGenerator::generate: bilinear needs a rectilinear lat-lon source here (the reference's BVH "
                                       "bilinear path is not provided)");
-            return detail::generate_bilinear_rect(src, dst, sax, s);
+            M = detail::generate_bilinear_rect(src, dst, sax, s);
+            break;
         }
         default:
             throw InvalidArgument("WeightGenerator::generate: only Bilinear and Conservative1stOrder are provided");
     }
+    const auto t0 = std::chrono::steady_clock::now();
+    detail::coastal_pass(src, dst, cfg.extrap_method, *M, s);  // :2018
+    M->gen_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return M;
 }
 
 }  // namespace axis_hip

@@ -632,7 +632,9 @@ void to_csr(Result &r, const std::vector<i64> &rows, const std::vector<i64> &col
 
 // generate_conservative_impl<3> (great circle),
 // weight_generator.cpp:3304-3418 (setup) and :3586-3769 (loop, packing).
-void conservative(const Mesh &src, const Mesh &dst, int norm_type, int unmapped_error, int brute, int dump_candidates, Result &r) {
+void coastal(const Mesh &src, const Mesh &dst, int extrap, std::vector<i64> &rows, std::vector<i64> &cols, std::vector<double> &vals);
+
+void conservative(const Mesh &src, const Mesh &dst, int norm_type, int unmapped_error, int brute, int dump_candidates, int extrap, Result &r) {
     const i64 n_src = src.n_cells, n_dst = dst.n_cells;
     r.n_src = n_src;
     r.n_dst = n_dst;
@@ -748,7 +750,116 @@ void conservative(const Mesh &src, const Mesh &dst, int norm_type, int unmapped_
     r.frac_b = fb;
     r.area_a = sa;
     r.area_b = da;
+    coastal(src, dst, extrap, rows, cols, vals);
     to_csr(r, rows, cols, vals);
+}
+
+// compute_cell_centroids_xy, weight_generator.cpp:74-138: on a sphere the
+// mean of the vertices' unit vectors, back to lon in [0, 2pi) and lat in the
+// mesh's unit; otherwise the mean of the first two coordinates.
+void centroid_lonlat(const Mesh &m, i64 c, double &cx, double &cy) {
+    const double pi = 3.14159265358979323846;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    const i64 b = m.offsets[c], e = m.offsets[c + 1];
+    if (m.coord_sys == 2) {
+        for (i64 k = b; k < e; ++k) {
+            sx += m.lon(m.indices[k]);
+            sy += m.lat(m.indices[k]);
+        }
+        const double inv = (e > b) ? 1.0 / (double)(e - b) : 0.0;
+        cx = sx * inv;
+        cy = sy * inv;
+        return;
+    }
+    for (i64 k = b; k < e; ++k) {
+        double lon = m.lon(m.indices[k]), lat = m.lat(m.indices[k]);
+        if (m.coord_sys == 0) {
+            lon = lon * pi / 180.0;
+            lat = lat * pi / 180.0;
+        }
+        sx += std::cos(lat) * std::cos(lon);
+        sy += std::cos(lat) * std::sin(lon);
+        sz += std::sin(lat);
+    }
+    const double inv = (e > b) ? 1.0 / (double)(e - b) : 0.0;
+    const double ax = sx * inv, ay = sy * inv, az = sz * inv;
+    double lat_avg = std::asin(az);
+    double lon_avg = std::atan2(ay, ax);
+    if (lon_avg < 0.0) lon_avg += 2.0 * pi;
+    if (m.coord_sys == 0) {
+        lon_avg = lon_avg * 180.0 / pi;
+        lat_avg = lat_avg * 180.0 / pi;
+    }
+    cx = lon_avg;
+    cy = lat_avg;
+}
+
+// coastal_renormalize_and_extrapolate, weight_generator.cpp:1736-1881, on
+// the generator's COO triplets in their generation order (before to_csr).
+// Wet weights (mask > 0) are kept, a row's kept weights divided by their sum
+// when that sum is in (0, 1); with NearestWet, a row left without weights
+// gets weight 1 at the wet (mask != 0) source cell whose float (lon, lat)
+// centroid is nearest in the plane (ArborX nearest on Point<2>; the
+// restatement takes the smallest squared float distance, then the smallest
+// index -- exact ties are the one case ArborX leaves open).
+void coastal(const Mesh &src, const Mesh &dst, int extrap, std::vector<i64> &rows, std::vector<i64> &cols, std::vector<double> &vals) {
+    if (src.mask == nullptr) return;
+    const i64 n_dst = dst.n_cells, n_src = src.n_cells;
+    std::vector<double> row_sums(n_dst, 0.0);
+    for (size_t k = 0; k < vals.size(); ++k)
+        if (src.mask[cols[k]] > 0) row_sums[rows[k]] += vals[k];
+    std::vector<i64> ur, uc;
+    std::vector<double> uv;
+    for (size_t k = 0; k < vals.size(); ++k) {
+        if (src.mask[cols[k]] > 0) {
+            double w = vals[k];
+            const double s = row_sums[rows[k]];
+            if (s > 0.0 && s < 1.0) w /= s;
+            ur.push_back(rows[k]);
+            uc.push_back(cols[k]);
+            uv.push_back(w);
+        }
+    }
+    std::vector<char> has(n_dst, 0);
+    for (i64 r : ur) has[r] = 1;
+    if (extrap == 1) {
+        std::vector<float> wx, wy;
+        std::vector<i64> wi;
+        for (i64 i = 0; i < n_src; ++i) {
+            if (src.mask[i] != 0) {
+                double cx, cy;
+                centroid_lonlat(src, i, cx, cy);
+                wx.push_back((float)cx);
+                wy.push_back((float)cy);
+                wi.push_back(i);
+            }
+        }
+        if (!wi.empty()) {
+            for (i64 j = 0; j < n_dst; ++j) {
+                if (has[j]) continue;
+                double cx, cy;
+                centroid_lonlat(dst, j, cx, cy);
+                const float qx = (float)cx, qy = (float)cy;
+                float best = std::numeric_limits<float>::infinity();
+                i64 bi = -1;
+                for (size_t q = 0; q < wi.size(); ++q) {
+                    const float dx = wx[q] - qx, dy = wy[q] - qy;
+                    const float d2 = dx * dx + dy * dy;
+                    if (d2 < best || (d2 == best && bi < 0)) {
+                        best = d2;
+                        bi = (i64)q;
+                    }
+                }
+                if (bi < 0) continue;  // NaN distances only
+                ur.push_back(j);
+                uc.push_back(wi[bi]);
+                uv.push_back(1.0);
+            }
+        }
+    }
+    rows.swap(ur);
+    cols.swap(uc);
+    vals.swap(uv);
 }
 
 // generate_bilinear_rect_nonuniform, weight_generator_bilinear_rect_nonuniform.cpp:22-149
@@ -950,7 +1061,7 @@ void om_aabbs(const double *coords, int64_t n_nodes, const int64_t *offsets, con
 void *om_generate(const double *s_coords, int64_t s_nn, const int64_t *s_off, const int64_t *s_idx, int64_t s_nc, const double *s_areas,
                   const int32_t *s_mask, const double *d_coords, int64_t d_nn, const int64_t *d_off, const int64_t *d_idx, int64_t d_nc,
                   const double *d_areas, const int32_t *d_mask, int coord_sys, int method, int norm_type, int unmapped_error, int flags,
-                  int n_threads) {
+                  int extrap, int n_threads) {
     if (n_threads > 0) omp_set_num_threads(n_threads);
     Result *r = new Result;
     Mesh src = make_mesh(s_coords, s_nn, s_off, s_idx, s_nc, coord_sys, s_areas, s_mask);
@@ -965,7 +1076,7 @@ void *om_generate(const double *s_coords, int64_t s_nn, const int64_t *s_off, co
             r->error = "regular/rectilinear fast path not restated";
             return r;
         }
-        conservative(src, dst, norm_type, unmapped_error, flags & 1, flags & 2, *r);
+        conservative(src, dst, norm_type, unmapped_error, flags & 1, flags & 2, extrap, *r);
     } else if (method == 0) {
         // generate_bilinear, weight_generator.cpp:2306-2345
         if (detect_regular(src).is_regular && detect_regular(dst).is_regular) {
@@ -978,6 +1089,17 @@ void *om_generate(const double *s_coords, int64_t s_nn, const int64_t *s_off, co
             return r;
         }
         bilinear_rect(src, dst, ri, *r);
+        if (src.mask != nullptr) {
+            // coastal pass (:2018) over the CSR triplets: the reference's
+            // COO lists a row's four corners in another order, which moves
+            // the wet-weight sums by rounding only.
+            std::vector<i64> rows, cols(r->col.begin(), r->col.end());
+            std::vector<double> vals(r->val.begin(), r->val.end());
+            for (i64 j = 0; j < dst.n_cells; ++j)
+                for (i64 e = r->row_ptr[j]; e < r->row_ptr[j + 1]; ++e) rows.push_back(j);
+            coastal(src, dst, extrap, rows, cols, vals);
+            to_csr(*r, rows, cols, vals);
+        }
     } else {
         r->error = "method not restated";
     }

@@ -49,7 +49,7 @@ def lib():
         L.om_aabbs.argtypes = [_D, C.c_int64, _I64, _I64, C.c_int64, C.c_int, _F]
         L.om_generate.restype = C.c_void_p
         L.om_generate.argtypes = [_D, C.c_int64, _I64, _I64, C.c_int64, _D, _I32, _D, C.c_int64, _I64, _I64, C.c_int64, _D,
-                                  _I32, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+                                  _I32, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.om_result_error.restype = C.c_char_p
         L.om_result_error.argtypes = [C.c_void_p]
         L.om_result_nnz.restype = C.c_int64
@@ -184,7 +184,7 @@ class OracleError(RuntimeError):
 
 
 def generate(src, dst, method: int = 2, norm_type: int = 0, unmapped_error: bool = False, brute: bool = False,
-             keep_candidates: bool = False, coord_sys: int = 0, n_threads: int = 0):
+             keep_candidates: bool = False, coord_sys: int = 0, n_threads: int = 0, extrap: int = 1):
     """Weights as a dict (CSR + fractions + areas + stats); raises OracleError like the reference throws."""
     sc, so, si = _mesh_args(src)
     dc, do, di = _mesh_args(dst)
@@ -201,7 +201,7 @@ def generate(src, dst, method: int = 2, norm_type: int = 0, unmapped_error: bool
     h = L.om_generate(_p(sc, _D), src["coords"].shape[0], _p(so, _I64), _p(si, _I64), so.shape[0] - 1, _p(s_area, _D),
                       _p(s_mask, _I32), _p(dc, _D), dst["coords"].shape[0], _p(do, _I64), _p(di, _I64), do.shape[0] - 1,
                       _p(d_area, _D), _p(d_mask, _I32), coord_sys, method, norm_type, 1 if unmapped_error else 0, flags,
-                      n_threads)
+                      extrap, n_threads)
     try:
         err = L.om_result_error(h).decode()
         if err:
