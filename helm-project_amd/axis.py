@@ -58,6 +58,7 @@ def lib():
         "axis_init_c": [],
         "axis_mesh_create_c": [_D, C.c_int64, C.c_int64, _I64, C.c_int64, _I64, C.c_int, _D, _I32, C.POINTER(C.c_int)],
         "axis_mesh_regular_latlon_c": [C.c_double] * 6 + [C.POINTER(C.c_int)],
+        "axis_mesh_cell_centroids_c": [C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), _I64],
         "axis_mesh_info_c": [C.c_int, _I64, _I64],
         "axis_mesh_grid_info_c": [C.c_int, C.POINTER(C.c_int), _I64, _D],
         "axis_generate_weights_c": [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)],
@@ -116,6 +117,7 @@ def lib():
 def exported_symbols():
     """Names of the C entry points the binding declares."""
     return ["axis_init_c", "axis_last_error_c", "axis_mesh_create_c", "axis_mesh_regular_latlon_c", "axis_mesh_info_c", "axis_mesh_grid_info_c",
+            "axis_mesh_cell_centroids_c",
             "axis_generate_weights_c", "axis_generate_weights_config_c", "axis_matrix_from_coo_c",
             "axis_matrix_to_csr_c", "axis_matrix_info_c", "axis_matrix_csr_c", "axis_matrix_fractions_c",
             "axis_matrix_stats_c", "axis_apply_c", "axis_batch_apply_c", "axis_batch_apply_async_c",
@@ -197,6 +199,17 @@ class Mesh:
         out6 = np.zeros(6)
         _check(lib().axis_mesh_grid_info_c(self.handle, C.byref(kind), ninj.ctypes.data_as(_I64), out6.ctypes.data_as(_D)))
         return kind.value, ninj, out6
+
+    def cell_centroids(self):
+        """(lon, lat, n_host): float32 cell centroids as the masked-source
+        extrapolation uses them, and how many cells were recomputed on the host."""
+        _, nc = self.info()
+        lon = np.zeros(nc, dtype=np.float32)
+        lat = np.zeros(nc, dtype=np.float32)
+        nh = C.c_int64(0)
+        _check(lib().axis_mesh_cell_centroids_c(self.handle, lon.ctypes.data_as(C.POINTER(C.c_float)),
+                                                lat.ctypes.data_as(C.POINTER(C.c_float)), C.byref(nh)))
+        return lon, lat, nh.value
 
     def close(self):
         if getattr(self, "handle", 0):

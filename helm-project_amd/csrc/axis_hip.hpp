@@ -299,6 +299,10 @@ void check_longitude_ranges(const Mesh &src, const Mesh &dst, bool structured_fa
 /// sum is in (0, 1)) and, for NearestWet, maps each row left empty to its
 /// nearest wet source cell with weight 1.  No-op without a source mask.
 void coastal_pass(const Mesh &src, const Mesh &dst, ExtrapolationAction extrap, InterpolationMatrix &M, hipStream_t s);
+/// Float (lon, lat) centroids of every cell (compute_cell_centroids_xy,
+/// weight_generator.cpp:74-138, rounded to float) into device memory, equal
+/// to host libm's; returns how many cells were recomputed on the host.
+index_t cell_centroids_f32(const Mesh &m, float2 *out, hipStream_t s);
 
 }  // namespace detail
 }  // namespace axis_hip

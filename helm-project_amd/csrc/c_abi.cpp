@@ -229,6 +229,25 @@ int axis_mesh_grid_info_c(int mesh_handle, int *kind, int64_t *ninj, double *out
     });
 }
 
+int axis_mesh_cell_centroids_c(int mesh_handle, float *lon, float *lat, int64_t *n_host) {
+    return guarded([&] {
+        if (!lon || !lat) throw InvalidArgument("null argument");
+        auto m = Registry::get().find<Mesh>(mesh_handle);
+        check_device();
+        const index_t n = m->n_cells();
+        DeviceBuffer<float2> d(std::max<index_t>(n, 1));
+        const index_t fixed = detail::cell_centroids_f32(*m, d.get(), nullptr);
+        std::vector<float2> h(n);
+        d.download(h.data(), n, nullptr);
+        AXIS_HIP_CHECK(hipStreamSynchronize(nullptr));
+        for (index_t i = 0; i < n; ++i) {
+            lon[i] = h[i].x;
+            lat[i] = h[i].y;
+        }
+        if (n_host) *n_host = fixed;
+    });
+}
+
 int axis_generate_weights_config_c(int src_handle, int dst_handle, const axis_regrid_config *config, int *matrix_handle) {
     return guarded([&] {
         if (!config || !matrix_handle) throw InvalidArgument("null argument");

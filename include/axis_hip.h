@@ -83,6 +83,14 @@ int axis_mesh_info_c(int mesh_handle, int64_t *n_nodes, int64_t *n_cells);
  * (detail/regular_grid_detector.hpp:64-144, 180-299). */
 int axis_mesh_grid_info_c(int mesh_handle, int *kind, int64_t *ninj, double *out6);
 
+/* Cell centroids as the masked-source extrapolation uses them:
+ * compute_cell_centroids_xy (weight_generator.cpp:74-138) rounded to float
+ * (:1813, :1843), into host arrays lon[n_cells], lat[n_cells].  Computed on the
+ * GPU; the cells whose float could differ from host libm's are recomputed on
+ * the host, and *n_host (optional) receives their count.  Diagnostic: the
+ * reference computes these inside coastal_renormalize_and_extrapolate only. */
+int axis_mesh_cell_centroids_c(int mesh_handle, float *lon, float *lat, int64_t *n_host);
+
 /* axis_generate_weights_c (axis_c_interop.cpp:117-145): method 0 Bilinear,
  * 2 Conservative1stOrder with the default RegridConfig otherwise. */
 int axis_generate_weights_c(int src_handle, int dst_handle, int method, int *matrix_handle);

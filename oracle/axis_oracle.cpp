@@ -1045,6 +1045,17 @@ void om_cap_data(const double *coords, int64_t n_nodes, const int64_t *offsets, 
 int om_cap_rejects(const double *cs, double rs, const double *cd, double rd) {
     return cap_rejects(V3{cs[0], cs[1], cs[2]}, rs, V3{cd[0], cd[1], cd[2]}, rd) ? 1 : 0;
 }
+// compute_cell_centroids_xy rounded to float (weight_generator.cpp:1813).
+void om_cell_centroids(const double *coords, int64_t n_nodes, const int64_t *offsets, const int64_t *indices, int64_t n_cells, int cs, float *lon,
+                       float *lat) {
+    Mesh m = make_mesh(coords, n_nodes, offsets, indices, n_cells, cs, nullptr, nullptr);
+    for (i64 c = 0; c < n_cells; ++c) {
+        double cx, cy;
+        centroid_lonlat(m, c, cx, cy);
+        lon[c] = (float)cx;
+        lat[c] = (float)cy;
+    }
+}
 void om_aabbs(const double *coords, int64_t n_nodes, const int64_t *offsets, const int64_t *indices, int64_t n_cells, int cs, float *out6) {
     Mesh m = make_mesh(coords, n_nodes, offsets, indices, n_cells, cs, nullptr, nullptr);
     auto b = aabbs3d(m);

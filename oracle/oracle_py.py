@@ -47,6 +47,7 @@ def lib():
         L.om_cap_rejects.restype = C.c_int
         L.om_cap_rejects.argtypes = [_D, C.c_double, _D, C.c_double]
         L.om_aabbs.argtypes = [_D, C.c_int64, _I64, _I64, C.c_int64, C.c_int, _F]
+        L.om_cell_centroids.argtypes = [_D, C.c_int64, _I64, _I64, C.c_int64, C.c_int, _F, _F]
         L.om_generate.restype = C.c_void_p
         L.om_generate.argtypes = [_D, C.c_int64, _I64, _I64, C.c_int64, _D, _I32, _D, C.c_int64, _I64, _I64, C.c_int64, _D,
                                   _I32, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
@@ -305,3 +306,13 @@ def available() -> bool:
 
 
 os.environ.setdefault("OMP_PROC_BIND", "false")
+
+
+def cell_centroids(mesh, coord_sys: int = 0):
+    """compute_cell_centroids_xy rounded to float: (lon, lat) float32 arrays."""
+    c, o, i = _mesh_args(mesh)
+    n = o.shape[0] - 1
+    lon = np.zeros(n, dtype=np.float32)
+    lat = np.zeros(n, dtype=np.float32)
+    lib().om_cell_centroids(_p(c, _D), mesh["coords"].shape[0], _p(o, _I64), _p(i, _I64), n, coord_sys, _p(lon, _F), _p(lat, _F))
+    return lon, lat

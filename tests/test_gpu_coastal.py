@@ -173,3 +173,36 @@ def test_large_properties():
     say("  rows", n.shape[0], "single-entry", cand.size, "sampled nearest", n_extrap, "exact", exact)
     assert n_extrap > 100 and exact >= n_extrap - 3
     m.close()
+
+
+def _radians(mesh):
+    m = dict(mesh)
+    m["coords"] = np.asfortranarray(mesh["coords"] * (np.pi / 180.0))
+    return m
+
+
+CENTROID_MESHES = {
+    "0.25deg": (lambda: grids.latlon_deg(0.25), 0),
+    "C384": (lambda: grids.cubed_sphere(384), 0),
+    "5deg[-180,180]": (lambda: grids.regular_latlon(72, 36, -180.0, -90.0, 5.0, 5.0), 0),
+    "voronoi4": (lambda: grids.jittered_voronoi(4, 77), 0),
+    "C48 radians": (lambda: _radians(grids.cubed_sphere(48)), 1),
+    "1deg radians": (lambda: _radians(grids.latlon_deg(1.0)), 1),
+}
+
+
+@pytest.mark.parametrize("name", list(CENTROID_MESHES))
+def test_centroids_bit_exact(name):
+    """The device centroids, after the host recomputes the cells inside the
+    error band, equal host libm's floats for every cell."""
+    mk, cs = CENTROID_MESHES[name]
+    mesh = mk()
+    say("case centroids", name, grids.n_cells(mesh))
+    want_lon, want_lat = O.cell_centroids(mesh, cs)
+    m = axis.Mesh(mesh, coord_sys=cs)
+    lon, lat, n_host = m.cell_centroids()
+    bad = np.nonzero((lon != want_lon) | (lat != want_lat))[0]
+    say("  host-recomputed", n_host, "mismatches", bad.size)
+    assert bad.size == 0, (bad[:5], lon[bad[:5]], want_lon[bad[:5]], lat[bad[:5]], want_lat[bad[:5]])
+    assert n_host <= max(64, grids.n_cells(mesh) // 1000)
+    m.close()

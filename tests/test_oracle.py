@@ -465,3 +465,15 @@ def test_coastal_all_dry_and_no_mask():
     b = O.generate(src, dst, 2, extrap=1)
     for k in ("row_ptr", "col", "val"):
         assert np.array_equal(a[k], b[k])
+
+
+def test_centroids_match_numpy_restatement():
+    for mesh in (grids.latlon_deg(2.0), grids.cubed_sphere(16), grids.regular_latlon(72, 36, -180.0, -90.0, 5.0, 5.0)):
+        lon, lat = O.cell_centroids(mesh)
+        nl, na = np_centroids(mesh)
+        nl32, na32 = nl.astype(np.float32), na.astype(np.float32)
+        dl = np.abs(lon.astype(np.float64) - nl32)
+        dl = np.minimum(dl, np.abs(dl - 360.0))  # the lon = 0 branch cut
+        assert np.all(dl < 1e-4) and np.all(np.abs(lat - na32) < 1e-4)
+        assert np.mean((lon == nl32) & (lat == na32)) > 0.99
+        assert np.all((lon >= 0) & (lon <= 360))
