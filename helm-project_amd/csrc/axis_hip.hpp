@@ -299,6 +299,11 @@ void check_longitude_ranges(const Mesh &src, const Mesh &dst, bool structured_fa
 /// sum is in (0, 1)) and, for NearestWet, maps each row left empty to its
 /// nearest wet source cell with weight 1.  No-op without a source mask.
 void coastal_pass(const Mesh &src, const Mesh &dst, ExtrapolationAction extrap, InterpolationMatrix &M, hipStream_t s);
+/// NearestNeighbor weights (generate_nearest, weight_generator.cpp:2025-2271):
+/// the analytic index rounding when both meshes are regular lat-lon grids
+/// (src_reg and dst_reg given), else the nearest source centroid point.
+std::unique_ptr<InterpolationMatrix> generate_nearest(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, const MeshGeom *src_reg,
+                                                      const MeshGeom *dst_reg, hipStream_t s);
 /// Float (lon, lat) centroids of every cell (compute_cell_centroids_xy,
 /// weight_generator.cpp:74-138, rounded to float) into device memory, equal
 /// to host libm's; returns how many cells were recomputed on the host.

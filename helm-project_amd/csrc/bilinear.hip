@@ -142,6 +142,9 @@ std::unique_ptr<InterpolationMatrix> WeightGenerator::generate_proposed(const Me
         throw InvalidArgument("WeightGenerator::generate: source and destination meshes have different CoordinateSystem values (src=" +
                               std::to_string(static_cast<int>(src.coord_system())) + ", dst=" + std::to_string(static_cast<int>(dst.coord_system())) +
                               ")");
+    if (src.coord_system() == CoordinateSystem::Cartesian3D && cfg.method != InterpolationMethod::NearestNeighbor)
+        throw InvalidArgument("WeightGenerator::generate: Cartesian3D meshes are served by NearestNeighbor only here (the great-circle "
+                              "conservative and bilinear paths need spherical coordinates)");
     std::unique_ptr<InterpolationMatrix> M;
     switch (cfg.method) {
         case InterpolationMethod::Conservative1stOrder: {
@@ -166,8 +169,21 @@ std::unique_ptr<InterpolationMatrix> WeightGenerator::generate_proposed(const Me
             M = detail::generate_bilinear_rect(src, dst, sax, s);
             break;
         }
+        case InterpolationMethod::NearestNeighbor: {
+            // generate_nearest (host dispatch, weight_generator.cpp:2107-2127)
+            detail::check_longitude_ranges(src, dst, false, s);
+            detail::GridAxes sax = detail::unique_axes(src, s);
+            detail::MeshGeom sg, dg;
+            bool reg = detail::regular_from_axes(src, sax, sg);
+            if (reg) {
+                detail::GridAxes dax = detail::unique_axes(dst, s);
+                reg = detail::regular_from_axes(dst, dax, dg);
+            }
+            M = detail::generate_nearest(src, dst, cfg, reg ? &sg : nullptr, reg ? &dg : nullptr, s);
+            break;
+        }
         default:
-            throw InvalidArgument("WeightGenerator::generate: only Bilinear and Conservative1stOrder are provided");
+            throw InvalidArgument("WeightGenerator::generate: only Bilinear, NearestNeighbor and Conservative1stOrder are provided");
     }
     const auto t0 = std::chrono::steady_clock::now();
     detail::coastal_pass(src, dst, cfg.extrap_method, *M, s);  // :2018

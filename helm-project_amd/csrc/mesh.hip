@@ -34,7 +34,6 @@ Mesh::Mesh(const double *node_coords, index_t n_nodes, index_t ndim, const index
     : n_nodes_(n_nodes), n_cells_(n_cells), cs_(cs) {
     if (n_nodes < 0 || n_cells < 0) throw InvalidArgument("Mesh: negative size");
     if (ndim != 2) throw InvalidArgument("Mesh: only [n_nodes, 2] (lon, lat) node coordinates are supported");
-    if (cs == CoordinateSystem::Cartesian3D) throw InvalidArgument("Mesh: Cartesian3D coordinates are not supported on the great-circle path");
     if (n_cells > 0 && conn_offsets == nullptr) throw InvalidArgument("Mesh: null connectivity offsets");
     h_coords.assign(node_coords, node_coords + 2 * n_nodes);
     h_offsets.assign(conn_offsets, conn_offsets + (n_cells + 1));

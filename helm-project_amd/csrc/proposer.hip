@@ -315,6 +315,8 @@ void proposer_candidates(const MeshGeom &sg, const MeshGeom &dg, const mlp::Mode
 }
 
 void proposer_training_data(const Mesh &src, const Mesh &dst, float feature_scale, float *x, float *t, hipStream_t s, index_t *n_unlabelled) {
+    if (src.coord_system() == CoordinateSystem::Cartesian3D || dst.coord_system() == CoordinateSystem::Cartesian3D)
+        throw InvalidArgument("proposer_training_data: spherical meshes only");
     MeshGeom sg, dg;
     GridAxes sax = unique_axes(src, s), dax = unique_axes(dst, s);
     regular_from_axes(src, sax, sg);

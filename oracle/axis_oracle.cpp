@@ -862,6 +862,120 @@ void coastal(const Mesh &src, const Mesh &dst, int extrap, std::vector<i64> &row
     vals.swap(uv);
 }
 
+// get_cell_areas / compute_single_cell_area, weight_generator.cpp:264-313:
+// the mesh's areas, else the shoelace area in raw coordinates.
+std::vector<double> planar_areas(const Mesh &m) {
+    std::vector<double> a(m.n_cells);
+    for (i64 c = 0; c < m.n_cells; ++c) {
+        if (m.areas) {
+            a[c] = m.areas[c];
+            continue;
+        }
+        const i64 b = m.offsets[c], n = m.offsets[c + 1] - b;
+        if (n < 3) {
+            a[c] = 0.0;
+            continue;
+        }
+        double s = 0.0;
+        for (i64 i = 0; i < n; ++i) {
+            const i64 p = m.indices[b + i], q = m.indices[b + (i + 1) % n];
+            s += m.lon(p) * m.lat(q) - m.lon(q) * m.lat(p);
+        }
+        a[c] = std::abs(s) * 0.5;
+    }
+    return a;
+}
+
+// generate_nearest_rect, weight_generator.cpp:2025-2083: the dst cell's
+// (lon_min + i dlon, lat_min + j dlat) rounded to the nearest source index,
+// longitude wrapped, latitude clamped; fractions and areas left zero.
+void nearest_rect(const Mesh &src, const Mesh &dst, const RegularInfo &sg, const RegularInfo &dg, std::vector<i64> &rows, std::vector<i64> &cols,
+                  std::vector<double> &vals, Result &r) {
+    for (i64 j = 0; j < dst.n_cells; ++j) {
+        const i64 di = j % dg.ni, dj = j / dg.ni;
+        const double lon = dg.lon_min + (double)di * dg.delta_lon;
+        const double lat = dg.lat_min + (double)dj * dg.delta_lat;
+        i64 si = (i64)std::round((lon - sg.lon_min) / sg.delta_lon);
+        si = si % sg.ni;
+        if (si < 0) si += sg.ni;
+        i64 sj = (i64)std::round((lat - sg.lat_min) / sg.delta_lat);
+        if (sj < 0) sj = 0;
+        if (sj >= sg.nj) sj = sg.nj - 1;
+        rows.push_back(j);
+        cols.push_back(sj * sg.ni + si);
+        vals.push_back(1.0);
+    }
+    r.frac_a.assign(src.n_cells, 0.0);
+    r.frac_b.assign(dst.n_cells, 0.0);
+    r.area_a.assign(src.n_cells, 0.0);
+    r.area_b.assign(dst.n_cells, 0.0);
+}
+
+// generate_nearest_impl<3> / <2>, weight_generator.cpp:2129-2271 (host
+// path): float points from the double centroids -- unit vectors
+// (cos lat cos lon, cos lat sin lon, sin lat) on a sphere, (lon, lat)
+// otherwise -- and each destination's nearest source point by float squared
+// distance (brute force here; ties to the smallest index, which ArborX leaves
+// open).  Fractions 1, areas from get_cell_areas.
+void nearest_points(const Mesh &src, const Mesh &dst, int unmapped_error, std::vector<i64> &rows, std::vector<i64> &cols,
+                    std::vector<double> &vals, Result &r) {
+    const double pi = 3.14159265358979323846;
+    const bool sph = src.coord_sys != 2;
+    auto points = [&](const Mesh &m) {
+        std::vector<float> p(3 * m.n_cells);
+        for (i64 c = 0; c < m.n_cells; ++c) {
+            double lon, lat;
+            centroid_lonlat(m, c, lon, lat);
+            if (sph) {
+                if (m.coord_sys == 0) {
+                    lon = lon * pi / 180.0;
+                    lat = lat * pi / 180.0;
+                }
+                p[3 * c] = (float)(std::cos(lat) * std::cos(lon));
+                p[3 * c + 1] = (float)(std::cos(lat) * std::sin(lon));
+                p[3 * c + 2] = (float)std::sin(lat);
+            } else {
+                p[3 * c] = (float)lon;
+                p[3 * c + 1] = (float)lat;
+                p[3 * c + 2] = 0.0f;
+            }
+        }
+        return p;
+    };
+    const std::vector<float> sp = points(src), dp = points(dst);
+    std::vector<i64> best(dst.n_cells, -1);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (i64 j = 0; j < dst.n_cells; ++j) {
+        float bd = std::numeric_limits<float>::infinity();
+        i64 bi = -1;
+        for (i64 i = 0; i < src.n_cells; ++i) {
+            const float dx = sp[3 * i] - dp[3 * j], dy = sp[3 * i + 1] - dp[3 * j + 1], dz = sp[3 * i + 2] - dp[3 * j + 2];
+            const float d2 = sph ? dx * dx + dy * dy + dz * dz : dx * dx + dy * dy;
+            if (d2 < bd || (d2 == bd && bi < 0)) {
+                bd = d2;
+                bi = i;
+            }
+        }
+        best[j] = bi;
+    }
+    for (i64 j = 0; j < dst.n_cells; ++j) {
+        if (best[j] < 0) {
+            if (unmapped_error) {
+                r.error = "WeightGenerator::generate_nearest: unmapped destination cell " + std::to_string(j);
+                return;
+            }
+            continue;
+        }
+        rows.push_back(j);
+        cols.push_back(best[j]);
+        vals.push_back(1.0);
+    }
+    r.frac_a.assign(src.n_cells, 1.0);
+    r.frac_b.assign(dst.n_cells, 1.0);
+    r.area_a = planar_areas(src);
+    r.area_b = planar_areas(dst);
+}
+
 // generate_bilinear_rect_nonuniform, weight_generator_bilinear_rect_nonuniform.cpp:22-149
 void bilinear_rect(const Mesh &src, const Mesh &dst, const RectInfo &ri, Result &r) {
     const i64 ni = ri.ni, nj = ri.nj, n_dst = dst.n_cells;
@@ -1111,6 +1225,21 @@ void *om_generate(const double *s_coords, int64_t s_nn, const int64_t *s_off, co
             coastal(src, dst, extrap, rows, cols, vals);
             to_csr(*r, rows, cols, vals);
         }
+    } else if (method == 1) {
+        // generate_nearest (host), weight_generator.cpp:2098-2127, then the
+        // coastal pass (:2018)
+        r->n_src = src.n_cells;
+        r->n_dst = dst.n_cells;
+        std::vector<i64> rows, cols;
+        std::vector<double> vals;
+        const RegularInfo sg = detect_regular(src), dg = detect_regular(dst);
+        if (sg.is_regular && dg.is_regular)
+            nearest_rect(src, dst, sg, dg, rows, cols, vals, *r);
+        else
+            nearest_points(src, dst, unmapped_error, rows, cols, vals, *r);
+        if (!r->error.empty()) return r;
+        coastal(src, dst, extrap, rows, cols, vals);
+        to_csr(*r, rows, cols, vals);
     } else {
         r->error = "method not restated";
     }

@@ -477,3 +477,67 @@ def test_centroids_match_numpy_restatement():
         assert np.all(dl < 1e-4) and np.all(np.abs(lat - na32) < 1e-4)
         assert np.mean((lon == nl32) & (lat == na32)) > 0.99
         assert np.all((lon >= 0) & (lon <= 360))
+
+
+# ---- NearestNeighbor (generate_nearest, weight_generator.cpp:2025-2271)
+def test_nearest_rect_closed_form():
+    src = grids.latlon_deg(2.0)
+    w = O.generate(src, src, 1)
+    assert np.array_equal(w["col"], np.arange(grids.n_cells(src)))
+    assert not w["area_a"].any() and not w["frac_b"].any()  # the rect path leaves them zero
+    dst = grids.regular_latlon(72, 36, -180.0, -90.0, 5.0, 5.0)
+    w = O.generate(grids.latlon_deg(2.5), dst, 1)
+    i, j = np.arange(72)[None, :], np.arange(36)[:, None]
+    si = np.floor((-180.0 + 5.0 * i) / 2.5 + 0.5).astype(np.int64) % 144  # lon wraps
+    sj = np.minimum(np.floor((5.0 * j) / 2.5 + 0.5).astype(np.int64), 71)
+    assert np.array_equal(w["col"], (sj * 144 + si).reshape(-1))
+    assert np.all(w["val"] == 1.0) and np.array_equal(w["row_ptr"], np.arange(72 * 36 + 1))
+
+
+def _unit(lon, lat):
+    lo, la = lon * math.pi / 180.0, lat * math.pi / 180.0
+    return np.stack([np.cos(la) * np.cos(lo), np.cos(la) * np.sin(lo), np.sin(la)], 1).astype(np.float32)
+
+
+def test_nearest_points_sphere():
+    src, dst = grids.cubed_sphere(24), grids.latlon_deg(3.0)
+    w = O.generate(src, dst, 1)
+    assert np.array_equal(w["row_ptr"], np.arange(grids.n_cells(dst) + 1)) and np.all(w["val"] == 1.0)
+    P, Q = _unit(*np_centroids(src)), _unit(*np_centroids(dst))
+    d2 = ((P[None, :, :] - Q[:, None, :]) ** 2).sum(-1)
+    best = d2.argmin(1)
+    got = d2[np.arange(len(Q)), w["col"]]
+    assert np.all(got <= d2.min(1) * (1 + 1e-5) + 1e-12)
+    assert np.mean(best == w["col"]) > 0.99
+    # fractions 1, areas the shoelace area in raw degrees
+    assert np.all(w["frac_a"] == 1.0) and np.all(w["frac_b"] == 1.0)
+    off, idx, c = dst["offsets"], dst["indices"], dst["coords"]
+    x, y = c[idx, 0].reshape(-1, 4), c[idx, 1].reshape(-1, 4)
+    shoe = 0.5 * np.abs((x * np.roll(y, -1, 1) - np.roll(x, -1, 1) * y).sum(1))
+    assert np.allclose(w["area_b"], shoe, rtol=1e-14)
+    # identical non-regular meshes: every cell is its own nearest
+    c8 = grids.cubed_sphere(8)
+    assert np.array_equal(O.generate(c8, c8, 1)["col"], np.arange(grids.n_cells(c8)))
+
+
+def test_nearest_points_planar():
+    src, dst = grids.jittered_voronoi(2, 3), grids.cubed_sphere(6)
+    w = O.generate(src, dst, 1, coord_sys=2)
+    def cen(m):
+        off, idx, c = m["offsets"], m["indices"], m["coords"]
+        n = np.diff(off).astype(np.float64)
+        return (np.add.reduceat(c[idx, 0], off[:-1]) * (1.0 / n)).astype(np.float32), \
+               (np.add.reduceat(c[idx, 1], off[:-1]) * (1.0 / n)).astype(np.float32)
+    (sx, sy), (qx, qy) = cen(src), cen(dst)
+    d2 = (sx[None, :] - qx[:, None]) ** 2 + (sy[None, :] - qy[:, None]) ** 2
+    assert np.array_equal(w["col"], d2.argmin(1))
+
+
+def test_nearest_masked_source_goes_wet():
+    src, dst = grids.cubed_sphere(12), grids.latlon_deg(4.0)
+    mask = land_mask(src)
+    w = O.generate(dict(src, mask=mask), dst, 1, extrap=1)
+    assert np.all(mask[w["col"]] != 0)
+    plain = O.generate(src, dst, 1)
+    keep = mask[plain["col"]] > 0
+    assert np.array_equal(w["col"][keep], plain["col"][keep])  # wet picks unchanged
