@@ -14,6 +14,7 @@
 #pragma once
 
 #include <cstdint>
+#include <atomic>
 #include <memory>
 #include <string>
 #include <vector>
@@ -258,7 +259,7 @@ bool rectilinear_from_axes(const Mesh &m, const GridAxes &ax);
 /// Node xyz (and, for radian meshes, the degree reading) with host libm;
 /// touches only g.h_xyz, g.h_xyz_deg and g.xyz_ready, so it may run on a host
 /// thread while the GPU works on other members.
-void node_xyz_host(const Mesh &m, MeshGeom &g);
+void node_xyz_host(const Mesh &m, MeshGeom &g, const std::atomic<bool> *cancel = nullptr);
 void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s);
 
 /// 30-bit Morton code of every cell's centroid (normalised mean of its vertex
@@ -299,6 +300,17 @@ void check_longitude_ranges(const Mesh &src, const Mesh &dst, bool structured_fa
 /// sum is in (0, 1)) and, for NearestWet, maps each row left empty to its
 /// nearest wet source cell with weight 1.  No-op without a source mask.
 void coastal_pass(const Mesh &src, const Mesh &dst, ExtrapolationAction extrap, InterpolationMatrix &M, hipStream_t s);
+/// Analytic paths for structured pairs (null when the path declines):
+/// generate_conservative_rect (weight_generator_conservative_rect.cpp:87-395),
+/// generate_conservative_rect_nonuniform
+/// (weight_generator_conservative_rect_nonuniform.cpp:52-313) and
+/// generate_bilinear_rect (weight_generator_bilinear_rect.cpp:182-266).
+std::unique_ptr<InterpolationMatrix> conservative_regular(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, const GridAxes &sax,
+                                                          const GridAxes &dax, const MeshGeom &sg, const MeshGeom &dg, hipStream_t s);
+std::unique_ptr<InterpolationMatrix> conservative_rectilinear(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, const GridAxes &sax,
+                                                              const GridAxes &dax, hipStream_t s);
+std::unique_ptr<InterpolationMatrix> bilinear_regular(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, const GridAxes &sax, const MeshGeom &sg,
+                                                      hipStream_t s);
 /// NearestNeighbor weights (generate_nearest, weight_generator.cpp:2025-2271):
 /// the analytic index rounding when both meshes are regular lat-lon grids
 /// (src_reg and dst_reg given), else the nearest source centroid point.

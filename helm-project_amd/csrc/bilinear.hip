@@ -142,9 +142,7 @@ std::unique_ptr<InterpolationMatrix> WeightGenerator::generate_proposed(const Me
         throw InvalidArgument("WeightGenerator::generate: source and destination meshes have different CoordinateSystem values (src=" +
                               std::to_string(static_cast<int>(src.coord_system())) + ", dst=" + std::to_string(static_cast<int>(dst.coord_system())) +
                               ")");
-    if (src.coord_system() == CoordinateSystem::Cartesian3D && cfg.method != InterpolationMethod::NearestNeighbor)
-        throw InvalidArgument("WeightGenerator::generate: Cartesian3D meshes are served by NearestNeighbor only here (the great-circle "
-                              "conservative and bilinear paths need spherical coordinates)");
+    const bool cartesian = src.coord_system() == CoordinateSystem::Cartesian3D;
     std::unique_ptr<InterpolationMatrix> M;
     switch (cfg.method) {
         case InterpolationMethod::Conservative1stOrder: {
@@ -159,10 +157,11 @@ std::unique_ptr<InterpolationMatrix> WeightGenerator::generate_proposed(const Me
             detail::MeshGeom sg, dg;
             if (detail::regular_from_axes(src, sax, sg)) {
                 detail::GridAxes dax = detail::unique_axes(dst, s);
-                if (detail::regular_from_axes(dst, dax, dg))
-                    throw InvalidArgument("WeightGenerator::generate: both meshes are regular lat-lon grids; the reference routes "
-                                          "bilinear to its analytic regular path, which this library does not provide");
+                if (detail::regular_from_axes(dst, dax, dg)) M = detail::bilinear_regular(src, dst, cfg, sax, sg, s);
             }
+            if (M) break;
+            if (cartesian)
+                throw InvalidArgument("WeightGenerator::generate: a Cartesian3D bilinear pair is served by the regular-grid path only here");
             if (!s_rect)
                 throw InvalidArgument("WeightGenerator::generate: bilinear needs a rectilinear lat-lon source here (the reference's BVH "
                                       "bilinear path is not provided)");

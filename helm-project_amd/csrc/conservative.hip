@@ -23,6 +23,7 @@
 // arbitration hands it to host libm.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <future>
 
 #include <algorithm>
@@ -601,26 +602,45 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     MeshGeom sg, dg;
     // host libm node xyz (source, then destination) on a host thread beside
     // the GPU grid detection and the source's upload and cell pass
+    // (a structured pair takes an analytic path instead: the task is told to stop)
+    std::atomic<bool> cancel{false};
     std::promise<void> src_done;
     std::future<void> src_ready = src_done.get_future();
+    const bool spherical = src.coord_system() != CoordinateSystem::Cartesian3D;
     auto xyz_task = std::async(std::launch::async, [&] {
         try {
-            node_xyz_host(src, sg);
+            if (spherical) node_xyz_host(src, sg, &cancel);
             src_done.set_value();
         } catch (...) {
             src_done.set_exception(std::current_exception());
             return;
         }
-        node_xyz_host(dst, dg);
+        if (spherical) node_xyz_host(dst, dg, &cancel);
     });
     GridAxes sax = unique_axes(src, s), dax = unique_axes(dst, s);
     const bool s_reg = regular_from_axes(src, sax, sg), d_reg = regular_from_axes(dst, dax, dg);
-    if (s_reg && d_reg)
-        throw InvalidArgument("generate_conservative: both meshes are regular lat-lon grids; the reference routes this pair to its analytic "
-                              "rectangle path, which this library does not provide");
-    if (rectilinear_from_axes(src, sax) && rectilinear_from_axes(dst, dax))
-        throw InvalidArgument("generate_conservative: both meshes are rectilinear; the reference routes this pair to its analytic "
-                              "rectilinear path, which this library does not provide");
+    // generate_conservative's dispatch (weight_generator.cpp:3259-3296): the
+    // regular, then the rectilinear, analytic path when both meshes qualify;
+    // a declined path, or an empty result under UnmappedAction::Error, falls
+    // through.
+    auto accept = [&](const std::unique_ptr<InterpolationMatrix> &R) {
+        return R && n_dst > 0 && (cfg.unmapped != UnmappedAction::Error || R->nnz > 0);
+    };
+    std::unique_ptr<InterpolationMatrix> fast;
+    if (s_reg && d_reg) {
+        fast = conservative_regular(src, dst, cfg, sax, dax, sg, dg, s);
+        if (!accept(fast)) fast.reset();
+    }
+    if (!fast && rectilinear_from_axes(src, sax) && rectilinear_from_axes(dst, dax)) {
+        fast = conservative_rectilinear(src, dst, cfg, sax, dax, s);
+        if (!accept(fast)) fast.reset();
+    }
+    if (fast) {
+        cancel.store(true);
+        xyz_task.get();
+        return fast;
+    }
+    if (!spherical) throw InvalidArgument("generate_conservative: Cartesian3D meshes are served by the rectangle paths only here");
     if (cfg.line_type != LineType::GreatCircle) throw InvalidArgument("generate_conservative: only LineType::GreatCircle is supported");
     tr.mark("detect");
     src_ready.get();

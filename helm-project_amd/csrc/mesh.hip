@@ -641,7 +641,7 @@ int host_classify(const double *xyz, const index_t *idx, int nv, double &fan_are
 
 }  // namespace
 
-void node_xyz_host(const Mesh &m, MeshGeom &g) {
+void node_xyz_host(const Mesh &m, MeshGeom &g, const std::atomic<bool> *cancel) {
     const index_t n_nodes = m.n_nodes();
     const bool deg = m.coord_system() == CoordinateSystem::SphericalDeg;
     g.h_xyz.resize(3 * static_cast<std::size_t>(n_nodes));
@@ -651,6 +651,7 @@ void node_xyz_host(const Mesh &m, MeshGeom &g) {
     parallel_range(n_nodes, [&](index_t b, index_t e) {
         SinCosMemo memo;
         for (index_t i = b; i < e; ++i) {
+            if (cancel && (i & 4095) == 0 && cancel->load(std::memory_order_relaxed)) return;
             geo::V3 p = deg ? memo.xyz(lon[i] * kDeg2Rad, lat[i] * kDeg2Rad) : memo.xyz(lon[i], lat[i]);
             g.h_xyz[3 * i] = p.x;
             g.h_xyz[3 * i + 1] = p.y;

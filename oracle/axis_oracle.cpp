@@ -608,6 +608,7 @@ struct Result {
     i64 n_candidates = 0, n_cap_rejected = 0, n_clipped = 0;
     std::vector<i64> cand_offsets, cand_idx;  // optional broad phase dump
     std::string error;
+    int path = 0;  // 0 general, 1 regular fast path, 2 rectilinear fast path
 };
 
 // to_csr, interpolation_matrix.hpp:180-255: sort (row, col), keep duplicates.
@@ -860,6 +861,267 @@ void coastal(const Mesh &src, const Mesh &dst, int extrap, std::vector<i64> &row
     rows.swap(ur);
     cols.swap(uc);
     vals.swap(uv);
+}
+
+// ------------------------------------------- regular / rectilinear fast paths
+// rect_overlap_spherical (weight_generator_conservative_rect.cpp:51-69, and
+// its twin in weight_generator_conservative_rect_nonuniform.cpp:23-40):
+// longitude overlap times the difference of the sines of the clamped
+// latitudes, degrees converted with the folded constant pi / 180.
+double rect_overlap_sph(double slx, double shx, double sly, double shy, double dlx, double dhx, double dly, double dhy, bool deg) {
+    double dx = std::fmax(0.0, std::fmin(shx, dhx) - std::fmax(slx, dlx));
+    double lo = std::fmax(sly, dly), hi = std::fmin(shy, dhy);
+    if (hi <= lo || dx <= 0.0) return 0.0;
+    if (deg) {
+        const double d2r = 3.14159265358979323846 / 180.0;
+        dx *= d2r;
+        lo *= d2r;
+        hi *= d2r;
+    }
+    return dx * (std::sin(hi) - std::sin(lo));
+}
+double rect_overlap_flat(double slx, double shx, double sly, double shy, double dlx, double dhx, double dly, double dhy) {
+    const double dx = std::fmax(0.0, std::fmin(shx, dhx) - std::fmax(slx, dlx));
+    const double dy = std::fmax(0.0, std::fmin(shy, dhy) - std::fmax(sly, dly));
+    return dx * dy;
+}
+// Band area dlon * (sin(hi) - sin(lo)) (or dlon * dlat in the plane).
+double band_area(double dlon, double lo, double hi, int cs) {
+    if (cs == 2) return dlon * (hi - lo);
+    if (cs == 0) {
+        const double d2r = 3.14159265358979323846 / 180.0;
+        dlon *= d2r;
+        lo *= d2r;
+        hi *= d2r;
+    }
+    return dlon * (std::sin(hi) - std::sin(lo));
+}
+
+// generate_conservative_rect, weight_generator_conservative_rect.cpp:87-395.
+// Returns false where the reference returns its empty "decline" matrix.
+bool conservative_regular(const Mesh &src, const Mesh &dst, const RegularInfo &sg, const RegularInfo &dg, int norm_type, std::vector<i64> &rows,
+                          std::vector<i64> &cols, std::vector<double> &vals, Result &r) {
+    const i64 n_src = src.n_cells, n_dst = dst.n_cells;
+    const bool deg = src.coord_sys == 0, sph = src.coord_sys != 2;
+    const double wrap = deg ? 350.0 : 6.0;
+    const bool s_glob = (double)sg.ni * sg.delta_lon > wrap, d_glob = (double)dg.ni * dg.delta_lon > wrap;
+    if (!s_glob && !d_glob && (dg.lon_min < sg.lon_min - sg.delta_lon * 0.5 || dg.lon_max > sg.lon_max + sg.delta_lon * 0.5)) return false;
+    std::vector<double> sa(n_src), da(n_dst), fa(n_src, 0.0), fb(n_dst, 0.0);
+    for (i64 c = 0; c < n_src; ++c) {
+        if (src.areas) {
+            sa[c] = src.areas[c];
+        } else {
+            const double lo = sg.lat_min + (double)(c / sg.ni) * sg.delta_lat;
+            sa[c] = sph ? band_area(sg.delta_lon, lo, lo + sg.delta_lat, src.coord_sys) : sg.delta_lon * sg.delta_lat;
+        }
+    }
+    for (i64 c = 0; c < n_dst; ++c) {
+        if (dst.areas) {
+            da[c] = dst.areas[c];
+        } else {
+            const double lo = dg.lat_min + (double)(c / dg.ni) * dg.delta_lat;
+            da[c] = sph ? band_area(dg.delta_lon, lo, lo + dg.delta_lat, src.coord_sys) : dg.delta_lon * dg.delta_lat;
+        }
+    }
+    for (i64 jd = 0; jd < dg.nj; ++jd) {
+        for (i64 id = 0; id < dg.ni; ++id) {
+            const i64 j = jd * dg.ni + id;
+            if (dst.mask && dst.mask[j] == 0) continue;
+            const double dlx = dg.lon_min + (double)id * dg.delta_lon, dhx = dlx + dg.delta_lon;
+            const double dly = dg.lat_min + (double)jd * dg.delta_lat, dhy = dly + dg.delta_lat;
+            double nlx = dlx, nhx = dhx;
+            if (s_glob) {
+                const double shift = 360.0 * std::floor((dlx - sg.lon_min) / 360.0);
+                nlx = dlx - shift;
+                nhx = dhx - shift;
+            }
+            const double ad = da[j];
+            if (ad <= 0.0) continue;
+            int is_lo = (int)std::floor((nlx - sg.lon_min) / sg.delta_lon);
+            int is_hi = (int)std::floor((nhx - sg.lon_min) / sg.delta_lon);
+            int js_lo = (int)std::floor((dly - sg.lat_min) / sg.delta_lat);
+            int js_hi = (int)std::floor((dhy - sg.lat_min) / sg.delta_lat);
+            if (!s_glob) {
+                is_lo = std::max(0, is_lo);
+                is_hi = std::min((int)sg.ni - 1, is_hi);
+            }
+            js_lo = std::max(0, js_lo);
+            js_hi = std::min((int)sg.nj - 1, js_hi);
+            const bool wraps = s_glob && is_lo > is_hi;
+            if (!wraps && is_lo > is_hi) continue;
+            if (js_lo > js_hi) continue;
+            for (int js = js_lo; js <= js_hi; ++js) {
+                for (int is = is_lo; is <= is_hi; ++is) {
+                    int ia = is;
+                    if (s_glob) ia = ((is % (int)sg.ni) + (int)sg.ni) % (int)sg.ni;
+                    const i64 i = (i64)js * sg.ni + ia;
+                    if (i >= n_src) continue;
+                    if (src.mask && src.mask[i] == 0) continue;
+                    const double as = sa[i];
+                    if (as <= 0.0) continue;
+                    const double slx = sg.lon_min + (double)is * sg.delta_lon, shx = slx + sg.delta_lon;
+                    const double sly = sg.lat_min + (double)js * sg.delta_lat, shy = sly + sg.delta_lat;
+                    const double ov = sph ? rect_overlap_sph(slx, shx, sly, shy, nlx, nhx, dly, dhy, deg) : rect_overlap_flat(slx, shx, sly, shy, nlx, nhx, dly, dhy);
+                    if (ov <= 0.0) continue;
+                    vals.push_back(std::max(ov / ad, 0.0));
+                    rows.push_back(j);
+                    cols.push_back(i);
+                    fa[i] += ov / as;
+                    fb[j] += ov / ad;
+                }
+            }
+        }
+    }
+    if (norm_type == 1)
+        for (size_t k = 0; k < vals.size(); ++k)
+            if (fb[rows[k]] > 0.0) vals[k] /= fb[rows[k]];
+    for (auto &x : fa) x = std::min(x, 1.0);
+    for (auto &x : fb) x = std::min(x, 1.0);
+    r.frac_a = fa;
+    r.frac_b = fb;
+    r.area_a = sa;
+    r.area_b = da;
+    return true;
+}
+
+// generate_conservative_rect_nonuniform,
+// weight_generator_conservative_rect_nonuniform.cpp:52-313.
+void conservative_rectilinear(const Mesh &src, const Mesh &dst, const RectInfo &sr, const RectInfo &dr, int norm_type, std::vector<i64> &rows,
+                              std::vector<i64> &cols, std::vector<double> &vals, Result &r) {
+    const i64 n_src = src.n_cells, n_dst = dst.n_cells, sni = sr.ni, snj = sr.nj, dni = dr.ni, dnj = dr.nj;
+    const bool deg = src.coord_sys == 0, sph = src.coord_sys != 2;
+    const double slon0 = sr.ulon[0], slon1 = sr.ulon[sni];
+    const bool s_glob = slon1 - slon0 > (deg ? 350.0 : 6.0);
+    std::vector<double> fa(n_src, 0.0), fb(n_dst, 0.0);
+    auto area = [&](double lx, double hx, double ly, double hy) { return sph ? band_area(hx - lx, ly, hy, src.coord_sys) : (hx - lx) * (hy - ly); };
+    for (i64 jd = 0; jd < dnj; ++jd) {
+        for (i64 id = 0; id < dni; ++id) {
+            const i64 j = id + jd * dni;
+            if (dst.mask && dst.mask[j] == 0) continue;
+            const double dlx = dr.ulon[id], dhx = dr.ulon[id + 1], dly = dr.ulat[jd], dhy = dr.ulat[jd + 1];
+            double nlx = dlx, nhx = dhx;
+            if (s_glob) {
+                const double shift = 360.0 * std::floor((dlx - slon0) / 360.0);
+                nlx = dlx - shift;
+                nhx = dhx - shift;
+            }
+            const double ad = dst.areas ? dst.areas[j] : area(nlx, nhx, dly, dhy);
+            if (ad <= 0.0) continue;
+            const i64 a0 = std::lower_bound(sr.ulon.begin(), sr.ulon.end(), nlx) - sr.ulon.begin();
+            const i64 a1 = std::upper_bound(sr.ulon.begin(), sr.ulon.end(), nhx) - sr.ulon.begin();
+            int is0 = std::max(0, (int)(a0 - 1)), is1 = (int)a1;
+            if (s_glob) {
+                if (nhx <= slon1) is1 = std::min((int)sni - 1, is1);
+            } else {
+                is1 = std::min((int)sni - 1, is1);
+            }
+            const i64 b0 = std::lower_bound(sr.ulat.begin(), sr.ulat.end(), dly) - sr.ulat.begin();
+            const i64 b1 = std::upper_bound(sr.ulat.begin(), sr.ulat.end(), dhy) - sr.ulat.begin();
+            const int js0 = std::max(0, (int)(b0 - 1)), js1 = std::min((int)snj - 1, (int)b1);
+            for (int js = js0; js <= js1; ++js) {
+                for (int is = is0; is <= is1; ++is) {
+                    const int ia = s_glob ? ((is % (int)sni) + (int)sni) % (int)sni : is;
+                    const i64 i = ia + (i64)js * sni;
+                    if (src.mask && src.mask[i] == 0) continue;
+                    double slx = sr.ulon[ia], shx = sr.ulon[ia + 1];
+                    if (s_glob && is >= (int)sni) {
+                        slx += 360.0;
+                        shx += 360.0;
+                    }
+                    const double sly = sr.ulat[js], shy = sr.ulat[js + 1];
+                    const double ov = sph ? rect_overlap_sph(slx, shx, sly, shy, nlx, nhx, dly, dhy, deg) : rect_overlap_flat(slx, shx, sly, shy, nlx, nhx, dly, dhy);
+                    if (ov > 1e-12) {
+                        const double as = src.areas ? src.areas[i] : area(slx, shx, sly, shy);
+                        if (as <= 0.0) continue;
+                        vals.push_back(ov / ad);
+                        rows.push_back(j);
+                        cols.push_back(i);
+                        fa[i] += ov / as;
+                        fb[j] += ov / ad;
+                    }
+                }
+            }
+        }
+    }
+    if (norm_type == 1)
+        for (size_t k = 0; k < vals.size(); ++k)
+            if (fb[rows[k]] > 0.0) vals[k] /= fb[rows[k]];
+    r.frac_a.resize(n_src);
+    r.area_a.resize(n_src);
+    r.frac_b.resize(n_dst);
+    r.area_b.resize(n_dst);
+    for (i64 i = 0; i < n_src; ++i) {
+        r.area_a[i] = src.areas ? src.areas[i] : area(sr.ulon[i % sni], sr.ulon[i % sni + 1], sr.ulat[i / sni], sr.ulat[i / sni + 1]);
+        r.frac_a[i] = std::min(fa[i], 1.0);
+    }
+    for (i64 j = 0; j < n_dst; ++j) {
+        r.area_b[j] = dst.areas ? dst.areas[j] : area(dr.ulon[j % dni], dr.ulon[j % dni + 1], dr.ulat[j / dni], dr.ulat[j / dni + 1]);
+        r.frac_b[j] = std::min(fb[j], 1.0);
+    }
+}
+
+// generate_bilinear_rect, weight_generator_bilinear_rect.cpp:33-266: the
+// planar mean of the cell's nodes, fractional cell-centred position, 2x2
+// stencil, longitude periodic when the grid spans 360 units.  Returns false
+// where the reference declines (ni or nj zero).
+bool bilinear_regular(const Mesh &src, const Mesh &dst, const RegularInfo &sg, int unmapped_error, std::vector<i64> &rows, std::vector<i64> &cols,
+                      std::vector<double> &vals, Result &r) {
+    const i64 ni = sg.ni, nj = sg.nj;
+    if (ni == 0 || nj == 0) return false;
+    const bool per = std::abs((sg.lon_max - sg.lon_min) - 360.0) < 1.0e-10 * sg.delta_lon;
+    r.frac_a.assign(src.n_cells, 1.0);
+    r.frac_b.assign(dst.n_cells, 0.0);
+    r.area_a.assign(src.n_cells, 0.0);
+    r.area_b.assign(dst.n_cells, 0.0);
+    for (i64 c = 0; c < dst.n_cells; ++c) {
+        const i64 b = dst.offsets[c], e = dst.offsets[c + 1];
+        double sx = 0.0, sy = 0.0;
+        for (i64 k = b; k < e; ++k) {
+            sx += dst.lon(dst.indices[k]);
+            sy += dst.lat(dst.indices[k]);
+        }
+        double lon = sx / (double)(e - b), lat = sy / (double)(e - b);
+        if (per) {
+            double off = lon - sg.lon_min;
+            off = off - std::floor(off / 360.0) * 360.0;
+            lon = sg.lon_min + off;
+        }
+        const double fi = (lon - sg.lon_min) / sg.delta_lon - 0.5, fj = (lat - sg.lat_min) / sg.delta_lat - 0.5;
+        const double tol = 1.0e-9;
+        bool out = fj < -0.5 - tol || fj > (double)nj - 0.5 + tol;
+        if (!per) out = out || fi < -0.5 - tol || fi > (double)ni - 0.5 + tol;
+        if (out) {
+            if (unmapped_error) {
+                r.error = "Unmapped destination cell " + std::to_string(c) + " in bilinear regular-grid fast-path";
+                return true;
+            }
+            continue;
+        }
+        int i = (int)std::floor(fi), j = (int)std::floor(fj);
+        if (j < 0) j = 0;
+        if (j >= (int)nj - 1) j = (int)nj - 2;
+        if (j < 0) j = 0;
+        if (!per) {
+            if (i < 0) i = 0;
+            if (i >= (int)ni - 1) i = (int)ni - 2;
+            if (i < 0) i = 0;
+        } else {
+            i = ((i % (int)ni) + (int)ni) % (int)ni;
+        }
+        const double tx = std::min(std::max(fi - (double)i, 0.0), 1.0), ty = std::min(std::max(fj - (double)j, 0.0), 1.0);
+        int i1 = i + 1, j1 = j + 1;
+        if (i1 >= (int)ni) i1 = per ? 0 : (int)ni - 1;
+        if (j1 >= (int)nj) j1 = (int)nj - 1;
+        const i64 ix[4] = {(i64)j * ni + i, (i64)j * ni + i1, (i64)j1 * ni + i, (i64)j1 * ni + i1};
+        const double w[4] = {(1.0 - tx) * (1.0 - ty), tx * (1.0 - ty), (1.0 - tx) * ty, tx * ty};
+        for (int k = 0; k < 4; ++k) {
+            rows.push_back(c);
+            cols.push_back(ix[k]);
+            vals.push_back(w[k]);
+        }
+        r.frac_b[c] = 1.0;
+    }
+    return true;
 }
 
 // get_cell_areas / compute_single_cell_area, weight_generator.cpp:264-313:
@@ -1193,20 +1455,58 @@ void *om_generate(const double *s_coords, int64_t s_nn, const int64_t *s_off, co
     Mesh dst = make_mesh(d_coords, d_nn, d_off, d_idx, d_nc, coord_sys, d_areas, d_mask);
     r->error = generate_checks(src, dst, method);
     if (!r->error.empty()) return r;
+    r->n_src = src.n_cells;
+    r->n_dst = dst.n_cells;
     if (method == 2) {
         // generate_conservative, weight_generator.cpp:3246-3302: the analytic
-        // regular/rectilinear paths are taken when BOTH meshes qualify; those
-        // paths are outside this restatement.
-        if ((detect_regular(src).is_regular && detect_regular(dst).is_regular) || (detect_rect(src).is_rect && detect_rect(dst).is_rect)) {
-            r->error = "regular/rectilinear fast path not restated";
-            return r;
+        // regular, then rectilinear, paths when BOTH meshes qualify; a
+        // declined or (under UnmappedAction::Error) empty result falls
+        // through to the next path.
+        const RegularInfo sg = detect_regular(src), dg = detect_regular(dst);
+        if (sg.is_regular && dg.is_regular) {
+            std::vector<i64> rows, cols;
+            std::vector<double> vals;
+            Result t;
+            t.n_src = src.n_cells;
+            t.n_dst = dst.n_cells;
+            if (conservative_regular(src, dst, sg, dg, norm_type, rows, cols, vals, t) && dst.n_cells > 0 && (!unmapped_error || !vals.empty())) {
+                *r = t;
+                r->path = 1;
+                coastal(src, dst, extrap, rows, cols, vals);
+                to_csr(*r, rows, cols, vals);
+                return r;
+            }
+        }
+        const RectInfo sr = detect_rect(src), dr = detect_rect(dst);
+        if (sr.is_rect && dr.is_rect) {
+            std::vector<i64> rows, cols;
+            std::vector<double> vals;
+            Result t;
+            t.n_src = src.n_cells;
+            t.n_dst = dst.n_cells;
+            conservative_rectilinear(src, dst, sr, dr, norm_type, rows, cols, vals, t);
+            if (dst.n_cells > 0 && (!unmapped_error || !vals.empty())) {
+                *r = t;
+                r->path = 2;
+                coastal(src, dst, extrap, rows, cols, vals);
+                to_csr(*r, rows, cols, vals);
+                return r;
+            }
         }
         conservative(src, dst, norm_type, unmapped_error, flags & 1, flags & 2, extrap, *r);
     } else if (method == 0) {
         // generate_bilinear, weight_generator.cpp:2306-2345
-        if (detect_regular(src).is_regular && detect_regular(dst).is_regular) {
-            r->error = "regular fast path not restated";
-            return r;
+        const RegularInfo sg = detect_regular(src);
+        if (sg.is_regular && detect_regular(dst).is_regular) {
+            std::vector<i64> rows, cols;
+            std::vector<double> vals;
+            if (bilinear_regular(src, dst, sg, unmapped_error, rows, cols, vals, *r)) {
+                if (!r->error.empty()) return r;
+                r->path = 1;
+                coastal(src, dst, extrap, rows, cols, vals);
+                to_csr(*r, rows, cols, vals);
+                return r;
+            }
         }
         RectInfo ri = detect_rect(src);
         if (!ri.is_rect) {
@@ -1247,6 +1547,7 @@ void *om_generate(const double *s_coords, int64_t s_nn, const int64_t *s_off, co
 }
 const char *om_result_error(void *h) { return static_cast<Result *>(h)->error.c_str(); }
 int64_t om_result_nnz(void *h) { return (int64_t) static_cast<Result *>(h)->val.size(); }
+int om_result_path(void *h) { return static_cast<Result *>(h)->path; }
 void om_result_stats(void *h, int64_t *out3) {
     Result *r = static_cast<Result *>(h);
     out3[0] = r->n_candidates;

@@ -55,6 +55,8 @@ def lib():
         L.om_result_error.argtypes = [C.c_void_p]
         L.om_result_nnz.restype = C.c_int64
         L.om_result_nnz.argtypes = [C.c_void_p]
+        L.om_result_path.restype = C.c_int
+        L.om_result_path.argtypes = [C.c_void_p]
         L.om_result_stats.argtypes = [C.c_void_p, _I64]
         L.om_result_csr.argtypes = [C.c_void_p, _I64, _I64, _D]
         L.om_result_fracs.argtypes = [C.c_void_p, _D, _D, _D, _D]
@@ -220,6 +222,7 @@ def generate(src, dst, method: int = 2, norm_type: int = 0, unmapped_error: bool
         st = np.zeros(3, dtype=np.int64)
         L.om_result_stats(h, _p(st, _I64))
         out["n_candidates"], out["n_cap_rejected"], out["n_clipped"] = (int(x) for x in st)
+        out["path"] = ("general", "regular", "rectilinear")[L.om_result_path(h)]
         if keep_candidates:
             nc = L.om_result_n_candidates(h)
             co = np.zeros(n_dst + 1, dtype=np.int64)

@@ -319,8 +319,6 @@ def test_to_csr_keeps_duplicates_and_sorts():
 
 def test_generate_error_paths():
     src, dst = grids.latlon_deg(10.0), grids.cubed_sphere(4)
-    with pytest.raises(O.OracleError):
-        O.generate(src, grids.latlon_deg(20.0), 2)  # both regular: analytic path, not restated
     far = dict(dst)
     far["coords"] = np.asfortranarray(dst["coords"] + np.array([720.0, 0.0]))
     with pytest.raises(O.OracleError, match="longitude"):
@@ -541,3 +539,94 @@ def test_nearest_masked_source_goes_wet():
     plain = O.generate(src, dst, 1)
     keep = mask[plain["col"]] > 0
     assert np.array_equal(w["col"][keep], plain["col"][keep])  # wet picks unchanged
+
+
+# ---- regular / rectilinear fast paths (weight_generator_conservative_rect.cpp,
+# weight_generator_conservative_rect_nonuniform.cpp, weight_generator_bilinear_rect.cpp),
+# pinned by the exact areas of lat-lon rectangles
+def _band(lo, hi, dlon):
+    return np.radians(dlon) * (np.sin(np.radians(hi)) - np.sin(np.radians(lo)))
+
+
+def test_conservative_regular_exact_rectangles():
+    src, dst = grids.latlon_deg(1.0), grids.latlon_deg(3.0)
+    w = O.generate(src, dst, 2)
+    assert w["path"] == "regular"
+    lat_s = -90.0 + np.arange(180)
+    assert np.allclose(w["area_a"], np.repeat(_band(lat_s, lat_s + 1.0, 1.0), 360), rtol=1e-13)
+    lat_d = -90.0 + 3.0 * np.arange(60)
+    assert np.allclose(w["area_b"], np.repeat(_band(lat_d, lat_d + 3.0, 3.0), 120), rtol=1e-13)
+    rp, col, val = w["row_ptr"], w["col"], w["val"]
+    big = val > 1e-12
+    # each 3 deg cell holds exactly 3 x 3 one-degree cells with weight area_src / area_dst
+    assert np.all(np.add.reduceat(big.astype(int), rp[:-1]) == 9)
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    assert np.allclose(val[big], w["area_a"][col[big]] / w["area_b"][rows[big]], rtol=1e-12)
+    assert np.all(np.abs(np.add.reduceat(val, rp[:-1]) - 1.0) < 1e-12)
+    assert np.all(np.abs(w["frac_a"] - 1.0) < 1e-12)
+    assert abs(w["area_a"].sum() - 4 * np.pi) < 1e-11
+
+
+def test_conservative_regular_identity_and_shifted():
+    g = grids.latlon_deg(4.0)
+    w = O.generate(g, g, 2)
+    rp, col, val = w["row_ptr"], w["col"], w["val"]
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    own = col == rows
+    assert np.all(np.abs(val[own] - 1.0) < 1e-13) and np.all(val[~own] < 1e-12)
+    # a [-180, 180] destination on a [0, 360] source: longitudes wrap
+    d = grids.regular_latlon(45, 45, -180.0, -90.0, 8.0, 4.0)
+    w = O.generate(g, d, 2)
+    assert w["path"] == "regular"
+    rp, val = w["row_ptr"], w["val"]
+    assert np.all(np.abs(np.add.reduceat(val, rp[:-1]) - 1.0) < 1e-12)
+    assert np.all(np.abs(w["frac_a"] - 1.0) < 1e-12)
+
+
+def _rectilinear(lon_edges, lat_edges):
+    ni, nj = len(lon_edges) - 1, len(lat_edges) - 1
+    lon = np.tile(lon_edges, nj + 1)
+    lat = np.repeat(lat_edges, ni + 1)
+    ci, cj = np.tile(np.arange(ni), nj), np.repeat(np.arange(nj), ni)
+    st = ni + 1
+    idx = np.stack([cj * st + ci, cj * st + ci + 1, (cj + 1) * st + ci + 1, (cj + 1) * st + ci], 1).reshape(-1)
+    return grids._mesh(lon, lat, np.arange(0, 4 * ni * nj + 1, 4), idx)
+
+
+def test_conservative_rectilinear_exact_rectangles():
+    rng = np.random.default_rng(12)
+    lat_e = np.concatenate([[-90.0], np.sort(rng.uniform(-89, 89, 40)), [90.0]])
+    src = _rectilinear(np.linspace(0.0, 360.0, 73), lat_e)
+    dst = grids.latlon_deg(6.0)
+    w = O.generate(src, dst, 2)
+    assert w["path"] == "rectilinear"
+    ni = 72
+    lo, hi = lat_e[:-1], lat_e[1:]
+    assert np.allclose(w["area_a"], np.repeat(_band(lo, hi, 5.0), ni), rtol=1e-12)
+    rp, col, val = w["row_ptr"], w["col"], w["val"]
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    # exact intersection of the two rectangles
+    sj, si = col // ni, col % ni
+    dj, di = rows // 60, rows % 60
+    dlon = np.minimum(5.0 * si + 5.0, 6.0 * di + 6.0) - np.maximum(5.0 * si, 6.0 * di)
+    ylo = np.maximum(lo[sj], -90.0 + 6.0 * dj)
+    yhi = np.minimum(hi[sj], -90.0 + 6.0 * dj + 6.0)
+    ov = _band(ylo, yhi, dlon)
+    assert np.allclose(val, ov / w["area_b"][rows], rtol=1e-11, atol=1e-15)
+    assert np.all(np.abs(np.add.reduceat(val, rp[:-1]) - 1.0) < 1e-12)
+    assert np.all(np.abs(w["frac_a"] - 1.0) < 1e-12)
+
+
+def test_bilinear_regular_affine_exact():
+    src, dst = grids.latlon_deg(2.0), grids.latlon_deg(3.0)
+    w = O.generate(src, dst, 0)
+    assert w["path"] == "regular"
+    lon_c = np.arange(180) * 2.0 + 1.0
+    lat_c = np.arange(90) * 2.0 - 89.0
+    f = (3.0 + 0.01 * lon_c[None, :] - 0.02 * lat_c[:, None]).reshape(-1)
+    y = O.batch_apply(w["row_ptr"], w["col"], w["val"], w["n_src"], f)
+    lon_d = np.tile(np.arange(120) * 3.0 + 1.5, 60)
+    lat_d = np.repeat(np.arange(60) * 3.0 - 88.5, 120)
+    inner = (lon_d > 1.0) & (lon_d < 359.0) & (np.abs(lat_d) < 89.0)
+    assert np.allclose(y[inner], 3.0 + 0.01 * lon_d[inner] - 0.02 * lat_d[inner], atol=1e-12)
+    assert np.all(np.diff(w["row_ptr"]) == 4) and np.all(w["frac_b"] == 1.0) and not w["area_a"].any()
