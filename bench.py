@@ -198,6 +198,56 @@ def bench_config5(axis, grids, torch):
     return res
 
 
+def bench_structured(axis, grids, torch, cpu):
+    """Analytic structured paths and nearest neighbour: 0.1deg (3600x1800) ->
+    0.25deg (1440x720) regular conservative and bilinear, the same pair's
+    nearest-neighbour index rounding, and C384 -> 0.25deg nearest neighbour
+    through the centroid point search.  Best of 3 warm generations each."""
+    import numpy as np
+
+    src, dst = grids.latlon_deg(0.1), grids.latlon_deg(0.25)
+    cs384 = grids.cubed_sphere(384)
+    S, D, C = axis.Mesh(src), axis.Mesh(dst), axis.Mesh(cs384)
+
+    def best(fn, reps=3):
+        fn().close()
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            m = fn()
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t) * 1000.0)
+            nnz = m.info()[2]
+            m.close()
+        return min(ts), nnz
+
+    out = {"pair": "0.1deg (3600x1800) -> 0.25deg (1440x720)"}
+    ms, nnz = best(lambda: axis.generate_weights(S, D, axis.CONSERVATIVE))
+    out.update(conservative_regular_ms=ms, conservative_regular_nnz=nnz)
+    ms, nnz = best(lambda: axis.generate_weights(S, D, axis.BILINEAR))
+    out.update(bilinear_regular_ms=ms, bilinear_regular_nnz=nnz)
+    ms, nnz = best(lambda: axis.generate_weights(S, D, axis.NEAREST))
+    out.update(nearest_regular_ms=ms)
+    ms, nnz = best(lambda: axis.generate_weights(C, D, axis.NEAREST))
+    out.update(nearest_points_c384_to_025deg_ms=ms, nearest_points_nnz=nnz)
+    if cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle_py as O
+
+        if O.available():
+            t = time.perf_counter()
+            ref = O.generate(src, dst, 2)
+            out["cpu_oracle_conservative_regular_s"] = time.perf_counter() - t
+            m = axis.generate_weights(S, D, axis.CONSERVATIVE)
+            rp, col, val = m.csr()
+            out["conservative_regular_bit_exact"] = bool(np.array_equal(rp, ref["row_ptr"]) and np.array_equal(col, ref["col"])
+                                                        and np.array_equal(val, ref["val"]))
+            m.close()
+    for m in (S, D, C):
+        m.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,9 +255,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--apply-fields", type=int, default=64, help="fields of the chosen-vs-plain apply comparison")
-    ap.add_argument("--sections", default="c3,c4,c5",
+    ap.add_argument("--sections", default="c3,c4,c5,structured",
                     help="comma list of the extra measurements after the headline: c3 (multi-pair batch), "
-                         "c4 (0.1deg -> C768 mixed apply), c5 (learned proposer); 'none' for the headline only")
+                         "c4 (0.1deg -> C768 mixed apply), c5 (learned proposer), structured (analytic regular/"
+                         "rectilinear paths and nearest neighbour); 'none' for the headline only")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (the apply and the weight generation "
@@ -417,6 +468,9 @@ def main():
     if "c5" in sections:
         log("config 5: learned proposer")
         extra["config5_learned_proposer"] = bench_config5(axis, grids, torch)
+    if "structured" in sections:
+        log("structured pairs: analytic paths and nearest neighbour")
+        extra["structured_paths"] = bench_structured(axis, grids, torch, not args.no_cpu_baseline)
 
     out = {
         "metric": "fields/s & HBM GB/s (apply), cells/s (weight gen): chosen vs plain @<=1e-6 err",
