@@ -36,6 +36,10 @@ class AxisError(RuntimeError):
     pass
 
 
+# int (*)(const int64_t *needed_global_src_ids, int64_t n_remote, double *out, void *user)
+GATHER_FN = C.CFUNCTYPE(C.c_int, C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_double), C.c_void_p)
+
+
 class RegridConfig(C.Structure):
     """axis_regrid_config; defaults are RegridConfig's (regrid_config.hpp:80-89)."""
 
@@ -68,6 +72,15 @@ def lib():
         "axis_matrix_serialize_c": [C.c_int, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)],
         "axis_nan_batch_apply_c": [C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_void_p],
         "axis_apply_masked_c": [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_void_p],
+        "axis_generate_weights_distributed_c": [C.c_int, C.c_int, C.c_void_p, _I64, C.c_int64, _I64, C.c_int64, _I32, C.c_int64,
+                                                C.POINTER(C.c_int), C.POINTER(C.c_int)],
+        "axis_halo_pattern_info_c": [C.c_int, _I64, _I64],
+        "axis_halo_pattern_arrays_c": [C.c_int, _I32, _I64, _I64, _I64],
+        "axis_destroy_halo_pattern_c": [C.c_int],
+        "axis_apply_distributed_c": [C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int64,
+                                     C.c_void_p],
+        "axis_apply_distributed_callback_c": [C.c_int, C.c_int, C.c_void_p, C.c_int64, GATHER_FN, C.c_void_p, C.c_void_p, C.c_int64,
+                                              C.c_void_p],
         "axis_matrix_deserialize_c": [C.c_void_p, C.c_uint64, C.POINTER(C.c_int)],
         "axis_matrix_write_esmf_c": [C.c_int, C.c_char_p],
         "axis_matrix_read_esmf_c": [C.c_char_p, C.POINTER(C.c_int)],
@@ -126,7 +139,9 @@ def exported_symbols():
             "axis_mlp_create_c", "axis_mlp_params_c", "axis_mlp_set_params_c", "axis_mlp_forward_c",
             "axis_mlp_gradient_c", "axis_mlp_train_c", "axis_destroy_mlp_c", "axis_generate_weights_proposed_c",
             "axis_proposer_training_data_c", "axis_matrix_serialize_c", "axis_matrix_deserialize_c",
-            "axis_matrix_write_esmf_c", "axis_matrix_read_esmf_c", "axis_nan_batch_apply_c", "axis_apply_masked_c"]
+            "axis_matrix_write_esmf_c", "axis_matrix_read_esmf_c", "axis_nan_batch_apply_c", "axis_apply_masked_c",
+            "axis_generate_weights_distributed_c", "axis_halo_pattern_info_c", "axis_halo_pattern_arrays_c",
+            "axis_destroy_halo_pattern_c", "axis_apply_distributed_c", "axis_apply_distributed_callback_c"]
 
 
 def _check(rc: int):
@@ -325,6 +340,41 @@ class Matrix:
                                          _stream(stream)))
         return dst
 
+    def apply_distributed(self, pattern, local_src, halo, dst, n_fields=1, stream=None):
+        """Distributed apply with the gathered halo buffer: dst = S [local_src ; halo]
+        (FP64 device fields, layout_left [n, n_fields])."""
+        _, n_dst, _ = self.info()
+        n_local = (local_src.numel() // n_fields) if local_src is not None else 0
+        n_halo = (halo.numel() // n_fields) if halo is not None else 0
+        _check(lib().axis_apply_distributed_c(self.handle, pattern.handle, _ptr(local_src), n_local, _ptr(halo), n_halo, _ptr(dst),
+                                              dst.numel() // n_fields, n_fields, _stream(stream)))
+        return dst
+
+    def apply_distributed_callback(self, pattern, local_src, gather, dst, stream=None):
+        """Callback overload: gather(needed_global_src_ids) -> values (host), called once."""
+        err = []
+
+        def cb(ids, n, out, user):
+            try:
+                v = np.asarray(gather(np.ctypeslib.as_array(ids, (n,)).copy() if n else np.zeros(0, np.int64)), dtype=np.float64)
+                if v.shape != (n,):
+                    err.append(f"gather returned {v.size} values but pattern.num_remote() is {n}")
+                    return 1
+                if n:
+                    np.ctypeslib.as_array(out, (n,))[:] = v
+                return 0
+            except Exception as e:  # noqa: BLE001 -- reported after the call
+                err.append(repr(e))
+                return 2
+
+        fn = GATHER_FN(cb)
+        rc = lib().axis_apply_distributed_callback_c(self.handle, pattern.handle, _ptr(local_src), local_src.numel(), fn, None, _ptr(dst),
+                                                     dst.numel(), _stream(stream))
+        if err:
+            raise AxisError("axis::solver::apply (callback): " + err[0])
+        _check(rc)
+        return dst
+
     def autotune(self, src, dst, n_fields, src_dtype, dst_dtype, mode, stream=None):
         """Times the launch shapes on these fields; returns (fields per step, field chunks, staged)."""
         t, r, st = C.c_int(0), C.c_int(0), C.c_int(0)
@@ -364,6 +414,47 @@ def generate(src: Mesh, dst: Mesh, config: RegridConfig) -> Matrix:
     h = C.c_int(0)
     _check(lib().axis_generate_weights_config_c(src.handle, dst.handle, C.byref(config), C.byref(h)))
     return Matrix(h.value)
+
+
+class HaloPattern:
+    """Off-rank source gather plan of one rank (HaloPattern, halo_pattern.hpp:29-48)."""
+
+    def __init__(self, handle: int):
+        self.handle = handle
+
+    def info(self):
+        nr, nk = C.c_int64(0), C.c_int64(0)
+        _check(lib().axis_halo_pattern_info_c(self.handle, C.byref(nr), C.byref(nk)))
+        return nr.value, nk.value
+
+    def arrays(self):
+        """dict of source_ranks, rank_offsets, needed_global_src_ids, gather_slot."""
+        nr, nk = self.info()
+        out = {"source_ranks": np.zeros(nk, np.int32), "rank_offsets": np.zeros(nk + 1, np.int64),
+               "needed_global_src_ids": np.zeros(nr, np.int64), "gather_slot": np.zeros(nr, np.int64)}
+        _check(lib().axis_halo_pattern_arrays_c(self.handle, out["source_ranks"].ctypes.data_as(_I32),
+                                                out["rank_offsets"].ctypes.data_as(_I64),
+                                                out["needed_global_src_ids"].ctypes.data_as(_I64),
+                                                out["gather_slot"].ctypes.data_as(_I64)))
+        return out
+
+    def close(self):
+        if getattr(self, "handle", 0):
+            _check(lib().axis_destroy_halo_pattern_c(self.handle))
+            self.handle = 0
+
+
+def generate_distributed(src: "Mesh", dst: "Mesh", config, src_global_ids, owner_of_src, dst_global_ids=None):
+    """Distributed generate of one rank: (Matrix with off-rank columns remapped, HaloPattern)."""
+    sg = np.ascontiguousarray(src_global_ids, dtype=np.int64)
+    ow = np.ascontiguousarray(owner_of_src, dtype=np.int32)
+    dg = None if dst_global_ids is None else np.ascontiguousarray(dst_global_ids, dtype=np.int64)
+    m, p = C.c_int(0), C.c_int(0)
+    _check(lib().axis_generate_weights_distributed_c(src.handle, dst.handle, None if config is None else C.byref(config),
+                                                     sg.ctypes.data_as(_I64), sg.shape[0],
+                                                     None if dg is None else dg.ctypes.data_as(_I64), 0 if dg is None else dg.shape[0],
+                                                     ow.ctypes.data_as(_I32), ow.shape[0], C.byref(m), C.byref(p)))
+    return Matrix(m.value), HaloPattern(p.value)
 
 
 def write_esmf(path, matrix: Matrix):

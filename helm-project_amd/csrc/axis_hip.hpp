@@ -128,6 +128,20 @@ class InterpolationMatrix {
     void prepare_tiles(hipStream_t s);
 };
 
+/// HaloPattern (halo_pattern.hpp:29-48): the off-rank source cells a rank's
+/// apply reads, grouped by owning rank in CSR form: the ids needed from
+/// source_ranks[r] are needed_global_src_ids[rank_offsets[r] ..
+/// rank_offsets[r + 1]), and gather_slot[i] is id i's slot in the gathered
+/// halo buffer.
+struct HaloPattern {
+    std::vector<int> source_ranks;
+    std::vector<index_t> rank_offsets{0};
+    std::vector<index_t> needed_global_src_ids;
+    std::vector<index_t> gather_slot;
+    index_t num_remote() const { return static_cast<index_t>(needed_global_src_ids.size()); }
+    index_t num_ranks() const { return static_cast<index_t>(source_ranks.size()); }
+};
+
 struct ConservationReport {
     double src_integral = 0, dst_integral = 0, absolute_error = 0, relative_error = 0;
 };
@@ -316,6 +330,17 @@ std::unique_ptr<InterpolationMatrix> bilinear_regular(const Mesh &src, const Mes
 /// (src_reg and dst_reg given), else the nearest source centroid point.
 std::unique_ptr<InterpolationMatrix> generate_nearest(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, const MeshGeom *src_reg,
                                                       const MeshGeom *dst_reg, hipStream_t s);
+/// Distributed generate of one rank (weight_generator.cpp:4525-4680): the
+/// local matrix with its off-rank columns remapped to n_local + slot, and
+/// the HaloPattern of those columns.  owner_of_src maps a global source id
+/// to its rank; this rank is the owner of the first local source cell.
+std::unique_ptr<InterpolationMatrix> generate_distributed(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, const index_t *src_global_ids,
+                                                          index_t n_src_ids, const std::int32_t *owner_of_src, index_t n_owner, HaloPattern &pattern,
+                                                          hipStream_t s);
+/// The gathered-buffer distributed apply (apply.hpp:151-211) on FP64
+/// layout_left device fields: dst = S [local_src ; halo].
+void apply_distributed(InterpolationMatrix &A, const HaloPattern &pattern, const double *local_src, index_t n_local, const double *halo,
+                       index_t n_halo, double *dst, index_t n_dst, index_t k, hipStream_t s);
 /// Float (lon, lat) centroids of every cell (compute_cell_centroids_xy,
 /// weight_generator.cpp:74-138, rounded to float) into device memory, equal
 /// to host libm's; returns how many cells were recomputed on the host.

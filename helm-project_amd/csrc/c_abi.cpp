@@ -31,7 +31,7 @@ struct Registry {
     std::mutex mu;
     int next = 1;
     std::unordered_map<int, std::variant<std::shared_ptr<Mesh>, std::shared_ptr<InterpolationMatrix>, std::shared_ptr<BatchPlan>,
-                                         std::shared_ptr<mlp::Model>>>
+                                         std::shared_ptr<mlp::Model>, std::shared_ptr<HaloPattern>>>
         items;
     static Registry &get() {
         static Registry r;
@@ -256,6 +256,79 @@ int axis_generate_weights_config_c(int src_handle, int dst_handle, const axis_re
         const RegridConfig cfg = config_of(config);
         std::shared_ptr<InterpolationMatrix> M = WeightGenerator::generate(*src, *dst, cfg, nullptr);
         *matrix_handle = Registry::get().add(std::move(M));
+    });
+}
+
+int axis_generate_weights_distributed_c(int src_handle, int dst_handle, const axis_regrid_config *config, const int64_t *src_global_ids,
+                                        int64_t n_src_ids, const int64_t *dst_global_ids, int64_t n_dst_ids, const int32_t *owner_of_src,
+                                        int64_t n_owner, int *matrix_handle, int *pattern_handle) {
+    return guarded([&] {
+        if (!matrix_handle || !pattern_handle || (n_src_ids > 0 && !src_global_ids) || (n_owner > 0 && !owner_of_src))
+            throw InvalidArgument("null argument");
+        (void)dst_global_ids;  // carried by the reference's signature, not read by it
+        (void)n_dst_ids;
+        auto src = Registry::get().find<Mesh>(src_handle);
+        auto dst = Registry::get().find<Mesh>(dst_handle);
+        const RegridConfig cfg = config ? config_of(config) : RegridConfig{};
+        auto pattern = std::make_shared<HaloPattern>();
+        std::shared_ptr<InterpolationMatrix> M =
+            detail::generate_distributed(*src, *dst, cfg, src_global_ids, n_src_ids, owner_of_src, n_owner, *pattern, nullptr);
+        *matrix_handle = Registry::get().add(std::move(M));
+        *pattern_handle = Registry::get().add(std::move(pattern));
+    });
+}
+
+int axis_halo_pattern_info_c(int pattern_handle, int64_t *num_remote, int64_t *num_ranks) {
+    return guarded([&] {
+        auto p = Registry::get().find<HaloPattern>(pattern_handle);
+        if (num_remote) *num_remote = p->num_remote();
+        if (num_ranks) *num_ranks = p->num_ranks();
+    });
+}
+
+int axis_halo_pattern_arrays_c(int pattern_handle, int32_t *source_ranks, int64_t *rank_offsets, int64_t *needed_global_src_ids,
+                               int64_t *gather_slot) {
+    return guarded([&] {
+        auto p = Registry::get().find<HaloPattern>(pattern_handle);
+        if (source_ranks) std::copy(p->source_ranks.begin(), p->source_ranks.end(), source_ranks);
+        if (rank_offsets) std::copy(p->rank_offsets.begin(), p->rank_offsets.end(), rank_offsets);
+        if (needed_global_src_ids) std::copy(p->needed_global_src_ids.begin(), p->needed_global_src_ids.end(), needed_global_src_ids);
+        if (gather_slot) std::copy(p->gather_slot.begin(), p->gather_slot.end(), gather_slot);
+    });
+}
+
+int axis_destroy_halo_pattern_c(int pattern_handle) {
+    return guarded([&] { Registry::get().release<HaloPattern>(pattern_handle); });
+}
+
+int axis_apply_distributed_c(int matrix_handle, int pattern_handle, const double *local_src, int64_t n_local, const double *gathered_halo_src,
+                             int64_t n_halo, double *dst, int64_t n_dst, int64_t n_fields, void *stream) {
+    return guarded([&] {
+        auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
+        auto p = Registry::get().find<HaloPattern>(pattern_handle);
+        check_device();
+        if ((!local_src && n_local > 0) || (!gathered_halo_src && n_halo > 0) || (!dst && n_dst > 0)) throw InvalidArgument("null argument");
+        detail::apply_distributed(*M, *p, local_src, n_local, gathered_halo_src, n_halo, dst, n_dst, n_fields, static_cast<hipStream_t>(stream));
+    });
+}
+
+int axis_apply_distributed_callback_c(int matrix_handle, int pattern_handle, const double *local_src, int64_t n_local, axis_gather_fn gather,
+                                      void *user, double *dst, int64_t n_dst, void *stream) {
+    return guarded([&] {
+        auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
+        auto p = Registry::get().find<HaloPattern>(pattern_handle);
+        check_device();
+        if (!gather) throw InvalidArgument("null gather callback");
+        if ((!local_src && n_local > 0) || (!dst && n_dst > 0)) throw InvalidArgument("null argument");
+        // the callback is invoked exactly once with the pattern (apply.hpp:233-248)
+        const index_t n = p->num_remote();
+        std::vector<double> halo(std::max<index_t>(n, 1));
+        const int rc = gather(p->needed_global_src_ids.data(), n, halo.data(), user);
+        if (rc != 0) throw InvalidArgument("axis::solver::apply (callback): gather failed with code " + std::to_string(rc));
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        DeviceBuffer<double> d_halo(std::max<index_t>(n, 1));
+        d_halo.upload(halo.data(), n, s);
+        detail::apply_distributed(*M, *p, local_src, n_local, d_halo.get(), n, dst, n_dst, 1, s);
     });
 }
 

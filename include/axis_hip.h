@@ -83,6 +83,40 @@ int axis_mesh_info_c(int mesh_handle, int64_t *n_nodes, int64_t *n_cells);
  * (detail/regular_grid_detector.hpp:64-144, 180-299). */
 int axis_mesh_grid_info_c(int mesh_handle, int *kind, int64_t *ninj, double *out6);
 
+/* ---- distributed mode of one rank (the exchange itself is the caller's) */
+
+/* HaloPattern handle: the off-rank source cells a rank's apply reads,
+ * grouped by owning rank (halo_pattern.hpp:29-48).  Produced by
+ * WeightGenerator::generate(src, dst, config, src_global_ids, dst_global_ids,
+ * owner_of_src) (weight_generator.hpp:83-90, weight_generator.cpp:4525-4680):
+ * the local matrix's off-rank columns are remapped to n_local + slot.  The
+ * rank is the owner of the first local source cell; config may be NULL
+ * (defaults).  dst_global_ids is accepted and, as in the reference, unused. */
+int axis_generate_weights_distributed_c(int src_handle, int dst_handle, const axis_regrid_config *config, const int64_t *src_global_ids,
+                                        int64_t n_src_ids, const int64_t *dst_global_ids, int64_t n_dst_ids, const int32_t *owner_of_src,
+                                        int64_t n_owner, int *matrix_handle, int *pattern_handle);
+/* HaloPattern::num_remote / num_ranks (halo_pattern.hpp:39-47). */
+int axis_halo_pattern_info_c(int pattern_handle, int64_t *num_remote, int64_t *num_ranks);
+/* Copies of the pattern's arrays into host buffers sized from
+ * axis_halo_pattern_info_c (rank_offsets has num_ranks + 1 entries); NULL
+ * skips an array. */
+int axis_halo_pattern_arrays_c(int pattern_handle, int32_t *source_ranks, int64_t *rank_offsets, int64_t *needed_global_src_ids,
+                               int64_t *gather_slot);
+int axis_destroy_halo_pattern_c(int pattern_handle);
+/* Distributed apply with the gathered halo buffer (apply.hpp:151-211):
+ * dst = S [local_src ; gathered_halo_src] on FP64 device fields, layout_left
+ * [n, n_fields]; the reference's extent checks and messages, before any
+ * write. */
+int axis_apply_distributed_c(int matrix_handle, int pattern_handle, const double *local_src, int64_t n_local, const double *gathered_halo_src,
+                             int64_t n_halo, double *dst, int64_t n_dst, int64_t n_fields, void *stream);
+/* Gather callback of the callback overload: fills out[n_remote] (host) with
+ * the values of needed_global_src_ids in gather_slot order; nonzero = error. */
+typedef int (*axis_gather_fn)(const int64_t *needed_global_src_ids, int64_t n_remote, double *out, void *user);
+/* Callback distributed apply (apply.hpp:233-248): gather is invoked exactly
+ * once, then the buffer overload runs (one FP64 device field). */
+int axis_apply_distributed_callback_c(int matrix_handle, int pattern_handle, const double *local_src, int64_t n_local, axis_gather_fn gather,
+                                      void *user, double *dst, int64_t n_dst, void *stream);
+
 /* Cell centroids as the masked-source extrapolation uses them:
  * compute_cell_centroids_xy (weight_generator.cpp:74-138) rounded to float
  * (:1813, :1843), into host arrays lon[n_cells], lat[n_cells].  Computed on the

@@ -319,3 +319,33 @@ def cell_centroids(mesh, coord_sys: int = 0):
     lat = np.zeros(n, dtype=np.float32)
     lib().om_cell_centroids(_p(c, _D), mesh["coords"].shape[0], _p(o, _I64), _p(i, _I64), n, coord_sys, _p(lon, _F), _p(lat, _F))
     return lon, lat
+
+
+def halo_pattern(cols, src_global_ids, owner_of_src):
+    """HaloPattern of one rank (weight_generator.cpp:4525-4680), restated over
+    the local matrix's column indices: the rank is the owner of the first
+    local source cell; used local columns whose global id another rank owns
+    are grouped by owner rank, ascending local column within a rank (the
+    reference takes first appearance in its COO order, then an unstable sort
+    by rank); slot g is remapped to column n_local + g.  Returns
+    (pattern dict, remapped columns)."""
+    cols = np.asarray(cols, dtype=np.int64)
+    gid = np.asarray(src_global_ids, dtype=np.int64)
+    owner = np.asarray(owner_of_src, dtype=np.int32)
+    n_local = gid.shape[0]
+    me = int(owner[gid[0]]) if n_local > 0 and 0 <= gid[0] < owner.shape[0] else -1
+    used = np.unique(cols[(cols >= 0) & (cols < n_local)])
+    g = gid[used]
+    ok = (g >= 0) & (g < owner.shape[0])
+    used, g = used[ok], g[ok]
+    off = used[owner[g] != me]
+    ranks = owner[gid[off]]
+    order = np.argsort(ranks, kind="stable")
+    off, ranks = off[order], ranks[order]
+    src_ranks, starts = np.unique(ranks, return_index=True)
+    pattern = {"source_ranks": src_ranks.astype(np.int32),
+               "rank_offsets": np.concatenate([starts, [off.shape[0]]]).astype(np.int64),
+               "needed_global_src_ids": gid[off], "gather_slot": np.arange(off.shape[0], dtype=np.int64)}
+    remap = np.arange(n_local, dtype=np.int64)
+    remap[off] = n_local + np.arange(off.shape[0])
+    return pattern, remap[cols]

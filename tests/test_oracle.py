@@ -630,3 +630,21 @@ def test_bilinear_regular_affine_exact():
     inner = (lon_d > 1.0) & (lon_d < 359.0) & (np.abs(lat_d) < 89.0)
     assert np.allclose(y[inner], 3.0 + 0.01 * lon_d[inner] - 0.02 * lat_d[inner], atol=1e-12)
     assert np.all(np.diff(w["row_ptr"]) == 4) and np.all(w["frac_b"] == 1.0) and not w["area_a"].any()
+
+
+# ---- HaloPattern (weight_generator.cpp:4525-4680), pinned on a hand example
+def test_halo_pattern_hand_example():
+    # local cells 0..5 with global ids; owners of the global ids; this rank = owner of gid 10 = 1
+    gid = np.array([10, 11, 20, 21, 30, 31])
+    owner = np.zeros(40, np.int32)
+    owner[10:20], owner[20:30], owner[30:40] = 1, 2, 0
+    cols = np.array([5, 0, 2, 3, 4, 2, 1])  # local 0,1 owned; 2,3 from rank 2; 4,5 from rank 0
+    pat, new = O.halo_pattern(cols, gid, owner)
+    assert list(pat["source_ranks"]) == [0, 2]
+    assert list(pat["rank_offsets"]) == [0, 2, 4]
+    assert list(pat["needed_global_src_ids"]) == [30, 31, 20, 21]
+    assert list(pat["gather_slot"]) == [0, 1, 2, 3]
+    assert list(new) == [6 + 1, 0, 6 + 2, 6 + 3, 6 + 0, 6 + 2, 1]
+    # a matrix that reads only owned cells publishes an empty pattern
+    pat, new = O.halo_pattern(np.array([0, 1, 1]), gid, owner)
+    assert pat["source_ranks"].size == 0 and list(pat["rank_offsets"]) == [0] and list(new) == [0, 1, 1]
