@@ -127,7 +127,7 @@ struct PT<2> {
     struct B {
         float4 v;
     };
-    static constexpr int kKeyBits = 16;
+    static constexpr int kKeyBits = 12;  // 24-bit keys: three radix passes
     __device__ static B empty() { return B{make_float4(INFINITY, INFINITY, -INFINITY, -INFINITY)}; }
     __device__ static B of(P p) { return B{make_float4(p.x, p.y, p.x, p.y)}; }
     __device__ static B join(B a, B b) { return B{make_float4(fminf(a.v.x, b.v.x), fminf(a.v.y, b.v.y), fmaxf(a.v.z, b.v.z), fmaxf(a.v.w, b.v.w))}; }
@@ -156,7 +156,7 @@ struct PT<3> {
     struct B {
         float4 l, h;
     };
-    static constexpr int kKeyBits = 10;
+    static constexpr int kKeyBits = 8;  // 24-bit keys: three radix passes
     __device__ static B empty() { return B{make_float4(INFINITY, INFINITY, INFINITY, 0.0f), make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f)}; }
     __device__ static B of(P p) { return B{p, p}; }
     __device__ static B join(B a, B b) {
@@ -752,8 +752,7 @@ index_t all_cell_points(const Mesh &m, typename PT<D>::P *out, hipStream_t s) {
 
 // Exact nearest point for every query: pts[n] (ids[q]: the id reported for
 // point q; null: q itself), queries[n_q]; out[qrow ? qrow[t] : t] = id of the
-// nearest point (smallest float squared distance, then smallest id).  The
-// queries are taken in Morton order so that a wave walks one neighbourhood.
+// nearest point (smallest float squared distance, then smallest id).
 // Synchronises the stream before returning (its work buffers are released).
 template <int D>
 void nearest_points(const typename PT<D>::P *pts, const std::int32_t *ids, index_t n, const typename PT<D>::P *queries, const std::int32_t *qrow,
@@ -774,7 +773,7 @@ void nearest_points(const typename PT<D>::P *pts, const std::int32_t *ids, index
     DeviceBuffer<std::uint32_t> key(n), kpos(n);
     point_keys_kernel<D><<<grid_for(n, 256), 256, 0, s>>>(pts, n, bounds.get(), key.get(), kpos.get());
     AXIS_LAUNCH_CHECK();
-    radix_sort_pairs_u32(key.get(), kpos.get(), static_cast<std::size_t>(n), 32, s);
+    radix_sort_pairs_u32(key.get(), kpos.get(), static_cast<std::size_t>(n), D * T::kKeyBits, s);
     DeviceBuffer<P> spts(n_pad);
     DeviceBuffer<std::int32_t> sidx(n_pad);
     gather_points_kernel<D><<<grid_for(n_pad, 256), 256, 0, s>>>(kpos.get(), pts, ids, n, n_pad, spts.get(), sidx.get());
@@ -786,12 +785,10 @@ void nearest_points(const typename PT<D>::P *pts, const std::int32_t *ids, index
         inner_boxes_kernel<D><<<grid_for(lo, 256), 256, 0, s>>>(box.get(), lo);
         AXIS_LAUNCH_CHECK();
     }
-    DeviceBuffer<std::uint32_t> qkey(n_q), qorder(n_q);
-    point_keys_kernel<D><<<grid_for(n_q, 256), 256, 0, s>>>(queries, n_q, bounds.get(), qkey.get(), qorder.get());
-    AXIS_LAUNCH_CHECK();
-    radix_sort_pairs_u32(qkey.get(), qorder.get(), static_cast<std::size_t>(n_q), 32, s);
-    nearest_kernel<D><<<grid_for(n_q, kNearThreads), kNearThreads, 0, s>>>(box.get(), spts.get(), sidx.get(), n_leaves, L, queries, qrow,
-                                                                           qorder.get(), n_q, out);
+    // queries in their own order: destination and dry-row indices are
+    // already spatially coherent (a Morton sort of them measured slower)
+    nearest_kernel<D><<<grid_for(n_q, kNearThreads), kNearThreads, 0, s>>>(box.get(), spts.get(), sidx.get(), n_leaves, L, queries, qrow, nullptr,
+                                                                           n_q, out);
     AXIS_LAUNCH_CHECK();
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
 }
