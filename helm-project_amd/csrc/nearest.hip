@@ -21,7 +21,7 @@
 //     lon = 0 branch cut) is listed and recomputed on the host, so the float
 //     points equal the reference's host-computed ones bit for bit;
 //   * exact nearest search over float points (2D or 3D): Morton sort, leaves
-//     of 32 points, an implicit complete binary tree of float boxes (heap
+//     of 4 points, an implicit complete binary tree of float boxes (heap
 //     order, root 1), one thread per query with its traversal stack in LDS;
 //     smallest float squared distance wins, ties go to the smallest source
 //     index (ArborX leaves exact ties open).
@@ -36,7 +36,7 @@
 namespace axis_hip::detail {
 namespace {
 
-constexpr int kLeafPts = 32;
+constexpr int kLeafPts = 4;  // 32: 7.6 ms, 16: 5.5, 8: 4.2, 4: 3.5, 2: 3.3 for 1.04 M queries against 885 K unit vectors
 constexpr int kStack = 32;
 // Rows longer than this (polar rows of a conservative matrix run to ~1600
 // nonzeros) are summed and compacted by a whole wave; shorter rows by one
@@ -419,8 +419,8 @@ __global__ void inner_boxes_kernel(typename PT<D>::B *__restrict__ box, index_t 
 
 // One thread per query (optionally taken in the order qorder); the traversal
 // stack sits in LDS ([depth][thread], conflict-free): a stack in scratch
-// memory costs a memory round trip per push and pop.  The tree height is at
-// most 26 (2^26 leaves of 32 points), so kStack entries always suffice.
+// memory costs a memory round trip per push and pop.  The host refuses trees
+// taller than kStack - 1 levels, so the stack never overflows.
 // out[qrow ? qrow[t] : t] receives the nearest point's id (left alone when
 // every distance is NaN).
 constexpr int kNearThreads = 128;
@@ -764,7 +764,9 @@ void nearest_points(const typename PT<D>::P *pts, const std::int32_t *ids, index
     const index_t n_leaves = (n + kLeafPts - 1) / kLeafPts, n_pad = n_leaves * kLeafPts;
     index_t L = 1;
     while (L < n_leaves) L *= 2;
-    if (L > (index_t(1) << 26)) throw InvalidArgument("nearest search: more than 2^31 points");
+    int height = 0;
+    while ((index_t(1) << height) < L) ++height;
+    if (n >= (index_t(1) << 31) || height + 1 > kStack) throw InvalidArgument("nearest search: too many points");
     DeviceBuffer<B> part(kBoundBlocks), bounds(1);
     bounds_kernel<D><<<kBoundBlocks, 256, 0, s>>>(pts, n, part.get());
     AXIS_LAUNCH_CHECK();
