@@ -487,12 +487,13 @@ def matrix_from_coo(rows, cols, vals, n_src, n_dst, frac_a=None, frac_b=None, ar
 
 
 class BatchPlan:
-    """Several (matrix, src, dst, k) regrid pairs applied by one persistent
-    work-queue launch recorded in a HIP graph (axis_batch_plan_*_c).  The
-    field tensors are bound at construction and must stay alive; run()
-    replays the graph."""
+    """Several (matrix, src, dst, k) regrid pairs applied by one HIP graph
+    (axis_batch_plan_*_c): a persistent work-queue launch, or -- when the
+    autotuner measures it faster -- the pairs' own tuned launches captured in
+    order.  The field tensors are bound at construction and must stay alive;
+    run() replays the graph."""
 
-    def __init__(self, pairs, src_dtype, dst_dtype, mode, autotune=False, stream=None):
+    def __init__(self, pairs, src_dtype, dst_dtype, mode, autotune=False, stream=None, pair_launches=False):
         n = len(pairs)
         hs = (C.c_int * n)(*[m.handle for m, _, _, _ in pairs])
         xs = (C.c_void_p * n)(*[_ptr(x) for _, x, _, _ in pairs])
@@ -500,7 +501,7 @@ class BatchPlan:
         ks = (C.c_int64 * n)(*[int(k) for _, _, _, k in pairs])
         self._keep = list(pairs)
         h = C.c_int(0)
-        _check(lib().axis_batch_plan_create_c(n, hs, xs, ys, ks, src_dtype, dst_dtype, mode, 1 if autotune else 0,
+        _check(lib().axis_batch_plan_create_c(n, hs, xs, ys, ks, src_dtype, dst_dtype, mode, 2 if pair_launches else (1 if autotune else 0),
                                               _stream(stream), C.byref(h)))
         self.handle = h.value
 
@@ -511,7 +512,8 @@ class BatchPlan:
         fs, fi, nw = C.c_int(0), C.c_int(0), C.c_int(0)
         ni = C.c_int64(0)
         _check(lib().axis_batch_plan_info_c(self.handle, C.byref(fs), C.byref(fi), C.byref(ni), C.byref(nw)))
-        return {"fields_per_step": fs.value, "fields_per_item": fi.value, "items": ni.value, "workgroups": nw.value}
+        rec = "pair_launches" if fs.value == 0 else "persistent"
+        return {"recording": rec, "fields_per_step": fs.value, "fields_per_item": fi.value, "items": ni.value, "workgroups": nw.value}
 
     def close(self):
         if getattr(self, "handle", 0):

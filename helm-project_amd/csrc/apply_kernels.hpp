@@ -601,6 +601,7 @@ inline double *partials(InterpolationMatrix &A, index_t k) {
 struct SpecialArgs {
     double thres = 0.0;
     const std::int32_t *mask = nullptr;
+    double *partials = nullptr;  // long-row partial sums (null: the matrix's own workspace)
 };
 
 template <int E, int FU, int NUM, class XT, class YT>
@@ -608,7 +609,7 @@ void launch_e(InterpolationMatrix &A, const void *x, void *y, index_t k, int chu
     Mat M = mat_of(A);
     M.thres = sa.thres;
     M.mask = sa.mask;
-    double *P = partials(A, k);
+    double *P = sa.partials ? sa.partials : partials(A, k);
     const unsigned nb = static_cast<unsigned>(std::max<index_t>(tiled ? A.n_tiles : (A.n_seg + kBlock - 1) / kBlock, 1));
     const std::uint64_t nwork = static_cast<std::uint64_t>(nb) * chunks;
     if (nwork >= (1ull << 31)) throw InvalidArgument("apply: launch too large");
@@ -653,20 +654,23 @@ void launch_xy(InterpolationMatrix &A, const void *x, void *y, index_t k, const 
 }
 
 template <int NUM>
-void launch_num(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t k, const Launch &L, hipStream_t s) {
-    if (xt == FieldType::F32 && yt == FieldType::F32) return launch_xy<NUM, float, float>(A, x, y, k, L, s);
-    if (xt == FieldType::F32 && yt == FieldType::F64) return launch_xy<NUM, float, double>(A, x, y, k, L, s);
-    if (xt == FieldType::BF16 && yt == FieldType::F32) return launch_xy<NUM, std::uint16_t, float>(A, x, y, k, L, s);
-    if (xt == FieldType::BF16 && yt == FieldType::F64) return launch_xy<NUM, std::uint16_t, double>(A, x, y, k, L, s);
+void launch_num(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t k, const Launch &L, hipStream_t s,
+                const SpecialArgs &sa) {
+    if (xt == FieldType::F32 && yt == FieldType::F32) return launch_xy<NUM, float, float>(A, x, y, k, L, s, sa);
+    if (xt == FieldType::F32 && yt == FieldType::F64) return launch_xy<NUM, float, double>(A, x, y, k, L, s, sa);
+    if (xt == FieldType::BF16 && yt == FieldType::F32) return launch_xy<NUM, std::uint16_t, float>(A, x, y, k, L, s, sa);
+    if (xt == FieldType::BF16 && yt == FieldType::F64) return launch_xy<NUM, std::uint16_t, double>(A, x, y, k, L, s, sa);
     throw InvalidArgument("apply: unsupported field type combination");
 }
 
+/// One apply with launch shape L; sa.partials, when given, replaces the
+/// matrix's own long-row workspace (n_slots * k doubles).
 inline void launch(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t k, ApplyMode mode, const Launch &L,
-                   hipStream_t s) {
+                   hipStream_t s, const SpecialArgs &sa = SpecialArgs{}) {
     if (L.tiled) A.prepare_tiles(s);
-    if (mode == ApplyMode::Plain) return launch_xy<kPlain, double, double>(A, x, y, k, L, s);
-    if (mode == ApplyMode::Mixed) return launch_num<kComp32>(A, x, xt, y, yt, k, L, s);
-    return launch_num<kAcc64>(A, x, xt, y, yt, k, L, s);
+    if (mode == ApplyMode::Plain) return launch_xy<kPlain, double, double>(A, x, y, k, L, s, sa);
+    if (mode == ApplyMode::Mixed) return launch_num<kComp32>(A, x, xt, y, yt, k, L, s, sa);
+    return launch_num<kAcc64>(A, x, xt, y, yt, k, L, s, sa);
 }
 
 // ------------------------------------------------------------- autotuner

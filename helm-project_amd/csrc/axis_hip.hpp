@@ -214,12 +214,18 @@ ApplyShape autotune_apply(InterpolationMatrix &A, const void *x, FieldType xt, v
 
 /// Launch shape of a persistent multi-pair batch: fields per step (staged
 /// together in LDS) and fields per work item.
+/// A plan's recording: the persistent work-queue kernel with its fields per
+/// step and per work item, or (pair_launches) each pair's own autotuned
+/// apply launches captured one after another in the graph.
 struct BatchShape {
     int fields_per_step = 4, fields_per_item = 16;
+    bool pair_launches = false;
 };
 
-/// Several regrid pairs applied in one persistent work-queue launch recorded
-/// in a HIP graph.  Pair p computes dst[p] = A[p] src[p] on device-resident
+/// Several regrid pairs applied in one HIP graph: a persistent work-queue
+/// launch over (pair, tile, field range) items, or -- when the autotuner
+/// measures it faster -- each pair's own tuned apply launches captured in
+/// order.  Pair p computes dst[p] = A[p] src[p] on device-resident
 /// layout_left fields [n_src, k[p]] -> [n_dst, k[p]]; all pairs share field
 /// types and mode.  The field pointers are bound at construction (the graph
 /// records them); run() replays the graph on any stream.  Replays of one
@@ -227,7 +233,7 @@ struct BatchShape {
 class BatchPlan {
    public:
     BatchPlan(std::vector<std::shared_ptr<InterpolationMatrix>> mats, std::vector<const void *> src, std::vector<void *> dst,
-              std::vector<index_t> k, FieldType xt, FieldType yt, ApplyMode mode, bool autotune, hipStream_t s);
+              std::vector<index_t> k, FieldType xt, FieldType yt, ApplyMode mode, int autotune, hipStream_t s);
     ~BatchPlan();
     void run(hipStream_t s, bool sync);
     BatchShape shape() const;

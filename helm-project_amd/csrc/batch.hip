@@ -230,11 +230,35 @@ struct BatchPlan::Impl {
         return y64 ? record_xy<kAcc64, float, double>(sh.fields_per_step, s) : record_xy<kAcc64, float, float>(sh.fields_per_step, s);
     }
 
+    // The pairs' own tuned launches, captured in order; each pair reads and
+    // writes the plan's partial-sum workspace, as the persistent kernel does.
+    std::vector<apply_detail::Launch> pair_shape;
+    void record_pairs() {
+        drop_graph();
+        n_items = 0;
+        n_wg = 0;
+        AXIS_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        for (std::size_t p = 0; p < mats.size(); ++p) {
+            if (k[p] <= 0 || mats[p]->n_dst == 0) continue;
+            apply_detail::SpecialArgs sa;
+            sa.partials = partial[p].get();
+            apply_detail::launch(*mats[p], src[p], xt, dst[p], yt, k[p], mode, pair_shape[p], cap, sa);
+        }
+        hipError_t launch_err = hipGetLastError();
+        hipGraph_t g = nullptr;
+        hipError_t cap_err = hipStreamEndCapture(cap, &g);
+        AXIS_HIP_CHECK(launch_err);
+        AXIS_HIP_CHECK(cap_err);
+        graph = g;
+        AXIS_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        shape = BatchShape{0, 0, true};
+    }
+
     void launch(hipStream_t s) { AXIS_HIP_CHECK(hipGraphLaunch(exec, s)); }
 };
 
 BatchPlan::BatchPlan(std::vector<std::shared_ptr<InterpolationMatrix>> mats, std::vector<const void *> src, std::vector<void *> dst,
-                     std::vector<index_t> k, FieldType xt, FieldType yt, ApplyMode mode, bool autotune, hipStream_t s)
+                     std::vector<index_t> k, FieldType xt, FieldType yt, ApplyMode mode, int autotune, hipStream_t s)
     : impl_(std::make_unique<Impl>()) {
     const std::size_t n = mats.size();
     if (n == 0) throw InvalidArgument("BatchPlan: no regrid pairs");
@@ -278,32 +302,55 @@ BatchPlan::BatchPlan(std::vector<std::shared_ptr<InterpolationMatrix>> mats, std
     BatchShape best;
     best.fields_per_step = mode == ApplyMode::Mixed ? 4 : 2;
     best.fields_per_item = static_cast<int>(std::clamp<index_t>(k_max, 1, 16));
-    if (autotune) {
+    // each pair's launch shape: its own autotuned (or cached, else heuristic) shape
+    I.pair_shape.resize(n);
+    for (std::size_t p = 0; p < n; ++p) {
+        if (I.k[p] <= 0) continue;
+        if (autotune != 0) {
+            const ApplyShape sh = autotune_apply(*I.mats[p], I.src[p], xt, I.dst[p], yt, I.k[p], mode, s);
+            I.pair_shape[p] = apply_detail::Launch{sh.fields_per_step, sh.field_chunks, sh.staged};
+        } else {
+            I.pair_shape[p] = apply_detail::choose(*I.mats[p], I.k[p], xt, yt, mode);
+        }
+        if (I.pair_shape[p].tiled) I.mats[p]->prepare_tiles(s);
+    }
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    if (autotune == 1) {
         hipEvent_t e0, e1;
         AXIS_HIP_CHECK(hipEventCreate(&e0));
         AXIS_HIP_CHECK(hipEventCreate(&e1));
         float best_ms = 1e30f;
+        auto time_graph = [&]() {
+            I.launch(s);  // warm
+            AXIS_HIP_CHECK(hipEventRecord(e0, s));
+            for (int r = 0; r < 3; ++r) I.launch(s);
+            AXIS_HIP_CHECK(hipEventRecord(e1, s));
+            AXIS_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            AXIS_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            return ms;
+        };
         for (int fu : {1, 2, 4})
             for (int fpi : {4, 8, 16, 32, 64, 128}) {
                 if (fpi > 4 && fpi / 2 >= k_max) continue;
                 if (fu > fpi) continue;
                 I.record(BatchShape{fu, fpi}, s);
-                I.launch(s);  // warm
-                AXIS_HIP_CHECK(hipEventRecord(e0, s));
-                for (int r = 0; r < 3; ++r) I.launch(s);
-                AXIS_HIP_CHECK(hipEventRecord(e1, s));
-                AXIS_HIP_CHECK(hipEventSynchronize(e1));
-                float ms = 0.0f;
-                AXIS_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                const float ms = time_graph();
                 if (ms < best_ms) {
                     best_ms = ms;
                     best = BatchShape{fu, fpi};
                 }
             }
+        I.record_pairs();
+        const float ms = time_graph();
+        if (ms < best_ms) best = I.shape;
         AXIS_HIP_CHECK(hipEventDestroy(e0));
         AXIS_HIP_CHECK(hipEventDestroy(e1));
     }
-    I.record(best, s);
+    if (best.pair_launches || autotune == 2)
+        I.record_pairs();
+    else
+        I.record(best, s);
 }
 
 BatchPlan::~BatchPlan() = default;

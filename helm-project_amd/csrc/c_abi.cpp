@@ -575,6 +575,7 @@ int axis_batch_plan_create_c(int n_pairs, const int *matrix_handles, const void 
     return guarded([&] {
         if (!plan_handle) throw InvalidArgument("null plan_handle");
         if (n_pairs <= 0 || !matrix_handles || !src || !dst || !n_fields) throw InvalidArgument("axis_batch_plan_create_c: empty or null pair list");
+        if (autotune < 0 || autotune > 2) throw InvalidArgument("axis_batch_plan_create_c: autotune must be 0, 1 or 2");
         std::vector<std::shared_ptr<InterpolationMatrix>> mats;
         std::vector<const void *> xs;
         std::vector<void *> ys;
@@ -588,7 +589,7 @@ int axis_batch_plan_create_c(int n_pairs, const int *matrix_handles, const void 
                 throw InvalidArgument("axis_batch_plan_create_c: fields of pair " + std::to_string(p) + " are not device memory");
         }
         auto plan = std::make_shared<BatchPlan>(std::move(mats), std::move(xs), std::move(ys), std::move(ks), ftype(src_dtype), ftype(dst_dtype),
-                                                amode(mode), autotune != 0, static_cast<hipStream_t>(stream));
+                                                amode(mode), autotune, static_cast<hipStream_t>(stream));
         *plan_handle = Registry::get().add(std::move(plan));
     });
 }

@@ -192,8 +192,10 @@ int axis_check_conservation_c(int matrix_handle, const double *src, const double
  * applies one matrix per batch_apply call, apply.hpp:275-343).  Binds
  * n_pairs (matrix, device src [n_src, n_fields[p]], device dst
  * [n_dst, n_fields[p]]) triples of one field type and mode, builds the work
- * queues and records one HIP graph (persistent work-queue kernel + long-row
- * reductions); autotune != 0 times the launch shapes first.  The matrices
+ * queues and records one HIP graph.  autotune 0: the persistent work-queue
+ * kernel (+ long-row reductions) with a default shape; 1: its shapes and the
+ * pairs' own autotuned launches captured in order are timed and the faster
+ * recording kept; 2: the pairs' own launches (tuned shapes).  The matrices
  * stay alive while the plan exists. */
 int axis_batch_plan_create_c(int n_pairs, const int *matrix_handles, const void *const *src, void *const *dst, const int64_t *n_fields,
                              int src_dtype, int dst_dtype, int mode, int autotune, void *stream, int *plan_handle);

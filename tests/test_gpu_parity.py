@@ -318,7 +318,8 @@ def batch_pairs():
 
 
 @pytest.mark.parametrize("mode,xdt,autotune", [(axis.MIXED, axis.F32, False), (axis.MIXED_ACC64, axis.BF16, True),
-                                               (axis.PLAIN, axis.F64, False)])
+                                               (axis.PLAIN, axis.F64, False), (axis.MIXED, axis.F32, "pairs"),
+                                               (axis.PLAIN, axis.F64, "pairs")])
 def test_batch_plan_matches_single_applies(batch_pairs, mode, xdt, autotune):
     """The persistent multi-pair graph gives, bit for bit, what one
     batch_apply per pair gives (every row is summed in the same order), on
@@ -333,8 +334,9 @@ def test_batch_plan_matches_single_applies(batch_pairs, mode, xdt, autotune):
         m.batch_apply(X, R, k, xdt, ydt, mode)
         pairs.append((m, X, Y, k))
         refs.append(R)
-    plan = axis.BatchPlan(pairs, xdt, ydt, mode, autotune=autotune)
+    plan = axis.BatchPlan(pairs, xdt, ydt, mode, autotune=autotune is True, pair_launches=autotune == "pairs")
     say("plan", plan.info())
+    assert (plan.info()["recording"] == "pair_launches") == (autotune == "pairs") or autotune is True
     for rep in range(3):
         for (_, _, Y, _) in pairs:
             Y.fill_(-1.0)  # eager work between replays
