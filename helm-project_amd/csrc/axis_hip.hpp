@@ -60,6 +60,7 @@ class Mesh {
     bool has_areas() const { return !h_areas.empty(); }
     bool has_mask() const { return !h_mask.empty(); }
     int max_cell_vertices() const { return max_nv_; }
+    int min_cell_vertices() const { return min_nv_; }
 
     // host copies
     std::vector<double> h_coords;  // [2 * n_nodes] lon block then lat block
@@ -75,7 +76,7 @@ class Mesh {
    private:
     index_t n_nodes_ = 0, n_cells_ = 0;
     CoordinateSystem cs_ = CoordinateSystem::SphericalDeg;
-    int max_nv_ = 0;
+    int max_nv_ = 0, min_nv_ = 0;
 };
 
 /// InterpolationMatrix, always held in CSR form sorted by (row, col)

@@ -43,6 +43,7 @@ Mesh::Mesh(const double *node_coords, index_t n_nodes, index_t ndim, const index
         index_t nv = h_offsets[c + 1] - h_offsets[c];
         if (nv < 0) throw InvalidArgument("Mesh: conn_offsets not monotone");
         max_nv_ = std::max<int>(max_nv_, static_cast<int>(std::min<index_t>(nv, 1 << 20)));
+        min_nv_ = c == 0 ? static_cast<int>(std::min<index_t>(nv, 1 << 20)) : std::min<int>(min_nv_, static_cast<int>(std::min<index_t>(nv, 1 << 20)));
     }
     const index_t nidx = h_offsets[n_cells];
     h_indices.assign(conn_indices, conn_indices + nidx);
@@ -306,12 +307,7 @@ GridAxes unique_axes(const Mesh &m, hipStream_t s) {
     GridAxes ax;
     const index_t n_cells = m.n_cells();
     if (n_cells == 0) return ax;
-    ax.all_quads = true;
-    for (index_t c = 0; c < n_cells; ++c)
-        if (m.h_offsets[c + 1] - m.h_offsets[c] != 4) {
-            ax.all_quads = false;
-            break;
-        }
+    ax.all_quads = m.min_cell_vertices() == 4 && m.max_cell_vertices() == 4;
     if (!ax.all_quads) return ax;
     const index_t n = m.n_nodes();
     PhaseTrace tr("axes", s);
