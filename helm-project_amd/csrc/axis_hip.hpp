@@ -255,6 +255,7 @@ struct MeshGeom {
     PinnedArray<double> h_xyz;          // node xyz [3 * n_nodes] (libm, reference bits)
     PinnedArray<double> h_xyz_deg;      // radian meshes: xyz of the coordinates read as degrees
     bool xyz_ready = false;             // h_xyz (and h_xyz_deg) filled by node_xyz_host
+    bool dev_xyz = false;               // d_xyz built on the device by node_xyz_device (h_xyz unused)
     DeviceBuffer<double> d_xyz;         // same on the device
     DeviceBuffer<double> d_area;        // [n_cells]
     DeviceBuffer<std::int32_t> d_flags; // bit0 degenerate, bit1 masked, bit2 area<=0
@@ -271,6 +272,7 @@ struct MeshGeom {
 struct GridAxes {
     bool all_quads = false;
     std::vector<double> ulon, ulat;  // tolerance-unique sorted node coordinates
+    std::vector<double> xlon, xlat;  // exact distinct node coordinates, sorted (when the detection collected them)
 };
 
 GridAxes unique_axes(const Mesh &m, hipStream_t s);
@@ -281,6 +283,21 @@ bool rectilinear_from_axes(const Mesh &m, const GridAxes &ax);
 /// touches only g.h_xyz, g.h_xyz_deg and g.xyz_ready, so it may run on a host
 /// thread while the GPU works on other members.
 void node_xyz_host(const Mesh &m, MeshGeom &g, const std::atomic<bool> *cancel = nullptr);
+/// xyz of one node as node_xyz_host computes it (same bits).
+void node_xyz_one(const Mesh &m, index_t node, double out[3]);
+/// Host-only necessary condition for node_xyz_device_eligible: degree
+/// coordinates, all cells quads, and node and cell counts of some
+/// nlon x nlat grid (nodes = nlon nlat, cells = (nlon - 1)(nlat - 1)).
+bool maybe_rectilinear_deg(const Mesh &m);
+/// Whether node_xyz_device can serve the mesh: degree coordinates on a
+/// rectilinear grid (and AXIS_HOST_NODE_XYZ unset).
+bool node_xyz_device_eligible(const Mesh &m, const GridAxes &ax);
+/// Node xyz of an eligible mesh built on the device into g.d_xyz from host
+/// libm sin/cos tables of the grid's axis values (bit-identical to
+/// node_xyz_host); nodes whose coordinates match no axis value bit for bit
+/// are computed on the host.  Returns false (and leaves g.dev_xyz unset) when
+/// the mesh is not eligible or too many nodes need the host.
+bool node_xyz_device(const Mesh &m, const GridAxes &ax, MeshGeom &g, hipStream_t s);
 void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s);
 
 /// 30-bit Morton code of every cell's centroid (normalised mean of its vertex

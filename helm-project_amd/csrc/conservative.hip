@@ -541,6 +541,15 @@ int bits_for(index_t n) {
 struct HostSide {
     const Mesh *m;
     const MeshGeom *g;
+    // node xyz with host libm: the host array, or (device-built xyz) the node's own
+    V3 node(index_t nd) const {
+        if (g->dev_xyz) {
+            double p[3];
+            node_xyz_one(*m, nd, p);
+            return V3{p[0], p[1], p[2]};
+        }
+        return V3{g->h_xyz[3 * nd], g->h_xyz[3 * nd + 1], g->h_xyz[3 * nd + 2]};
+    }
     V3 vertex(index_t c, int k) const {
         if (g->regular) {
             const index_t nl = g->ni + 1, na = g->nj + 1;
@@ -550,8 +559,7 @@ struct HostSide {
             index_t b = (k >= 2) ? cj + 1 : cj;
             return V3{clat[b] * clon[a], clat[b] * slon[a], slat[b]};
         }
-        index_t nd = m->h_indices[m->h_offsets[c] + k];
-        return V3{g->h_xyz[3 * nd], g->h_xyz[3 * nd + 1], g->h_xyz[3 * nd + 2]};
+        return node(m->h_indices[m->h_offsets[c] + k]);
     }
     int nverts(index_t c) const {
         if (g->regular) return 4;
@@ -565,16 +573,15 @@ struct HostSide {
         if (e - b < 1) return;
         double sx = 0, sy = 0, sz = 0;
         for (index_t k = b; k < e; ++k) {
-            index_t nd = m->h_indices[k];
-            sx += g->h_xyz[3 * nd];
-            sy += g->h_xyz[3 * nd + 1];
-            sz += g->h_xyz[3 * nd + 2];
+            const V3 p = node(m->h_indices[k]);
+            sx += p.x;
+            sy += p.y;
+            sz += p.z;
         }
         double inv = 1.0 / static_cast<double>(e - b);
         cen = geo::normalize(V3{sx * inv, sy * inv, sz * inv});
         for (index_t k = b; k < e; ++k) {
-            index_t nd = m->h_indices[k];
-            double d = geo::dot(cen, V3{g->h_xyz[3 * nd], g->h_xyz[3 * nd + 1], g->h_xyz[3 * nd + 2]});
+            double d = geo::dot(cen, node(m->h_indices[k]));
             d = std::fmin(1.0, std::fmax(-1.0, d));
             double ang = std::acos(d);
             if (ang > rad) rad = ang;
@@ -706,13 +713,19 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     // host libm node xyz (source, then destination) on a host thread beside
     // the GPU grid detection and the source's upload and cell pass
     // (a structured pair takes an analytic path instead: the task is told to stop)
-    std::atomic<bool> cancel{false};
+    // A rectilinear degree source has its node xyz built on the device from
+    // axis tables instead.  When the source's counts allow a rectilinear grid
+    // the task starts on the destination at once (a source that then turns
+    // out to need the host is done on this thread); otherwise it does the
+    // source first, and is told to skip it should its axes allow the device.
+    std::atomic<bool> cancel{false}, skip_src{false};
     std::promise<void> src_done;
     std::future<void> src_ready = src_done.get_future();
     const bool spherical = src.coord_system() != CoordinateSystem::Cartesian3D;
+    const bool dst_first = spherical && maybe_rectilinear_deg(src);
     auto xyz_task = std::async(std::launch::async, [&] {
         try {
-            if (spherical) node_xyz_host(src, sg, &cancel);
+            if (spherical && !dst_first) node_xyz_host(src, sg, &skip_src);
             src_done.set_value();
         } catch (...) {
             src_done.set_exception(std::current_exception());
@@ -720,7 +733,10 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
         }
         if (spherical) node_xyz_host(dst, dg, &cancel);
     });
-    GridAxes sax = unique_axes(src, s), dax = unique_axes(dst, s);
+    GridAxes sax = unique_axes(src, s);
+    const bool src_dev_xyz = spherical && node_xyz_device_eligible(src, sax);
+    if (src_dev_xyz) skip_src.store(true);
+    GridAxes dax = unique_axes(dst, s);
     const bool s_reg = regular_from_axes(src, sax, sg), d_reg = regular_from_axes(dst, dax, dg);
     // generate_conservative's dispatch (weight_generator.cpp:3259-3296): the
     // regular, then the rectilinear, analytic path when both meshes qualify;
@@ -739,6 +755,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
         if (!accept(fast)) fast.reset();
     }
     if (fast) {
+        skip_src.store(true);
         cancel.store(true);
         xyz_task.get();
         return fast;
@@ -746,6 +763,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     if (!spherical) throw InvalidArgument("generate_conservative: Cartesian3D meshes are served by the rectangle paths only here");
     if (cfg.line_type != LineType::GreatCircle) throw InvalidArgument("generate_conservative: only LineType::GreatCircle is supported");
     tr.mark("detect");
+    if (src_dev_xyz) node_xyz_device(src, sax, sg, s);  // (false: host xyz below)
     src_ready.get();
     prepare_geometry(src, sg, s);
     tr.mark("geometry_src");

@@ -351,12 +351,17 @@ GridAxes unique_axes(const Mesh &m, hipStream_t s) {
                 std::vector<double> h(2 * static_cast<std::size_t>(kSetMax));
                 vals.download(h.data(), h.size(), s);
                 AXIS_HIP_CHECK(hipStreamSynchronize(s));
-                std::vector<double> ulon = host_unique(std::vector<double>(h.begin(), h.begin() + hq.n_distinct[0]));
-                std::vector<double> ulat = host_unique(std::vector<double>(h.begin() + kSetMax, h.begin() + kSetMax + hq.n_distinct[1]));
+                std::vector<double> xlon(h.begin(), h.begin() + hq.n_distinct[0]);
+                std::vector<double> xlat(h.begin() + kSetMax, h.begin() + kSetMax + hq.n_distinct[1]);
+                std::sort(xlon.begin(), xlon.end());
+                std::sort(xlat.begin(), xlat.end());
+                std::vector<double> ulon = host_unique(xlon), ulat = host_unique(xlat);
                 const index_t nlon = static_cast<index_t>(ulon.size()), nlat = static_cast<index_t>(ulat.size());
                 if (nlon >= 2 && nlat >= 2 && (nlon - 1) * (nlat - 1) == n_cells) {
                     ax.ulon = std::move(ulon);
                     ax.ulat = std::move(ulat);
+                    ax.xlon = std::move(xlon);
+                    ax.xlat = std::move(xlat);
                 }
                 tr.mark("distinct");
                 return ax;
@@ -635,7 +640,144 @@ int host_classify(const double *xyz, const index_t *idx, int nv, double &fan_are
     return 0;
 }
 
+__device__ inline int lower_bound_dev(const double *u, int n, double x) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (u[mid] < x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// xyz of every node of a rectilinear degree mesh from the sin/cos tables of
+// its axis values: tab = [ulon | ulat | sin lon | cos lon | sin lat | cos lat].
+// A node whose lon or lat equals no axis value bit for bit is listed for the
+// host (its xyz is left zero here).
+__global__ void axis_xyz_kernel(const double *__restrict__ lon, const double *__restrict__ lat, index_t n, const double *__restrict__ tab, int nlon,
+                                int nlat, double *__restrict__ xyz, index_t *__restrict__ fix, int *__restrict__ n_fix, int fix_cap) {
+    const index_t i = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *ulon = tab, *ulat = tab + nlon, *slon = ulat + nlat, *clon = slon + nlon, *slat = clon + nlon, *clat = slat + nlat;
+    const double x = lon[i], y = lat[i];
+    const int a = lower_bound_dev(ulon, nlon, x), b = lower_bound_dev(ulat, nlat, y);
+    const bool ok = a < nlon && b < nlat && __double_as_longlong(ulon[a]) == __double_as_longlong(x) &&
+                    __double_as_longlong(ulat[b]) == __double_as_longlong(y);
+    double px = 0.0, py = 0.0, pz = 0.0;
+    if (ok) {
+        const double cl = clat[b];
+        px = cl * clon[a];
+        py = cl * slon[a];
+        pz = slat[b];
+    } else {
+        const int k = atomicAdd(n_fix, 1);
+        if (k < fix_cap) fix[k] = i;
+    }
+    xyz[3 * i] = px;
+    xyz[3 * i + 1] = py;
+    xyz[3 * i + 2] = pz;
+}
+
+__global__ void scatter_xyz_kernel(const index_t *__restrict__ nodes, const double *__restrict__ vals, int n, double *__restrict__ xyz) {
+    const int k = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    for (int d = 0; d < 3; ++d) xyz[3 * nodes[k] + d] = vals[3 * k + d];
+}
+
+constexpr int kXyzFixCap = 8192;
+
 }  // namespace
+
+void node_xyz_one(const Mesh &m, index_t node, double out[3]) {
+    const index_t n = m.n_nodes();
+    const double lon = m.h_coords[node], lat = m.h_coords[n + node];
+    double sl, cl, so, co;
+    if (m.coord_system() == CoordinateSystem::SphericalDeg) {
+        sin_cos(lat * kDeg2Rad, sl, cl);
+        sin_cos(lon * kDeg2Rad, so, co);
+    } else {
+        sin_cos(lat, sl, cl);
+        sin_cos(lon, so, co);
+    }
+    out[0] = cl * co;
+    out[1] = cl * so;
+    out[2] = sl;
+}
+
+bool maybe_rectilinear_deg(const Mesh &m) {
+    if (m.coord_system() != CoordinateSystem::SphericalDeg || m.n_cells() == 0 || std::getenv("AXIS_HOST_NODE_XYZ") != nullptr) return false;
+    if (m.min_cell_vertices() != 4 || m.max_cell_vertices() != 4) return false;
+    // nlon + nlat = nodes - cells + 1 and nlon nlat = nodes: integer roots
+    // (nlon, nlat) = (S +- d) / 2 with d^2 = S^2 - 4 nodes
+    const __int128 S = static_cast<__int128>(m.n_nodes()) - m.n_cells() + 1, P = m.n_nodes();
+    const __int128 disc = S * S - 4 * P;
+    if (S < 4 || disc < 0) return false;
+    const index_t r = static_cast<index_t>(std::llround(std::sqrt(static_cast<long double>(disc))));
+    for (index_t d = r - 1; d <= r + 1; ++d)
+        if (d >= 0 && static_cast<__int128>(d) * d == disc && (S + d) % 2 == 0 && (S - d) / 2 >= 2) return true;
+    return false;
+}
+
+bool node_xyz_device_eligible(const Mesh &m, const GridAxes &ax) {
+    return m.coord_system() == CoordinateSystem::SphericalDeg && rectilinear_from_axes(m, ax) && std::getenv("AXIS_HOST_NODE_XYZ") == nullptr &&
+           ax.xlon.size() < (std::size_t(1) << 30) && ax.xlat.size() < (std::size_t(1) << 30);
+}
+
+bool node_xyz_device(const Mesh &m, const GridAxes &ax, MeshGeom &g, hipStream_t s) {
+    g.dev_xyz = false;
+    if (!node_xyz_device_eligible(m, ax)) return false;
+    PhaseTrace tr("geometry", s);
+    const index_t n = m.n_nodes();
+    // exact distinct coordinates where the detection kept them (else the
+    // tolerance-unique ones: nodes off those values go to the host)
+    const std::vector<double> &vlon = ax.xlon.empty() ? ax.ulon : ax.xlon, &vlat = ax.xlat.empty() ? ax.ulat : ax.xlat;
+    const int nlon = static_cast<int>(vlon.size()), nlat = static_cast<int>(vlat.size());
+    // the tables with host libm, as node_xyz_host's per-node calls would see them
+    std::vector<double> tab(3 * static_cast<std::size_t>(nlon + nlat));
+    double *ulon = tab.data(), *ulat = ulon + nlon, *slon = ulat + nlat, *clon = slon + nlon, *slat = clon + nlon, *clat = slat + nlat;
+    for (int a = 0; a < nlon; ++a) {
+        ulon[a] = vlon[a];
+        sin_cos(vlon[a] * kDeg2Rad, slon[a], clon[a]);
+    }
+    for (int b = 0; b < nlat; ++b) {
+        ulat[b] = vlat[b];
+        sin_cos(vlat[b] * kDeg2Rad, slat[b], clat[b]);
+    }
+    DeviceBuffer<double> d_tab(tab.size());
+    d_tab.upload(tab.data(), tab.size(), s);
+    g.d_xyz.resize(std::max<std::size_t>(3 * static_cast<std::size_t>(n), 3));
+    DeviceBuffer<index_t> fix(kXyzFixCap);
+    DeviceBuffer<int> n_fix(1);
+    AXIS_HIP_CHECK(hipMemsetAsync(n_fix.get(), 0, sizeof(int), s));
+    if (n > 0) {
+        axis_xyz_kernel<<<grid_for(n, 256), 256, 0, s>>>(m.d_coords.get(), m.d_coords.get() + n, n, d_tab.get(), nlon, nlat, g.d_xyz.get(), fix.get(),
+                                                         n_fix.get(), kXyzFixCap);
+        AXIS_LAUNCH_CHECK();
+    }
+    int h_fix = 0;
+    AXIS_HIP_CHECK(hipMemcpyAsync(&h_fix, n_fix.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    if (h_fix > kXyzFixCap) return false;
+    if (h_fix > 0) {
+        std::vector<index_t> nodes(h_fix);
+        fix.download(nodes.data(), h_fix, s);
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+        std::vector<double> vals(3 * static_cast<std::size_t>(h_fix));
+        for (int k = 0; k < h_fix; ++k) node_xyz_one(m, nodes[k], &vals[3 * static_cast<std::size_t>(k)]);
+        DeviceBuffer<index_t> d_nodes(h_fix);
+        DeviceBuffer<double> d_vals(vals.size());
+        d_nodes.upload(nodes.data(), h_fix, s);
+        d_vals.upload(vals.data(), vals.size(), s);
+        scatter_xyz_kernel<<<grid_for(h_fix, 256), 256, 0, s>>>(d_nodes.get(), d_vals.get(), h_fix, g.d_xyz.get());
+        AXIS_LAUNCH_CHECK();
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    g.dev_xyz = true;
+    tr.mark("device_xyz");
+    return true;
+}
 
 void node_xyz_host(const Mesh &m, MeshGeom &g, const std::atomic<bool> *cancel) {
     const index_t n_nodes = m.n_nodes();
@@ -660,7 +802,7 @@ void node_xyz_host(const Mesh &m, MeshGeom &g, const std::atomic<bool> *cancel) 
             }
         }
     });
-    g.xyz_ready = true;
+    g.xyz_ready = !(cancel && cancel->load());
 }
 
 void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s) {
@@ -670,11 +812,14 @@ void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s) {
     const bool deg = m.coord_system() == CoordinateSystem::SphericalDeg;
     PhaseTrace tr("geometry", s);
     // node xyz with host libm (extract_cell_polygon_spherical, weight_generator.cpp:462-470)
-    if (!g.xyz_ready) node_xyz_host(m, g);
+    // (or built on the device, node_xyz_device)
+    if (!g.dev_xyz) {
+        if (!g.xyz_ready) node_xyz_host(m, g);
+        g.d_xyz.resize(std::max<std::size_t>(g.h_xyz.size(), 3));
+        g.d_xyz.upload(g.h_xyz.data(), g.h_xyz.size(), s);
+    }
     const PinnedArray<double> &h_xyz_deg = g.h_xyz_deg;
     tr.mark("host_sincos");
-    g.d_xyz.resize(std::max<std::size_t>(g.h_xyz.size(), 3));
-    g.d_xyz.upload(g.h_xyz.data(), g.h_xyz.size(), s);
     DeviceBuffer<double> d_xyz_deg;
     if (!deg) {
         d_xyz_deg.resize(std::max<std::size_t>(h_xyz_deg.size(), 3));
@@ -724,12 +869,25 @@ void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s) {
         DeviceBuffer<std::int32_t> tmp_flags(h_n);
         AXIS_HIP_CHECK(hipStreamSynchronize(s));
         const double *xyz_deg = deg ? g.h_xyz.data() : h_xyz_deg.data();
+        std::vector<double> own_xyz;  // device-built xyz: the cell's nodes with host libm
+        std::vector<index_t> own_idx;
         for (int k = 0; k < h_n; ++k) {
             index_t c = cells[k];
             index_t b = m.h_offsets[c];
             int nv = static_cast<int>(std::min<index_t>(m.h_offsets[c + 1] - b, 32));
             double fan_deg;
-            int cls = host_classify(xyz_deg, m.h_indices.data() + b, nv, fan_deg);
+            int cls;
+            if (g.dev_xyz) {  // (degree meshes only)
+                own_xyz.resize(3 * static_cast<std::size_t>(nv));
+                own_idx.resize(nv);
+                for (int v = 0; v < nv; ++v) {
+                    node_xyz_one(m, m.h_indices[b + v], &own_xyz[3 * static_cast<std::size_t>(v)]);
+                    own_idx[v] = v;
+                }
+                cls = host_classify(own_xyz.data(), own_idx.data(), nv, fan_deg);
+            } else {
+                cls = host_classify(xyz_deg, m.h_indices.data() + b, nv, fan_deg);
+            }
             double a;
             if (m.has_areas())
                 a = m.h_areas[c];

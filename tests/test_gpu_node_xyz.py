@@ -1,0 +1,136 @@
+"""Node xyz of a rectilinear degree source built on the device (axis tables
+of host-libm sin/cos, exact-bit lookups, host fix-ups for nodes matching no
+axis value) against the host-libm build of every node, through the general
+conservative path (WeightGenerator::generate, weight_generator.cpp:3304-3770,
+whose node xyz is extract_cell_polygon_spherical's lonlat_to_xyz,
+weight_generator.cpp:462-470).
+
+Each pair is generated twice in one process, with AXIS_HOST_NODE_XYZ set
+(every node on the host) and unset; the matrices must agree bit for bit:
+row_ptr, columns, weights, fractions, areas and the arbitration counts.  With
+AXIS_TRACE set the library reports the device build (`geometry.device_xyz`),
+so each case also checks which path ran.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from helm_project_amd import axis, grids
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+CONS = axis.RegridConfig(method=axis.M_CONSERVATIVE1)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init():
+    if not torch.cuda.is_available():
+        pytest.fail("no GPU visible")
+    axis.init()
+
+
+def rectilinear(lon_edges, lat_edges):
+    ni, nj = len(lon_edges) - 1, len(lat_edges) - 1
+    lon = np.tile(lon_edges, nj + 1)
+    lat = np.repeat(lat_edges, ni + 1)
+    ci, cj = np.tile(np.arange(ni), nj), np.repeat(np.arange(nj), ni)
+    st = ni + 1
+    idx = np.stack([cj * st + ci, cj * st + ci + 1, (cj + 1) * st + ci + 1, (cj + 1) * st + ci], 1).reshape(-1)
+    return grids._mesh(lon, lat, np.arange(0, 4 * ni * nj + 1, 4), idx)
+
+
+def perturbed(mesh, n_nodes, seed):
+    """Some node longitudes moved by 1e-13 deg: the grid stays rectilinear
+    within the axis tolerance, with two or three exact values per axis value."""
+    m = dict(mesh)
+    c = np.array(mesh["coords"], dtype=np.float64, order="F")
+    rng = np.random.default_rng(seed)
+    nodes = rng.choice(c.shape[0], n_nodes, replace=False)
+    c[nodes, 0] += rng.choice([-1.0, 1.0], n_nodes) * 1.0e-13
+    m["coords"] = c
+    return m
+
+
+def spread(mesh, n_nodes, seed):
+    """Node longitudes moved up by up to 8e-13 deg in many distinct steps: more
+    distinct values than the detection keeps exactly, so the tables hold only
+    the tolerance-unique values and too many nodes would need the host; the
+    whole source is then built on the host."""
+    m = dict(mesh)
+    c = np.array(mesh["coords"], dtype=np.float64, order="F")
+    rng = np.random.default_rng(seed)
+    nodes = rng.choice(c.shape[0], n_nodes, replace=False)
+    c[nodes, 0] += rng.integers(1, 41, n_nodes) * 2.0e-14
+    m["coords"] = c
+    return m
+
+
+def jittered(mesh, seed):
+    """Every node longitude moved by up to 0.3 deg: the node and cell counts
+    of a 180 x 90 grid, but no rectilinear axes."""
+    m = dict(mesh)
+    c = np.array(mesh["coords"], dtype=np.float64, order="F")
+    c[:, 0] += np.random.default_rng(seed).uniform(-0.3, 0.3, c.shape[0])
+    m["coords"] = c
+    return m
+
+
+def generate(src, dst, host, capfd):
+    if host:
+        os.environ["AXIS_HOST_NODE_XYZ"] = "1"
+    os.environ["AXIS_TRACE"] = "1"
+    try:
+        capfd.readouterr()
+        m = axis.generate(axis.Mesh(src), axis.Mesh(dst), CONS)
+        err = capfd.readouterr().err
+    finally:
+        os.environ.pop("AXIS_HOST_NODE_XYZ", None)
+        os.environ.pop("AXIS_TRACE", None)
+    return m, "geometry.device_xyz" in err
+
+
+CASES = {
+    "regular 1deg -> C48": (lambda: grids.latlon_deg(1.0), lambda: grids.cubed_sphere(48), True),
+    "rectilinear random latitudes -> C24": (
+        lambda: rectilinear(np.linspace(-180.0, 180.0, 145), np.concatenate([[-90.0], np.sort(np.random.default_rng(7).uniform(-89, 89, 70)), [90.0]])),
+        lambda: grids.cubed_sphere(24), True),
+    "2deg with 300 perturbed nodes -> C24": (lambda: perturbed(grids.latlon_deg(2.0), 300, 3), lambda: grids.cubed_sphere(24), True),
+    "0.5deg with 10000 perturbed nodes -> C48": (lambda: perturbed(grids.latlon_deg(0.5), 10000, 4), lambda: grids.cubed_sphere(48), True),
+    "0.5deg with 60000 nodes off the tolerance-unique axes -> C48 (host build)": (lambda: spread(grids.latlon_deg(0.5), 60000, 5),
+                                                                                  lambda: grids.cubed_sphere(48), False),
+    "C24 -> 2deg (source not rectilinear)": (lambda: grids.cubed_sphere(24), lambda: grids.latlon_deg(2.0), False),
+    "2deg with jittered longitudes -> C24 (grid counts, not rectilinear)": (lambda: jittered(grids.latlon_deg(2.0), 6),
+                                                                             lambda: grids.cubed_sphere(24), False),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_device_node_xyz_matches_host(case, capfd):
+    mk_src, mk_dst, want_device = CASES[case]
+    src, dst = mk_src(), mk_dst()
+    with capfd.disabled():
+        say("case node xyz:", case)
+    mh, used_h = generate(src, dst, True, capfd)
+    md, used_d = generate(src, dst, False, capfd)
+    assert not used_h
+    assert used_d == want_device
+    assert mh.info() == md.info()
+    for a, b in zip(mh.csr(), md.csr()):
+        assert np.array_equal(a, b)
+    fh, fd = mh.fractions(), md.fractions()
+    for k in fh:
+        assert np.array_equal(fh[k].view(np.int64), fd[k].view(np.int64)), k
+    sh, sd = mh.stats(), md.stats()
+    assert (sh["host_arbitrated"], sh["cap_ties"], sh["candidates"]) == (sd["host_arbitrated"], sd["cap_ties"], sd["candidates"])
+    assert md.nnz > 0
+    mh.close()
+    md.close()
