@@ -256,6 +256,7 @@ struct MeshGeom {
     PinnedArray<double> h_xyz_deg;      // radian meshes: xyz of the coordinates read as degrees
     bool xyz_ready = false;             // h_xyz (and h_xyz_deg) filled by node_xyz_host
     bool dev_xyz = false;               // d_xyz built on the device by node_xyz_device (h_xyz unused)
+    bool d_xyz_ready = false;           // d_xyz holds h_xyz (uploaded by node_xyz_host; ordered by the caller)
     DeviceBuffer<double> d_xyz;         // same on the device
     DeviceBuffer<double> d_area;        // [n_cells]
     DeviceBuffer<std::int32_t> d_flags; // bit0 degenerate, bit1 masked, bit2 area<=0
@@ -282,7 +283,16 @@ bool rectilinear_from_axes(const Mesh &m, const GridAxes &ax);
 /// Node xyz (and, for radian meshes, the degree reading) with host libm;
 /// touches only g.h_xyz, g.h_xyz_deg and g.xyz_ready, so it may run on a host
 /// thread while the GPU works on other members.
-void node_xyz_host(const Mesh &m, MeshGeom &g, const std::atomic<bool> *cancel = nullptr);
+///
+/// With `up` given, g.d_xyz (already sized) receives each host thread's slice
+/// of h_xyz on stream `up` as soon as that slice is done, and `done` is
+/// recorded on `up` after the last one; returns whether it was (false when
+/// cancelled).  The caller orders its stream after `done` and sets
+/// g.d_xyz_ready.
+bool node_xyz_host(const Mesh &m, MeshGeom &g, const std::atomic<bool> *cancel = nullptr, hipStream_t up = nullptr, hipEvent_t done = nullptr);
+/// A non-blocking stream of the current device for host-to-device uploads
+/// beside the caller's stream (created once per device, default priority).
+hipStream_t upload_stream();
 /// xyz of one node as node_xyz_host computes it (same bits).
 void node_xyz_one(const Mesh &m, index_t node, double out[3]);
 /// Host-only necessary condition for node_xyz_device_eligible: degree

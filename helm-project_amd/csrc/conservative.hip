@@ -723,6 +723,21 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     std::future<void> src_ready = src_done.get_future();
     const bool spherical = src.coord_system() != CoordinateSystem::Cartesian3D;
     const bool dst_first = spherical && maybe_rectilinear_deg(src);
+    // the destination's xyz goes to the device slice by slice as the host
+    // threads finish them, on a side stream that this stream then waits for
+    struct EventGuard {
+        hipEvent_t e = nullptr;
+        ~EventGuard() {
+            if (e) (void)hipEventDestroy(e);
+        }
+    } dst_up;
+    hipStream_t up = nullptr;
+    std::atomic<bool> dst_uploaded{false};
+    if (spherical) {
+        up = upload_stream();
+        AXIS_HIP_CHECK(hipEventCreateWithFlags(&dst_up.e, hipEventDisableTiming));
+        dg.d_xyz.resize(std::max<std::size_t>(3 * static_cast<std::size_t>(dst.n_nodes()), 3));
+    }
     auto xyz_task = std::async(std::launch::async, [&] {
         try {
             if (spherical && !dst_first) node_xyz_host(src, sg, &skip_src);
@@ -731,7 +746,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
             src_done.set_exception(std::current_exception());
             return;
         }
-        if (spherical) node_xyz_host(dst, dg, &cancel);
+        if (spherical) dst_uploaded.store(node_xyz_host(dst, dg, &cancel, up, dst_up.e));
     });
     GridAxes sax = unique_axes(src, s);
     const bool src_dev_xyz = spherical && node_xyz_device_eligible(src, sax);
@@ -773,6 +788,10 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     lbvh_build(sg.d_box.get(), n_src, tree, s);
     tr.mark("lbvh_build");
     xyz_task.get();
+    if (dst_uploaded.load()) {
+        AXIS_HIP_CHECK(hipStreamWaitEvent(s, dst_up.e, 0));
+        dg.d_xyz_ready = true;
+    }
     prepare_geometry(dst, dg, s);
     tr.mark("geometry_dst");
 

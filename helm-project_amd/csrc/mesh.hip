@@ -19,6 +19,8 @@
 #include <cstring>
 #include <cfloat>
 #include <limits>
+#include <map>
+#include <mutex>
 
 #include "axis_hip.hpp"
 #include "geometry.hpp"
@@ -779,13 +781,29 @@ bool node_xyz_device(const Mesh &m, const GridAxes &ax, MeshGeom &g, hipStream_t
     return true;
 }
 
-void node_xyz_host(const Mesh &m, MeshGeom &g, const std::atomic<bool> *cancel) {
+hipStream_t upload_stream() {
+    static std::mutex mu;
+    static std::map<int, hipStream_t> streams;
+    int dev = 0;
+    AXIS_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(mu);
+    auto it = streams.find(dev);
+    if (it != streams.end()) return it->second;
+    hipStream_t st = nullptr;
+    AXIS_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    streams[dev] = st;
+    return st;
+}
+
+bool node_xyz_host(const Mesh &m, MeshGeom &g, const std::atomic<bool> *cancel, hipStream_t up, hipEvent_t done) {
     const index_t n_nodes = m.n_nodes();
     const bool deg = m.coord_system() == CoordinateSystem::SphericalDeg;
     g.h_xyz.resize(3 * static_cast<std::size_t>(n_nodes));
     g.h_xyz_deg.clear();
     if (!deg) g.h_xyz_deg.resize(3 * static_cast<std::size_t>(n_nodes));
     const double *lon = m.h_coords.data(), *lat = m.h_coords.data() + n_nodes;
+    const bool upload = up != nullptr && g.d_xyz.size() >= 3 * static_cast<std::size_t>(n_nodes);
+    std::atomic<int> up_err{0};
     parallel_range(n_nodes, [&](index_t b, index_t e) {
         SinCosMemo memo;
         for (index_t i = b; i < e; ++i) {
@@ -801,8 +819,20 @@ void node_xyz_host(const Mesh &m, MeshGeom &g, const std::atomic<bool> *cancel) 
                 g.h_xyz_deg[3 * i + 2] = q.z;
             }
         }
+        // this slice to the device now (errors are reported after the join:
+        // a worker thread must not throw)
+        if (upload && e > b) {
+            const hipError_t r = hipMemcpyAsync(g.d_xyz.get() + 3 * b, g.h_xyz.data() + 3 * b, 3 * sizeof(double) * static_cast<std::size_t>(e - b),
+                                                hipMemcpyHostToDevice, up);
+            if (r != hipSuccess) up_err.store(static_cast<int>(r));
+        }
     });
-    g.xyz_ready = !(cancel && cancel->load());
+    const bool complete = !(cancel && cancel->load());
+    g.xyz_ready = complete;
+    if (up_err.load() != 0) AXIS_HIP_CHECK(static_cast<hipError_t>(up_err.load()));
+    if (!upload || !complete) return false;
+    AXIS_HIP_CHECK(hipEventRecord(done, up));
+    return true;
 }
 
 void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s) {
@@ -813,7 +843,7 @@ void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s) {
     PhaseTrace tr("geometry", s);
     // node xyz with host libm (extract_cell_polygon_spherical, weight_generator.cpp:462-470)
     // (or built on the device, node_xyz_device)
-    if (!g.dev_xyz) {
+    if (!g.dev_xyz && !g.d_xyz_ready) {
         if (!g.xyz_ready) node_xyz_host(m, g);
         g.d_xyz.resize(std::max<std::size_t>(g.h_xyz.size(), 3));
         g.d_xyz.upload(g.h_xyz.data(), g.h_xyz.size(), s);
