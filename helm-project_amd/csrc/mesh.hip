@@ -350,11 +350,12 @@ GridAxes unique_axes(const Mesh &m, hipStream_t s) {
             AXIS_HIP_CHECK(hipMemcpyAsync(&hq, dq.get(), sizeof hq, hipMemcpyDeviceToHost, s));
             AXIS_HIP_CHECK(hipStreamSynchronize(s));
             if (!hq.overflow[0] && !hq.overflow[1]) {
-                std::vector<double> h(2 * static_cast<std::size_t>(kSetMax));
-                vals.download(h.data(), h.size(), s);
+                std::vector<double> xlon(hq.n_distinct[0]), xlat(hq.n_distinct[1]);
+                if (!xlon.empty())
+                    AXIS_HIP_CHECK(hipMemcpyAsync(xlon.data(), vals.get(), sizeof(double) * xlon.size(), hipMemcpyDeviceToHost, s));
+                if (!xlat.empty())
+                    AXIS_HIP_CHECK(hipMemcpyAsync(xlat.data(), vals.get() + kSetMax, sizeof(double) * xlat.size(), hipMemcpyDeviceToHost, s));
                 AXIS_HIP_CHECK(hipStreamSynchronize(s));
-                std::vector<double> xlon(h.begin(), h.begin() + hq.n_distinct[0]);
-                std::vector<double> xlat(h.begin() + kSetMax, h.begin() + kSetMax + hq.n_distinct[1]);
                 std::sort(xlon.begin(), xlon.end());
                 std::sort(xlat.begin(), xlat.end());
                 std::vector<double> ulon = host_unique(xlon), ulat = host_unique(xlat);
