@@ -290,10 +290,27 @@ __global__ void fill_rows_mapped_kernel(const index_t *off, index_t n_rows, cons
     for (index_t e = off[q]; e < off[q + 1]; ++e) pair_row[e] = rows[q];
 }
 
-__global__ void fill_rows_kernel(const index_t *cand_off, index_t n_dst, std::int32_t *pair_row) {
-    index_t j = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (j >= n_dst) return;
-    for (index_t e = cand_off[j]; e < cand_off[j + 1]; ++e) pair_row[e] = static_cast<std::int32_t>(j);
+// dst row of every candidate pair: a block stages the offsets of its 256 rows
+// and writes their pair range contiguously, each pair's row found by binary
+// search in LDS
+__global__ void __launch_bounds__(256) fill_rows_kernel(const index_t *cand_off, index_t n_dst, std::int32_t *pair_row) {
+    __shared__ index_t so[257];
+    const index_t r0 = static_cast<index_t>(blockIdx.x) * 256;
+    const int nr = static_cast<int>(min<index_t>(256, n_dst - r0));
+    for (int k = threadIdx.x; k <= nr; k += 256) so[k] = cand_off[r0 + k];
+    __syncthreads();
+    const index_t b = so[0], e = so[nr];
+    for (index_t p = b + threadIdx.x; p < e; p += 256) {
+        int lo = 0, hi = nr;  // so[lo] <= p < so[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (so[mid] <= p)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        pair_row[p] = static_cast<std::int32_t>(r0 + lo);
+    }
 }
 
 // (pair, dst row, src cell) of the pairs handed to the host, in one buffer
@@ -415,6 +432,7 @@ __global__ void unmapped_kernel(const index_t *row_ptr, index_t n_dst, const std
 constexpr int kShortRow = 16;     // sorted by one thread
 constexpr int kLongRowMax = 4096; // sorted by one workgroup in LDS
 constexpr int kLongRowBlock = 256;
+constexpr int kMidRow = 64;        // sorted by one wave in a small LDS tile
 
 __global__ void compact_grouped_kernel(const double *overlap, const index_t *pos, index_t n, const std::int32_t *pair_src, std::int32_t *col_c,
                                        double *ov_c) {
@@ -437,7 +455,11 @@ __global__ void __launch_bounds__(256) short_rows_kernel(const index_t *cand_off
         const int len = static_cast<int>(min<index_t>(e - b, index_t(1) << 30));
         longest = max(longest, len);
         if (len > kShortRow) {
-            long_rows[atomicAdd(n_long, 1)] = static_cast<std::int32_t>(r);
+            // rows of up to kMidRow entries from the front of the list, longer ones from the back
+            if (len <= kMidRow)
+                long_rows[atomicAdd(&n_long[0], 1)] = static_cast<std::int32_t>(r);
+            else
+                long_rows[n_dst - 1 - atomicAdd(&n_long[1], 1)] = static_cast<std::int32_t>(r);
             continue;
         }
         std::int32_t cc[kShortRow];
@@ -476,28 +498,32 @@ __global__ void __launch_bounds__(256) short_rows_kernel(const index_t *cand_off
     }
 }
 
-// rows of kShortRow < len <= kLongRowMax: bitonic column sort in LDS, the
-// ordered sum by one thread, the rest in parallel
-__global__ void __launch_bounds__(kLongRowBlock) long_rows_kernel(const std::int32_t *long_rows, const index_t *row_ptr, const std::int32_t *col_c,
-                                                                  const double *ov_c, const double *area_dst, int frac_area, index_t *col,
-                                                                  double *val, double *ov_s, double *frac_b) {
-    __shared__ std::int32_t sc[kLongRowMax];
-    __shared__ double so[kLongRowMax];
+// rows of kShortRow < len <= CAP: bitonic column sort in LDS, the ordered sum
+// by one thread, the rest in parallel (columns are distinct within a row, so
+// the sorted order does not depend on the sort).  CAP = kMidRow with one wave
+// for most long rows (LDS of a few hundred bytes: many rows per CU), CAP =
+// kLongRowMax with kLongRowBlock threads for the rest.
+template <int CAP, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) long_rows_kernel(const std::int32_t *long_rows, const index_t *row_ptr, const std::int32_t *col_c,
+                                                          const double *ov_c, const double *area_dst, int frac_area, index_t *col, double *val,
+                                                          double *ov_s, double *frac_b) {
+    __shared__ std::int32_t sc[CAP];
+    __shared__ double so[CAP];
     __shared__ double s_fb;
     const int tid = static_cast<int>(threadIdx.x);
     const index_t r = long_rows[blockIdx.x];
     const index_t b = row_ptr[r];
-    const int len = static_cast<int>(row_ptr[r + 1] - b);  // <= kLongRowMax (checked by the host)
+    const int len = static_cast<int>(row_ptr[r + 1] - b);  // <= CAP (checked by the host)
     int P = 2;
     while (P < len) P <<= 1;
-    for (int k = tid; k < P; k += kLongRowBlock) {
+    for (int k = tid; k < P; k += BLOCK) {
         sc[k] = k < len ? col_c[b + k] : 0x7FFFFFFF;
         so[k] = k < len ? ov_c[b + k] : 0.0;
     }
     __syncthreads();
     for (int size = 2; size <= P; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int i = tid; i < P / 2; i += kLongRowBlock) {
+            for (int i = tid; i < P / 2; i += BLOCK) {
                 const int lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
                 const bool asc = (lo & size) == 0;
                 if ((sc[lo] > sc[hi]) == asc) {
@@ -512,22 +538,28 @@ __global__ void __launch_bounds__(kLongRowBlock) long_rows_kernel(const std::int
             __syncthreads();
         }
     }
+    // sorted entries out, then the quotients overlap / area in place (in
+    // parallel), then their sum in column order by one thread
     const double a = area_dst[r];
+    for (int k = tid; k < len; k += BLOCK) {
+        col[b + k] = sc[k];
+        ov_s[b + k] = so[k];
+        so[k] = so[k] / a;
+    }
+    __syncthreads();
     if (tid == 0) {
         double fb = 0.0;
-        for (int k = 0; k < len; ++k) fb += so[k] / a;
+        for (int k = 0; k < len; ++k) fb += so[k];
         s_fb = fb;
         frac_b[r] = fmin(fb, 1.0);
     }
     __syncthreads();
     const double fb = s_fb;
     const bool divide = frac_area && fb > 0.0;
-    for (int k = tid; k < len; k += kLongRowBlock) {
-        double w = fmax(so[k] / a, 0.0);
+    for (int k = tid; k < len; k += BLOCK) {
+        double w = fmax(so[k], 0.0);
         if (divide) w /= fb;
         val[b + k] = w;
-        col[b + k] = sc[k];
-        ov_s[b + k] = so[k];
     }
 }
 
@@ -1028,8 +1060,8 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     if (!appended && n_dst > 0) {
         DeviceBuffer<std::int32_t> col_c(nz1), long_rows(std::max<index_t>(n_dst, 1));
         DeviceBuffer<double> ov_c(nz1);
-        DeviceBuffer<int> n_long(1);
-        AXIS_HIP_CHECK(hipMemsetAsync(n_long.get(), 0, sizeof(int), s));
+        DeviceBuffer<int> n_long(2);  // rows of up to kMidRow entries, longer rows
+        AXIS_HIP_CHECK(hipMemsetAsync(n_long.get(), 0, 2 * sizeof(int), s));
         if (nnz > 0) {
             compact_grouped_kernel<<<grid_for(n_pairs, 256), 256, 0, s>>>(overlap.get(), pos.get(), n_pairs, cand_src.get(), col_c.get(), ov_c.get());
             AXIS_LAUNCH_CHECK();
@@ -1039,15 +1071,21 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
                                                M->row_ptr.get(), M->col.get(), M->val.get(), ov.get(), M->frac_b.get(), max_row.get(),
                                                long_rows.get(), n_long.get());
         AXIS_LAUNCH_CHECK();
-        int h[2] = {0, 0};
+        int h[3] = {0, 0, 0};
         AXIS_HIP_CHECK(hipMemcpyAsync(&h[0], max_row.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-        AXIS_HIP_CHECK(hipMemcpyAsync(&h[1], n_long.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+        AXIS_HIP_CHECK(hipMemcpyAsync(&h[1], n_long.get(), 2 * sizeof(int), hipMemcpyDeviceToHost, s));
         AXIS_HIP_CHECK(hipStreamSynchronize(s));
         if (h[0] <= kLongRowMax) {
             if (h[1] > 0) {
-                long_rows_kernel<<<static_cast<unsigned>(h[1]), kLongRowBlock, 0, s>>>(long_rows.get(), M->row_ptr.get(), col_c.get(), ov_c.get(),
-                                                                                      dg.d_area.get(), frac_area, M->col.get(), M->val.get(),
-                                                                                      ov.get(), M->frac_b.get());
+                long_rows_kernel<kMidRow, kMidRow><<<static_cast<unsigned>(h[1]), kMidRow, 0, s>>>(
+                    long_rows.get(), M->row_ptr.get(), col_c.get(), ov_c.get(), dg.d_area.get(), frac_area, M->col.get(), M->val.get(), ov.get(),
+                    M->frac_b.get());
+                AXIS_LAUNCH_CHECK();
+            }
+            if (h[2] > 0) {
+                long_rows_kernel<kLongRowMax, kLongRowBlock><<<static_cast<unsigned>(h[2]), kLongRowBlock, 0, s>>>(
+                    long_rows.get() + (n_dst - h[2]), M->row_ptr.get(), col_c.get(), ov_c.get(), dg.d_area.get(), frac_area, M->col.get(),
+                    M->val.get(), ov.get(), M->frac_b.get());
                 AXIS_LAUNCH_CHECK();
             }
             assembled = true;
