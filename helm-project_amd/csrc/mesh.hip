@@ -222,9 +222,14 @@ __global__ void __launch_bounds__(kQuickBlock) distinct_kernel(const double *lon
     const index_t i = static_cast<index_t>(blockIdx.x) * kQuickBlock + threadIdx.x;
     const bool act = i < n;
     for (int c = 0; c < 2; ++c) {
-        const unsigned long long key = act ? order_key(c == 0 ? lon[i] : lat[i]) : 0ull;
+        const double v = act ? (c == 0 ? lon[i] : lat[i]) : 0.0;
+        const unsigned long long key = act ? order_key(v) : 0ull;
         const unsigned long long left = __shfl_up(key, 1);
         if (!act || ((threadIdx.x & 63) != 0 && left == key)) continue;
+        if (!isfinite(v)) {  // reported to the host, which then takes the sorting path
+            atomicOr(&q->nonfinite, 1);
+            continue;
+        }
         if (__hip_atomic_load(&q->overflow[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
         unsigned long long *tab = set + c * kSetSlots;
         std::uint32_t h = mix_key(key) & (kSetSlots - 1);
@@ -305,6 +310,20 @@ index_t unique_column(const double *d_col, const double *h_col, index_t n, bool 
 
 }  // namespace
 
+// Node and cell counts of some nlon x nlat grid (nlon, nlat >= 2):
+// nlon + nlat = nodes - cells + 1 = S and nlon nlat = nodes, so
+// (nlon, nlat) = (S +- d) / 2 with d^2 = S^2 - 4 nodes.
+bool grid_counts(const Mesh &m) {
+    if (m.n_cells() == 0) return false;
+    const __int128 S = static_cast<__int128>(m.n_nodes()) - m.n_cells() + 1, P = m.n_nodes();
+    const __int128 disc = S * S - 4 * P;
+    if (S < 4 || disc < 0) return false;
+    const index_t r = static_cast<index_t>(std::llround(std::sqrt(static_cast<long double>(disc))));
+    for (index_t d = r - 1; d <= r + 1; ++d)
+        if (d >= 0 && static_cast<__int128>(d) * d == disc && (S + d) % 2 == 0 && (S - d) / 2 >= 2) return true;
+    return false;
+}
+
 GridAxes unique_axes(const Mesh &m, hipStream_t s) {
     GridAxes ax;
     const index_t n_cells = m.n_cells();
@@ -315,29 +334,37 @@ GridAxes unique_axes(const Mesh &m, hipStream_t s) {
     PhaseTrace tr("axes", s);
     if (n > 0) {
         // quick pass: refute with the lower bounds, or take both columns'
-        // distinct values from the hash sets
+        // distinct values from the hash sets.  Node and cell counts of some
+        // nlon x nlat grid leave nothing for the bounds to refute: then the
+        // distinct pass alone runs (and checks for non-finite values itself).
         DeviceBuffer<QuickAxes> dq(1);
-        DeviceBuffer<std::uint32_t> bins(2 * kBinWords);
         QuickAxes hq{};
         hq.mm[0] = hq.mm[2] = ~0ull;
         const double *lon = m.d_coords.get(), *lat = m.d_coords.get() + n;
         AXIS_HIP_CHECK(hipMemcpyAsync(dq.get(), &hq, sizeof hq, hipMemcpyHostToDevice, s));
-        AXIS_HIP_CHECK(hipMemsetAsync(bins.get(), 0, sizeof(std::uint32_t) * 2 * kBinWords, s));
-        const unsigned mm_grid = static_cast<unsigned>(std::min<index_t>((n + kQuickBlock - 1) / kQuickBlock, 1024));
-        minmax_kernel<<<mm_grid, kQuickBlock, 0, s>>>(lon, lat, n, dq.get());
-        AXIS_LAUNCH_CHECK();
-        const index_t ns = std::min(n, kSample);
-        sample_bins_kernel<<<grid_for(ns, kQuickBlock), kQuickBlock, 0, s>>>(lon, lat, ns, dq.get(), bins.get());
-        AXIS_LAUNCH_CHECK();
-        count_bins_kernel<<<2 * kBinWords / 256, 256, 0, s>>>(bins.get(), dq.get());
-        AXIS_LAUNCH_CHECK();
-        AXIS_HIP_CHECK(hipMemcpyAsync(&hq, dq.get(), sizeof hq, hipMemcpyDeviceToHost, s));
-        AXIS_HIP_CHECK(hipStreamSynchronize(s));
-        tr.mark("bound");
-        if (!hq.nonfinite) {
-            // (the bins above are meaningless with non-finite coordinates)
-            const index_t lb_lon = (hq.occupied[0] + 1) / 2, lb_lat = (hq.occupied[1] + 1) / 2;
-            if (lb_lon >= 2 && lb_lat >= 2 && (lb_lon - 1) > n_cells / (lb_lat - 1)) return ax;
+        bool run_distinct = true;
+        if (!grid_counts(m)) {
+            DeviceBuffer<std::uint32_t> bins(2 * kBinWords);
+            AXIS_HIP_CHECK(hipMemsetAsync(bins.get(), 0, sizeof(std::uint32_t) * 2 * kBinWords, s));
+            const unsigned mm_grid = static_cast<unsigned>(std::min<index_t>((n + kQuickBlock - 1) / kQuickBlock, 1024));
+            minmax_kernel<<<mm_grid, kQuickBlock, 0, s>>>(lon, lat, n, dq.get());
+            AXIS_LAUNCH_CHECK();
+            const index_t ns = std::min(n, kSample);
+            sample_bins_kernel<<<grid_for(ns, kQuickBlock), kQuickBlock, 0, s>>>(lon, lat, ns, dq.get(), bins.get());
+            AXIS_LAUNCH_CHECK();
+            count_bins_kernel<<<2 * kBinWords / 256, 256, 0, s>>>(bins.get(), dq.get());
+            AXIS_LAUNCH_CHECK();
+            AXIS_HIP_CHECK(hipMemcpyAsync(&hq, dq.get(), sizeof hq, hipMemcpyDeviceToHost, s));
+            AXIS_HIP_CHECK(hipStreamSynchronize(s));
+            tr.mark("bound");
+            if (hq.nonfinite) {
+                run_distinct = false;  // (the bins are meaningless with non-finite coordinates)
+            } else {
+                const index_t lb_lon = (hq.occupied[0] + 1) / 2, lb_lat = (hq.occupied[1] + 1) / 2;
+                if (lb_lon >= 2 && lb_lat >= 2 && (lb_lon - 1) > n_cells / (lb_lat - 1)) return ax;
+            }
+        }
+        if (run_distinct) {
             DeviceBuffer<unsigned long long> set(2 * kSetSlots);
             DeviceBuffer<double> vals(2 * kSetMax);
             DeviceBuffer<int> pos(2);
@@ -349,7 +376,7 @@ GridAxes unique_axes(const Mesh &m, hipStream_t s) {
             AXIS_LAUNCH_CHECK();
             AXIS_HIP_CHECK(hipMemcpyAsync(&hq, dq.get(), sizeof hq, hipMemcpyDeviceToHost, s));
             AXIS_HIP_CHECK(hipStreamSynchronize(s));
-            if (!hq.overflow[0] && !hq.overflow[1]) {
+            if (!hq.nonfinite && !hq.overflow[0] && !hq.overflow[1]) {
                 std::vector<double> xlon(hq.n_distinct[0]), xlat(hq.n_distinct[1]);
                 if (!xlon.empty())
                     AXIS_HIP_CHECK(hipMemcpyAsync(xlon.data(), vals.get(), sizeof(double) * xlon.size(), hipMemcpyDeviceToHost, s));
@@ -710,17 +737,8 @@ void node_xyz_one(const Mesh &m, index_t node, double out[3]) {
 }
 
 bool maybe_rectilinear_deg(const Mesh &m) {
-    if (m.coord_system() != CoordinateSystem::SphericalDeg || m.n_cells() == 0 || std::getenv("AXIS_HOST_NODE_XYZ") != nullptr) return false;
-    if (m.min_cell_vertices() != 4 || m.max_cell_vertices() != 4) return false;
-    // nlon + nlat = nodes - cells + 1 and nlon nlat = nodes: integer roots
-    // (nlon, nlat) = (S +- d) / 2 with d^2 = S^2 - 4 nodes
-    const __int128 S = static_cast<__int128>(m.n_nodes()) - m.n_cells() + 1, P = m.n_nodes();
-    const __int128 disc = S * S - 4 * P;
-    if (S < 4 || disc < 0) return false;
-    const index_t r = static_cast<index_t>(std::llround(std::sqrt(static_cast<long double>(disc))));
-    for (index_t d = r - 1; d <= r + 1; ++d)
-        if (d >= 0 && static_cast<__int128>(d) * d == disc && (S + d) % 2 == 0 && (S - d) / 2 >= 2) return true;
-    return false;
+    if (m.coord_system() != CoordinateSystem::SphericalDeg || std::getenv("AXIS_HOST_NODE_XYZ") != nullptr) return false;
+    return m.min_cell_vertices() == 4 && m.max_cell_vertices() == 4 && grid_counts(m);
 }
 
 bool node_xyz_device_eligible(const Mesh &m, const GridAxes &ax) {
