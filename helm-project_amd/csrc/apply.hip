@@ -106,6 +106,58 @@ __global__ void seg_fill_kernel(const std::uint64_t *key, const index_t *rp, ind
     long_slot[long_off[p]] = static_cast<std::int32_t>(slot_off[p]);
 }
 
+// Direct plain apply of a matrix that has no apply layout yet (a few FP64
+// fields): the same sums as the segmented apply, without building it.  A row
+// of at most E nonzeros is one FMA chain in CSR order (one thread); a longer
+// row is cut into E-entry segments, each an FMA chain from zero, and the
+// segment sums are added in order from zero (one wave per row), exactly as
+// the segmented kernels and long_reduce_kernel do.
+__global__ void __launch_bounds__(256) direct_rows_kernel(const index_t *__restrict__ rp, const index_t *__restrict__ col,
+                                                          const double *__restrict__ val, index_t n_dst, index_t n_src, const double *__restrict__ X,
+                                                          double *__restrict__ Y, index_t k, int E, std::int32_t *__restrict__ long_list,
+                                                          int *__restrict__ n_long) {
+    const index_t r = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= n_dst) return;
+    const index_t b = rp[r], len = rp[r + 1] - b;
+    if (len > E) {
+        long_list[atomicAdd(n_long, 1)] = static_cast<std::int32_t>(r);
+        return;
+    }
+    for (index_t f = 0; f < k; ++f) {
+        const double *xf = X + f * n_src;
+        double acc = 0.0;
+        for (index_t e = b; e < b + len; ++e) acc = fma(val[e], xf[col[e]], acc);
+        Y[f * n_dst + r] = acc;
+    }
+}
+
+__global__ void __launch_bounds__(256) direct_long_kernel(const index_t *__restrict__ rp, const index_t *__restrict__ col,
+                                                          const double *__restrict__ val, index_t n_dst, index_t n_src, const double *__restrict__ X,
+                                                          double *__restrict__ Y, index_t k, int E, const std::int32_t *__restrict__ long_list,
+                                                          int n_long) {
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    const index_t i = static_cast<index_t>(blockIdx.x) * 4 + threadIdx.x / 64;
+    if (i >= n_long) return;  // whole wave (no workgroup barriers below)
+    const index_t r = long_list[i];
+    const index_t b = rp[r], len = rp[r + 1] - b;
+    const index_t ns = (len + E - 1) / E;
+    for (index_t f = 0; f < k; ++f) {
+        const double *xf = X + f * n_src;
+        double tot = 0.0;
+        for (index_t q0 = 0; q0 < ns; q0 += 64) {
+            const index_t q = q0 + lane;
+            double part = 0.0;
+            if (q < ns) {
+                const index_t sb = b + q * E, se = min(sb + E, b + len);
+                for (index_t e = sb; e < se; ++e) part = fma(val[e], xf[col[e]], part);
+            }
+            const int m = static_cast<int>(min<index_t>(64, ns - q0));
+            for (int j = 0; j < m; ++j) tot += __shfl(part, j, 64);
+        }
+        if (lane == 0) Y[f * n_dst + r] = tot;
+    }
+}
+
 // exclusive scan of one int per thread over a tile workgroup
 __device__ inline int block_exclusive_scan(int v, int *wsum, int &total) {
     const int lane = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
@@ -211,6 +263,30 @@ __global__ void __launch_bounds__(apply_detail::kTileSeg) tile_build_kernel(cons
 
 }  // namespace
 
+int InterpolationMatrix::row_length_hist(hipStream_t s) {
+    constexpr int kH = apply_detail::kMaxE + 2;
+    static_assert(kH == sizeof(len_hist) / sizeof(len_hist[0]), "histogram bins");
+    if (!len_hist_ready) {
+        DeviceBuffer<index_t> d_hist(kH);
+        AXIS_HIP_CHECK(hipMemsetAsync(d_hist.get(), 0, sizeof(index_t) * kH, s));
+        if (n_dst > 0) {
+            len_hist_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(row_ptr.get(), n_dst, d_hist.get());
+            AXIS_LAUNCH_CHECK();
+        }
+        d_hist.download(len_hist, kH, s);
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+        len_hist_ready = true;
+    }
+    // E = the smallest register segment (4, 8, 12, 16) that leaves at most
+    // 0.1% of the rows longer than one segment
+    for (int cap : {4, 8, 12, 16}) {
+        index_t above = 0;
+        for (int l = cap + 1; l < kH; ++l) above += len_hist[l];
+        if (above * 1000 <= n_dst) return cap;
+    }
+    return apply_detail::kMaxE;
+}
+
 void InterpolationMatrix::prepare_apply(hipStream_t s) {
     if (apply_ready) return;
     PhaseTrace tr("prepare_apply", s);
@@ -224,28 +300,8 @@ void InterpolationMatrix::prepare_apply(hipStream_t s) {
         split_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(val.get(), col.get(), nnz, val_hi.get(), val_lo.get(), col32.get());
         AXIS_LAUNCH_CHECK();
     }
-    // E = the smallest register segment (4, 8, 12, 16) that leaves at most
-    // 0.1% of the rows longer than one segment
-    constexpr int kH = apply_detail::kMaxE + 2;
-    DeviceBuffer<index_t> d_hist(kH);
-    AXIS_HIP_CHECK(hipMemsetAsync(d_hist.get(), 0, sizeof(index_t) * kH, s));
-    if (n_dst > 0) {
-        len_hist_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(row_ptr.get(), n_dst, d_hist.get());
-        AXIS_LAUNCH_CHECK();
-    }
-    index_t hist[kH] = {0};
-    d_hist.download(hist, kH, s);
-    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    seg_cap = row_length_hist(s);
     tr.mark("split_hist");
-    seg_cap = apply_detail::kMaxE;
-    for (int cap : {4, 8, 12, 16}) {
-        index_t above = 0;
-        for (int l = cap + 1; l < kH; ++l) above += hist[l];
-        if (above * 1000 <= n_dst) {
-            seg_cap = cap;
-            break;
-        }
-    }
     // Processing order: rows grouped by a coarse spatial cell of their
     // centroid (about 256 destination cells per cell), index order inside a
     // cell.  A tile of 256 segments then covers a compact patch, whose source
@@ -329,6 +385,10 @@ void InterpolationMatrix::prepare_tiles(hipStream_t s) {
     tiles_ready = true;
 }
 
+// plain applies of at most this many fields through a matrix without an
+// apply layout take the direct kernels
+constexpr index_t kDirectMaxFields = 2;
+
 void batch_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t n_src, index_t n_dst, index_t k, ApplyMode mode,
                  hipStream_t s) {
     // apply.hpp:63-71 / :285-296: validate before touching dst
@@ -342,6 +402,27 @@ void batch_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, F
     if (mode != ApplyMode::Plain && (xt == FieldType::F64 || yt == FieldType::BF16))
         throw InvalidArgument("batch_apply: the mixed apply takes FP32/BF16 fields and writes FP32/FP64");
     if (k == 0 || n_dst == 0) return;
+    if (mode == ApplyMode::Plain && !A.apply_ready && k <= kDirectMaxFields && n_dst < (index_t(1) << 31)) {
+        // a few fields through a matrix without an apply layout: the direct
+        // kernels give the segmented apply's bits without building the layout
+        const int E = A.row_length_hist(s);
+        index_t n_long = 0;
+        for (int l = E + 1; l < apply_detail::kMaxE + 2; ++l) n_long += A.len_hist[l];
+        DeviceBuffer<std::int32_t> list(std::max<index_t>(n_long, 1));
+        DeviceBuffer<int> cnt(1);
+        AXIS_HIP_CHECK(hipMemsetAsync(cnt.get(), 0, sizeof(int), s));
+        direct_rows_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(A.row_ptr.get(), A.col.get(), A.val.get(), n_dst, n_src, static_cast<const double *>(x),
+                                                                static_cast<double *>(y), k, E, list.get(), cnt.get());
+        AXIS_LAUNCH_CHECK();
+        if (n_long > 0) {
+            direct_long_kernel<<<static_cast<unsigned>((n_long + 3) / 4), 256, 0, s>>>(A.row_ptr.get(), A.col.get(), A.val.get(), n_dst, n_src,
+                                                                                        static_cast<const double *>(x), static_cast<double *>(y), k, E,
+                                                                                        list.get(), static_cast<int>(n_long));
+            AXIS_LAUNCH_CHECK();
+        }
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));  // list and cnt are released on return
+        return;
+    }
     A.prepare_apply(s);
     apply_detail::Launch L = apply_detail::choose(A, k, xt, yt, mode);
     apply_detail::launch(A, x, xt, y, yt, k, mode, L, s);

@@ -118,6 +118,10 @@ class InterpolationMatrix {
     DeviceBuffer<std::int32_t> tile_nu, tile_col;
     DeviceBuffer<std::uint16_t> t_lcol;
     bool apply_ready = false, tiles_ready = false;
+    // row-length histogram (lengths 0 .. 16, then "longer"), cached by
+    // row_length_hist(); it fixes seg_cap
+    index_t len_hist[18] = {0};
+    bool len_hist_ready = false;
     // diagnostics of the generation that produced it
     index_t n_candidates = 0, n_host_arbitrated = 0, n_cap_borderline = 0;
     double gen_ms = 0.0;
@@ -127,6 +131,9 @@ class InterpolationMatrix {
     void prepare_apply(hipStream_t s);
     /// Builds the tiles once (on the first tiled launch).
     void prepare_tiles(hipStream_t s);
+    /// Fills len_hist once; returns the segment length E it implies (the
+    /// smallest of 4, 8, 12, 16 leaving at most 0.1% of the rows longer).
+    int row_length_hist(hipStream_t s);
 };
 
 /// HaloPattern (halo_pattern.hpp:29-48): the off-rank source cells a rank's

@@ -446,7 +446,12 @@ __global__ void compact_grouped_kernel(const double *overlap, const index_t *pos
 __global__ void __launch_bounds__(256) short_rows_kernel(const index_t *cand_off, const index_t *pos, index_t n_dst, const std::int32_t *col_c,
                                                          const double *ov_c, const double *area_dst, int frac_area, index_t *row_ptr,
                                                          index_t *col, double *val, double *ov_s, double *frac_b,
-                                                         int *max_row, std::int32_t *long_rows, int *n_long) {
+                                                         int *max_row, std::int32_t *long_rows, int *n_long, unsigned long long *len_hist) {
+    // row-length histogram for the apply (lengths 0 .. kHistMax, then longer)
+    constexpr int kHistMax = 16;
+    __shared__ unsigned hist[kHistMax + 2];
+    if (threadIdx.x < kHistMax + 2) hist[threadIdx.x] = 0;
+    __syncthreads();
     int longest = 0;
     for (index_t r = static_cast<index_t>(blockIdx.x) * 256 + threadIdx.x; r < n_dst; r += static_cast<index_t>(gridDim.x) * 256) {
         const index_t b = pos[cand_off[r]], e = pos[cand_off[r + 1]];
@@ -454,6 +459,7 @@ __global__ void __launch_bounds__(256) short_rows_kernel(const index_t *cand_off
         if (r == n_dst - 1) row_ptr[n_dst] = e;
         const int len = static_cast<int>(min<index_t>(e - b, index_t(1) << 30));
         longest = max(longest, len);
+        atomicAdd(&hist[len > kHistMax ? kHistMax + 1 : len], 1u);
         if (len > kShortRow) {
             // rows of up to kMidRow entries from the front of the list, longer ones from the back
             if (len <= kMidRow)
@@ -496,6 +502,7 @@ __global__ void __launch_bounds__(256) short_rows_kernel(const index_t *cand_off
         longest = max(max(part[0], part[1]), max(part[2], part[3]));
         if (longest > 0) atomicMax(max_row, longest);
     }
+    if (threadIdx.x < kHistMax + 2 && hist[threadIdx.x]) atomicAdd(&len_hist[threadIdx.x], static_cast<unsigned long long>(hist[threadIdx.x]));
 }
 
 // rows of kShortRow < len <= CAP: bitonic column sort in LDS, the ordered sum
@@ -1062,6 +1069,9 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
         DeviceBuffer<double> ov_c(nz1);
         DeviceBuffer<int> n_long(2);  // rows of up to kMidRow entries, longer rows
         AXIS_HIP_CHECK(hipMemsetAsync(n_long.get(), 0, 2 * sizeof(int), s));
+        constexpr int kH = sizeof(M->len_hist) / sizeof(M->len_hist[0]);
+        DeviceBuffer<unsigned long long> d_hist(kH);
+        AXIS_HIP_CHECK(hipMemsetAsync(d_hist.get(), 0, sizeof(unsigned long long) * kH, s));
         if (nnz > 0) {
             compact_grouped_kernel<<<grid_for(n_pairs, 256), 256, 0, s>>>(overlap.get(), pos.get(), n_pairs, cand_src.get(), col_c.get(), ov_c.get());
             AXIS_LAUNCH_CHECK();
@@ -1069,12 +1079,16 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
         const unsigned grid = static_cast<unsigned>(std::min<index_t>((n_dst + 255) / 256, 2048));
         short_rows_kernel<<<grid, 256, 0, s>>>(cand_off.get(), pos.get(), n_dst, col_c.get(), ov_c.get(), dg.d_area.get(), frac_area,
                                                M->row_ptr.get(), M->col.get(), M->val.get(), ov.get(), M->frac_b.get(), max_row.get(),
-                                               long_rows.get(), n_long.get());
+                                               long_rows.get(), n_long.get(), d_hist.get());
         AXIS_LAUNCH_CHECK();
         int h[3] = {0, 0, 0};
+        unsigned long long hh[kH];
         AXIS_HIP_CHECK(hipMemcpyAsync(&h[0], max_row.get(), sizeof(int), hipMemcpyDeviceToHost, s));
         AXIS_HIP_CHECK(hipMemcpyAsync(&h[1], n_long.get(), 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+        AXIS_HIP_CHECK(hipMemcpyAsync(hh, d_hist.get(), sizeof hh, hipMemcpyDeviceToHost, s));
         AXIS_HIP_CHECK(hipStreamSynchronize(s));
+        for (int l = 0; l < kH; ++l) M->len_hist[l] = static_cast<index_t>(hh[l]);
+        M->len_hist_ready = true;
         if (h[0] <= kLongRowMax) {
             if (h[1] > 0) {
                 long_rows_kernel<kMidRow, kMidRow><<<static_cast<unsigned>(h[1]), kMidRow, 0, s>>>(

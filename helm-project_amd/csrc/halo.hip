@@ -144,6 +144,7 @@ std::unique_ptr<InterpolationMatrix> generate_distributed(const Mesh &src, const
     }
     M->n_src = n_ext;
     M->apply_ready = false;
+    M->len_hist_ready = false;
     M->tiles_ready = false;
     return M;
 }

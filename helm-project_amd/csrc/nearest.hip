@@ -874,6 +874,7 @@ void coastal_pass(const Mesh &src, const Mesh &dst, ExtrapolationAction extrap, 
     M.nnz = new_nnz;
     M.max_row_nnz = new_nnz > 0 ? std::max(h_ints[0], 1) : 0;
     M.apply_ready = false;
+    M.len_hist_ready = false;
     M.tiles_ready = false;
 }
 
