@@ -94,8 +94,7 @@ std::unique_ptr<InterpolationMatrix> generate_bilinear_rect(const Mesh &src, con
     std::vector<double> clon(ni), clat(nj);
     for (int i = 0; i < ni; ++i) clon[i] = 0.5 * (ax.ulon[i] + ax.ulon[i + 1]);
     for (int j = 0; j < nj; ++j) clat[j] = 0.5 * (ax.ulat[j] + ax.ulat[j + 1]);
-    for (index_t c = 0; c < n_dst; ++c)
-        if (dst.h_offsets[c + 1] == dst.h_offsets[c]) throw InvalidArgument("generate_bilinear: destination cell without vertices");
+    if (n_dst > 0 && dst.min_cell_vertices() == 0) throw InvalidArgument("generate_bilinear: destination cell without vertices");
     DeviceBuffer<double> d_clon(std::max(ni, 1)), d_clat(std::max(nj, 1));
     d_clon.upload(clon.data(), ni, s);
     d_clat.upload(clat.data(), nj, s);
@@ -111,6 +110,8 @@ std::unique_ptr<InterpolationMatrix> generate_bilinear_rect(const Mesh &src, con
     M->area_a.resize(std::max<index_t>(n_src, 1));
     M->area_b.resize(std::max<index_t>(n_dst, 1));
     M->max_row_nnz = n_dst > 0 ? 4 : 0;
+    M->len_hist[4] = n_dst;  // every row holds its four stencil entries
+    M->len_hist_ready = true;
     cell_morton_keys(dst, M->row_key, s);
     bilinear_kernel<<<grid_for(n_dst + 1, 256), 256, 0, s>>>(dst.d_offsets.get(), dst.d_indices.get(), dst.d_coords.get(), dst.n_nodes(), n_dst,
                                                              d_clon.get(), ni, d_clat.get(), nj, M->row_ptr.get(), M->col.get(), M->val.get());

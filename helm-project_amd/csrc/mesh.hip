@@ -310,17 +310,78 @@ index_t unique_column(const double *d_col, const double *h_col, index_t n, bool 
 
 }  // namespace
 
+// Whether node p holds the longitude of node p % nlon and the latitude of
+// node (p / nlon) nlon, bit for bit, and finite values (a longitude-fastest
+// nlon x nlat node grid); one flag per failing wave.
+__global__ void __launch_bounds__(256) canonical_grid_kernel(const double *__restrict__ lon, const double *__restrict__ lat, index_t n, index_t nlon,
+                                                             int *__restrict__ bad) {
+    const index_t p = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool ok = true;
+    if (p < n) {
+        const double a = lon[p], b = lat[p];
+        ok = isfinite(a) && isfinite(b) && __double_as_longlong(a) == __double_as_longlong(lon[p % nlon]) &&
+             __double_as_longlong(b) == __double_as_longlong(lat[(p / nlon) * nlon]);
+    }
+    if (__ballot(!ok) != 0ull && (threadIdx.x & 63) == 0) atomicOr(bad, 1);
+}
+
+// total order for the distinct coordinate values: by value, ties (-0, +0) by bits
+inline bool value_then_bits(double a, double b) {
+    if (a < b) return true;
+    if (b < a) return false;
+    std::uint64_t x, y;
+    std::memcpy(&x, &a, sizeof x);
+    std::memcpy(&y, &b, sizeof y);
+    return x < y;
+}
+
 // Node and cell counts of some nlon x nlat grid (nlon, nlat >= 2):
 // nlon + nlat = nodes - cells + 1 = S and nlon nlat = nodes, so
 // (nlon, nlat) = (S +- d) / 2 with d^2 = S^2 - 4 nodes.
-bool grid_counts(const Mesh &m) {
+bool grid_counts(const Mesh &m, index_t *na = nullptr, index_t *nb = nullptr) {
     if (m.n_cells() == 0) return false;
     const __int128 S = static_cast<__int128>(m.n_nodes()) - m.n_cells() + 1, P = m.n_nodes();
     const __int128 disc = S * S - 4 * P;
     if (S < 4 || disc < 0) return false;
     const index_t r = static_cast<index_t>(std::llround(std::sqrt(static_cast<long double>(disc))));
     for (index_t d = r - 1; d <= r + 1; ++d)
-        if (d >= 0 && static_cast<__int128>(d) * d == disc && (S + d) % 2 == 0 && (S - d) / 2 >= 2) return true;
+        if (d >= 0 && static_cast<__int128>(d) * d == disc && (S + d) % 2 == 0 && (S - d) / 2 >= 2) {
+            if (na) *na = static_cast<index_t>((S + d) / 2);
+            if (nb) *nb = static_cast<index_t>((S - d) / 2);
+            return true;
+        }
+    return false;
+}
+
+// Exact distinct coordinate values of a mesh whose nodes form a
+// longitude-fastest grid (checked on the device), read from the first row
+// and column of the host coordinates; false when the nodes are in another
+// layout.
+bool canonical_axes(const Mesh &m, index_t na, index_t nb, std::vector<double> &xlon, std::vector<double> &xlat, hipStream_t s) {
+    const index_t n = m.n_nodes();
+    for (int t = 0; t < 2; ++t) {
+        const index_t nlon = t == 0 ? na : nb, nlat = t == 0 ? nb : na;
+        if (t == 1 && na == nb) continue;  // (the same layout again)
+        DeviceBuffer<int> bad(1);
+        AXIS_HIP_CHECK(hipMemsetAsync(bad.get(), 0, sizeof(int), s));
+        canonical_grid_kernel<<<grid_for(n, 256), 256, 0, s>>>(m.d_coords.get(), m.d_coords.get() + n, n, nlon, bad.get());
+        AXIS_LAUNCH_CHECK();
+        int h_bad = 1;
+        AXIS_HIP_CHECK(hipMemcpyAsync(&h_bad, bad.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+        if (h_bad) continue;
+        auto distinct = [](std::vector<double> v) {
+            std::sort(v.begin(), v.end(), value_then_bits);
+            v.erase(std::unique(v.begin(), v.end(), [](double a, double b) { return !value_then_bits(a, b) && !value_then_bits(b, a); }), v.end());
+            return v;
+        };
+        const double *lon = m.h_coords.data(), *lat = m.h_coords.data() + n;
+        std::vector<double> vlat(nlat);
+        for (index_t j = 0; j < nlat; ++j) vlat[j] = lat[j * nlon];
+        xlon = distinct(std::vector<double>(lon, lon + nlon));
+        xlat = distinct(std::move(vlat));
+        return true;
+    }
     return false;
 }
 
@@ -343,7 +404,22 @@ GridAxes unique_axes(const Mesh &m, hipStream_t s) {
         const double *lon = m.d_coords.get(), *lat = m.d_coords.get() + n;
         AXIS_HIP_CHECK(hipMemcpyAsync(dq.get(), &hq, sizeof hq, hipMemcpyHostToDevice, s));
         bool run_distinct = true;
-        if (!grid_counts(m)) {
+        index_t na = 0, nb = 0;
+        const bool counts = grid_counts(m, &na, &nb);
+        std::vector<double> xlon, xlat;
+        if (counts && canonical_axes(m, na, nb, xlon, xlat, s)) {
+            std::vector<double> ulon = host_unique(xlon), ulat = host_unique(xlat);
+            const index_t nlon = static_cast<index_t>(ulon.size()), nlat = static_cast<index_t>(ulat.size());
+            if (nlon >= 2 && nlat >= 2 && (nlon - 1) * (nlat - 1) == n_cells) {
+                ax.ulon = std::move(ulon);
+                ax.ulat = std::move(ulat);
+                ax.xlon = std::move(xlon);
+                ax.xlat = std::move(xlat);
+            }
+            tr.mark("canonical");
+            return ax;
+        }
+        if (!counts) {
             DeviceBuffer<std::uint32_t> bins(2 * kBinWords);
             AXIS_HIP_CHECK(hipMemsetAsync(bins.get(), 0, sizeof(std::uint32_t) * 2 * kBinWords, s));
             const unsigned mm_grid = static_cast<unsigned>(std::min<index_t>((n + kQuickBlock - 1) / kQuickBlock, 1024));
@@ -377,14 +453,15 @@ GridAxes unique_axes(const Mesh &m, hipStream_t s) {
             AXIS_HIP_CHECK(hipMemcpyAsync(&hq, dq.get(), sizeof hq, hipMemcpyDeviceToHost, s));
             AXIS_HIP_CHECK(hipStreamSynchronize(s));
             if (!hq.nonfinite && !hq.overflow[0] && !hq.overflow[1]) {
-                std::vector<double> xlon(hq.n_distinct[0]), xlat(hq.n_distinct[1]);
+                xlon.assign(hq.n_distinct[0], 0.0);
+                xlat.assign(hq.n_distinct[1], 0.0);
                 if (!xlon.empty())
                     AXIS_HIP_CHECK(hipMemcpyAsync(xlon.data(), vals.get(), sizeof(double) * xlon.size(), hipMemcpyDeviceToHost, s));
                 if (!xlat.empty())
                     AXIS_HIP_CHECK(hipMemcpyAsync(xlat.data(), vals.get() + kSetMax, sizeof(double) * xlat.size(), hipMemcpyDeviceToHost, s));
                 AXIS_HIP_CHECK(hipStreamSynchronize(s));
-                std::sort(xlon.begin(), xlon.end());
-                std::sort(xlat.begin(), xlat.end());
+                std::sort(xlon.begin(), xlon.end(), value_then_bits);
+                std::sort(xlat.begin(), xlat.end(), value_then_bits);
                 std::vector<double> ulon = host_unique(xlon), ulat = host_unique(xlat);
                 const index_t nlon = static_cast<index_t>(ulon.size()), nlat = static_cast<index_t>(ulat.size());
                 if (nlon >= 2 && nlat >= 2 && (nlon - 1) * (nlat - 1) == n_cells) {

@@ -134,3 +134,37 @@ def test_device_node_xyz_matches_host(case, capfd):
     assert md.nnz > 0
     mh.close()
     md.close()
+
+
+def permuted_nodes(mesh, seed):
+    """The same cells over the same node coordinates, nodes stored in a
+    random order (grid detection then cannot read the axes off the first row
+    and column and takes its hash-set path)."""
+    c = np.asarray(mesh["coords"], dtype=np.float64)
+    perm = np.random.default_rng(seed).permutation(c.shape[0])
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.size)
+    m = dict(mesh)
+    m["coords"] = np.asfortranarray(c[perm])
+    m["indices"] = inv[np.asarray(mesh["indices"])].astype(np.int64)
+    return m
+
+
+@pytest.mark.parametrize("method", [axis.M_CONSERVATIVE1, axis.M_BILINEAR])
+def test_grid_detection_paths_agree(method):
+    """A lat-lon source in node-grid order (axes read from its first row and
+    column after a device check) and the same source with permuted nodes
+    (axes from the hash set) give the same matrix bit for bit."""
+    say("case detection paths, method", method)
+    src, dst = grids.latlon_deg(1.0), grids.cubed_sphere(32)
+    cfg = axis.RegridConfig(method=method)
+    a = axis.generate(axis.Mesh(src), axis.Mesh(dst), cfg)
+    b = axis.generate(axis.Mesh(permuted_nodes(src, 11)), axis.Mesh(dst), cfg)
+    assert a.info() == b.info()
+    for x, y in zip(a.csr(), b.csr()):
+        assert np.array_equal(x, y)
+    fa, fb = a.fractions(), b.fractions()
+    for k in fa:
+        assert np.array_equal(fa[k].view(np.int64), fb[k].view(np.int64)), k
+    a.close()
+    b.close()
