@@ -12,6 +12,12 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
+#include <condition_variable>
+#include <deque>
+#include <exception>
+#include <functional>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -33,7 +39,99 @@ inline int host_threads() {
     return n;
 }
 
-/// Runs fn(begin, end) over [0, n) split into contiguous chunks.
+namespace pool_detail {
+
+/// One parallel_range call: chunk indices handed out by an atomic counter.
+struct Job {
+    std::function<void(std::int64_t)> fn;
+    std::int64_t chunks = 0;
+    std::atomic<std::int64_t> next{0};
+    std::int64_t done = 0;  // guarded by mu
+    int refs = 0;           // workers inside the job, guarded by the pool's mutex
+    std::exception_ptr err; // guarded by mu
+    std::mutex mu;
+    std::condition_variable cv;
+};
+
+/// Persistent host workers (created once, host_threads() - 1 of them; the
+/// caller of run() works on its own job too, so a call always makes progress
+/// even when every worker is busy with another caller's job).
+class Pool {
+   public:
+    explicit Pool(int workers) {
+        for (int i = 0; i < workers; ++i) std::thread([this] { loop(); }).detach();
+        workers_ = workers;
+    }
+    void run(Job &j) {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            const std::int64_t helpers = std::min<std::int64_t>(workers_, j.chunks - 1);
+            for (std::int64_t i = 0; i < helpers; ++i) q_.push_back(&j);
+        }
+        cv_.notify_all();
+        work(j);
+        {
+            std::unique_lock<std::mutex> l(j.mu);
+            j.cv.wait(l, [&] { return j.done == j.chunks; });
+        }
+        {
+            // queued entries nobody took are dropped; wait for workers still inside
+            std::unique_lock<std::mutex> l(mu_);
+            q_.erase(std::remove(q_.begin(), q_.end(), &j), q_.end());
+            left_.wait(l, [&] { return j.refs == 0; });
+        }
+        if (j.err) std::rethrow_exception(j.err);
+    }
+
+   private:
+    static void work(Job &j) {
+        for (;;) {
+            const std::int64_t c = j.next.fetch_add(1);
+            if (c >= j.chunks) return;
+            try {
+                j.fn(c);
+            } catch (...) {
+                std::lock_guard<std::mutex> g(j.mu);
+                if (!j.err) j.err = std::current_exception();
+            }
+            std::lock_guard<std::mutex> g(j.mu);
+            if (++j.done == j.chunks) j.cv.notify_all();
+        }
+    }
+    void loop() {
+        for (;;) {
+            Job *j = nullptr;
+            {
+                std::unique_lock<std::mutex> l(mu_);
+                cv_.wait(l, [&] { return !q_.empty(); });
+                j = q_.front();
+                q_.pop_front();
+                ++j->refs;
+            }
+            work(*j);
+            {
+                std::lock_guard<std::mutex> l(mu_);
+                --j->refs;
+            }
+            left_.notify_all();
+        }
+    }
+    std::mutex mu_;
+    std::condition_variable cv_, left_;
+    std::deque<Job *> q_;
+    int workers_ = 0;
+};
+
+inline Pool &pool() {
+    static Pool *p = new Pool(std::max(host_threads() - 1, 0));  // never destroyed: the workers outlive static destruction
+    return *p;
+}
+
+}  // namespace pool_detail
+
+/// Runs fn(begin, end) over [0, n) split into contiguous chunks, on the
+/// persistent host workers and the calling thread.  An exception thrown by
+/// fn is rethrown here after every chunk has finished.
 template <class Fn>
 void parallel_range(std::int64_t n, Fn fn, std::int64_t min_chunk = 4096) {
     int nt = host_threads();
@@ -42,13 +140,10 @@ void parallel_range(std::int64_t n, Fn fn, std::int64_t min_chunk = 4096) {
         fn(std::int64_t(0), n);
         return;
     }
-    std::vector<std::thread> th;
-    th.reserve(chunks);
-    for (std::int64_t c = 0; c < chunks; ++c) {
-        std::int64_t b = n * c / chunks, e = n * (c + 1) / chunks;
-        th.emplace_back([=] { fn(b, e); });
-    }
-    for (auto &t : th) t.join();
+    pool_detail::Job j;
+    j.chunks = chunks;
+    j.fn = [&](std::int64_t c) { fn(n * c / chunks, n * (c + 1) / chunks); };
+    pool_detail::pool().run(j);
 }
 
 /// Phase timer for AXIS_TRACE=1: each mark() waits for the stream and prints
