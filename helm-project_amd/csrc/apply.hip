@@ -420,7 +420,9 @@ void batch_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, F
                                                                                         list.get(), static_cast<int>(n_long));
             AXIS_LAUNCH_CHECK();
         }
-        AXIS_HIP_CHECK(hipStreamSynchronize(s));  // list and cnt are released on return
+        // (list and cnt go back to the block cache, which hands a block out
+        // again only after a device synchronisation: no wait here, so the
+        // async entry point stays asynchronous)
         return;
     }
     A.prepare_apply(s);
