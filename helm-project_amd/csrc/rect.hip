@@ -225,6 +225,26 @@ __global__ void column_sum_kernel(const std::uint32_t *__restrict__ key, const s
     out[c] = fmin(s, 1.0);
 }
 
+// row-length histogram (lengths 0 .. 16, then longer) in stats[0 .. 17] and
+// the longest row in stats[18]
+__global__ void __launch_bounds__(256) row_stats_kernel(const index_t *__restrict__ row_ptr, index_t n_rows, unsigned long long *__restrict__ stats) {
+    constexpr int kMax = 16;
+    __shared__ unsigned hist[kMax + 2];
+    __shared__ unsigned long long longest;
+    if (threadIdx.x < kMax + 2) hist[threadIdx.x] = 0;
+    if (threadIdx.x == 0) longest = 0;
+    __syncthreads();
+    const index_t r = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r < n_rows) {
+        const index_t len = row_ptr[r + 1] - row_ptr[r];
+        atomicAdd(&hist[len > kMax ? kMax + 1 : len], 1u);
+        atomicMax(&longest, static_cast<unsigned long long>(len));
+    }
+    __syncthreads();
+    if (threadIdx.x < kMax + 2 && hist[threadIdx.x]) atomicAdd(&stats[threadIdx.x], static_cast<unsigned long long>(hist[threadIdx.x]));
+    if (threadIdx.x == 0 && longest) atomicMax(&stats[kMax + 2], longest);
+}
+
 __global__ void zero_kernel(double *__restrict__ p, index_t n) {
     const index_t i = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
@@ -452,13 +472,18 @@ std::unique_ptr<InterpolationMatrix> assemble(const Mesh &src, const Mesh &dst, 
         radix_sort_pairs_u32(key.get(), perm.get(), static_cast<std::size_t>(nnz), bits, s);
         column_sum_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(key.get(), perm.get(), nnz, part.get(), M->frac_a.get());
         AXIS_LAUNCH_CHECK();
-        // the longest row
-        std::vector<index_t> rp(n_dst + 1);
-        M->row_ptr.download(rp.data(), n_dst + 1, s);
+        // the longest row and the row-length histogram (for the apply), on the device
+        constexpr int kH = sizeof(M->len_hist) / sizeof(M->len_hist[0]);
+        DeviceBuffer<unsigned long long> stats(kH + 1);
+        AXIS_HIP_CHECK(hipMemsetAsync(stats.get(), 0, sizeof(unsigned long long) * (kH + 1), s));
+        row_stats_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(M->row_ptr.get(), n_dst, stats.get());
+        AXIS_LAUNCH_CHECK();
+        unsigned long long h[kH + 1];
+        AXIS_HIP_CHECK(hipMemcpyAsync(h, stats.get(), sizeof h, hipMemcpyDeviceToHost, s));
         AXIS_HIP_CHECK(hipStreamSynchronize(s));
-        index_t mx = 0;
-        for (index_t r = 0; r < n_dst; ++r) mx = std::max(mx, rp[r + 1] - rp[r]);
-        M->max_row_nnz = static_cast<int>(mx);
+        M->max_row_nnz = static_cast<int>(std::min<unsigned long long>(h[kH], 1ull << 30));
+        for (int l = 0; l < kH; ++l) M->len_hist[l] = static_cast<index_t>(h[l]);
+        M->len_hist_ready = true;
     }
     cell_morton_keys(dst, M->row_key, s);
     AXIS_HIP_CHECK(hipStreamSynchronize(s));  // the tables and work buffers are released on return
