@@ -400,21 +400,40 @@ int walk_start(int is0, int is1, index_t sni, bool glob) {
 std::unique_ptr<InterpolationMatrix> assemble(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, RectArgs a, const RectTables &t,
                                               hipStream_t s, std::chrono::steady_clock::time_point t0) {
     const index_t n_src = src.n_cells(), n_dst = dst.n_cells();
-    DeviceBuffer<double> b_nlx, b_nhx, b_dly, b_dhy, b_ssl, b_ssh, b_dsl, b_dsh, b_sw, b_dw, b_srl, b_srh, b_drl, b_drh;
-    DeviceBuffer<int> b_is0, b_is1, b_k0, b_js0, b_js1;
-    a.c_nlx = upload(b_nlx, t.c_nlx, s);
-    a.c_nhx = upload(b_nhx, t.c_nhx, s);
-    a.r_dly = upload(b_dly, t.r_dly, s);
-    a.r_dhy = upload(b_dhy, t.r_dhy, s);
-    a.s_sin_lo = upload(b_ssl, t.s_sin_lo, s);
-    a.s_sin_hi = upload(b_ssh, t.s_sin_hi, s);
-    a.d_sin_lo = upload(b_dsl, t.d_sin_lo, s);
-    a.d_sin_hi = upload(b_dsh, t.d_sin_hi, s);
-    a.c_is0 = upload(b_is0, t.c_is0, s);
-    a.c_is1 = upload(b_is1, t.c_is1, s);
-    a.c_k0 = upload(b_k0, t.c_k0, s);
-    a.r_js0 = upload(b_js0, t.r_js0, s);
-    a.r_js1 = upload(b_js1, t.r_js1, s);
+    // every table in one double and one int upload
+    const std::vector<double> *dtab[] = {&t.c_nlx,    &t.c_nhx,    &t.r_dly,   &t.r_dhy,    &t.s_sin_lo, &t.s_sin_hi, &t.d_sin_lo,
+                                         &t.d_sin_hi, &t.s_width, &t.s_row_lo, &t.s_row_hi, &t.d_width, &t.d_row_lo, &t.d_row_hi};
+    const std::vector<int> *itab[] = {&t.c_is0, &t.c_is1, &t.c_k0, &t.r_js0, &t.r_js1};
+    std::vector<double> hd;
+    std::vector<int> hi;
+    std::size_t od[14], oi[5];
+    for (int k = 0; k < 14; ++k) {
+        od[k] = hd.size();
+        hd.insert(hd.end(), dtab[k]->begin(), dtab[k]->end());
+    }
+    for (int k = 0; k < 5; ++k) {
+        oi[k] = hi.size();
+        hi.insert(hi.end(), itab[k]->begin(), itab[k]->end());
+    }
+    DeviceBuffer<double> bd(std::max<std::size_t>(hd.size(), 1));
+    DeviceBuffer<int> bi(std::max<std::size_t>(hi.size(), 1));
+    if (!hd.empty()) bd.upload(hd.data(), hd.size(), s);
+    if (!hi.empty()) bi.upload(hi.data(), hi.size(), s);
+    auto D = [&](int k) -> const double * { return bd.get() + od[k]; };
+    auto I = [&](int k) -> const int * { return bi.get() + oi[k]; };
+    a.c_nlx = D(0);
+    a.c_nhx = D(1);
+    a.r_dly = D(2);
+    a.r_dhy = D(3);
+    a.s_sin_lo = D(4);
+    a.s_sin_hi = D(5);
+    a.d_sin_lo = D(6);
+    a.d_sin_hi = D(7);
+    a.c_is0 = I(0);
+    a.c_is1 = I(1);
+    a.c_k0 = I(2);
+    a.r_js0 = I(3);
+    a.r_js1 = I(4);
     a.s_mask = src.has_mask() ? src.d_mask.get() : nullptr;
     a.d_mask = dst.has_mask() ? dst.d_mask.get() : nullptr;
     auto M = std::make_unique<InterpolationMatrix>();
@@ -427,11 +446,9 @@ std::unique_ptr<InterpolationMatrix> assemble(const Mesh &src, const Mesh &dst, 
     M->area_b.resize(nd);
     M->row_ptr.resize(n_dst + 1);
     // final areas first: the regular path's weights divide by them
-    rect_area_kernel<<<grid_for(ns, 256), 256, 0, s>>>(a.cs, a.sni, a.snj, upload(b_sw, t.s_width, s), upload(b_srl, t.s_row_lo, s),
-                                                       upload(b_srh, t.s_row_hi, s), a.s_sin_lo, a.s_sin_hi,
+    rect_area_kernel<<<grid_for(ns, 256), 256, 0, s>>>(a.cs, a.sni, a.snj, D(8), D(9), D(10), a.s_sin_lo, a.s_sin_hi,
                                                        src.has_areas() ? src.d_areas.get() : nullptr, M->area_a.get());
-    rect_area_kernel<<<grid_for(nd, 256), 256, 0, s>>>(a.cs, a.dni, a.dnj, upload(b_dw, t.d_width, s), upload(b_drl, t.d_row_lo, s),
-                                                       upload(b_drh, t.d_row_hi, s), a.d_sin_lo, a.d_sin_hi,
+    rect_area_kernel<<<grid_for(nd, 256), 256, 0, s>>>(a.cs, a.dni, a.dnj, D(11), D(12), D(13), a.d_sin_lo, a.d_sin_hi,
                                                        dst.has_areas() ? dst.d_areas.get() : nullptr, M->area_b.get());
     AXIS_LAUNCH_CHECK();
     if (a.path == 0) {
