@@ -772,21 +772,28 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     } dst_up;
     hipStream_t up = nullptr;
     std::atomic<bool> dst_uploaded{false};
-    if (spherical) {
+    // both meshes with grid counts: an analytic path will most likely answer,
+    // so no host task is started (should both decline, the geometry below
+    // computes the node xyz on this thread)
+    const bool likely_structured = dst_first && maybe_rectilinear_deg(dst);
+    std::future<void> xyz_task;
+    if (spherical && !likely_structured) {
         up = upload_stream();
         AXIS_HIP_CHECK(hipEventCreateWithFlags(&dst_up.e, hipEventDisableTiming));
         dg.d_xyz.resize(std::max<std::size_t>(3 * static_cast<std::size_t>(dst.n_nodes()), 3));
+        xyz_task = std::async(std::launch::async, [&] {
+            try {
+                if (!dst_first) node_xyz_host(src, sg, &skip_src);
+                src_done.set_value();
+            } catch (...) {
+                src_done.set_exception(std::current_exception());
+                return;
+            }
+            dst_uploaded.store(node_xyz_host(dst, dg, &cancel, up, dst_up.e));
+        });
+    } else {
+        src_done.set_value();
     }
-    auto xyz_task = std::async(std::launch::async, [&] {
-        try {
-            if (spherical && !dst_first) node_xyz_host(src, sg, &skip_src);
-            src_done.set_value();
-        } catch (...) {
-            src_done.set_exception(std::current_exception());
-            return;
-        }
-        if (spherical) dst_uploaded.store(node_xyz_host(dst, dg, &cancel, up, dst_up.e));
-    });
     GridAxes sax = unique_axes(src, s);
     const bool src_dev_xyz = spherical && node_xyz_device_eligible(src, sax);
     if (src_dev_xyz) skip_src.store(true);
@@ -811,7 +818,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     if (fast) {
         skip_src.store(true);
         cancel.store(true);
-        xyz_task.get();
+        if (xyz_task.valid()) xyz_task.get();
         return fast;
     }
     if (!spherical) throw InvalidArgument("generate_conservative: Cartesian3D meshes are served by the rectangle paths only here");
@@ -826,7 +833,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     LbvhTree tree;
     lbvh_build(sg.d_box.get(), n_src, tree, s);
     tr.mark("lbvh_build");
-    xyz_task.get();
+    if (xyz_task.valid()) xyz_task.get();
     if (dst_uploaded.load()) {
         AXIS_HIP_CHECK(hipStreamWaitEvent(s, dst_up.e, 0));
         dg.d_xyz_ready = true;
