@@ -176,12 +176,12 @@ def bench_config5(axis, grids, torch):
     net = proposer.load(str(ROOT / "helm-project_amd" / "data" / "proposer_voronoi8_c384.npz"))
     S, D = axis.Mesh(src), axis.Mesh(dst)
     res = {"workload": "Voronoi level 8 (655,362 cells, seed 5020) -> C384"}
-    for rep in range(2):
+    for rep in range(2):  # the better of the two runs of each generation is reported
         t = time.perf_counter()
         plain = axis.generate_weights(S, D, axis.CONSERVATIVE)
         torch.cuda.synchronize()
         t_plain = (time.perf_counter() - t) * 1e3
-        res["plain_ms"] = t_plain
+        res["plain_ms"] = min(t_plain, res.get("plain_ms", t_plain))
         res["plain_candidates"] = plain.stats()["candidates"]
         for tol in (1e-12, 1e-10):
             t = time.perf_counter()
@@ -191,6 +191,8 @@ def bench_config5(axis, grids, torch):
             ra, ca, va = plain.csr()
             rb, cb, vb = prop.csr()
             same = bool((ra == rb).all() and (ca == cb).all() and (va == vb).all())
+            prev = res.get(f"proposed_area_tol_{tol:g}")
+            t_prop = min(t_prop, prev["ms"]) if prev else t_prop
             res[f"proposed_area_tol_{tol:g}"] = dict(st, ms=t_prop, identical_to_plain=same,
                                                      served_fraction=1 - (st["guard_rejected"] + st["area_failed"]) / st["cells"],
                                                      candidate_pairs=st["pairs_proposed"] + st["pairs_fallback"])

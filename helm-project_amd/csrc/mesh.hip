@@ -795,6 +795,14 @@ __global__ void scatter_xyz_kernel(const index_t *__restrict__ nodes, const doub
 
 constexpr int kXyzFixCap = 8192;
 
+__global__ void scatter_recheck_kernel(const index_t *__restrict__ cells, const double *__restrict__ area, const std::int32_t *__restrict__ flags, int n,
+                                       double *__restrict__ d_area, std::int32_t *__restrict__ d_flags) {
+    const int k = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    d_area[cells[k]] = area[k];
+    d_flags[cells[k]] = flags[k];
+}
+
 }  // namespace
 
 void node_xyz_one(const Mesh &m, index_t node, double out[3]) {
@@ -1027,10 +1035,13 @@ void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s) {
             area[k] = a;
             flags[k] = (cls != 0 ? 1 : 0) | ((m.has_mask() && m.h_mask[c] == 0) ? 2 : 0) | (a <= 0.0 ? 4 : 0);
         }
-        for (int k = 0; k < h_n; ++k) {
-            AXIS_HIP_CHECK(hipMemcpyAsync(g.d_area.get() + cells[k], &area[k], sizeof(double), hipMemcpyHostToDevice, s));
-            AXIS_HIP_CHECK(hipMemcpyAsync(g.d_flags.get() + cells[k], &flags[k], sizeof(std::int32_t), hipMemcpyHostToDevice, s));
-        }
+        // the replayed values in one upload each, scattered on the device
+        DeviceBuffer<index_t> d_cells(h_n);
+        d_cells.upload(cells.data(), h_n, s);
+        tmp_area.upload(area.data(), h_n, s);
+        tmp_flags.upload(flags.data(), h_n, s);
+        scatter_recheck_kernel<<<grid_for(h_n, 256), 256, 0, s>>>(d_cells.get(), tmp_area.get(), tmp_flags.get(), h_n, g.d_area.get(), g.d_flags.get());
+        AXIS_LAUNCH_CHECK();
         AXIS_HIP_CHECK(hipStreamSynchronize(s));
     }
 }
