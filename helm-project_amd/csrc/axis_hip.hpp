@@ -291,12 +291,14 @@ bool rectilinear_from_axes(const Mesh &m, const GridAxes &ax);
 /// touches only g.h_xyz, g.h_xyz_deg and g.xyz_ready, so it may run on a host
 /// thread while the GPU works on other members.
 ///
-/// With `up` given, g.d_xyz (already sized) receives each host thread's slice
-/// of h_xyz on stream `up` as soon as that slice is done, and `done` is
-/// recorded on `up` after the last one; returns whether it was (false when
-/// cancelled).  The caller orders its stream after `done` and sets
-/// g.d_xyz_ready.
-bool node_xyz_host(const Mesh &m, MeshGeom &g, const std::atomic<bool> *cancel = nullptr, hipStream_t up = nullptr, hipEvent_t done = nullptr);
+/// With `up` given, `d_out` (3 * n_nodes doubles, g.d_xyz's memory taken by
+/// the caller before starting this on another thread) receives each host
+/// thread's slice of h_xyz on stream `up` as soon as that slice is done, and
+/// `done` is recorded on `up` after the last one; returns whether it was
+/// (false when cancelled).  The caller orders its stream after `done` and
+/// sets g.d_xyz_ready.
+bool node_xyz_host(const Mesh &m, MeshGeom &g, const std::atomic<bool> *cancel = nullptr, hipStream_t up = nullptr, hipEvent_t done = nullptr,
+                   double *d_out = nullptr);
 /// A non-blocking stream of the current device for host-to-device uploads
 /// beside the caller's stream (created once per device, default priority).
 hipStream_t upload_stream();

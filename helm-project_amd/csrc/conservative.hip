@@ -769,9 +769,9 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
         ~EventGuard() {
             if (e) (void)hipEventDestroy(e);
         }
-    } dst_up;
+    } dst_up, src_up;
     hipStream_t up = nullptr;
-    std::atomic<bool> dst_uploaded{false};
+    std::atomic<bool> dst_uploaded{false}, src_uploaded{false};
     // both meshes with grid counts: an analytic path will most likely answer,
     // so no host task is started (should both decline, the geometry below
     // computes the node xyz on this thread)
@@ -781,15 +781,21 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
         up = upload_stream();
         AXIS_HIP_CHECK(hipEventCreateWithFlags(&dst_up.e, hipEventDisableTiming));
         dg.d_xyz.resize(std::max<std::size_t>(3 * static_cast<std::size_t>(dst.n_nodes()), 3));
-        xyz_task = std::async(std::launch::async, [&] {
+        double *src_out = nullptr, *dst_out = dg.d_xyz.get();  // (the task never touches the DeviceBuffer objects)
+        if (!dst_first) {  // the source's slices go up the same way
+            AXIS_HIP_CHECK(hipEventCreateWithFlags(&src_up.e, hipEventDisableTiming));
+            sg.d_xyz.resize(std::max<std::size_t>(3 * static_cast<std::size_t>(src.n_nodes()), 3));
+            src_out = sg.d_xyz.get();
+        }
+        xyz_task = std::async(std::launch::async, [&, src_out, dst_out] {
             try {
-                if (!dst_first) node_xyz_host(src, sg, &skip_src);
+                if (!dst_first) src_uploaded.store(node_xyz_host(src, sg, &skip_src, up, src_up.e, src_out));
                 src_done.set_value();
             } catch (...) {
                 src_done.set_exception(std::current_exception());
                 return;
             }
-            dst_uploaded.store(node_xyz_host(dst, dg, &cancel, up, dst_up.e));
+            dst_uploaded.store(node_xyz_host(dst, dg, &cancel, up, dst_up.e, dst_out));
         });
     } else {
         src_done.set_value();
@@ -826,6 +832,10 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     tr.mark("detect");
     if (src_dev_xyz) node_xyz_device(src, sax, sg, s);  // (false: host xyz below)
     src_ready.get();
+    if (!sg.dev_xyz && src_uploaded.load()) {
+        AXIS_HIP_CHECK(hipStreamWaitEvent(s, src_up.e, 0));
+        sg.d_xyz_ready = true;
+    }
     prepare_geometry(src, sg, s);
     tr.mark("geometry_src");
     // the source tree does not need the destination: build it while the host

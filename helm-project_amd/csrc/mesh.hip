@@ -853,7 +853,8 @@ bool node_xyz_device(const Mesh &m, const GridAxes &ax, MeshGeom &g, hipStream_t
     }
     DeviceBuffer<double> d_tab(tab.size());
     d_tab.upload(tab.data(), tab.size(), s);
-    g.d_xyz.resize(std::max<std::size_t>(3 * static_cast<std::size_t>(n), 3));
+    // a fresh buffer: the host path may have queued slice uploads into the old one
+    g.d_xyz = DeviceBuffer<double>(std::max<std::size_t>(3 * static_cast<std::size_t>(n), 3));
     DeviceBuffer<index_t> fix(kXyzFixCap);
     DeviceBuffer<int> n_fix(1);
     AXIS_HIP_CHECK(hipMemsetAsync(n_fix.get(), 0, sizeof(int), s));
@@ -899,14 +900,14 @@ hipStream_t upload_stream() {
     return st;
 }
 
-bool node_xyz_host(const Mesh &m, MeshGeom &g, const std::atomic<bool> *cancel, hipStream_t up, hipEvent_t done) {
+bool node_xyz_host(const Mesh &m, MeshGeom &g, const std::atomic<bool> *cancel, hipStream_t up, hipEvent_t done, double *d_out) {
     const index_t n_nodes = m.n_nodes();
     const bool deg = m.coord_system() == CoordinateSystem::SphericalDeg;
     g.h_xyz.resize(3 * static_cast<std::size_t>(n_nodes));
     g.h_xyz_deg.clear();
     if (!deg) g.h_xyz_deg.resize(3 * static_cast<std::size_t>(n_nodes));
     const double *lon = m.h_coords.data(), *lat = m.h_coords.data() + n_nodes;
-    const bool upload = up != nullptr && g.d_xyz.size() >= 3 * static_cast<std::size_t>(n_nodes);
+    const bool upload = up != nullptr && d_out != nullptr;
     std::atomic<int> up_err{0};
     parallel_range(n_nodes, [&](index_t b, index_t e) {
         SinCosMemo memo;
@@ -926,7 +927,7 @@ bool node_xyz_host(const Mesh &m, MeshGeom &g, const std::atomic<bool> *cancel, 
         // this slice to the device now (errors are reported after the join:
         // a worker thread must not throw)
         if (upload && e > b) {
-            const hipError_t r = hipMemcpyAsync(g.d_xyz.get() + 3 * b, g.h_xyz.data() + 3 * b, 3 * sizeof(double) * static_cast<std::size_t>(e - b),
+            const hipError_t r = hipMemcpyAsync(d_out + 3 * b, g.h_xyz.data() + 3 * b, 3 * sizeof(double) * static_cast<std::size_t>(e - b),
                                                 hipMemcpyHostToDevice, up);
             if (r != hipSuccess) up_err.store(static_cast<int>(r));
         }
