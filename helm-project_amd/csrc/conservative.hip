@@ -701,26 +701,33 @@ double key_to_double(unsigned long long k) {
     return d;
 }
 
-bool lon_range(const Mesh &m, double &lo, double &hi, hipStream_t s) {
-    DeviceBuffer<unsigned long long> r(2);
-    unsigned long long init[2] = {~0ull, 0ull};
+// longitude ranges of the vertices of both meshes' cells: one pair of
+// reductions, one download (an empty mesh gives lo = +max, hi = -max)
+void lon_ranges(const Mesh &a, const Mesh &b, double range[4], hipStream_t s) {
+    DeviceBuffer<unsigned long long> r(4);
+    unsigned long long init[4] = {~0ull, 0ull, ~0ull, 0ull};
     AXIS_HIP_CHECK(hipMemcpyAsync(r.get(), init, sizeof init, hipMemcpyHostToDevice, s));
-    if (m.n_cells() > 0) {
+    const Mesh *ms[2] = {&a, &b};
+    for (int k = 0; k < 2; ++k) {
+        const Mesh &m = *ms[k];
+        if (m.n_cells() == 0) continue;
         const unsigned grid = static_cast<unsigned>(std::min<index_t>((m.n_cells() + kRangeBlock - 1) / kRangeBlock, 1024));
-        lon_range_kernel<<<grid, kRangeBlock, 0, s>>>(m.d_offsets.get(), m.d_indices.get(), m.d_coords.get(), m.n_cells(), r.get(), r.get() + 1);
+        lon_range_kernel<<<grid, kRangeBlock, 0, s>>>(m.d_offsets.get(), m.d_indices.get(), m.d_coords.get(), m.n_cells(), r.get() + 2 * k,
+                                                      r.get() + 2 * k + 1);
         AXIS_LAUNCH_CHECK();
     }
-    unsigned long long h[2];
+    unsigned long long h[4];
     AXIS_HIP_CHECK(hipMemcpyAsync(h, r.get(), sizeof h, hipMemcpyDeviceToHost, s));
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
-    if (h[0] == ~0ull) {
-        lo = std::numeric_limits<double>::max();
-        hi = -std::numeric_limits<double>::max();
-        return false;
+    for (int k = 0; k < 2; ++k) {
+        if (h[2 * k] == ~0ull) {
+            range[2 * k] = std::numeric_limits<double>::max();
+            range[2 * k + 1] = -std::numeric_limits<double>::max();
+        } else {
+            range[2 * k] = key_to_double(h[2 * k]);
+            range[2 * k + 1] = key_to_double(h[2 * k + 1]);
+        }
     }
-    lo = key_to_double(h[0]);
-    hi = key_to_double(h[1]);
-    return true;
 }
 
 }  // namespace
@@ -728,9 +735,9 @@ bool lon_range(const Mesh &m, double &lo, double &hi, hipStream_t s) {
 void check_longitude_ranges(const Mesh &src, const Mesh &dst, bool structured_fast_path, hipStream_t s) {
     const auto cs = src.coord_system();
     if (cs != CoordinateSystem::SphericalDeg && cs != CoordinateSystem::SphericalRad) return;
-    double s0, s1, d0, d1;
-    lon_range(src, s0, s1, s);
-    lon_range(dst, d0, d1, s);
+    double range[4];
+    lon_ranges(src, dst, range, s);
+    const double s0 = range[0], s1 = range[1], d0 = range[2], d1 = range[3];
     bool disjoint = (s1 < d0) || (d1 < s0);
     if (!disjoint) return;
     const double period = (cs == CoordinateSystem::SphericalDeg) ? 360.0 : (2.0 * kPi);
