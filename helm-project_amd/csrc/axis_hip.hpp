@@ -314,8 +314,9 @@ bool node_xyz_device_eligible(const Mesh &m, const GridAxes &ax);
 /// Node xyz of an eligible mesh built on the device into g.d_xyz from host
 /// libm sin/cos tables of the grid's axis values (bit-identical to
 /// node_xyz_host); nodes whose coordinates match no axis value bit for bit
-/// are computed on the host.  Returns false (and leaves g.dev_xyz unset) when
-/// the mesh is not eligible or too many nodes need the host.
+/// are computed on the host.  Returns false (and leaves g.dev_xyz unset and
+/// g.d_xyz untouched) when the mesh is not eligible or too many nodes need the
+/// host.  The caller must not have slice uploads into g.d_xyz still to queue.
 bool node_xyz_device(const Mesh &m, const GridAxes &ax, MeshGeom &g, hipStream_t s);
 void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s);
 

@@ -837,8 +837,11 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     if (!spherical) throw InvalidArgument("generate_conservative: Cartesian3D meshes are served by the rectangle paths only here");
     if (cfg.line_type != LineType::GreatCircle) throw InvalidArgument("generate_conservative: only LineType::GreatCircle is supported");
     tr.mark("detect");
-    if (src_dev_xyz) node_xyz_device(src, sax, sg, s);  // (false: host xyz below)
+    // the host task is done with the source (finished, or stopped by
+    // skip_src) before the device build may replace sg.d_xyz, so no slice
+    // upload is queued into a buffer after its release
     src_ready.get();
+    if (src_dev_xyz) node_xyz_device(src, sax, sg, s);  // (false: host xyz below)
     if (!sg.dev_xyz && src_uploaded.load()) {
         AXIS_HIP_CHECK(hipStreamWaitEvent(s, src_up.e, 0));
         sg.d_xyz_ready = true;

@@ -853,13 +853,14 @@ bool node_xyz_device(const Mesh &m, const GridAxes &ax, MeshGeom &g, hipStream_t
     }
     DeviceBuffer<double> d_tab(tab.size());
     d_tab.upload(tab.data(), tab.size(), s);
-    // a fresh buffer: the host path may have queued slice uploads into the old one
-    g.d_xyz = DeviceBuffer<double>(std::max<std::size_t>(3 * static_cast<std::size_t>(n), 3));
+    // a fresh buffer, moved into g.d_xyz only on success: on failure g.d_xyz
+    // keeps whatever the host path uploaded into it
+    DeviceBuffer<double> xyz(std::max<std::size_t>(3 * static_cast<std::size_t>(n), 3));
     DeviceBuffer<index_t> fix(kXyzFixCap);
     DeviceBuffer<int> n_fix(1);
     AXIS_HIP_CHECK(hipMemsetAsync(n_fix.get(), 0, sizeof(int), s));
     if (n > 0) {
-        axis_xyz_kernel<<<grid_for(n, 256), 256, 0, s>>>(m.d_coords.get(), m.d_coords.get() + n, n, d_tab.get(), nlon, nlat, g.d_xyz.get(), fix.get(),
+        axis_xyz_kernel<<<grid_for(n, 256), 256, 0, s>>>(m.d_coords.get(), m.d_coords.get() + n, n, d_tab.get(), nlon, nlat, xyz.get(), fix.get(),
                                                          n_fix.get(), kXyzFixCap);
         AXIS_LAUNCH_CHECK();
     }
@@ -877,10 +878,11 @@ bool node_xyz_device(const Mesh &m, const GridAxes &ax, MeshGeom &g, hipStream_t
         DeviceBuffer<double> d_vals(vals.size());
         d_nodes.upload(nodes.data(), h_fix, s);
         d_vals.upload(vals.data(), vals.size(), s);
-        scatter_xyz_kernel<<<grid_for(h_fix, 256), 256, 0, s>>>(d_nodes.get(), d_vals.get(), h_fix, g.d_xyz.get());
+        scatter_xyz_kernel<<<grid_for(h_fix, 256), 256, 0, s>>>(d_nodes.get(), d_vals.get(), h_fix, xyz.get());
         AXIS_LAUNCH_CHECK();
         AXIS_HIP_CHECK(hipStreamSynchronize(s));
     }
+    g.d_xyz = std::move(xyz);
     g.dev_xyz = true;
     tr.mark("device_xyz");
     return true;
