@@ -50,6 +50,97 @@ class RegridConfig(C.Structure):
         super().__init__(method, norm_type, line_type, unmapped, use_limiter, extrap_method)
 
 
+class _StdString(C.Structure):
+    """A libstdc++ std::string as the C++ side reads it: data pointer, length,
+    then a 16-byte buffer that holds short strings (the pointer then points at
+    it) or the capacity of a longer one."""
+
+    _fields_ = [("p", C.c_void_p), ("n", C.c_size_t), ("buf", C.c_char * 16)]
+
+    def set(self, text: str, keep: list):
+        b = text.encode()
+        self.n = len(b)
+        if len(b) < 16:
+            self.buf = b
+            self.p = C.addressof(self) + _StdString.buf.offset
+        else:
+            mem = C.create_string_buffer(b)
+            keep.append(mem)
+            self.p = C.addressof(mem)
+            C.cast(C.addressof(self) + _StdString.buf.offset, C.POINTER(C.c_size_t))[0] = len(b)
+
+
+class _View1(C.Structure):  # field_view<T, 1>: data handle, extent
+    _fields_ = [("data", C.c_void_p), ("n", C.c_size_t)]
+
+
+class _View2(C.Structure):  # field_view<T, 2> (layout_left): data handle, extents
+    _fields_ = [("data", C.c_void_p), ("n0", C.c_size_t), ("n1", C.c_size_t)]
+
+
+class GridDescriptor(C.Structure):
+    """axis::ingest::GridDescriptor (ingest/grid_descriptor.hpp:37-216) laid
+    out field by field in its declaration order, for
+    axis_mesh_from_descriptor_c.  Build with grid_descriptor()."""
+
+    _fields_ = [
+        ("kind", C.c_uint8), ("coord_system", C.c_uint8),
+        # CfParams
+        ("cf_grid_mapping_name", _StdString), ("cf_earth_radius", C.c_double), ("cf_semi_major", C.c_double),
+        ("cf_semi_minor", C.c_double), ("cf_inverse_flattening", C.c_double),
+        # UgridParams
+        ("ugrid_topology_dimension", C.c_int), ("ugrid_start_index", C.c_int), ("ugrid_mesh_name", _StdString),
+        # GribParams
+        ("grib_grid_type", _StdString), ("grib_ni", C.c_int64), ("grib_nj", C.c_int64), ("grib_gaussian_n", C.c_int64),
+        ("grib_scanning_mode", C.c_uint8), ("grib_earth_shape_code", C.c_int),
+        # ProjectedParams, NamedGridParams
+        ("proj_string", _StdString), ("named_grid_name", _StdString),
+        # GridRulesParams
+        ("rules_kind", _StdString), ("rules_min_x", C.c_double), ("rules_max_x", C.c_double), ("rules_min_y", C.c_double),
+        ("rules_max_y", C.c_double), ("rules_r_x", C.c_double), ("rules_r_y", C.c_double), ("rules_gaussian_n", C.c_int64),
+        ("rules_proj_string", _StdString),
+        # BufferViews
+        ("center_x", _View1), ("center_y", _View1), ("corner_x", _View1), ("corner_y", _View1), ("node_coords", _View2),
+        ("conn_offsets", _View1), ("conn_indices", _View1), ("cell_areas", _View1), ("cell_mask", _View1),
+        ("ni", C.c_size_t), ("nj", C.c_size_t),
+    ]
+
+
+# ConventionKind (grid_descriptor.hpp:37-44)
+DESC_CF, DESC_UGRID, DESC_GRIB, DESC_PROJECTED, DESC_NAMED, DESC_RULES = range(6)
+
+
+def grid_descriptor(kind: int, coord_system: int = 0, strings=None, scalars=None, arrays=None):
+    """A GridDescriptor with the reference's defaults, then the given string
+    fields (e.g. {"rules_kind": "RegularLatLon"}), scalar fields and buffers
+    (numpy arrays: 1-D for the views, [n, ndim] for node_coords).  The arrays
+    are kept alive on the returned object."""
+    d = GridDescriptor()
+    keep = []
+    d.kind, d.coord_system = kind, coord_system
+    d.ugrid_topology_dimension = 2
+    d.rules_min_x, d.rules_max_x, d.rules_min_y, d.rules_max_y, d.rules_r_x, d.rules_r_y = -180.0, 180.0, -90.0, 90.0, 1.0, 1.0
+    for name, _ in GridDescriptor._fields_:
+        if name not in ("kind", "coord_system") and isinstance(getattr(d, name), _StdString):
+            getattr(d, name).set("", keep)
+    for name, text in (strings or {}).items():
+        getattr(d, name).set(text, keep)
+    for name, v in (scalars or {}).items():
+        setattr(d, name, v)
+    for name, a in (arrays or {}).items():
+        if name == "node_coords":
+            a = np.asfortranarray(a, dtype=np.float64)
+            d.node_coords.data, d.node_coords.n0, d.node_coords.n1 = a.ctypes.data, a.shape[0], a.shape[1]
+        else:
+            dt = {"conn_offsets": np.int64, "conn_indices": np.int64, "cell_mask": np.int32}.get(name, np.float64)
+            a = np.ascontiguousarray(a, dtype=dt)
+            v = getattr(d, name)
+            v.data, v.n = a.ctypes.data, a.shape[0]
+        keep.append(a)
+    d._keep = keep
+    return d
+
+
 def lib():
     """Loads the library; raises if it was not built (there is no fallback)."""
     global _lib
@@ -62,6 +153,10 @@ def lib():
         "axis_init_c": [],
         "axis_mesh_create_c": [_D, C.c_int64, C.c_int64, _I64, C.c_int64, _I64, C.c_int, _D, _I32, C.POINTER(C.c_int)],
         "axis_mesh_regular_latlon_c": [C.c_double] * 6 + [C.POINTER(C.c_int)],
+        "axis_mesh_from_descriptor_c": [C.c_void_p, C.POINTER(C.c_int)],
+        "axis_mesh_from_named_c": [C.c_char_p, C.POINTER(C.c_int)],
+        "axis_mesh_arrays_c": [C.c_int, _D, _I64, _I64],
+        "axis_generate_vector_weights_c": [C.c_int, C.c_int, _D, _D, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)],
         "axis_mesh_cell_centroids_c": [C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), _I64],
         "axis_mesh_info_c": [C.c_int, _I64, _I64],
         "axis_mesh_grid_info_c": [C.c_int, C.POINTER(C.c_int), _I64, _D],
@@ -141,7 +236,8 @@ def exported_symbols():
             "axis_proposer_training_data_c", "axis_matrix_serialize_c", "axis_matrix_deserialize_c",
             "axis_matrix_write_esmf_c", "axis_matrix_read_esmf_c", "axis_nan_batch_apply_c", "axis_apply_masked_c",
             "axis_generate_weights_distributed_c", "axis_halo_pattern_info_c", "axis_halo_pattern_arrays_c",
-            "axis_destroy_halo_pattern_c", "axis_apply_distributed_c", "axis_apply_distributed_callback_c"]
+            "axis_destroy_halo_pattern_c", "axis_apply_distributed_c", "axis_apply_distributed_callback_c",
+            "axis_mesh_from_descriptor_c", "axis_mesh_from_named_c", "axis_mesh_arrays_c", "axis_generate_vector_weights_c"]
 
 
 def _check(rc: int):
@@ -201,10 +297,37 @@ class Mesh:
         self.handle = h.value
         return self
 
+    @classmethod
+    def from_descriptor(cls, desc: "GridDescriptor"):
+        """MeshFactory::from_descriptor through axis_mesh_from_descriptor_c."""
+        self = cls.__new__(cls)
+        h = C.c_int(0)
+        _check(lib().axis_mesh_from_descriptor_c(C.addressof(desc), C.byref(h)))
+        self.handle = h.value
+        return self
+
+    @classmethod
+    def from_named(cls, name: str):
+        self = cls.__new__(cls)
+        h = C.c_int(0)
+        _check(lib().axis_mesh_from_named_c(name.encode(), C.byref(h)))
+        self.handle = h.value
+        return self
+
     def info(self):
         nn, nc = C.c_int64(0), C.c_int64(0)
         _check(lib().axis_mesh_info_c(self.handle, C.byref(nn), C.byref(nc)))
         return nn.value, nc.value
+
+    def arrays(self):
+        """(coords [n_nodes, 2], offsets, indices) copied out of the mesh."""
+        nn, nc = self.info()
+        coords = np.zeros(2 * nn)
+        off = np.zeros(nc + 1, dtype=np.int64)
+        _check(lib().axis_mesh_arrays_c(self.handle, coords.ctypes.data_as(_D), off.ctypes.data_as(_I64), None))
+        idx = np.zeros(max(int(off[-1]), 1), dtype=np.int64)
+        _check(lib().axis_mesh_arrays_c(self.handle, None, None, idx.ctypes.data_as(_I64)))
+        return coords.reshape(2, nn).T, off, idx[:int(off[-1])]
 
     def grid_info(self):
         """(kind, (ni, nj), out6): kind 0 unstructured, 1 rectilinear, 2 regular;

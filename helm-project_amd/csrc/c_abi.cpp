@@ -18,6 +18,7 @@
 
 #include "../../include/axis_hip.h"
 #include "axis_hip.hpp"
+#include "grid_descriptor.hpp"
 #include "mlp.hpp"
 #include "proposer.hpp"
 
@@ -172,32 +173,32 @@ int axis_mesh_create_c(const double *node_coords, int64_t n_nodes, int64_t ndim,
 int axis_mesh_regular_latlon_c(double min_x, double max_x, double min_y, double max_y, double r_x, double r_y, int *mesh_handle) {
     return guarded([&] {
         if (!mesh_handle) throw InvalidArgument("null mesh_handle");
-        // validate_bbox / validate_resolution, rule_generator.cpp:110-111
-        if (!(max_x > min_x) || !(max_y > min_y)) throw InvalidArgument("RuleGenerator: invalid bounding box");
-        if (!(r_x > 0.0) || !(r_y > 0.0)) throw InvalidArgument("RuleGenerator: resolution must be positive");
-        const auto ni = static_cast<index_t>(std::floor((max_x - min_x) / r_x));
-        const auto nj = static_cast<index_t>(std::floor((max_y - min_y) / r_y));
-        if (ni == 0 || nj == 0) throw InvalidArgument("RuleGenerator: RegularLatLon resolution produces zero cells");
-        const index_t nn = (ni + 1) * (nj + 1), nc = ni * nj;
-        std::vector<double> coords(2 * nn);
-        for (index_t idx = 0; idx < nn; ++idx) {
-            const index_t i = idx % (ni + 1), j = idx / (ni + 1);
-            coords[idx] = min_x + static_cast<double>(i) * r_x;
-            coords[nn + idx] = min_y + static_cast<double>(j) * r_y;
-        }
-        std::vector<index_t> off(nc + 1), ind(4 * nc);
-        for (index_t c = 0; c < nc; ++c) {
-            const index_t i = c % ni, j = c / ni, st = ni + 1;
-            off[c] = 4 * c;
-            ind[4 * c + 0] = j * st + i;
-            ind[4 * c + 1] = j * st + i + 1;
-            ind[4 * c + 2] = (j + 1) * st + i + 1;
-            ind[4 * c + 3] = (j + 1) * st + i;
-        }
-        off[nc] = 4 * nc;
         check_device();
-        auto m = std::make_shared<Mesh>(coords.data(), nn, 2, off.data(), nc, ind.data(), CoordinateSystem::SphericalDeg, nullptr, nullptr);
-        *mesh_handle = Registry::get().add(std::move(m));
+        *mesh_handle = Registry::get().add(ingest::mesh_regular_latlon(min_x, max_x, min_y, max_y, r_x, r_y));
+    });
+}
+
+int axis_mesh_from_descriptor_c(const void *descriptor_ptr, int *mesh_handle) {
+    return guarded([&] {
+        if (!descriptor_ptr || !mesh_handle) throw InvalidArgument("null argument");
+        check_device();
+        *mesh_handle = Registry::get().add(ingest::mesh_from_descriptor(*static_cast<const ingest::GridDescriptor *>(descriptor_ptr)));
+    });
+}
+
+int axis_mesh_from_named_c(const char *name, int *mesh_handle) {
+    return guarded([&] {
+        if (!name || !mesh_handle) throw InvalidArgument("null argument");
+        throw InvalidArgument(std::string("MeshFactory::from_named: named grids are not built ('") + name + "')");
+    });
+}
+
+int axis_mesh_arrays_c(int mesh_handle, double *node_coords, int64_t *conn_offsets, int64_t *conn_indices) {
+    return guarded([&] {
+        auto m = Registry::get().find<Mesh>(mesh_handle);
+        if (node_coords) std::memcpy(node_coords, m->h_coords.data(), sizeof(double) * m->h_coords.size());
+        if (conn_offsets) std::memcpy(conn_offsets, m->h_offsets.data(), sizeof(int64_t) * m->h_offsets.size());
+        if (conn_indices && !m->h_indices.empty()) std::memcpy(conn_indices, m->h_indices.data(), sizeof(int64_t) * m->h_indices.size());
     });
 }
 
@@ -377,6 +378,17 @@ int axis_generate_weights_c(int src_handle, int dst_handle, int method, int *mat
         default: g_last_error = "unknown method code " + std::to_string(method); return AXIS_ERROR;
     }
     return axis_generate_weights_config_c(src_handle, dst_handle, &cfg, matrix_handle);
+}
+
+int axis_generate_vector_weights_c(int src_handle, int dst_handle, const double *src_alpha, const double *dst_alpha, int method, int *matrix_u_handle,
+                                   int *matrix_v_handle) {
+    return guarded([&] {
+        (void)Registry::get().find<Mesh>(src_handle);
+        (void)Registry::get().find<Mesh>(dst_handle);
+        if (!src_alpha || !dst_alpha || !matrix_u_handle || !matrix_v_handle) throw InvalidArgument("null argument");
+        if (method < 0 || method > 2) throw InvalidArgument("unknown method code " + std::to_string(method));
+        throw InvalidArgument("VectorWeightGenerator::generate: rotated vector weights are not built (scalar weights: axis_generate_weights_c)");
+    });
 }
 
 int axis_matrix_from_coo_c(const int64_t *rows, const int64_t *cols, const double *values, int64_t nnz, int64_t n_src, int64_t n_dst,

@@ -73,6 +73,28 @@ int axis_mesh_create_c(const double *node_coords, int64_t n_nodes, int64_t ndim,
  * (rule_generator.cpp:105-169, reached through MeshFactory::from_rules). */
 int axis_mesh_regular_latlon_c(double min_x, double max_x, double min_y, double max_y, double r_x, double r_y, int *mesh_handle);
 
+/* axis_mesh_from_descriptor_c (axis_c_interop.cpp:105-109): descriptor_ptr is
+ * the address of a C++ axis::ingest::GridDescriptor
+ * (ingest/grid_descriptor.hpp:192-216) filled by the caller and read in place
+ * (MeshFactory::from_descriptor, mesh_factory.cpp:396-431, with its validation
+ * messages).  Served kinds: UGRID, CF and GRIB (StructuredGrid::to_unstructured,
+ * corners synthesized from the centres when absent) and GridRules of kind
+ * RegularLatLon; Projected, NamedGrid and the other rule kinds return
+ * AXIS_ERROR with a message. */
+int axis_mesh_from_descriptor_c(const void *descriptor_ptr, int *mesh_handle);
+
+/* axis_mesh_from_named_c (axis_c_interop.cpp:96-99): the named-grid registry
+ * (named_grid_registry.cpp) is not part of this library; always AXIS_ERROR
+ * with the message "named grids are not built". */
+int axis_mesh_from_named_c(const char *name, int *mesh_handle);
+
+/* node_coords() / conn_offsets() / conn_indices() of UnstructuredMesh
+ * (unstructured_mesh.hpp:113-137): copies into host buffers sized from
+ * axis_mesh_info_c (node_coords 2 * n_nodes doubles, all lon then all lat;
+ * conn_offsets n_cells + 1; conn_indices conn_offsets[n_cells]); any pointer
+ * may be NULL. */
+int axis_mesh_arrays_c(int mesh_handle, double *node_coords, int64_t *conn_offsets, int64_t *conn_indices);
+
 /* Sizes of a mesh (UnstructuredMesh::n_nodes / n_cells, unstructured_mesh.hpp:86-97). */
 int axis_mesh_info_c(int mesh_handle, int64_t *n_nodes, int64_t *n_cells);
 
@@ -128,6 +150,13 @@ int axis_mesh_cell_centroids_c(int mesh_handle, float *lon, float *lat, int64_t 
 /* axis_generate_weights_c (axis_c_interop.cpp:117-145): method 0 Bilinear,
  * 2 Conservative1stOrder with the default RegridConfig otherwise. */
 int axis_generate_weights_c(int src_handle, int dst_handle, int method, int *matrix_handle);
+
+/* axis_generate_vector_weights_c (axis_c_interop.cpp:156-205,
+ * VectorWeightGenerator, vector_regridder.cpp:14): rotated U/V weights are
+ * not part of this library; the handles are checked, then AXIS_ERROR with an
+ * explicit message and the output handles untouched. */
+int axis_generate_vector_weights_c(int src_handle, int dst_handle, const double *src_alpha, const double *dst_alpha, int method, int *matrix_u_handle,
+                                   int *matrix_v_handle);
 
 /* WeightGenerator::generate with a full RegridConfig (weight_generator.hpp:60-62). */
 int axis_generate_weights_config_c(int src_handle, int dst_handle, const axis_regrid_config *config, int *matrix_handle);

@@ -349,3 +349,65 @@ def halo_pattern(cols, src_global_ids, owner_of_src):
     remap = np.arange(n_local, dtype=np.int64)
     remap[off] = n_local + np.arange(off.shape[0])
     return pattern, remap[cols]
+
+
+# ----------------------------------------------------------------- mesh ingest
+def rule_regular_latlon(min_x, max_x, min_y, max_y, r_x, r_y):
+    """RuleGenerator RegularLatLon (rule_generator.cpp:105-169): ni =
+    floor((max_x - min_x) / r_x), nj likewise; node idx = i + j (ni + 1) at
+    (min_x + i r_x, min_y + j r_y); cell c = i + j ni with corners BL, BR, TR,
+    TL.  Returns (coords [n_nodes, 2], offsets, indices)."""
+    ni = int(np.floor((max_x - min_x) / r_x))
+    nj = int(np.floor((max_y - min_y) / r_y))
+    nodes = []
+    for idx in range((ni + 1) * (nj + 1)):
+        i, j = idx % (ni + 1), idx // (ni + 1)
+        nodes.append((min_x + float(i) * r_x, min_y + float(j) * r_y))
+    ind = []
+    for c in range(ni * nj):
+        i, j, st = c % ni, c // ni, ni + 1
+        ind += [j * st + i, j * st + i + 1, (j + 1) * st + i + 1, (j + 1) * st + i]
+    return np.array(nodes, dtype=np.float64).reshape(-1, 2), np.arange(0, 4 * ni * nj + 1, 4, dtype=np.int64), np.array(ind, dtype=np.int64)
+
+
+def structured_to_unstructured(ni, nj, clon, clat, corner_lon=None, corner_lat=None):
+    """StructuredGrid(ni, nj, centres).to_unstructured()
+    (structured_grid.cpp:23-83 constructor, :107-174 corner synthesis,
+    :209-272 conversion): with no corners each corner (ci, cj) is the mean of
+    the centres of the existing cells (ci + di, cj + dj), dj then di over
+    {-1, 0}; a row spanning 350-370 degrees (ni |clon[1] - clon[0]|) wraps, the
+    wrapped cell's longitude moved by -360 (left) or +360 (right).  Node
+    ci + cj (ni + 1); cell i + j ni with corners BL, BR, TR, TL."""
+    clon = np.asarray(clon, dtype=np.float64)
+    clat = np.asarray(clat, dtype=np.float64)
+    if corner_lon is None:
+        periodic = ni > 1 and 350.0 < ni * abs(clon[1] - clon[0]) < 370.0
+        corner_lon = np.zeros((ni + 1) * (nj + 1))
+        corner_lat = np.zeros((ni + 1) * (nj + 1))
+        for cj in range(nj + 1):
+            for ci in range(ni + 1):
+                sx = sy = 0.0
+                n = 0
+                for dj in (-1, 0):
+                    for di in (-1, 0):
+                        i, j, off = ci + di, cj + dj, 0.0
+                        if periodic:
+                            if i < 0:
+                                i, off = ni - 1, -360.0
+                            elif i >= ni:
+                                i, off = 0, 360.0
+                        elif not 0 <= i < ni:
+                            continue
+                        if not 0 <= j < nj:
+                            continue
+                        sx += clon[i + j * ni] + off
+                        sy += clat[i + j * ni]
+                        n += 1
+                corner_lon[ci + cj * (ni + 1)] = sx / n
+                corner_lat[ci + cj * (ni + 1)] = sy / n
+    coords = np.stack([np.asarray(corner_lon, dtype=np.float64), np.asarray(corner_lat, dtype=np.float64)], 1)
+    ind = []
+    for c in range(ni * nj):
+        i, j, st = c % ni, c // ni, ni + 1
+        ind += [i + j * st, i + 1 + j * st, i + 1 + (j + 1) * st, i + (j + 1) * st]
+    return coords, np.arange(0, 4 * ni * nj + 1, 4, dtype=np.int64), np.array(ind, dtype=np.int64)

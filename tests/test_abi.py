@@ -30,6 +30,43 @@ def test_library_exports_header_symbols():
         assert hasattr(lib, n), n
 
 
+# every bind(C, name=...) of the reference's Fortran module
+# (libs/axis/fortran/axis_mod.f90:68-152): an unmodified axis_mod links
+# against this library only if all of them are exported
+FORTRAN_BOUND = ["axis_init_c", "axis_mesh_from_named_c", "axis_mesh_from_descriptor_c", "axis_generate_weights_c",
+                 "axis_generate_vector_weights_c", "axis_apply_c", "axis_destroy_mesh_c", "axis_destroy_matrix_c"]
+
+
+def test_library_exports_fortran_module_symbols():
+    lib = ctypes.CDLL(str(axis.LIB_PATH))
+    for n in FORTRAN_BOUND:
+        assert hasattr(lib, n), n
+    assert set(FORTRAN_BOUND) <= set(header_functions())
+
+
+def test_grid_descriptor_layout():
+    """The ctypes GridDescriptor (used by the tests) has the x86-64 layout of
+    the reference's declaration that the library's C++ mirror asserts:
+    libstdc++ strings of 32 bytes, field views as pointer + extents."""
+    G = axis.GridDescriptor
+    assert ctypes.sizeof(G) == 528
+    assert (G.cf_grid_mapping_name.offset, G.ugrid_topology_dimension.offset, G.grib_grid_type.offset, G.proj_string.offset,
+            G.named_grid_name.offset, G.rules_kind.offset, G.center_x.offset, G.node_coords.offset, G.conn_offsets.offset,
+            G.cell_mask.offset, G.ni.offset) == (8, 72, 112, 176, 208, 240, 360, 424, 448, 496, 512)
+    src = (ROOT / "helm-project_amd" / "csrc" / "grid_descriptor.hpp").read_text()
+    for frag in ("offsetof(GridDescriptor, buffers) == 360", "sizeof(GridDescriptor) == 528", "offsetof(BufferViews, node_coords) == 64",
+                 "offsetof(BufferViews, ni) == 152"):
+        assert frag in src
+
+
+def test_std_string_mirror():
+    d = axis.grid_descriptor(axis.DESC_RULES, strings={"rules_kind": "RegularLatLon", "proj_string": "x" * 40}, arrays={})
+    assert ctypes.string_at(d.rules_kind.p, d.rules_kind.n) == b"RegularLatLon"
+    assert d.rules_kind.p == ctypes.addressof(d) + axis.GridDescriptor.rules_kind.offset + 16  # short string: in-object buffer
+    assert ctypes.string_at(d.proj_string.p, d.proj_string.n) == b"x" * 40
+    assert d.named_grid_name.n == 0 and d.ugrid_topology_dimension == 2 and d.rules_max_x == 180.0
+
+
 def test_binding_covers_header():
     assert sorted(axis.exported_symbols()) == header_functions()
 

@@ -648,3 +648,35 @@ def test_halo_pattern_hand_example():
     # a matrix that reads only owned cells publishes an empty pattern
     pat, new = O.halo_pattern(np.array([0, 1, 1]), gid, owner)
     assert pat["source_ranks"].size == 0 and list(pat["rank_offsets"]) == [0] and list(new) == [0, 1, 1]
+
+
+def test_rule_regular_latlon_hand_worked():
+    """RegularLatLon over a bbox that the resolution does not divide: floor
+    gives 3 x 1 cells (10 / 3 and 5 / 4), nodes at min + i r (the last node
+    stops short of max), corners BL, BR, TR, TL."""
+    c, off, ind = O.rule_regular_latlon(0.0, 10.0, -2.0, 3.0, 3.0, 4.0)
+    assert c.tolist() == [[0.0, -2.0], [3.0, -2.0], [6.0, -2.0], [9.0, -2.0], [0.0, 2.0], [3.0, 2.0], [6.0, 2.0], [9.0, 2.0]]
+    assert off.tolist() == [0, 4, 8, 12]
+    assert ind.tolist() == [0, 1, 5, 4, 1, 2, 6, 5, 2, 3, 7, 6]
+
+
+def test_structured_corner_synthesis_hand_worked():
+    """Corners of a global 90-degree grid of centres (periodic: 4 x 90 = 360):
+    interior corners are midpoints of four centres, the seam corners see the
+    wrapped cell moved by -/+360 (0 and 360 exactly), the top and bottom rows
+    average the one row of cells they touch."""
+    ni, nj = 4, 2
+    clon = np.tile([45.0, 135.0, 225.0, 315.0], nj)
+    clat = np.repeat([-45.0, 45.0], ni)
+    c, off, ind = O.structured_to_unstructured(ni, nj, clon, clat)
+    lon = c[:, 0].reshape(nj + 1, ni + 1)
+    lat = c[:, 1].reshape(nj + 1, ni + 1)
+    assert lon.tolist() == [[0.0, 90.0, 180.0, 270.0, 360.0]] * 3
+    assert lat.tolist() == [[-45.0] * 5, [0.0] * 5, [45.0] * 5]
+    assert ind[:4].tolist() == [0, 1, 6, 5] and off[-1] == 4 * ni * nj
+    # not periodic (3 x 30 = 90 degrees): edge corners take one column only
+    c, _, _ = O.structured_to_unstructured(3, 1, [10.0, 40.0, 70.0], [5.0, 5.0, 5.0])
+    assert c[:4, 0].tolist() == [10.0, 25.0, 55.0, 70.0] and c[:, 1].tolist() == [5.0] * 8
+    # corners given: taken as they are
+    c, _, _ = O.structured_to_unstructured(1, 1, [0.5], [0.5], [0.0, 1.0, 0.0, 1.0], [0.0, 0.0, 1.0, 1.0])
+    assert c.tolist() == [[0.0, 0.0], [1.0, 0.0], [0.0, 1.0], [1.0, 1.0]]
