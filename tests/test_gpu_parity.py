@@ -294,11 +294,17 @@ def test_c384_properties_and_sampled_rows():
         assert np.array_equal(col[a:b], ref["col"][ra:rb]), f"row {r}"
         w = ref["val"][ra:rb]
         assert np.all(np.abs(val[a:b] - w) <= 1e-12 * np.maximum(w, 1e-13)), f"row {r}"
-    # bilinear: exactly four entries per row summing to one
+    # bilinear: exactly four entries per row summing to one, and the sampled
+    # rows bit for bit against the oracle's stencils
     bil = run_pair(src, dst, axis.BILINEAR)
     brp, bcol, bval = bil.csr()
     assert bil.nnz == 4 * grids.n_cells(dst)
     assert np.max(np.abs(np.add.reduceat(bval, brp[:-1]) - 1.0)) < 1e-12
+    bref = O.generate(src, sub, 0)
+    for k, r in enumerate(sample):
+        a, b = brp[r], brp[r + 1]
+        ra, rb = bref["row_ptr"][k], bref["row_ptr"][k + 1]
+        assert np.array_equal(bcol[a:b], bref["col"][ra:rb]) and np.array_equal(bval[a:b], bref["val"][ra:rb]), f"bilinear row {r}"
 
 
 @pytest.fixture(scope="module")
@@ -345,6 +351,80 @@ def test_batch_plan_matches_single_applies(batch_pairs, mode, xdt, autotune):
         for (_, _, Y, _), R in zip(pairs, refs):
             assert torch.equal(Y, R), rep
     plan.close()
+
+
+@pytest.mark.parametrize("mode,xdt,ydt", [(axis.PLAIN, axis.F64, axis.F64), (axis.MIXED, axis.F32, axis.F32),
+                                          (axis.MIXED_ACC64, axis.BF16, axis.F64)])
+def test_batch_plan_against_oracle(batch_pairs, mode, xdt, ydt):
+    """Each pair's output of the multi-pair graph against the oracle's weights
+    and row loop on the decoded inputs: 1e-13 (FP64), 1e-6 (FP32 output) and
+    1e-14 (FP64 accumulation, FP64 output) relative per field."""
+    tdt = {axis.F64: torch.float64, axis.F32: torch.float32, axis.BF16: torch.bfloat16}
+    srcs = [(grids.latlon_deg(2.0), grids.cubed_sphere(24)), (grids.latlon_deg(1.0), grids.cubed_sphere(48)),
+            (grids.latlon_deg(2.0), grids.cubed_sphere(12))]
+    pairs = []
+    for (m, X64, n_src, n_dst, k) in batch_pairs:
+        pairs.append((m, X64.to(tdt[xdt]).contiguous(), torch.full((k, n_dst), float("nan"), dtype=tdt[ydt], device="cuda"), k))
+    plan = axis.BatchPlan(pairs, xdt, ydt, mode, autotune=True)
+    plan.run()
+    tol = {axis.PLAIN: 1e-13, axis.MIXED: 1e-6, axis.MIXED_ACC64: 1e-14}[mode]
+    for (m, X, Y, k), (s_m, d_m) in zip(pairs, srcs):
+        ref = O.generate(s_m, d_m, 2)
+        compare_weights(m, ref, fracs=False)
+        for f in range(k):
+            yo = O.batch_apply(ref["row_ptr"], ref["col"], ref["val"], ref["n_src"], X[f].double().cpu().numpy())
+            err = np.max(np.abs(Y[f].double().cpu().numpy() - yo)) / np.max(np.abs(yo))
+            assert err <= tol, (k, f, err)
+    plan.close()
+
+
+def degenerate_variants():
+    """A 5-degree lat-lon source and a C12 destination, each with cells made
+    degenerate in the three ways DegenerateCellHandler::classify tells apart
+    (degenerate_cell_handler.hpp:115-142): fewer than three distinct
+    vertices, zero area (vertices on one great circle) and a self-intersecting
+    (bow-tie) quad."""
+    src = grids.latlon_deg(5.0)
+    dst = grids.cubed_sphere(12)
+    out = []
+    for mesh, cells in ((src, (100, 1000, 1700)), (dst, (7, 300, 611))):
+        m = {k: np.array(v) for k, v in mesh.items()}
+        coords = np.asarray(m["coords"], dtype=np.float64)
+        idx = m["indices"].copy()
+        off = m["offsets"]
+        a, b, c = cells
+        q = idx[off[a]:off[a + 1]].copy()
+        idx[off[a]:off[a + 1]] = [q[0], q[1], q[1], q[0]]            # two distinct vertices
+        q = idx[off[b]:off[b + 1]].copy()
+        idx[off[b]:off[b + 1]] = [q[0], q[2], q[1], q[3]]            # bow tie
+        n0 = coords.shape[0]
+        lon0 = float(coords[idx[off[c]], 0])
+        extra = np.array([[lon0, 0.0], [lon0 + 1.0, 0.0], [lon0 + 2.0, 0.0], [lon0 + 3.0, 0.0]])
+        coords = np.concatenate([coords, extra])                      # four points on the equator
+        idx[off[c]:off[c + 1]] = [n0, n0 + 1, n0 + 2, n0 + 3]
+        m["coords"] = np.asfortranarray(coords)
+        m["indices"] = idx
+        out.append(m)
+    return out
+
+
+@pytest.mark.parametrize("which", ["src", "dst", "both"])
+def test_degenerate_cells_against_oracle(which):
+    say("case degenerate cells in", which)
+    src0, dst0 = grids.latlon_deg(5.0), grids.cubed_sphere(12)
+    bad_src, bad_dst = degenerate_variants()
+    src = bad_src if which in ("src", "both") else src0
+    dst = bad_dst if which in ("dst", "both") else dst0
+    deg_s, deg_d = O.degenerate(src), O.degenerate(dst)
+    assert sorted(set(deg_s[deg_s > 0]) | set(deg_d[deg_d > 0])) == [1, 2, 3] or which != "both"
+    ref = O.generate(src, dst, 2)
+    m = run_pair(src, dst)
+    compare_weights(m, ref)
+    # degenerate destination cells get no row, degenerate sources no column
+    rp, col, _ = m.csr()
+    for c in np.nonzero(deg_d)[0]:
+        assert rp[c + 1] == rp[c]
+    assert not np.isin(col, np.nonzero(deg_s)[0]).any()
 
 
 def test_batch_plan_keeps_matrices_and_rejects_bad_input(batch_pairs):

@@ -4,6 +4,7 @@ matrix equals the plain method's bit for bit (the containment guard, the
 most cells with fewer candidate pairs than the LBVH; the training labels
 contain every source the reference cap test keeps."""
 import numpy as np
+import oracle_py as O
 import pytest
 
 from helm_project_amd import axis, grids, proposer
@@ -90,3 +91,19 @@ def test_trained_proposer_matches_plain_and_needs_fewer_pairs():
     assert same_matrix(plain, prop)
     assert served > 0.8
     assert cand_prop < cand_plain
+
+
+def test_proposer_against_oracle(small_pair):
+    """The proposer's matrix against the oracle's weights directly (not only
+    against the library's plain method)."""
+    src, dst = small_pair
+    S, D = axis.Mesh(src), axis.Mesh(dst)
+    net = axis.MLP(seed=3, feature_scale=proposer.feature_scale(grids.n_cells(dst)))
+    prop, st = axis.generate_weights_proposed(S, D, net)
+    ref = O.generate(src, dst, 2)
+    rp, col, val = prop.csr()
+    assert np.array_equal(rp, ref["row_ptr"]) and np.array_equal(col, ref["col"])
+    assert np.max(np.abs(val - ref["val"]) / np.abs(ref["val"])) <= 1e-12
+    fr = prop.fractions()
+    for k in ("frac_a", "frac_b", "area_a", "area_b"):
+        assert np.max(np.abs(fr[k] - ref[k]) / np.maximum(np.abs(ref[k]), 1e-13)) <= 1e-12, k
