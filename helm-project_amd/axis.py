@@ -186,6 +186,8 @@ def lib():
         "axis_apply_c": [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int],
         "axis_batch_apply_c": [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_int,
                                C.c_void_p],
+        "axis_batch_apply_noalloc_c": [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int64, C.c_int64, C.c_int64,
+                                       C.c_int, C.c_void_p],
         "axis_batch_apply_async_c": [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int64, C.c_int64, C.c_int64,
                                      C.c_int, C.c_void_p],
         "axis_autotune_apply_c": [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int64, C.c_int, C.c_void_p,
@@ -237,7 +239,8 @@ def exported_symbols():
             "axis_matrix_write_esmf_c", "axis_matrix_read_esmf_c", "axis_nan_batch_apply_c", "axis_apply_masked_c",
             "axis_generate_weights_distributed_c", "axis_halo_pattern_info_c", "axis_halo_pattern_arrays_c",
             "axis_destroy_halo_pattern_c", "axis_apply_distributed_c", "axis_apply_distributed_callback_c",
-            "axis_mesh_from_descriptor_c", "axis_mesh_from_named_c", "axis_mesh_arrays_c", "axis_generate_vector_weights_c"]
+            "axis_mesh_from_descriptor_c", "axis_mesh_from_named_c", "axis_mesh_arrays_c", "axis_generate_vector_weights_c",
+            "axis_batch_apply_noalloc_c"]
 
 
 def _check(rc: int):

@@ -19,6 +19,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include "apply_kernels.hpp"
 #include "axis_hip.hpp"
@@ -428,6 +430,59 @@ void batch_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, F
     A.prepare_apply(s);
     apply_detail::Launch L = apply_detail::choose(A, k, xt, yt, mode);
     apply_detail::launch(A, x, xt, y, yt, k, mode, L, s);
+}
+
+// ------------------------------------------------ apply without allocating
+namespace {
+
+// Long-row partial sums of batch_apply_noalloc: one array per GPU the library
+// runs on, held until the process ends.  Host code must use the per-device
+// address from hipGetSymbolAddress, never the array's name.
+constexpr std::size_t kNoAllocScratchDoubles = std::size_t(1) << 22;  // 32 MiB
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+double *scratch_on_current_device() {
+    static std::mutex mu;
+    static std::map<int, double *> ptrs;
+    int dev = 0;
+    AXIS_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(mu);
+    auto it = ptrs.find(dev);
+    if (it != ptrs.end()) return it->second;
+    double *p = nullptr;
+    AXIS_HIP_CHECK(hipGetSymbolAddress(reinterpret_cast<void **>(&p), HIP_SYMBOL(scratch)));
+    ptrs[dev] = p;
+    return p;
+}
+
+}  // namespace
+
+std::size_t noalloc_scratch_doubles() { return kNoAllocScratchDoubles; }
+
+void batch_apply_noalloc(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t n_src, index_t n_dst, index_t k,
+                         ApplyMode mode, hipStream_t s) {
+    if (n_src != A.n_src) throw InvalidArgument("axis::solver::batch_apply: src.extent(0) (" + std::to_string(n_src) + ") != matrix.n_src() (" +
+                                                std::to_string(A.n_src) + ")");
+    if (n_dst != A.n_dst) throw InvalidArgument("axis::solver::batch_apply: dst.extent(0) (" + std::to_string(n_dst) + ") != matrix.n_dst() (" +
+                                                std::to_string(A.n_dst) + ")");
+    if (k < 0) throw InvalidArgument("batch_apply: negative field count");
+    if (mode == ApplyMode::Plain && (xt != FieldType::F64 || yt != FieldType::F64))
+        throw InvalidArgument("batch_apply: the plain apply takes FP64 fields");
+    if (mode != ApplyMode::Plain && (xt == FieldType::F64 || yt == FieldType::BF16))
+        throw InvalidArgument("batch_apply: the mixed apply takes FP32/BF16 fields and writes FP32/FP64");
+    if (k == 0 || n_dst == 0) return;
+    if (!A.apply_ready)
+        throw InvalidArgument("batch_apply_noalloc: the matrix has no apply layout yet (one axis_batch_apply_c with these field types builds it)");
+    const apply_detail::Launch L = apply_detail::choose(A, k, xt, yt, mode);
+    if (L.tiled && !A.tiles_ready)
+        throw InvalidArgument("batch_apply_noalloc: the matrix has no staged tiles yet (one axis_batch_apply_c with these field types builds them)");
+    const std::size_t need = static_cast<std::size_t>(A.n_slots) * static_cast<std::size_t>(k);
+    if (need > kNoAllocScratchDoubles)
+        throw InvalidArgument("batch_apply_noalloc: " + std::to_string(A.n_slots) + " long-row partial sums x " + std::to_string(k) +
+                              " fields exceed the scratch maximum of " + std::to_string(kNoAllocScratchDoubles) + " doubles");
+    apply_detail::SpecialArgs sa;
+    if (A.n_slots > 0) sa.partials = scratch_on_current_device();
+    apply_detail::launch(A, x, xt, y, yt, k, mode, L, s, sa);
 }
 
 ApplyShape autotune_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t k, ApplyMode mode, hipStream_t s) {

@@ -206,6 +206,15 @@ std::unique_ptr<InterpolationMatrix> matrix_from_coo(const index_t *rows, const 
 void batch_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t n_src, index_t n_dst, index_t k,
                  ApplyMode mode, hipStream_t s);
 
+/// batch_apply that allocates nothing: the matrix's apply layout (and, for a
+/// staged launch, its tiles) must already exist, and long-row partial sums go
+/// to a fixed per-device array of noalloc_scratch_doubles() doubles; larger
+/// requests are rejected.  Calls that overlap on one GPU share that array and
+/// race.  Enqueues only (no synchronisation, no memset).
+void batch_apply_noalloc(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t n_src, index_t n_dst, index_t k,
+                         ApplyMode mode, hipStream_t s);
+std::size_t noalloc_scratch_doubles();
+
 ConservationReport check_conservation(const InterpolationMatrix &A, const double *d_src, const double *d_dst, NormType norm, hipStream_t s);
 
 /// Launch shape of the apply: fields per step, field chunks (more chunks =

@@ -518,6 +518,14 @@ int axis_batch_apply_async_c(int matrix_handle, const void *src, int src_dtype, 
     });
 }
 
+int axis_batch_apply_noalloc_c(int matrix_handle, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n_src, int64_t n_dst,
+                               int64_t n_fields, int mode, void *stream) {
+    return guarded([&] {
+        auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
+        batch_apply_noalloc(*M, src, ftype(src_dtype), dst, ftype(dst_dtype), n_src, n_dst, n_fields, amode(mode), static_cast<hipStream_t>(stream));
+    });
+}
+
 int axis_batch_apply_c(int matrix_handle, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n_src, int64_t n_dst,
                        int64_t n_fields, int mode, void *stream) {
     return guarded([&] {

@@ -204,6 +204,21 @@ int axis_batch_apply_c(int matrix_handle, const void *src, int src_dtype, void *
 int axis_batch_apply_async_c(int matrix_handle, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n_src, int64_t n_dst,
                              int64_t n_fields, int mode, void *stream);
 
+/* Batched apply that allocates nothing (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as axis_batch_apply_async_c (enqueue only: no synchronisation, no
+ * memset), with two limits: the matrix's apply layout must already exist (it
+ * is built by the first axis_batch_apply_c / axis_batch_apply_async_c /
+ * axis_autotune_apply_c of the matrix with the same field types and count;
+ * otherwise AXIS_ERROR), and the long-row partial sums (long-row segments x
+ * n_fields doubles) must fit a fixed per-GPU array of 4,194,304 doubles
+ * (32 MiB, held until the process ends; larger requests return AXIS_ERROR).
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * race. */
+int axis_batch_apply_noalloc_c(int matrix_handle, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n_src, int64_t n_dst,
+                               int64_t n_fields, int mode, void *stream);
+
 /* Autotuner (no reference counterpart): times the launch shapes of
  * axis_batch_apply_c (fields per step x field chunks x LDS-staged tiles or
  * direct gathers) on these device fields, caches the fastest for this
