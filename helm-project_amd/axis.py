@@ -186,6 +186,7 @@ def lib():
         "axis_apply_c": [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int],
         "axis_batch_apply_c": [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_int,
                                C.c_void_p],
+        "axis_apply_set_shape_c": [C.c_int, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
         "axis_batch_apply_noalloc_c": [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int64, C.c_int64, C.c_int64,
                                        C.c_int, C.c_void_p],
         "axis_batch_apply_async_c": [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int64, C.c_int64, C.c_int64,
@@ -240,7 +241,7 @@ def exported_symbols():
             "axis_generate_weights_distributed_c", "axis_halo_pattern_info_c", "axis_halo_pattern_arrays_c",
             "axis_destroy_halo_pattern_c", "axis_apply_distributed_c", "axis_apply_distributed_callback_c",
             "axis_mesh_from_descriptor_c", "axis_mesh_from_named_c", "axis_mesh_arrays_c", "axis_generate_vector_weights_c",
-            "axis_batch_apply_noalloc_c"]
+            "axis_batch_apply_noalloc_c", "axis_apply_set_shape_c"]
 
 
 def _check(rc: int):
@@ -502,11 +503,17 @@ class Matrix:
         return dst
 
     def autotune(self, src, dst, n_fields, src_dtype, dst_dtype, mode, stream=None):
-        """Times the launch shapes on these fields; returns (fields per step, field chunks, staged)."""
+        """Times the launch shapes on these fields; returns (fields per step, field chunks, staging:
+        0 direct gathers, 1 workgroup tiles, 2 wave tiles)."""
         t, r, st = C.c_int(0), C.c_int(0), C.c_int(0)
         _check(lib().axis_autotune_apply_c(self.handle, _ptr(src), src_dtype, _ptr(dst), dst_dtype, n_fields, mode,
                                            _stream(stream), C.byref(t), C.byref(r), C.byref(st)))
-        return t.value, r.value, bool(st.value)
+        return t.value, r.value, st.value
+
+    def set_shape(self, n_fields, src_dtype, dst_dtype, mode, fields_per_step, field_chunks, staged):
+        """Pins the launch shape of batch_apply for these field types and count
+        (staged: 0 direct gathers, 1 workgroup tiles, 2 wave tiles)."""
+        _check(lib().axis_apply_set_shape_c(self.handle, n_fields, src_dtype, dst_dtype, mode, fields_per_step, field_chunks, int(staged)))
 
     def check_conservation(self, src, dst, norm_type=0):
         out = np.zeros(4)

@@ -263,7 +263,125 @@ __global__ void __launch_bounds__(apply_detail::kTileSeg) tile_build_kernel(cons
     if (tid == 0) tile_nu[t] = total;
 }
 
+// Wave tile t = segments [64 t, 64 t + 64): one wave sorts the aligned
+// chunks (CW columns each) its nonzeros touch, keeps the distinct ones (at
+// most kWaveChunks, else wnu[t] = -1) and records each nonzero's element
+// position in the staged chunks (CW * chunk position + column % CW).
+template <int E, int CW>
+__global__ void __launch_bounds__(apply_detail::kWaveSeg) wave_tile_build_kernel(const index_t *seg_begin, const std::int32_t *seg_len, index_t n_seg,
+                                                                                 const std::int32_t *col32, std::int32_t *wnu, std::int32_t *wcol,
+                                                                                 std::uint16_t *lpos) {
+    constexpr int kB = apply_detail::kWaveSeg;
+    constexpr int SZ = pow2_at_least(kB * E);
+    constexpr int C = SZ / kB;
+    constexpr int kEmpty = 0x7FFFFFFF;
+    __shared__ int key[SZ];
+    __shared__ int uniq[apply_detail::kWaveChunks];
+    const index_t t = blockIdx.x;
+    const int tid = static_cast<int>(threadIdx.x);
+    const index_t i = t * kB + tid;
+    const int n = i < n_seg ? seg_len[i] : 0;
+    const index_t b = i < n_seg ? seg_begin[i] : 0;
+    for (int q = tid; q < SZ; q += kB) key[q] = kEmpty;
+    __syncthreads();
+    for (int e = 0; e < n; ++e) key[tid * E + e] = col32[b + e] / CW;
+    __syncthreads();
+    for (int kk = 2; kk <= SZ; kk <<= 1) {
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            for (int q = tid; q < SZ; q += kB) {
+                const int p = q ^ j;
+                if (p > q) {
+                    const int x = key[q], y = key[p];
+                    const bool up = (q & kk) == 0;
+                    if ((x > y) == up) {
+                        key[q] = y;
+                        key[p] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    int cnt = 0;
+    for (int q = tid * C; q < tid * C + C; ++q) {
+        const int v = key[q];
+        if (v != kEmpty && (q == 0 || key[q - 1] != v)) ++cnt;
+    }
+    // exclusive scan over the wave
+    int x = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (tid >= d) x += y;
+    }
+    const int total = __shfl(x, 63, 64);
+    int pos = x - cnt;
+    if (total > apply_detail::kWaveChunks) {
+        if (tid == 0) wnu[t] = -1;
+        return;  // whole wave
+    }
+    for (int q = tid * C; q < tid * C + C; ++q) {
+        const int v = key[q];
+        if (v != kEmpty && (q == 0 || key[q - 1] != v)) {
+            uniq[pos] = v;
+            wcol[t * apply_detail::kWaveChunks + pos] = v * CW;
+            ++pos;
+        }
+    }
+    __syncthreads();
+    for (int e = 0; e < n; ++e) {
+        const int col = col32[b + e];
+        const int c = col / CW;
+        int lo = 0, hi = total;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (uniq[mid] < c)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        lpos[b + e] = static_cast<std::uint16_t>(lo * CW + col % CW);
+    }
+    if (tid == 0) wnu[t] = total;
+}
+
+template <int E>
+void launch_wave_build(int cw, unsigned g, const InterpolationMatrix &A, std::int32_t *wnu, std::int32_t *wcol, std::uint16_t *lpos, hipStream_t s) {
+    constexpr int B = apply_detail::kWaveSeg;
+    if (cw == 2)
+        wave_tile_build_kernel<E, 2><<<g, B, 0, s>>>(A.seg_begin.get(), A.seg_len.get(), A.n_seg, A.col32.get(), wnu, wcol, lpos);
+    else if (cw == 4)
+        wave_tile_build_kernel<E, 4><<<g, B, 0, s>>>(A.seg_begin.get(), A.seg_len.get(), A.n_seg, A.col32.get(), wnu, wcol, lpos);
+    else
+        wave_tile_build_kernel<E, 8><<<g, B, 0, s>>>(A.seg_begin.get(), A.seg_len.get(), A.n_seg, A.col32.get(), wnu, wcol, lpos);
+    AXIS_LAUNCH_CHECK();
+}
+
 }  // namespace
+
+void InterpolationMatrix::prepare_wave_tiles(int cw, hipStream_t s) {
+    const int ws = apply_detail::width_slot(cw);
+    if (wave_ready[ws]) return;
+    prepare_apply(s);
+    PhaseTrace tr("prepare_wave_tiles", s);
+    n_wtiles = (n_seg + apply_detail::kWaveSeg - 1) / apply_detail::kWaveSeg;
+    const std::size_t nw = static_cast<std::size_t>(std::max<index_t>(n_wtiles, 1));
+    wnu[ws].resize(nw);
+    wcol[ws].resize(nw * apply_detail::kWaveChunks);
+    wlpos[ws].resize(static_cast<std::size_t>(std::max<index_t>(nnz, 1)));
+    if (n_wtiles > 0) {
+        const unsigned g = static_cast<unsigned>(n_wtiles);
+        switch (seg_cap) {
+            case 4: launch_wave_build<4>(cw, g, *this, wnu[ws].get(), wcol[ws].get(), wlpos[ws].get(), s); break;
+            case 8: launch_wave_build<8>(cw, g, *this, wnu[ws].get(), wcol[ws].get(), wlpos[ws].get(), s); break;
+            case 12: launch_wave_build<12>(cw, g, *this, wnu[ws].get(), wcol[ws].get(), wlpos[ws].get(), s); break;
+            default: launch_wave_build<16>(cw, g, *this, wnu[ws].get(), wcol[ws].get(), wlpos[ws].get(), s); break;
+        }
+    }
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    tr.mark("wave_tiles");
+    wave_ready[ws] = true;
+}
 
 int InterpolationMatrix::row_length_hist(hipStream_t s) {
     constexpr int kH = apply_detail::kMaxE + 2;
@@ -359,6 +477,7 @@ void InterpolationMatrix::prepare_apply(hipStream_t s) {
     tr.mark("segments");
     n_tiles = (n_seg + apply_detail::kTileSeg - 1) / apply_detail::kTileSeg;
     tiles_ready = false;
+    for (bool &w : wave_ready) w = false;
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
     apply_ready = true;
 }
@@ -404,6 +523,15 @@ void batch_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, F
     if (mode != ApplyMode::Plain && (xt == FieldType::F64 || yt == FieldType::BF16))
         throw InvalidArgument("batch_apply: the mixed apply takes FP32/BF16 fields and writes FP32/FP64");
     if (k == 0 || n_dst == 0) return;
+    std::lock_guard<std::mutex> lock(A.apply_mu);
+    if (apply_detail::capturing(s)) {
+        // recorded into a graph: nothing may be built, allocated or synchronised
+        const apply_detail::Launch L = apply_detail::choose(A, k, xt, yt, mode);
+        if (!apply_detail::layout_ready(A, L, xt, x))
+            throw InvalidArgument("batch_apply: a stream under graph capture needs the matrix's apply layout from an earlier apply with these field types");
+        apply_detail::launch(A, x, xt, y, yt, k, mode, L, s);
+        return;
+    }
     if (mode == ApplyMode::Plain && !A.apply_ready && k <= kDirectMaxFields && n_dst < (index_t(1) << 31)) {
         // a few fields through a matrix without an apply layout: the direct
         // kernels give the segmented apply's bits without building the layout
@@ -471,10 +599,11 @@ void batch_apply_noalloc(InterpolationMatrix &A, const void *x, FieldType xt, vo
     if (mode != ApplyMode::Plain && (xt == FieldType::F64 || yt == FieldType::BF16))
         throw InvalidArgument("batch_apply: the mixed apply takes FP32/BF16 fields and writes FP32/FP64");
     if (k == 0 || n_dst == 0) return;
+    std::lock_guard<std::mutex> lock(A.apply_mu);
     if (!A.apply_ready)
         throw InvalidArgument("batch_apply_noalloc: the matrix has no apply layout yet (one axis_batch_apply_c with these field types builds it)");
     const apply_detail::Launch L = apply_detail::choose(A, k, xt, yt, mode);
-    if (L.tiled && !A.tiles_ready)
+    if (!apply_detail::layout_ready(A, L, xt, x))
         throw InvalidArgument("batch_apply_noalloc: the matrix has no staged tiles yet (one axis_batch_apply_c with these field types builds them)");
     const std::size_t need = static_cast<std::size_t>(A.n_slots) * static_cast<std::size_t>(k);
     if (need > kNoAllocScratchDoubles)
@@ -485,15 +614,30 @@ void batch_apply_noalloc(InterpolationMatrix &A, const void *x, FieldType xt, vo
     apply_detail::launch(A, x, xt, y, yt, k, mode, L, s, sa);
 }
 
+void set_apply_shape(InterpolationMatrix &A, index_t k, FieldType xt, FieldType yt, ApplyMode mode, const ApplyShape &shape) {
+    if (k <= 0) throw InvalidArgument("set_apply_shape: no fields");
+    if (shape.fields_per_step != 1 && shape.fields_per_step != 2 && shape.fields_per_step != 4)
+        throw InvalidArgument("set_apply_shape: fields per step must be 1, 2 or 4");
+    if (shape.field_chunks < 1 || shape.field_chunks > 1024) throw InvalidArgument("set_apply_shape: field chunks must be in [1, 1024]");
+    apply_detail::Launch L;
+    L.fu = shape.fields_per_step;
+    L.chunks = shape.field_chunks;
+    L.tiled = shape.staged && !shape.wave;
+    L.wave = shape.wave;
+    std::lock_guard<std::mutex> g(apply_detail::tune_mutex());
+    apply_detail::tune_cache()[apply_detail::TuneKey{A.n_dst, A.nnz, k, static_cast<int>(xt), static_cast<int>(yt), static_cast<int>(mode)}] = L;
+}
+
 ApplyShape autotune_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t k, ApplyMode mode, hipStream_t s) {
     if (k <= 0) throw InvalidArgument("autotune_apply: no fields");
+    std::lock_guard<std::mutex> lock(A.apply_mu);
     A.prepare_apply(s);
     hipEvent_t e0, e1;
     AXIS_HIP_CHECK(hipEventCreate(&e0));
     AXIS_HIP_CHECK(hipEventCreate(&e1));
     apply_detail::Launch best = apply_detail::heuristic(A, k, mode);
     float best_ms = 1e30f;
-    for (bool tiled : {false, true})
+    for (int kind : {0, 1, 2})  // direct gathers, staged tiles, wave tiles
     for (int fu : {1, 2, 4}) {
         if (fu > 1 && fu / 2 >= k) continue;
         for (int chunks : {1, 2, 4, 8, 16, 32, 64}) {
@@ -501,7 +645,8 @@ ApplyShape autotune_apply(InterpolationMatrix &A, const void *x, FieldType xt, v
             apply_detail::Launch L;
             L.fu = fu;
             L.chunks = chunks;
-            L.tiled = tiled;
+            L.tiled = kind == 1;
+            L.wave = kind == 2;
             apply_detail::launch(A, x, xt, y, yt, k, mode, L, s);  // warm
             AXIS_HIP_CHECK(hipEventRecord(e0, s));
             const int reps = 3;
@@ -526,7 +671,7 @@ ApplyShape autotune_apply(InterpolationMatrix &A, const void *x, FieldType xt, v
     // leave the output of the chosen shape in y
     apply_detail::launch(A, x, xt, y, yt, k, mode, best, s);
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
-    return ApplyShape{best.fu, best.chunks, best.tiled};
+    return ApplyShape{best.fu, best.chunks, best.tiled || best.wave, best.wave};
 }
 
 // ------------------------------------------------------ conservation check
@@ -626,6 +771,7 @@ void nan_batch_apply(InterpolationMatrix &A, const double *x, double *y, index_t
     if (n_dst != A.n_dst) throw InvalidArgument("axis::solver::nan_batch_apply: dst.extent(0) != matrix.n_dst()");
     if (k < 0) throw InvalidArgument("axis::solver::nan_batch_apply: src.extent(1) != dst.extent(1)");
     if (k == 0 || n_dst == 0) return;
+    std::lock_guard<std::mutex> lock(A.apply_mu);
     const apply_detail::Launch L = special_launch(A, k, s);
     apply_detail::SpecialArgs sa;
     sa.thres = na_thres;
@@ -643,6 +789,7 @@ void masked_apply(InterpolationMatrix &A, const double *x, double *y, const std:
                               std::to_string(A.n_dst) + ")");
     if (k < 0) throw InvalidArgument("apply (masked): negative field count");
     if (k == 0 || n_dst == 0) return;
+    std::lock_guard<std::mutex> lock(A.apply_mu);
     const apply_detail::Launch L = special_launch(A, k, s);
     apply_detail::SpecialArgs sa;
     sa.mask = (dst_mask && n_mask == n_dst) ? dst_mask : nullptr;  // apply.hpp:464

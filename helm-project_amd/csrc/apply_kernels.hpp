@@ -67,7 +67,25 @@ struct Launch {
     int fu = 4;
     int chunks = 1;
     bool tiled = false;  // stage each tile's source values in LDS
+    bool wave = false;   // wave tiles: each wave stages its own 64 segments' source chunks (no workgroup barriers)
 };
+
+// Wave tiles: the segments [64 w, 64 w + 64) of the processing order form wave
+// tile w.  Its distinct aligned 16-byte chunks of a field (kWaveNQ per lane at
+// most) are listed by first column in wcol[w * kWaveChunks ...]; per nonzero,
+// wlpos holds its column's element position in the staged chunks.  The chunk
+// width depends on the field type (16 bytes: 2 FP64, 4 FP32, 8 BF16 columns),
+// so the layout is built per width.  wnu[w] = -1: the wave gathers directly.
+constexpr int kWaveSeg = 64;
+constexpr int kWaveNQ = 2;
+constexpr int kWaveChunks = kWaveSeg * kWaveNQ;
+constexpr int kWaveStage = kWaveChunks * 16;  // bytes of one field's staged chunks
+template <class XT>
+constexpr int chunk_cols() {
+    return 16 / static_cast<int>(sizeof(XT));
+}
+/// index of a chunk width (2, 4, 8 columns) in the per-width layouts
+constexpr int width_slot(int cw) { return cw == 2 ? 0 : (cw == 4 ? 1 : 2); }
 
 __device__ inline float ld_f32(const float *p) { return *p; }
 __device__ inline float ld_f32(const std::uint16_t *p) { return __uint_as_float(static_cast<unsigned>(*p) << 16); }
@@ -119,6 +137,10 @@ struct Mat {
     // destination mask (nullptr: none)
     double thres;
     const std::int32_t *mask;
+    // wave tiles of the launch's field type (see kWaveSeg)
+    const std::int32_t *wnu, *wcol;
+    const std::uint16_t *wlpos;
+    index_t n_wt;
 };
 
 /// Hardware places workgroup b on XCD b % 8; give XCD x a contiguous range of
@@ -512,6 +534,153 @@ __global__ void __launch_bounds__(kTileSeg) spmm_tile_kernel(Mat A, const XT *X,
     tile_apply<E, FU, NUM, XT, YT>(A, X, Y, P, t, f0, f1, S);
 }
 
+/// A staged field value at byte offset o, in the accumulation type.
+template <class XT, class V>
+__device__ inline V lds_val(const char *base, unsigned o) {
+    if constexpr (std::is_same_v<XT, std::uint16_t>)
+        return static_cast<V>(__uint_as_float(static_cast<unsigned>(*reinterpret_cast<const std::uint16_t *>(base + o)) << 16));
+    else
+        return static_cast<V>(*reinterpret_cast<const XT *>(base + o));
+}
+
+/// Orders this wave's LDS accesses in program order for the compiler (the
+/// hardware executes one wave's LDS instructions in order, so a lane's read
+/// after another lane's write in program order sees the write).
+__device__ inline void wave_lds_order() { asm volatile("" ::: "memory"); }
+
+/// Wave-tile apply: each wave owns 64 consecutive segments and runs all its
+/// fields [f0, f1) without workgroup barriers.  Per step of FU fields the
+/// wave's distinct 16-byte source chunks (one or two per lane) go from
+/// registers into the wave's own LDS area, then every lane forms its
+/// segment's sums from LDS; the chunks of the fields kDepth steps ahead are
+/// loaded into registers meanwhile.  Padding terms (past a segment's length,
+/// up to the wave's longest) read a zero slot with zero weights.  The sums
+/// of every row are those of the other apply kernels (CSR order per segment,
+/// long-row partials reduced in segment order), so results are identical.
+template <int E, int FU, int NUM, class XT, class YT>
+__global__ void __launch_bounds__(256) spmm_wave_kernel(Mat A, const XT *X, YT *Y, double *P, index_t k, unsigned n_wt, int chunks) {
+    using V = typename NumTraits<NUM>::V;
+    using W = typename NumTraits<NUM>::W;
+    constexpr int kDepth = FU >= 4 ? 1 : 4 / FU;
+    constexpr index_t kAhead = static_cast<index_t>(kDepth) * FU;
+    // field u's staged chunks at u * kStride, followed by 16 zero bytes (the
+    // slot padding terms read)
+    constexpr int kStride = kWaveStage + 16;
+    constexpr unsigned kZero = kWaveStage;
+    __shared__ __align__(16) char lds[4][FU * kStride];
+    const int lane = static_cast<int>(threadIdx.x) & 63, wv = static_cast<int>(threadIdx.x) >> 6;
+    const unsigned wid = xcd_remap(blockIdx.x, gridDim.x) * 4u + static_cast<unsigned>(wv);
+    // chunk-major: the waves running at one time share their fields, whose
+    // source lines then come from L2 (tile-major order measured slower)
+    const unsigned t = wid % n_wt, ch = wid / n_wt;
+    if (ch >= static_cast<unsigned>(chunks)) return;  // whole wave (no workgroup barriers in this kernel)
+    const index_t per = (k + chunks - 1) / chunks;
+    const index_t f0 = static_cast<index_t>(ch) * per;
+    const index_t f1 = f0 + per < k ? f0 + per : k;
+    if (f0 >= f1) return;
+    const index_t i = static_cast<index_t>(t) * kWaveSeg + lane;
+    const bool valid = i < A.n_seg;
+    const int nu = A.wnu[t];
+    if (nu < 0) {  // too many distinct chunks: direct gathers
+        if (valid) segment_apply<E, 1, NUM>(A, X, Y, P, i, f0, f1);
+        return;
+    }
+    index_t r = 0, b = 0;
+    int n = 0, slot = -1;
+    if (valid) {
+        r = A.seg_row[i];
+        b = A.seg_begin[i];
+        n = A.seg_len[i];
+        slot = A.seg_slot[i];
+    }
+    int n_max = n;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) n_max = max(n_max, __shfl_xor(n_max, d, 64));
+    const int n_wave = __builtin_amdgcn_readfirstlane(n_max);
+    char *stg = lds[wv];
+    if (lane < 4 * FU) *reinterpret_cast<unsigned *>(stg + (lane >> 2) * kStride + kZero + 4 * (lane & 3)) = 0u;
+    unsigned lc[E];
+    W h[E], l[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        lc[e] = kZero;
+        h[e] = W(0);
+        l[e] = W(0);
+        if (e < n) {
+            lc[e] = static_cast<unsigned>(A.wlpos[b + e]) * static_cast<unsigned>(sizeof(XT));
+            load_w<NUM>(A, b + e, h[e], l[e]);
+        }
+    }
+    // this lane's chunks (lanes without one load the field's first chunk, whose
+    // slot no nonzero reads)
+    const std::int32_t *wc = A.wcol + static_cast<index_t>(t) * kWaveChunks;
+    const bool two = nu > 64;  // wave-uniform
+    const XT *xa = X + (lane < nu ? wc[lane] : 0);
+    const XT *xb = X + (two && lane + 64 < nu ? wc[lane + 64] : 0);
+    using u32x4 = unsigned __attribute__((ext_vector_type(4)));
+    u32x4 va[kDepth][FU], vb[kDepth][FU];
+#pragma unroll
+    for (int d = 0; d < kDepth; ++d)
+#pragma unroll
+        for (int u = 0; u < FU; ++u) {
+            const index_t fu = min(f0 + d * FU + u, f1 - 1);
+            va[d][u] = *reinterpret_cast<const u32x4 *>(xa + fu * A.n_src);
+            vb[d][u] = two ? *reinterpret_cast<const u32x4 *>(xb + fu * A.n_src) : u32x4{0u, 0u, 0u, 0u};
+        }
+    for (index_t fb = f0; fb < f1; fb += kAhead) {
+#pragma unroll
+        for (int d = 0; d < kDepth; ++d) {
+            const index_t f = fb + d * FU;
+            if (f < f1) {  // wave-uniform
+                wave_lds_order();
+#pragma unroll
+                for (int u = 0; u < FU; ++u) {
+                    *reinterpret_cast<u32x4 *>(stg + u * kStride + 16 * lane) = va[d][u];
+                    if (two) *reinterpret_cast<u32x4 *>(stg + u * kStride + 16 * (lane + 64)) = vb[d][u];
+                }
+                wave_lds_order();
+                if (f + kAhead < f1) {
+#pragma unroll
+                    for (int u = 0; u < FU; ++u) {
+                        const index_t fu = min(f + kAhead + u, f1 - 1);
+                        va[d][u] = *reinterpret_cast<const u32x4 *>(xa + fu * A.n_src);
+                        if (two) vb[d][u] = *reinterpret_cast<const u32x4 *>(xb + fu * A.n_src);
+                    }
+                }
+                RowAcc<NUM> acc[FU];
+#pragma unroll
+                for (int u = 0; u < FU; ++u) begin_row(acc[u], A, h, n);
+#pragma unroll
+                for (int g = 0; g < E; g += 4) {
+                    if (g < n_wave) {
+                        V xv[FU][4];
+#pragma unroll
+                        for (int u = 0; u < FU; ++u)
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) xv[u][q] = lds_val<XT, V>(stg + u * kStride, lc[g + q]);
+#pragma unroll
+                        for (int u = 0; u < FU; ++u)
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) acc[u].add(h[g + q], l[g + q], xv[u][q]);
+                    }
+                }
+                if (valid) {
+#pragma unroll
+                    for (int u = 0; u < FU; ++u) {
+                        if (f + u < f1) {
+                            const double val = acc[u].value();
+                            if (slot < 0)
+                                st(Y + (f + u) * A.n_dst + r, val);
+                            else
+                                P[(f + u) * A.n_slots + slot] = val;
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
 /// Y of the long rows: the partials of each row summed in segment order.
 template <class YT>
 __device__ inline void long_reduce(const Mat &A, const double *P, YT *Y, index_t k, index_t t) {
@@ -585,16 +754,41 @@ inline Mat mat_of(const InterpolationMatrix &A) {
     M.n_tiles = A.n_tiles;
     M.thres = 0.0;
     M.mask = nullptr;
+    M.wnu = nullptr;
+    M.wcol = nullptr;
+    M.wlpos = nullptr;
+    M.n_wt = A.n_wtiles;
     return M;
 }
 
-/// FP64 partial-sum workspace for k fields, owned by the matrix (applies of
-/// one matrix that overlap on different streams share it and must not overlap).
-inline double *partials(InterpolationMatrix &A, index_t k) {
+/// Whether the wave-tile kernel can serve fields of XT: chunks are read with
+/// 16-byte loads, so the fields must be 16-byte aligned and hold whole chunks.
+template <class XT>
+inline bool wave_ok(const InterpolationMatrix &A, const void *x) {
+    return A.n_src > 0 && A.n_src % chunk_cols<XT>() == 0 && (reinterpret_cast<std::uintptr_t>(x) % 16) == 0;
+}
+
+/// Whether work enqueued on s is being recorded into a graph: nothing may be
+/// allocated, synchronised or memset then.
+inline bool capturing(hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    AXIS_HIP_CHECK(hipStreamIsCapturing(s, &st));
+    return st != hipStreamCaptureStatusNone;
+}
+
+/// FP64 partial-sum workspace for k fields of applies on stream s, owned by
+/// the matrix (one per stream; the caller holds A.apply_mu).  A stream under
+/// graph capture must find it allocated by an earlier apply on that stream.
+inline double *partials(InterpolationMatrix &A, index_t k, hipStream_t s) {
     if (A.n_slots == 0) return nullptr;
     const std::size_t need = static_cast<std::size_t>(A.n_slots) * static_cast<std::size_t>(k);
-    if (A.partial_ws.size() < need) A.partial_ws.resize(need);
-    return A.partial_ws.get();
+    DeviceBuffer<double> &ws = A.partial_ws[s];
+    if (ws.size() < need) {
+        if (capturing(s))
+            throw InvalidArgument("apply: a stream under graph capture needs its long-row workspace from an earlier apply of as many fields on it");
+        ws.resize(need);
+    }
+    return ws.get();
 }
 
 /// Extra operands of the special modes.
@@ -605,15 +799,26 @@ struct SpecialArgs {
 };
 
 template <int E, int FU, int NUM, class XT, class YT>
-void launch_e(InterpolationMatrix &A, const void *x, void *y, index_t k, int chunks, bool tiled, hipStream_t s, const SpecialArgs &sa) {
+void launch_e(InterpolationMatrix &A, const void *x, void *y, index_t k, int chunks, bool tiled, bool wave, hipStream_t s, const SpecialArgs &sa) {
     Mat M = mat_of(A);
     M.thres = sa.thres;
     M.mask = sa.mask;
-    double *P = sa.partials ? sa.partials : partials(A, k);
+    double *P = sa.partials ? sa.partials : partials(A, k, s);
     const unsigned nb = static_cast<unsigned>(std::max<index_t>(tiled ? A.n_tiles : (A.n_seg + kBlock - 1) / kBlock, 1));
     const std::uint64_t nwork = static_cast<std::uint64_t>(nb) * chunks;
     if (nwork >= (1ull << 31)) throw InvalidArgument("apply: launch too large");
-    if (A.n_seg > 0) {
+    if (wave && A.n_seg > 0) {
+        const int ws = width_slot(chunk_cols<XT>());
+        if (!A.wave_ready[ws]) throw InvalidArgument("apply: wave tiles not built");
+        M.wnu = A.wnu[ws].get();
+        M.wcol = A.wcol[ws].get();
+        M.wlpos = A.wlpos[ws].get();
+        const std::uint64_t items = static_cast<std::uint64_t>(A.n_wtiles) * chunks;
+        if (items >= (1ull << 33)) throw InvalidArgument("apply: launch too large");
+        spmm_wave_kernel<E, FU, NUM, XT, YT><<<static_cast<unsigned>((items + 3) / 4), 256, 0, s>>>(
+            M, static_cast<const XT *>(x), static_cast<YT *>(y), P, k, static_cast<unsigned>(A.n_wtiles), chunks);
+        AXIS_LAUNCH_CHECK();
+    } else if (A.n_seg > 0) {
         if (tiled)
             spmm_tile_kernel<E, FU, NUM, XT, YT><<<static_cast<unsigned>(nwork), kTileSeg, 0, s>>>(M, static_cast<const XT *>(x),
                                                                                                     static_cast<YT *>(y), P, k, nb, chunks);
@@ -634,21 +839,23 @@ void launch_e(InterpolationMatrix &A, const void *x, void *y, index_t k, int chu
 }
 
 template <int FU, int NUM, class XT, class YT>
-void launch_fu(InterpolationMatrix &A, const void *x, void *y, index_t k, int chunks, bool tiled, hipStream_t s, const SpecialArgs &sa) {
+void launch_fu(InterpolationMatrix &A, const void *x, void *y, index_t k, int chunks, bool tiled, bool wave, hipStream_t s, const SpecialArgs &sa) {
     switch (A.seg_cap) {
-        case 4: return launch_e<4, FU, NUM, XT, YT>(A, x, y, k, chunks, tiled, s, sa);
-        case 8: return launch_e<8, FU, NUM, XT, YT>(A, x, y, k, chunks, tiled, s, sa);
-        case 12: return launch_e<12, FU, NUM, XT, YT>(A, x, y, k, chunks, tiled, s, sa);
-        default: return launch_e<16, FU, NUM, XT, YT>(A, x, y, k, chunks, tiled, s, sa);
+        case 4: return launch_e<4, FU, NUM, XT, YT>(A, x, y, k, chunks, tiled, wave, s, sa);
+        case 8: return launch_e<8, FU, NUM, XT, YT>(A, x, y, k, chunks, tiled, wave, s, sa);
+        case 12: return launch_e<12, FU, NUM, XT, YT>(A, x, y, k, chunks, tiled, wave, s, sa);
+        default: return launch_e<16, FU, NUM, XT, YT>(A, x, y, k, chunks, tiled, wave, s, sa);
     }
 }
 
 template <int NUM, class XT, class YT>
 void launch_xy(InterpolationMatrix &A, const void *x, void *y, index_t k, const Launch &L, hipStream_t s, const SpecialArgs &sa = SpecialArgs{}) {
+    const bool wave = L.wave && !is_special(NUM) && wave_ok<XT>(A, x);
+    const bool tiled = L.tiled || (L.wave && !wave);  // a wave shape on fields it cannot serve runs staged tiles
     switch (L.fu) {
-        case 1: return launch_fu<1, NUM, XT, YT>(A, x, y, k, L.chunks, L.tiled, s, sa);
-        case 2: return launch_fu<2, NUM, XT, YT>(A, x, y, k, L.chunks, L.tiled, s, sa);
-        case 4: return launch_fu<4, NUM, XT, YT>(A, x, y, k, L.chunks, L.tiled, s, sa);
+        case 1: return launch_fu<1, NUM, XT, YT>(A, x, y, k, L.chunks, tiled, wave, s, sa);
+        case 2: return launch_fu<2, NUM, XT, YT>(A, x, y, k, L.chunks, tiled, wave, s, sa);
+        case 4: return launch_fu<4, NUM, XT, YT>(A, x, y, k, L.chunks, tiled, wave, s, sa);
         default: throw InvalidArgument("apply: unsupported fields per step");
     }
 }
@@ -665,9 +872,27 @@ void launch_num(InterpolationMatrix &A, const void *x, FieldType xt, void *y, Fi
 
 /// One apply with launch shape L; sa.partials, when given, replaces the
 /// matrix's own long-row workspace (n_slots * k doubles).
+inline int chunk_cols_of(FieldType xt) { return xt == FieldType::F64 ? 2 : (xt == FieldType::F32 ? 4 : 8); }
+
+/// Whether launch shape L on fields x of type xt runs wave tiles (a wave
+/// shape on fields they cannot serve runs staged tiles instead).
+inline bool runs_wave(const InterpolationMatrix &A, const Launch &L, FieldType xt, const void *x) {
+    const int cw = chunk_cols_of(xt);
+    return L.wave && A.n_src > 0 && A.n_src % cw == 0 && (reinterpret_cast<std::uintptr_t>(x) % 16) == 0;
+}
+
+/// Whether the layouts launch shape L needs on these fields exist already.
+inline bool layout_ready(const InterpolationMatrix &A, const Launch &L, FieldType xt, const void *x) {
+    if (!A.apply_ready) return false;
+    if (runs_wave(A, L, xt, x)) return A.wave_ready[width_slot(chunk_cols_of(xt))];
+    return !(L.tiled || L.wave) || A.tiles_ready;
+}
+
 inline void launch(InterpolationMatrix &A, const void *x, FieldType xt, void *y, FieldType yt, index_t k, ApplyMode mode, const Launch &L,
                    hipStream_t s, const SpecialArgs &sa = SpecialArgs{}) {
-    if (L.tiled) A.prepare_tiles(s);
+    const bool wave_fit = runs_wave(A, L, xt, x);
+    if (L.tiled || (L.wave && !wave_fit)) A.prepare_tiles(s);
+    if (wave_fit) A.prepare_wave_tiles(chunk_cols_of(xt), s);
     if (mode == ApplyMode::Plain) return launch_xy<kPlain, double, double>(A, x, y, k, L, s, sa);
     if (mode == ApplyMode::Mixed) return launch_num<kComp32>(A, x, xt, y, yt, k, L, s, sa);
     return launch_num<kAcc64>(A, x, xt, y, yt, k, L, s, sa);

@@ -15,7 +15,9 @@
 
 #include <cstdint>
 #include <atomic>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -102,7 +104,12 @@ class InterpolationMatrix {
     DeviceBuffer<index_t> seg_begin;
     int n_long_rows = 0, n_slots = 0;
     DeviceBuffer<std::int32_t> long_rows, long_slot;
-    DeviceBuffer<double> partial_ws;
+    // Long-row partial-sum workspaces, one per stream the matrix is applied
+    // on: applies on one stream are ordered by it, applies on different
+    // streams write different workspaces, so concurrent applies of one matrix
+    // do not race.  apply_mu guards them and the lazily built layouts.
+    std::map<hipStream_t, DeviceBuffer<double>> partial_ws;
+    std::mutex apply_mu;
     // Spatial processing order: 30-bit Morton code of each destination cell's
     // centroid (unit-sphere xyz), set by the weight generators; empty for
     // matrices built from triplets (rows then keep index order).
@@ -118,6 +125,15 @@ class InterpolationMatrix {
     DeviceBuffer<std::int32_t> tile_nu, tile_col;
     DeviceBuffer<std::uint16_t> t_lcol;
     bool apply_ready = false, tiles_ready = false;
+    // Wave tiles: wave tile w = segments [64 w, 64 w + 64); per chunk width
+    // (16-byte chunks of 2, 4 or 8 columns: FP64, FP32, BF16 fields) the
+    // distinct chunks each wave stages (at most 128; wnu[w] = -1: too many,
+    // the wave gathers directly), their first columns wcol[w * 128 ...], and
+    // per nonzero its column's position in the staged chunks.
+    index_t n_wtiles = 0;
+    DeviceBuffer<std::int32_t> wnu[3], wcol[3];
+    DeviceBuffer<std::uint16_t> wlpos[3];
+    bool wave_ready[3] = {false, false, false};
     // row-length histogram (lengths 0 .. 16, then "longer"), cached by
     // row_length_hist(); it fixes seg_cap
     index_t len_hist[18] = {0};
@@ -131,6 +147,14 @@ class InterpolationMatrix {
     void prepare_apply(hipStream_t s);
     /// Builds the tiles once (on the first tiled launch).
     void prepare_tiles(hipStream_t s);
+    /// Builds the wave tiles for chunks of cw columns once (first wave launch
+    /// with fields of that width).
+    void prepare_wave_tiles(int cw, hipStream_t s);
+    /// Forgets every apply-side layout (after the CSR arrays change).
+    void reset_apply_layout() {
+        apply_ready = tiles_ready = false;
+        for (bool &w : wave_ready) w = false;
+    }
     /// Fills len_hist once; returns the segment length E it implies (the
     /// smallest of 4, 8, 12, 16 leaving at most 0.1% of the rows longer).
     int row_length_hist(hipStream_t s);
@@ -223,7 +247,12 @@ ConservationReport check_conservation(const InterpolationMatrix &A, const double
 struct ApplyShape {
     int fields_per_step = 4, field_chunks = 1;
     bool staged = false;
+    bool wave = false;  // staged per wave (wave tiles) rather than per workgroup
 };
+
+/// Pins the launch shape batch_apply uses for this (n_dst, nnz, k, types,
+/// mode), as the autotuner's cache entry would.
+void set_apply_shape(InterpolationMatrix &A, index_t k, FieldType xt, FieldType yt, ApplyMode mode, const ApplyShape &shape);
 
 /// Times the candidate launch shapes of batch_apply on the given device
 /// fields and caches the fastest for this (n_dst, nnz, k, types, mode).

@@ -287,7 +287,10 @@ BatchPlan::BatchPlan(std::vector<std::shared_ptr<InterpolationMatrix>> mats, std
     index_t k_max = 0;
     for (std::size_t p = 0; p < n; ++p) {
         auto &A = *I.mats[p];
-        A.prepare_tiles(s);
+        {
+            std::lock_guard<std::mutex> lock(A.apply_mu);
+            A.prepare_tiles(s);
+        }
         I.max_e = std::max(I.max_e, A.seg_cap);
         pd[p].A = apply_detail::mat_of(A);
         pd[p].X = I.src[p];
@@ -308,11 +311,16 @@ BatchPlan::BatchPlan(std::vector<std::shared_ptr<InterpolationMatrix>> mats, std
         if (I.k[p] <= 0) continue;
         if (autotune != 0) {
             const ApplyShape sh = autotune_apply(*I.mats[p], I.src[p], xt, I.dst[p], yt, I.k[p], mode, s);
-            I.pair_shape[p] = apply_detail::Launch{sh.fields_per_step, sh.field_chunks, sh.staged};
+            I.pair_shape[p] = apply_detail::Launch{sh.fields_per_step, sh.field_chunks, sh.staged && !sh.wave, sh.wave};
         } else {
             I.pair_shape[p] = apply_detail::choose(*I.mats[p], I.k[p], xt, yt, mode);
         }
-        if (I.pair_shape[p].tiled) I.mats[p]->prepare_tiles(s);
+        const apply_detail::Launch &L = I.pair_shape[p];
+        std::lock_guard<std::mutex> lock(I.mats[p]->apply_mu);
+        if (apply_detail::runs_wave(*I.mats[p], L, xt, I.src[p]))
+            I.mats[p]->prepare_wave_tiles(apply_detail::chunk_cols_of(xt), s);
+        else if (L.tiled || L.wave)
+            I.mats[p]->prepare_tiles(s);
     }
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
     if (autotune == 1) {

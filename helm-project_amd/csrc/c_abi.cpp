@@ -563,7 +563,21 @@ int axis_autotune_apply_c(int matrix_handle, const void *src, int src_dtype, voi
         auto r = autotune_apply(*M, src, ftype(src_dtype), dst, ftype(dst_dtype), n_fields, amode(mode), static_cast<hipStream_t>(stream));
         if (fields_per_step) *fields_per_step = r.fields_per_step;
         if (field_chunks) *field_chunks = r.field_chunks;
-        if (staged) *staged = r.staged ? 1 : 0;
+        if (staged) *staged = r.wave ? 2 : (r.staged ? 1 : 0);
+    });
+}
+
+int axis_apply_set_shape_c(int matrix_handle, int64_t n_fields, int src_dtype, int dst_dtype, int mode, int fields_per_step, int field_chunks,
+                           int staged) {
+    return guarded([&] {
+        auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
+        if (staged < 0 || staged > 2) throw InvalidArgument("staged must be 0, 1 or 2");
+        ApplyShape sh;
+        sh.fields_per_step = fields_per_step;
+        sh.field_chunks = field_chunks;
+        sh.staged = staged != 0;
+        sh.wave = staged == 2;
+        set_apply_shape(*M, n_fields, ftype(src_dtype), ftype(dst_dtype), amode(mode), sh);
     });
 }
 

@@ -143,9 +143,8 @@ std::unique_ptr<InterpolationMatrix> generate_distributed(const Mesh &src, const
         M->area_a = std::move(aa);
     }
     M->n_src = n_ext;
-    M->apply_ready = false;
+    M->reset_apply_layout();
     M->len_hist_ready = false;
-    M->tiles_ready = false;
     return M;
 }
 

@@ -873,9 +873,8 @@ void coastal_pass(const Mesh &src, const Mesh &dst, ExtrapolationAction extrap, 
     M.val = std::move(new_val);
     M.nnz = new_nnz;
     M.max_row_nnz = new_nnz > 0 ? std::max(h_ints[0], 1) : 0;
-    M.apply_ready = false;
+    M.reset_apply_layout();
     M.len_hist_ready = false;
-    M.tiles_ready = false;
 }
 
 std::unique_ptr<InterpolationMatrix> generate_nearest(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, const MeshGeom *src_reg,

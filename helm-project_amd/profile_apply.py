@@ -39,6 +39,12 @@ def main():
     ap.add_argument("--dst", default="f32", choices=["f32", "f64"], help="output storage of the mixed modes")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--no-tune", action="store_true", help="keep the library's default launch shapes")
+    ap.add_argument("--shape", default=None,
+                    help="pin the launch shape 'fields_per_step,field_chunks,staged' (staged: 0 direct, 1 workgroup tiles, 2 wave tiles)")
+    ap.add_argument("--sweep", action="store_true", help="time every launch shape and print one line each")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="end with three copies of 256 Mi FP32 values (1 GiB read, 1 GiB written): the known-size kernel "
+                         "against which memory counters are scaled")
     args = ap.parse_args()
 
     import torch
@@ -64,20 +70,42 @@ def main():
             xt, yt, mode = args.src, args.dst, axis.MIXED if name == "comp32" else axis.MIXED_ACC64
         X = X64.to(codes[xt][1]).contiguous()
         Y = torch.empty((k, n_dst), dtype=codes[yt][1], device=dev)
-        shape = None if args.no_tune else m.autotune(X, Y, k, codes[xt][0], codes[yt][0], mode, stream=stream.cuda_stream)
-        m.batch_apply(X, Y, k, codes[xt][0], codes[yt][0], mode, stream=stream.cuda_stream)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(args.reps):
-            m.batch_apply(X, Y, k, codes[xt][0], codes[yt][0], mode, stream=stream.cuda_stream, sync=False)
-        e1.record(stream)
-        e1.synchronize()
-        ms = e0.elapsed_time(e1) / args.reps
         nbytes = n_src * k * codes[xt][2] + n_dst * k * codes[yt][2] + nnz * 12 + (n_dst + 1) * 8
-        print(json.dumps({"pair": args.pair, "mode": name, "src": xt, "dst": yt, "fields": k, "shape": shape, "ms": ms,
-                          "algorithmic_bytes": nbytes, "GBs": nbytes / (ms * 1e-3) / 1e9, "nnz": nnz, "n_src": n_src,
-                          "n_dst": n_dst}), flush=True)
+
+        def timed(shape):
+            m.batch_apply(X, Y, k, codes[xt][0], codes[yt][0], mode, stream=stream.cuda_stream)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(args.reps):
+                m.batch_apply(X, Y, k, codes[xt][0], codes[yt][0], mode, stream=stream.cuda_stream, sync=False)
+            e1.record(stream)
+            e1.synchronize()
+            ms = e0.elapsed_time(e1) / args.reps
+            print(json.dumps({"pair": args.pair, "mode": name, "src": xt, "dst": yt, "fields": k, "shape": shape, "ms": ms,
+                              "algorithmic_bytes": nbytes, "GBs": nbytes / (ms * 1e-3) / 1e9, "nnz": nnz, "n_src": n_src,
+                              "n_dst": n_dst}), flush=True)
+
+        if args.sweep:
+            for staged in (1, 2):
+                for fu in (1, 2, 4):
+                    for chunks in (1, 2, 4, 8):
+                        m.set_shape(k, codes[xt][0], codes[yt][0], mode, fu, chunks, staged)
+                        timed([fu, chunks, staged])
+        elif args.shape:
+            shape = [int(v) for v in args.shape.split(",")]
+            m.set_shape(k, codes[xt][0], codes[yt][0], mode, *shape)
+            timed(shape)
+        else:
+            shape = None if args.no_tune else list(m.autotune(X, Y, k, codes[xt][0], codes[yt][0], mode, stream=stream.cuda_stream))
+            timed(shape)
         del X, Y
+    if args.calibrate:
+        a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes_read": a.numel() * 4, "calibration_copy_bytes_written": a.numel() * 4}), flush=True)
 
 
 if __name__ == "__main__":

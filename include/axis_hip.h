@@ -220,12 +220,21 @@ int axis_batch_apply_noalloc_c(int matrix_handle, const void *src, int src_dtype
                                int64_t n_fields, int mode, void *stream);
 
 /* Autotuner (no reference counterpart): times the launch shapes of
- * axis_batch_apply_c (fields per step x field chunks x LDS-staged tiles or
- * direct gathers) on these device fields, caches the fastest for this
- * (matrix size, n_fields, types, mode) and reports it; output pointers may be
- * NULL. */
+ * axis_batch_apply_c (fields per step x field chunks x direct gathers,
+ * LDS-staged workgroup tiles or LDS-staged wave tiles) on these device
+ * fields, caches the fastest for this (matrix size, n_fields, types, mode)
+ * and reports it (*staged: 0 direct, 1 workgroup tiles, 2 wave tiles);
+ * output pointers may be NULL. */
 int axis_autotune_apply_c(int matrix_handle, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n_fields, int mode, void *stream,
                           int *fields_per_step, int *field_chunks, int *staged);
+
+/* Pins the launch shape of axis_batch_apply_c for this (matrix size,
+ * n_fields, types, mode) as the autotuner's cache entry would (no reference
+ * counterpart): fields per step 1, 2 or 4, field chunks >= 1, staged 0
+ * direct gathers, 1 workgroup tiles, 2 wave tiles.  Every shape gives the
+ * same bits. */
+int axis_apply_set_shape_c(int matrix_handle, int64_t n_fields, int src_dtype, int dst_dtype, int mode, int fields_per_step, int field_chunks,
+                           int staged);
 
 /* check_conservation (conservation.hpp:95-97, conservation.cpp:82-103) on
  * FP64 fields (host or device): out4 = {src_integral, dst_integral,

@@ -255,6 +255,48 @@ def test_autotune_keeps_results(cfg1_weights, mode):
     assert torch.equal(Y1, Y2)
 
 
+@pytest.mark.parametrize("xt,yt,mode", [(axis.F32, axis.F32, axis.MIXED), (axis.BF16, axis.F64, axis.MIXED_ACC64),
+                                         (axis.F64, axis.F64, axis.PLAIN), (axis.F32, axis.F64, axis.MIXED_ACC64)])
+def test_every_launch_shape_same_bits(xt, yt, mode):
+    """Every pinned launch shape (direct gathers, workgroup tiles, wave tiles;
+    fields per step; field chunks) gives the bits of the oracle-checked
+    default, on conservative weights (1 deg -> C48) and on a matrix with long
+    rows and rows touching many distinct source chunks."""
+    tdt = {axis.F64: torch.float64, axis.F32: torch.float32, axis.BF16: torch.bfloat16}
+    m1 = axis.generate_weights(axis.Mesh(grids.latlon_deg(1.0)), axis.Mesh(grids.cubed_sphere(48)), axis.CONSERVATIVE)
+    rng = np.random.default_rng(11)
+    rows, cols, vals = [], [], []
+    for r in range(500):
+        n = 300 if r % 50 == 0 else int(rng.integers(1, 15))
+        base = int(rng.integers(0, 7000))
+        c = np.unique(np.clip(base + rng.integers(-400, 400, n), 0, 7999))
+        rows += [r] * len(c)
+        cols += list(c)
+        vals += list(rng.uniform(0, 1, len(c)))
+    m2 = axis.matrix_from_coo(rows, cols, vals, 8000, 500)
+    for m in (m1, m2):
+        n_src, n_dst, _ = m.info()
+        for k in (3, 13):
+            X = torch.randn((k, n_src), dtype=torch.float64, device="cuda").to(tdt[xt]).contiguous()
+            R = torch.empty((k, n_dst), dtype=tdt[yt], device="cuda")
+            m.batch_apply(X, R, k, xt, yt, mode)
+            if mode == axis.PLAIN:
+                Xh = X.cpu().numpy()
+                rp, col, val = m.csr()
+                for f in (0, k - 1):
+                    yo = O.batch_apply(rp, col, val, n_src, Xh[f])
+                    assert np.max(np.abs(R[f].cpu().numpy() - yo)) <= 1e-13 * np.max(np.abs(yo))
+            for staged in (0, 1, 2):
+                for fu in (1, 2, 4):
+                    for chunks in (1, 2, 5):
+                        m.set_shape(k, xt, yt, mode, fu, chunks, staged)
+                        Y = torch.full_like(R, float("nan"))
+                        m.batch_apply(X, Y, k, xt, yt, mode)
+                        assert torch.equal(Y, R), (m.info(), k, staged, fu, chunks)
+    m1.close()
+    m2.close()
+
+
 @pytest.mark.timeout(900)
 def test_c384_properties_and_sampled_rows():
     """Config-2 sizes: 0.25 deg (1,036,800 cells) -> C384 (884,736 cells).
