@@ -74,6 +74,21 @@ class BlockCache {
     void release(void *p, std::size_t bytes);
     /// Returns every cached block to the runtime (waits for the device).
     void trim();
+    /// While one of these lives on a thread, that thread's allocations never
+    /// wait for the device to recycle released blocks (a miss is served by a
+    /// new block instead): for pipelines whose other streams (host-fed
+    /// uploads) must not be waited for.  Device and pinned blocks alike.
+    struct NoWaitScope {
+        NoWaitScope();
+        ~NoWaitScope();
+        NoWaitScope(const NoWaitScope &) = delete;
+        NoWaitScope &operator=(const NoWaitScope &) = delete;
+    };
+    static bool no_wait();
+    /// Waits for the device once and makes every released device and pinned
+    /// block reusable: called where the device is about to be idle anyway,
+    /// before a NoWaitScope, so that the scope's allocations find blocks.
+    static void recycle_now();
     static std::size_t rounded(std::size_t bytes) {
         if (bytes <= (std::size_t(2) << 20)) {
             std::size_t r = 256;

@@ -753,6 +753,11 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
                                                            const mlp::Model *model, ProposerStats *pst, double area_tol) {
     const auto t0 = std::chrono::steady_clock::now();
     PhaseTrace tr("conservative", s);
+    // the host node-xyz task uploads on its own stream: this thread's
+    // allocations must not wait for the device (and thereby for the task);
+    // blocks released before this call are made reusable first
+    BlockCache::recycle_now();
+    BlockCache::NoWaitScope no_wait;
     const index_t n_src = src.n_cells(), n_dst = dst.n_cells();
     if (n_src >= (index_t(1) << 31) || n_dst >= (index_t(1) << 31)) throw InvalidArgument("generate_conservative: more than 2^31 cells");
     MeshGeom sg, dg;
@@ -795,6 +800,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
             src_out = sg.d_xyz.get();
         }
         xyz_task = std::async(std::launch::async, [&, src_out, dst_out] {
+            BlockCache::NoWaitScope task_no_wait;  // (its pinned blocks)
             try {
                 if (!dst_first) src_uploaded.store(node_xyz_host(src, sg, &skip_src, up, src_up.e, src_out));
                 src_done.set_value();

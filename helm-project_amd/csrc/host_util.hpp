@@ -129,12 +129,13 @@ inline Pool &pool() {
 
 }  // namespace pool_detail
 
-/// Runs fn(begin, end) over [0, n) split into contiguous chunks, on the
-/// persistent host workers and the calling thread.  An exception thrown by
-/// fn is rethrown here after every chunk has finished.
+/// Runs fn(begin, end) over [0, n) split into contiguous chunks (at most
+/// max_threads of them when given), on the persistent host workers and the
+/// calling thread.  An exception thrown by fn is rethrown here after every
+/// chunk has finished.
 template <class Fn>
-void parallel_range(std::int64_t n, Fn fn, std::int64_t min_chunk = 4096) {
-    int nt = host_threads();
+void parallel_range(std::int64_t n, Fn fn, std::int64_t min_chunk = 4096, int max_threads = 0) {
+    int nt = max_threads > 0 ? std::min(max_threads, host_threads()) : host_threads();
     std::int64_t chunks = std::max<std::int64_t>(1, std::min<std::int64_t>(nt, (n + min_chunk - 1) / min_chunk));
     if (chunks <= 1) {
         fn(std::int64_t(0), n);

@@ -841,7 +841,8 @@ bool node_xyz_device(const Mesh &m, const GridAxes &ax, MeshGeom &g, hipStream_t
     const std::vector<double> &vlon = ax.xlon.empty() ? ax.ulon : ax.xlon, &vlat = ax.xlat.empty() ? ax.ulat : ax.xlat;
     const int nlon = static_cast<int>(vlon.size()), nlat = static_cast<int>(vlat.size());
     // the tables with host libm, as node_xyz_host's per-node calls would see them
-    std::vector<double> tab(3 * static_cast<std::size_t>(nlon + nlat));
+    PinnedArray<double> tab;  // pinned: a pageable upload would wait on the host threads
+    tab.resize(3 * static_cast<std::size_t>(nlon + nlat));
     double *ulon = tab.data(), *ulat = ulon + nlon, *slon = ulat + nlat, *clon = slon + nlon, *slat = clon + nlon, *clat = slat + nlat;
     for (int a = 0; a < nlon; ++a) {
         ulon[a] = vlon[a];
@@ -911,6 +912,9 @@ bool node_xyz_host(const Mesh &m, MeshGeom &g, const std::atomic<bool> *cancel, 
     const double *lon = m.h_coords.data(), *lat = m.h_coords.data() + n_nodes;
     const bool upload = up != nullptr && d_out != nullptr;
     std::atomic<int> up_err{0};
+    // beside a running generation (upload given), two cores stay free for the
+    // generating thread and the HIP runtime's own threads
+    const int max_threads = upload ? std::max(1, host_threads() - 2) : 0;
     parallel_range(n_nodes, [&](index_t b, index_t e) {
         SinCosMemo memo;
         for (index_t i = b; i < e; ++i) {
@@ -933,7 +937,7 @@ bool node_xyz_host(const Mesh &m, MeshGeom &g, const std::atomic<bool> *cancel, 
                                                 hipMemcpyHostToDevice, up);
             if (r != hipSuccess) up_err.store(static_cast<int>(r));
         }
-    });
+    }, 4096, max_threads);
     const bool complete = !(cancel && cancel->load());
     g.xyz_ready = complete;
     if (up_err.load() != 0) AXIS_HIP_CHECK(static_cast<hipError_t>(up_err.load()));

@@ -50,6 +50,22 @@ BlockCache &BlockCache::get() {
     return b;
 }
 
+namespace {
+thread_local int g_no_wait = 0;
+}
+BlockCache::NoWaitScope::NoWaitScope() { ++g_no_wait; }
+BlockCache::NoWaitScope::~NoWaitScope() { --g_no_wait; }
+bool BlockCache::no_wait() { return g_no_wait > 0; }
+
+void BlockCache::recycle_now() {
+    CacheState &c = cache_state(), &h = pinned_state();
+    std::scoped_lock g(c.mu, h.mu);
+    if (c.pending.empty() && h.pending.empty()) return;
+    AXIS_HIP_CHECK(hipDeviceSynchronize());
+    c.recycle();
+    h.recycle();
+}
+
 void *BlockCache::alloc(std::size_t bytes) {
     const std::size_t r = rounded(bytes);
     const int dev = current_device();
@@ -63,7 +79,7 @@ void *BlockCache::alloc(std::size_t bytes) {
             c.owner[p] = dev;
             return p;
         }
-        if (pass == 0 && !c.pending.empty()) {
+        if (pass == 0 && !c.pending.empty() && !no_wait()) {
             AXIS_HIP_CHECK(hipDeviceSynchronize());  // work queued before the releases has finished
             c.recycle();
             continue;
@@ -117,7 +133,7 @@ void *PinnedCache::alloc(std::size_t bytes) {
             c.free.erase(it);
             return p;
         }
-        if (pass == 0 && !c.pending.empty()) {
+        if (pass == 0 && !c.pending.empty() && !BlockCache::no_wait()) {
             AXIS_HIP_CHECK(hipDeviceSynchronize());  // uploads queued from the released blocks have finished
             c.recycle();
             continue;
@@ -234,8 +250,8 @@ void exclusive_scan_i64(const index_t *in, index_t *out, index_t n, hipStream_t 
     }
     scan_down_kernel<<<static_cast<unsigned>(nb), kScanBlock, 0, s>>>(in, n, offs.get(), out);
     AXIS_LAUNCH_CHECK();
-    // the temporaries are freed only after the kernels that read them ran
-    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    // (no wait: the temporaries go back to the block cache, which hands a
+    // block out again only after a device synchronisation)
 }
 
 // --------------------------------------------------------------- radix sort
@@ -333,7 +349,6 @@ void radix_sort_impl(K *keys, std::uint32_t *vals, std::size_t n, int bits, hipS
         AXIS_HIP_CHECK(hipMemcpyAsync(keys, kin, n * sizeof(K), hipMemcpyDeviceToDevice, s));
         if constexpr (kVals) AXIS_HIP_CHECK(hipMemcpyAsync(vals, vin, n * sizeof(std::uint32_t), hipMemcpyDeviceToDevice, s));
     }
-    AXIS_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 }  // namespace
