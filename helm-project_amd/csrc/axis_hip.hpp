@@ -337,6 +337,15 @@ bool rectilinear_from_axes(const Mesh &m, const GridAxes &ax);
 /// sets g.d_xyz_ready.
 bool node_xyz_host(const Mesh &m, MeshGeom &g, const std::atomic<bool> *cancel = nullptr, hipStream_t up = nullptr, hipEvent_t done = nullptr,
                    double *d_out = nullptr);
+/// Node xyz of a degree mesh built on the device into d_out [3 * n_nodes] on
+/// stream `st`, with host libm's bits: double-double sin/cos give the
+/// correctly rounded values, which host libm (within ~0.55 ULP) also returns
+/// unless the exact value lies near a rounding midpoint; those angles are
+/// evaluated by host libm and patched in.  Waits for `st`.  Returns false
+/// (d_out partly written) for a mesh not in degrees, with AXIS_HOST_NODE_XYZ
+/// set, with too many coordinates beyond the reduction's range, or when
+/// `cancel` is set.
+bool node_xyz_trig(const Mesh &m, double *d_out, hipStream_t st, const std::atomic<bool> *cancel = nullptr);
 /// A non-blocking stream of the current device for host-to-device uploads
 /// beside the caller's stream (created once per device, default priority).
 hipStream_t upload_stream();

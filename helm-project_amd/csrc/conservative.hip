@@ -784,6 +784,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     } dst_up, src_up;
     hipStream_t up = nullptr;
     std::atomic<bool> dst_uploaded{false}, src_uploaded{false};
+    std::atomic<bool> dst_trig{false}, src_trig{false};  // built by node_xyz_trig (else host libm slices)
     // both meshes with grid counts: an analytic path will most likely answer,
     // so no host task is started (should both decline, the geometry below
     // computes the node xyz on this thread)
@@ -801,14 +802,23 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
         }
         xyz_task = std::async(std::launch::async, [&, src_out, dst_out] {
             BlockCache::NoWaitScope task_no_wait;  // (its pinned blocks)
+            // on the device where the mesh allows, else host libm by slices
+            auto build = [&](const Mesh &m, MeshGeom &g, const std::atomic<bool> *stop, hipEvent_t done, double *out, std::atomic<bool> &trig) {
+                if (node_xyz_trig(m, out, up, stop)) {
+                    trig.store(true);
+                    AXIS_HIP_CHECK(hipEventRecord(done, up));
+                    return true;
+                }
+                return node_xyz_host(m, g, stop, up, done, out);
+            };
             try {
-                if (!dst_first) src_uploaded.store(node_xyz_host(src, sg, &skip_src, up, src_up.e, src_out));
+                if (!dst_first) src_uploaded.store(build(src, sg, &skip_src, src_up.e, src_out, src_trig));
                 src_done.set_value();
             } catch (...) {
                 src_done.set_exception(std::current_exception());
                 return;
             }
-            dst_uploaded.store(node_xyz_host(dst, dg, &cancel, up, dst_up.e, dst_out));
+            dst_uploaded.store(build(dst, dg, &cancel, dst_up.e, dst_out, dst_trig));
         });
     } else {
         src_done.set_value();
@@ -850,7 +860,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     if (src_dev_xyz) node_xyz_device(src, sax, sg, s);  // (false: host xyz below)
     if (!sg.dev_xyz && src_uploaded.load()) {
         AXIS_HIP_CHECK(hipStreamWaitEvent(s, src_up.e, 0));
-        sg.d_xyz_ready = true;
+        (src_trig.load() ? sg.dev_xyz : sg.d_xyz_ready) = true;
     }
     prepare_geometry(src, sg, s);
     tr.mark("geometry_src");
@@ -862,7 +872,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     if (xyz_task.valid()) xyz_task.get();
     if (dst_uploaded.load()) {
         AXIS_HIP_CHECK(hipStreamWaitEvent(s, dst_up.e, 0));
-        dg.d_xyz_ready = true;
+        (dst_trig.load() ? dg.dev_xyz : dg.d_xyz_ready) = true;
     }
     prepare_geometry(dst, dg, s);
     tr.mark("geometry_dst");

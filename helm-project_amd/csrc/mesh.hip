@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <bit>
 #include <cmath>
 #include <cstring>
 #include <cfloat>
@@ -23,6 +24,7 @@
 #include <mutex>
 
 #include "axis_hip.hpp"
+#include "ddtrig.hpp"
 #include "geometry.hpp"
 #include "host_util.hpp"
 #include "morton.hpp"
@@ -795,6 +797,66 @@ __global__ void scatter_xyz_kernel(const index_t *__restrict__ nodes, const doub
 
 constexpr int kXyzFixCap = 8192;
 
+// xyz of every node of a degree mesh from double-double sin/cos
+// (lonlat_to_xyz, spherical_geometry.hpp:80-83: x = cos lat cos lon,
+// y = cos lat sin lon, z = sin lat, one product each).  Bit i of word i/64 of
+// flag_lat / flag_lon marks a node whose latitude / longitude has a sine or
+// cosine within dd::kLibmBand ULP of a rounding midpoint: its xyz here may be
+// one ULP off host libm's until trig_fix_kernel.  Nodes outside the
+// reduction's range, or with a sine or cosine below dd::kTinyResult, are
+// listed in `full` for the host.
+__global__ void __launch_bounds__(256) trig_xyz_kernel(const double *__restrict__ lon, const double *__restrict__ lat, index_t n, index_t words,
+                                                       double *__restrict__ xyz, std::uint64_t *__restrict__ flag_lat,
+                                                       std::uint64_t *__restrict__ flag_lon, index_t *__restrict__ full, int *__restrict__ n_full,
+                                                       int full_cap) {
+    const index_t i = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool f_lat = false, f_lon = false;
+    if (i < n) {
+        dd::DD sl, cl, so, co;
+        if (dd::sincos_dd(lat[i] * kDeg2Rad, sl, cl) && dd::sincos_dd(lon[i] * kDeg2Rad, so, co) && !dd::tiny(sl) && !dd::tiny(cl) &&
+            !dd::tiny(so) && !dd::tiny(co)) {
+            f_lat = dd::near_midpoint(sl) || dd::near_midpoint(cl);
+            f_lon = dd::near_midpoint(so) || dd::near_midpoint(co);
+            xyz[3 * i] = cl.hi * co.hi;
+            xyz[3 * i + 1] = cl.hi * so.hi;
+            xyz[3 * i + 2] = sl.hi;
+        } else {
+            const int k = atomicAdd(n_full, 1);
+            if (k < full_cap) full[k] = i;
+        }
+    }
+    // one word per wave (the block is four whole waves)
+    const std::uint64_t bl = __ballot(f_lat), bo = __ballot(f_lon);
+    const index_t w = i >> 6;
+    if ((threadIdx.x & 63) == 0 && w < words) {
+        flag_lat[w] = bl;
+        flag_lon[w] = bo;
+    }
+}
+
+// The flagged nodes again, each sine and cosine of a flagged angle replaced by
+// host libm's double (libm_bits: the last bit of host libm's sin lat, cos lat,
+// sin lon, cos lon, words [0, words), [words, 2 words), ...).
+__global__ void __launch_bounds__(256) trig_fix_kernel(const double *__restrict__ lon, const double *__restrict__ lat, index_t n, index_t words,
+                                                       const std::uint64_t *__restrict__ flag_lat, const std::uint64_t *__restrict__ flag_lon,
+                                                       const std::uint64_t *__restrict__ libm_bits, double *__restrict__ xyz) {
+    const index_t i = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const index_t w = i >> 6;
+    const int t = static_cast<int>(i & 63);
+    const bool fl = (flag_lat[w] >> t) & 1, fo = (flag_lon[w] >> t) & 1;
+    if (!fl && !fo) return;
+    dd::DD sl, cl, so, co;
+    dd::sincos_dd(lat[i] * kDeg2Rad, sl, cl);  // (in range: the node was not listed for the host)
+    dd::sincos_dd(lon[i] * kDeg2Rad, so, co);
+    auto bit = [&](int q) { return static_cast<unsigned>((libm_bits[q * words + w] >> t) & 1); };
+    const double vsl = fl ? dd::pick(sl, bit(0)) : sl.hi, vcl = fl ? dd::pick(cl, bit(1)) : cl.hi;
+    const double vso = fo ? dd::pick(so, bit(2)) : so.hi, vco = fo ? dd::pick(co, bit(3)) : co.hi;
+    xyz[3 * i] = vcl * vco;
+    xyz[3 * i + 1] = vcl * vso;
+    xyz[3 * i + 2] = vsl;
+}
+
 __global__ void scatter_recheck_kernel(const index_t *__restrict__ cells, const double *__restrict__ area, const std::int32_t *__restrict__ flags, int n,
                                        double *__restrict__ d_area, std::int32_t *__restrict__ d_flags) {
     const int k = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -889,6 +951,81 @@ bool node_xyz_device(const Mesh &m, const GridAxes &ax, MeshGeom &g, hipStream_t
     return true;
 }
 
+bool node_xyz_trig(const Mesh &m, double *d_out, hipStream_t st, const std::atomic<bool> *cancel) {
+    if (m.coord_system() != CoordinateSystem::SphericalDeg || std::getenv("AXIS_HOST_NODE_XYZ") != nullptr) return false;
+    const index_t n = m.n_nodes();
+    if (n == 0) return true;
+    PhaseTrace tr("geometry", st);
+    const index_t words = (n + 63) / 64;
+    // flags of lat, lon; then host libm's last bits of sin lat, cos lat, sin lon, cos lon
+    DeviceBuffer<std::uint64_t> bits(6 * static_cast<std::size_t>(words));
+    // (nodes on the multiples of 90 degrees: a few per thousand on a cubed sphere)
+    const int full_cap = static_cast<int>(std::min<index_t>(n / 16 + kXyzFixCap, index_t(1) << 30));
+    DeviceBuffer<index_t> full(full_cap);
+    DeviceBuffer<int> n_full(1);
+    PinnedArray<std::uint64_t> hb;
+    hb.resize(6 * static_cast<std::size_t>(words) + 1);
+    std::uint64_t *flags = hb.data(), *res = flags + 2 * words, *h_nfull = flags + 6 * words;
+    AXIS_HIP_CHECK(hipMemsetAsync(n_full.get(), 0, sizeof(int), st));
+    const double *d_lon = m.d_coords.get(), *d_lat = d_lon + n;
+    trig_xyz_kernel<<<grid_for(n, 256), 256, 0, st>>>(d_lon, d_lat, n, words, d_out, bits.get(), bits.get() + words, full.get(), n_full.get(),
+                                                      full_cap);
+    AXIS_LAUNCH_CHECK();
+    AXIS_HIP_CHECK(hipMemcpyAsync(flags, bits.get(), 2 * sizeof(std::uint64_t) * words, hipMemcpyDeviceToHost, st));
+    *h_nfull = 0;
+    AXIS_HIP_CHECK(hipMemcpyAsync(h_nfull, n_full.get(), sizeof(int), hipMemcpyDeviceToHost, st));
+    AXIS_HIP_CHECK(hipStreamSynchronize(st));
+    const int nf = static_cast<int>(*h_nfull & 0xffffffffu);
+    if (nf > full_cap) return false;
+    // host libm for the flagged angles: one bit per sine and cosine
+    const double *lon = m.h_coords.data(), *lat = lon + n;
+    std::atomic<bool> stopped{false};
+    parallel_range(
+        words,
+        [&](index_t b, index_t e) {
+            for (index_t w = b; w < e; ++w) {
+                if (cancel && (w & 63) == 0 && cancel->load(std::memory_order_relaxed)) {
+                    stopped.store(true);
+                    return;
+                }
+                std::uint64_t r[4] = {0, 0, 0, 0};
+                for (int a = 0; a < 2; ++a) {
+                    const double *v = a == 0 ? lat : lon;
+                    for (std::uint64_t f = flags[a * words + w]; f != 0; f &= f - 1) {
+                        const int t = __builtin_ctzll(f);
+                        double sv, cv;
+                        sin_cos(v[64 * w + t] * kDeg2Rad, sv, cv);
+                        r[2 * a] |= (std::bit_cast<std::uint64_t>(sv) & 1u) << t;
+                        r[2 * a + 1] |= (std::bit_cast<std::uint64_t>(cv) & 1u) << t;
+                    }
+                }
+                for (int q = 0; q < 4; ++q) res[q * words + w] = r[q];
+            }
+        },
+        256, std::max(1, host_threads() - 2));
+    if (stopped.load()) return false;
+    AXIS_HIP_CHECK(hipMemcpyAsync(bits.get() + 2 * words, res, 4 * sizeof(std::uint64_t) * words, hipMemcpyHostToDevice, st));
+    trig_fix_kernel<<<grid_for(n, 256), 256, 0, st>>>(d_lon, d_lat, n, words, bits.get(), bits.get() + words, bits.get() + 2 * words, d_out);
+    AXIS_LAUNCH_CHECK();
+    if (nf > 0) {  // every value of these nodes from host libm
+        std::vector<index_t> nodes(nf);
+        full.download(nodes.data(), nf, st);
+        AXIS_HIP_CHECK(hipStreamSynchronize(st));
+        std::vector<double> vals(3 * static_cast<std::size_t>(nf));
+        for (int k = 0; k < nf; ++k) node_xyz_one(m, nodes[k], &vals[3 * static_cast<std::size_t>(k)]);
+        DeviceBuffer<index_t> d_nodes(nf);
+        DeviceBuffer<double> d_vals(vals.size());
+        d_nodes.upload(nodes.data(), nf, st);
+        d_vals.upload(vals.data(), vals.size(), st);
+        scatter_xyz_kernel<<<grid_for(nf, 256), 256, 0, st>>>(d_nodes.get(), d_vals.get(), nf, d_out);
+        AXIS_LAUNCH_CHECK();
+    }
+    // the pinned words go back to their cache on return: the copies are done first
+    AXIS_HIP_CHECK(hipStreamSynchronize(st));
+    tr.mark("trig_xyz");
+    return true;
+}
+
 hipStream_t upload_stream() {
     static std::mutex mu;
     static std::map<int, hipStream_t> streams;
@@ -954,6 +1091,13 @@ void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s) {
     PhaseTrace tr("geometry", s);
     // node xyz with host libm (extract_cell_polygon_spherical, weight_generator.cpp:462-470)
     // (or built on the device, node_xyz_device)
+    if (!g.dev_xyz && !g.d_xyz_ready && !g.xyz_ready && deg) {
+        DeviceBuffer<double> xyz(std::max<std::size_t>(3 * static_cast<std::size_t>(n_nodes), 3));
+        if (node_xyz_trig(m, xyz.get(), s)) {
+            g.d_xyz = std::move(xyz);
+            g.dev_xyz = true;
+        }
+    }
     if (!g.dev_xyz && !g.d_xyz_ready) {
         if (!g.xyz_ready) node_xyz_host(m, g);
         g.d_xyz.resize(std::max<std::size_t>(g.h_xyz.size(), 3));

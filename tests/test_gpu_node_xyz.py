@@ -1,15 +1,17 @@
-"""Node xyz of a rectilinear degree source built on the device (axis tables
-of host-libm sin/cos, exact-bit lookups, host fix-ups for nodes matching no
-axis value) against the host-libm build of every node, through the general
-conservative path (WeightGenerator::generate, weight_generator.cpp:3304-3770,
+"""Node xyz built on the device -- axis tables of host-libm sin/cos for a
+rectilinear degree source (exact-bit lookups, host fix-ups for nodes matching
+no axis value), double-double sin/cos for every other degree mesh (host libm
+for the angles near a rounding midpoint, the tiny results and the arguments
+beyond the reduction's range) -- against the host-libm build of every node,
+through the general conservative path (WeightGenerator::generate, weight_generator.cpp:3304-3770,
 whose node xyz is extract_cell_polygon_spherical's lonlat_to_xyz,
 weight_generator.cpp:462-470).
 
 Each pair is generated twice in one process, with AXIS_HOST_NODE_XYZ set
 (every node on the host) and unset; the matrices must agree bit for bit:
 row_ptr, columns, weights, fractions, areas and the arbitration counts.  With
-AXIS_TRACE set the library reports the device build (`geometry.device_xyz`),
-so each case also checks which path ran.
+AXIS_TRACE set the library reports the device builds (`geometry.device_xyz`,
+`geometry.trig_xyz`), so each case also checks which paths ran.
 """
 import os
 import sys
@@ -95,6 +97,8 @@ def corners(mesh):
 
 
 def generate(src, dst, host, capfd):
+    """The matrix, whether the axis-table build ran, and how many meshes the
+    double-double build served."""
     if host:
         os.environ["AXIS_HOST_NODE_XYZ"] = "1"
     os.environ["AXIS_TRACE"] = "1"
@@ -105,7 +109,17 @@ def generate(src, dst, host, capfd):
     finally:
         os.environ.pop("AXIS_HOST_NODE_XYZ", None)
         os.environ.pop("AXIS_TRACE", None)
-    return m, "geometry.device_xyz" in err
+    return m, "geometry.device_xyz" in err, err.count("geometry.trig_xyz")
+
+
+def wrapped(mesh, every, turns):
+    """Every `every`-th node's longitude moved by `turns` full turns: the same
+    points, other bits, some beyond the device reduction's range."""
+    m = dict(mesh)
+    c = np.array(mesh["coords"], dtype=np.float64, order="F")
+    c[::every, 0] += 360.0 * turns
+    m["coords"] = c
+    return m
 
 
 CASES = {
@@ -123,6 +137,9 @@ CASES = {
     "C24 -> 2deg (source not rectilinear)": (lambda: grids.cubed_sphere(24), lambda: grids.latlon_deg(2.0), False),
     "2deg with jittered longitudes -> C24 (grid counts, not rectilinear)": (lambda: jittered(grids.latlon_deg(2.0), 6),
                                                                              lambda: grids.cubed_sphere(24), False),
+    "0.5deg -> C192": (lambda: grids.latlon_deg(0.5), lambda: grids.cubed_sphere(192), True),
+    "C96 -> 1deg": (lambda: grids.cubed_sphere(96), lambda: grids.latlon_deg(1.0), False),
+    "C24 with every third longitude ten turns on -> 2deg": (lambda: wrapped(grids.cubed_sphere(24), 3, 10), lambda: grids.latlon_deg(2.0), False),
 }
 
 
@@ -132,10 +149,12 @@ def test_device_node_xyz_matches_host(case, capfd):
     src, dst = mk_src(), mk_dst()
     with capfd.disabled():
         say("case node xyz:", case)
-    mh, used_h = generate(src, dst, True, capfd)
-    md, used_d = generate(src, dst, False, capfd)
-    assert not used_h
+    mh, used_h, trig_h = generate(src, dst, True, capfd)
+    md, used_d, trig_d = generate(src, dst, False, capfd)
+    assert not used_h and trig_h == 0
     assert used_d == want_device
+    # the double-double build serves every mesh the axis tables do not
+    assert trig_d == (1 if want_device else 2)
     assert mh.info() == md.info()
     for a, b in zip(mh.csr(), md.csr()):
         assert np.array_equal(a, b)
