@@ -387,6 +387,24 @@ struct LbvhTree {
     const float *src_boxes = nullptr;
     DeviceBuffer<std::uint8_t> nodes;
 };
+/// Broad phase of a rectilinear degree source through its (lon, lat) cell
+/// grid (gridquery.hip): the same candidate sets as lbvh_query.
+struct GridIndex {
+    bool ready = false;
+    bool grid_order = false;               // cell j (nlon - 1) + i is slot (i, j) (cell_at is the identity)
+    int nlon = 0, nlat = 0;                // node counts per axis
+    double ulon0 = 0.0, dlon = 0.0;        // first longitude, mean spacing
+    DeviceBuffer<double> axes;             // tolerance-unique lon values, then lat values
+    DeviceBuffer<std::int32_t> cell_at;    // source cell of grid slot j (nlon - 1) + i
+    DeviceBuffer<float> rows;              // per cell row: z-band suffix minima, prefix maxima, longitude overhang
+    const float *src_boxes = nullptr;
+};
+/// Builds the grid of a source whose axes passed rectilinear_from_axes;
+/// false (index not ready) when its cells are not the 2 x 2 node blocks of
+/// that grid, each once, or its tables exceed 64 KiB of LDS.  Waits for `s`.
+bool grid_index_build(const Mesh &src, const GridAxes &ax, const float *src_boxes, GridIndex &G, hipStream_t s);
+void grid_query(const GridIndex &G, const float *dst_boxes, index_t n_dst, DeviceBuffer<index_t> &cand_off, DeviceBuffer<std::int32_t> &cand_src,
+                hipStream_t s);
 void lbvh_build(const float *src_boxes, index_t n_src, LbvhTree &t, hipStream_t s);
 void lbvh_query(const LbvhTree &t, const float *dst_boxes, index_t n_dst, DeviceBuffer<index_t> &cand_off, DeviceBuffer<std::int32_t> &cand_src,
                 hipStream_t s);

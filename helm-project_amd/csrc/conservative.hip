@@ -864,11 +864,16 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     }
     prepare_geometry(src, sg, s);
     tr.mark("geometry_src");
-    // the source tree does not need the destination: build it while the host
-    // finishes the destination's node coordinates
+    // broad phase: the cell grid of a rectilinear degree source, else a BVH
+    // (the proposer's fallback rows query the BVH too); neither needs the
+    // destination, so it is built while the destination's geometry finishes
+    GridIndex grid;
+    const char *bp = std::getenv("AXIS_BROAD_PHASE");
+    const bool grid_ok = !model && !(bp && std::string(bp) == "lbvh") && rectilinear_from_axes(src, sax) &&
+                         grid_index_build(src, sax, sg.d_box.get(), grid, s);
     LbvhTree tree;
-    lbvh_build(sg.d_box.get(), n_src, tree, s);
-    tr.mark("lbvh_build");
+    if (!grid_ok) lbvh_build(sg.d_box.get(), n_src, tree, s);
+    tr.mark("broad_phase_build");
     if (xyz_task.valid()) xyz_task.get();
     if (dst_uploaded.load()) {
         AXIS_HIP_CHECK(hipStreamWaitEvent(s, dst_up.e, 0));
@@ -881,9 +886,11 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     DeviceBuffer<std::int32_t> cand_src, accepted;
     if (model)
         proposer_candidates(sg, dg, *model, cand_off, cand_src, accepted, s);
+    else if (grid_ok)
+        grid_query(grid, dg.d_box.get(), n_dst, cand_off, cand_src, s);
     else
         lbvh_query(tree, dg.d_box.get(), n_dst, cand_off, cand_src, s);
-    tr.mark(model ? "propose" : "lbvh");
+    tr.mark(model ? "propose" : "broad_phase");
     index_t n_pairs = 0;
     AXIS_HIP_CHECK(hipMemcpyAsync(&n_pairs, cand_off.get() + n_dst, sizeof(index_t), hipMemcpyDeviceToHost, s));
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
