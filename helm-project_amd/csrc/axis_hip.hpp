@@ -403,8 +403,10 @@ struct GridIndex {
 /// false (index not ready) when its cells are not the 2 x 2 node blocks of
 /// that grid, each once, or its tables exceed 64 KiB of LDS.  Waits for `s`.
 bool grid_index_build(const Mesh &src, const GridAxes &ax, const float *src_boxes, GridIndex &G, hipStream_t s);
-void grid_query(const GridIndex &G, const float *dst_boxes, index_t n_dst, DeviceBuffer<index_t> &cand_off, DeviceBuffer<std::int32_t> &cand_src,
-                hipStream_t s);
+/// Candidates (cand_off [n_dst + 1], cand_src) and each one's destination row
+/// (pair_row); returns their number.  Waits for `s`.
+index_t grid_query(const GridIndex &G, const float *dst_boxes, index_t n_dst, DeviceBuffer<index_t> &cand_off, DeviceBuffer<std::int32_t> &cand_src,
+                   DeviceBuffer<std::int32_t> &pair_row, hipStream_t s);
 void lbvh_build(const float *src_boxes, index_t n_src, LbvhTree &t, hipStream_t s);
 void lbvh_query(const LbvhTree &t, const float *dst_boxes, index_t n_dst, DeviceBuffer<index_t> &cand_off, DeviceBuffer<std::int32_t> &cand_src,
                 hipStream_t s);

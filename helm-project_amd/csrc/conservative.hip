@@ -884,16 +884,19 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
 
     DeviceBuffer<index_t> cand_off;
     DeviceBuffer<std::int32_t> cand_src, accepted;
-    if (model)
-        proposer_candidates(sg, dg, *model, cand_off, cand_src, accepted, s);
-    else if (grid_ok)
-        grid_query(grid, dg.d_box.get(), n_dst, cand_off, cand_src, s);
-    else
-        lbvh_query(tree, dg.d_box.get(), n_dst, cand_off, cand_src, s);
-    tr.mark(model ? "propose" : "broad_phase");
+    DeviceBuffer<std::int32_t> grid_rows;  // (the grid query writes each candidate's row)
     index_t n_pairs = 0;
-    AXIS_HIP_CHECK(hipMemcpyAsync(&n_pairs, cand_off.get() + n_dst, sizeof(index_t), hipMemcpyDeviceToHost, s));
-    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    if (grid_ok) {
+        n_pairs = grid_query(grid, dg.d_box.get(), n_dst, cand_off, cand_src, grid_rows, s);
+    } else {
+        if (model)
+            proposer_candidates(sg, dg, *model, cand_off, cand_src, accepted, s);
+        else
+            lbvh_query(tree, dg.d_box.get(), n_dst, cand_off, cand_src, s);
+        AXIS_HIP_CHECK(hipMemcpyAsync(&n_pairs, cand_off.get() + n_dst, sizeof(index_t), hipMemcpyDeviceToHost, s));
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    tr.mark(model ? "propose" : "broad_phase");
 
     auto M = std::make_unique<InterpolationMatrix>();
     M->n_src = n_src;
@@ -903,7 +906,11 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     // with a proposer, room for the LBVH pairs of redone rows (~10%) so that
     // round 2 usually appends in place
     const index_t cap_pairs = model ? n_pairs + n_pairs / 8 + 4096 : std::max<index_t>(n_pairs, 1);
-    DeviceBuffer<std::int32_t> pair_row(cap_pairs);
+    DeviceBuffer<std::int32_t> pair_row;
+    if (grid_ok)
+        pair_row = std::move(grid_rows);
+    else
+        pair_row.resize(cap_pairs);
     DeviceBuffer<double> overlap(cap_pairs);
     DeviceBuffer<std::int32_t> arb(cap_pairs);
     if (model && cand_src.size() < static_cast<std::size_t>(cap_pairs)) {
@@ -913,7 +920,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     }
     DeviceBuffer<int> counters(2);
     AXIS_HIP_CHECK(hipMemsetAsync(counters.get(), 0, 2 * sizeof(int), s));
-    if (n_dst > 0) {
+    if (n_dst > 0 && !grid_ok) {
         fill_rows_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(cand_off.get(), n_dst, pair_row.get());
         AXIS_LAUNCH_CHECK();
     }

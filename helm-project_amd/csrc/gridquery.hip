@@ -34,13 +34,15 @@
 //      overhang and radial band (order-preserving integer keys, atomics
 //      after a wave reduction);
 //   3. row_finish_kernel: the keys to floats, suffix minima / prefix maxima;
-//   4. grid_count_kernel: one lane per destination cell counts its hits (the
-//      row tables and, when it fits, the longitude axis in LDS); cells with more than
-//      kLaneItems grid cells to test are listed for grid_long_kernel (one wave
-//      per cell, row by row, ordered writes by ballot), then an exclusive scan
-//      of the counts and the same two kernels fill.
+//   4. grid_count_kernel: one lane per destination cell counts its hits
+//      (the row tables and, when it fits, the longitude axis in LDS); cells
+//      with more than kLaneItems grid cells to test are listed for
+//      grid_long_kernel (one wave per cell, row by row); then an exclusive
+//      scan of the counts, and the same two kernels fill the candidate list
+//      and each candidate's destination row (writes ordered by ballot).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -52,13 +54,16 @@ namespace axis_hip::detail {
 namespace {
 
 constexpr int kLaneItems = 64;        // grid cells one lane tests before the cell goes to a wave
+constexpr int kUnroll = 4;            // box loads a lane keeps in flight
 constexpr int kMaxLonRanges = 3;      // longitude index ranges per row (wraps of the span)
 constexpr double kAxisGap = 1e-6;     // rectangle this near the polar axis: every longitude
-constexpr double kLonMarginDeg = 1e-7;
+constexpr double kLonMarginDeg = 1e-7;     // source overhangs (double arithmetic)
+constexpr float kDstLonMarginDeg = 1e-3f;  // destination spans (single precision)
 constexpr float kAllLon = 1e30f;      // row overhang meaning "every cell of the row"
 constexpr double kRad2Deg = 180.0 / kPi;
 constexpr int kQueryBlock = 256;
 constexpr int kLongBlocks = 1024;     // workgroups of grid_long_kernel (they loop over the list)
+constexpr index_t kCountBlocks = 2048;  // workgroups of grid_count_kernel (they loop over the cells)
 constexpr std::size_t kMaxLds = 64 * 1024;
 // per cell row: z-band suffix minimum and prefix maximum, longitude overhang,
 // radial band (distance of the boxes' xy-rectangles from the polar axis)
@@ -272,7 +277,7 @@ struct GridView {
     const float *rows;   // kRowFields arrays of nrow floats
     int nlon, nrow;      // lon node count; cell rows
     bool lon_lds;        // ulon copied to LDS (else read from HBM)
-    double u0, du;       // ulon[0] and the mean spacing (index guesses)
+    double u0, inv_du;   // ulon[0] and the inverse mean spacing (index guesses)
     const std::int32_t *cell_at;  // null: cell j (nlon - 1) + i is grid slot (i, j)
     const float *src_box;
 };
@@ -294,10 +299,10 @@ __device__ inline Tables load_tables(const GridView &G, double *s) {
 
 struct Dst {
     float bx[6];
-    int j0, j1;     // candidate rows
-    bool all_lon;   // rectangle near the axis
-    double a0, a1;  // angular span (degrees)
-    float r0, r1;   // distance range from the axis (widened)
+    int j0, j1;    // candidate rows
+    bool all_lon;  // rectangle near the axis
+    float a0, a1;  // angular span (degrees, widened)
+    float r0, r1;  // distance range from the axis (widened)
 };
 
 __device__ inline void dst_setup(const GridView &G, const Tables &T, const float *box, index_t d, Dst &D) {
@@ -307,22 +312,34 @@ __device__ inline void dst_setup(const GridView &G, const Tables &T, const float
     if (!(D.bx[0] <= D.bx[3] && D.bx[1] <= D.bx[4] && D.bx[2] <= D.bx[5])) return;  // empty (or NaN) box: no hit
     D.j0 = lower_bound_t(T.zhi, G.nrow, D.bx[2]);     // first row whose band top reaches the box
     D.j1 = upper_bound_t(T.zlo, G.nrow, D.bx[5]) - 1;  // last row whose band bottom is below the box top
-    double r0, r1;
-    rect_rho(D.bx[0], D.bx[3], D.bx[1], D.bx[4], r0, r1);
-    D.r0 = __double2float_rd(r0 - 1e-9);
-    D.r1 = __double2float_ru(r1 + 1e-9);
-    double lo, hi;
-    const double ref = atan2(0.5 * (static_cast<double>(D.bx[1]) + D.bx[4]), 0.5 * (static_cast<double>(D.bx[0]) + D.bx[3])) * kRad2Deg;
-    D.all_lon = !rect_span(D.bx[0], D.bx[3], D.bx[1], D.bx[4], ref, lo, hi);
-    D.a0 = ref + lo - kLonMarginDeg;
-    D.a1 = ref + hi + kLonMarginDeg;
+    // single precision with margins far above its rounding (a wider range
+    // only adds grid cells to test)
+    const float *b = D.bx;
+    const float cx = b[0] > 0.0f ? b[0] : (b[3] < 0.0f ? -b[3] : 0.0f), cy = b[1] > 0.0f ? b[1] : (b[4] < 0.0f ? -b[4] : 0.0f);
+    const float fx = fmaxf(fabsf(b[0]), fabsf(b[3])), fy = fmaxf(fabsf(b[1]), fabsf(b[4]));
+    D.r0 = sqrtf(cx * cx + cy * cy) * (1.0f - 1e-5f) - 1e-7f;
+    D.r1 = sqrtf(fx * fx + fy * fy) * (1.0f + 1e-5f) + 1e-7f;
+    D.all_lon = b[0] <= static_cast<float>(kAxisGap) && b[3] >= -static_cast<float>(kAxisGap) && b[1] <= static_cast<float>(kAxisGap) &&
+                b[4] >= -static_cast<float>(kAxisGap);
+    if (D.all_lon) return;
+    const float rad2deg = static_cast<float>(kRad2Deg);
+    const float ref = atan2f(0.5f * (b[1] + b[4]), 0.5f * (b[0] + b[3])) * rad2deg;
+    float lo = 0.0f, hi = 0.0f;
+    for (int q = 0; q < 4; ++q) {
+        float d = atan2f((q & 2) ? b[4] : b[1], (q & 1) ? b[3] : b[0]) * rad2deg - ref;
+        d -= 360.0f * rintf(d * (1.0f / 360.0f));  // into [-180, 180]
+        lo = fminf(lo, d);
+        hi = fmaxf(hi, d);
+    }
+    D.a0 = ref + lo - kDstLonMarginDeg;
+    D.a1 = ref + hi + kDstLonMarginDeg;
 }
 
 // first i in [0, last] with ulon[i + 1] >= w (last + 1 if none): a guess from
 // the mean spacing walked a few steps, else a binary search
 __device__ inline int lon_first(const GridView &G, const double *ulon, double w) {
     const int last = G.nlon - 2;
-    int g = static_cast<int>(fmin(fmax(floor((w - G.u0) / G.du), 0.0), static_cast<double>(last + 1)));
+    int g = static_cast<int>(fmin(fmax(floor((w - G.u0) * G.inv_du), 0.0), static_cast<double>(last + 1)));
     for (int step = 0; step < 8; ++step) {
         if (g > 0 && ulon[g] >= w)
             --g;
@@ -336,7 +353,7 @@ __device__ inline int lon_first(const GridView &G, const double *ulon, double w)
 // last i in [0, last] with ulon[i] <= w (-1 if none)
 __device__ inline int lon_last(const GridView &G, const double *ulon, double w) {
     const int last = G.nlon - 2;
-    int g = static_cast<int>(fmin(fmax(floor((w - G.u0) / G.du), -1.0), static_cast<double>(last)));
+    int g = static_cast<int>(fmin(fmax(floor((w - G.u0) * G.inv_du), -1.0), static_cast<double>(last)));
     for (int step = 0; step < 8; ++step) {
         if (g < last && ulon[g + 1] <= w)
             ++g;
@@ -362,7 +379,7 @@ __device__ inline int row_ranges(const GridView &G, const Tables &T, int j, cons
         r1[0] = last;
         return 1;
     }
-    const double kmin = ceil((ulon[0] - w1) / 360.0), kmax = floor((ulon[G.nlon - 1] - w0) / 360.0);
+    const double kmin = ceil((ulon[0] - w1) * (1.0 / 360.0) - 1e-12), kmax = floor((ulon[G.nlon - 1] - w0) * (1.0 / 360.0) + 1e-12);
     if (kmax - kmin + 1 > kMaxLonRanges) {
         r0[0] = 0;
         r1[0] = last;
@@ -388,13 +405,8 @@ __device__ inline bool boxes_meet(const float *a, const float *b) {
 __device__ inline std::int32_t cell_of(const GridView &G, std::int64_t k) { return G.cell_at ? G.cell_at[k] : static_cast<std::int32_t>(k); }
 
 template <bool kFill>
-__global__ void __launch_bounds__(kQueryBlock) grid_count_kernel(GridView G, const float *dst_box, index_t n_dst, index_t *count,
-                                                                 const index_t *cand_off, std::int32_t *cand_src, std::int32_t *long_rows,
-                                                                 int *n_long) {
-    extern __shared__ double s_tab[];
-    const Tables T = load_tables(G, s_tab);
-    const index_t d = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (d >= n_dst) return;
+__device__ inline void count_one(const GridView &G, const Tables &T, const float *dst_box, index_t d, index_t *count, const index_t *cand_off,
+                                 std::int32_t *cand_src, std::int32_t *pair_row, std::int32_t *long_rows, int *n_long) {
     Dst D;
     dst_setup(G, T, dst_box, d, D);
     int r0[kMaxLonRanges], r1[kMaxLonRanges];
@@ -416,12 +428,12 @@ __global__ void __launch_bounds__(kQueryBlock) grid_count_kernel(GridView G, con
         const int n = row_ranges(G, T, j, D, r0, r1);
         const std::int64_t base = static_cast<std::int64_t>(j) * (G.nlon - 1);
         for (int q = 0; q < n; ++q)
-            for (int a = r0[q]; a <= r1[q]; a += 4) {
-                // four boxes in flight, then their tests
-                float bs[4][6];
-                std::int32_t cs[4];
+            for (int a = r0[q]; a <= r1[q]; a += kUnroll) {
+                // kUnroll boxes in flight, then their tests
+                float bs[kUnroll][6];
+                std::int32_t cs[kUnroll];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < kUnroll; ++u) {
                     cs[u] = a + u <= r1[q] ? cell_of(G, base + a + u) : -1;
                     if (cs[u] >= 0) {
                         const float *p = G.src_box + 6 * static_cast<std::int64_t>(cs[u]);
@@ -430,9 +442,12 @@ __global__ void __launch_bounds__(kQueryBlock) grid_count_kernel(GridView G, con
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
+                for (int u = 0; u < kUnroll; ++u)
                     if (cs[u] >= 0 && boxes_meet(bs[u], D.bx)) {
-                        if (kFill) cand_src[pos] = cs[u];
+                        if (kFill) {
+                            cand_src[pos] = cs[u];
+                            pair_row[pos] = static_cast<std::int32_t>(d);
+                        }
                         ++pos;
                     }
             }
@@ -440,10 +455,23 @@ __global__ void __launch_bounds__(kQueryBlock) grid_count_kernel(GridView G, con
     if (!kFill) count[d] = pos;
 }
 
+// one lane per destination cell; workgroups loop over the cells (the tables
+// are loaded once per workgroup)
+template <bool kFill>
+__global__ void __launch_bounds__(kQueryBlock) grid_count_kernel(GridView G, const float *dst_box, index_t n_dst, index_t *count,
+                                                                 const index_t *cand_off, std::int32_t *cand_src, std::int32_t *pair_row,
+                                                                 std::int32_t *long_rows, int *n_long) {
+    extern __shared__ double s_tab[];
+    const Tables T = load_tables(G, s_tab);
+    for (index_t d = static_cast<index_t>(blockIdx.x) * kQueryBlock + threadIdx.x; d < n_dst; d += static_cast<index_t>(gridDim.x) * kQueryBlock)
+        count_one<kFill>(G, T, dst_box, d, count, cand_off, cand_src, pair_row, long_rows, n_long);
+}
+
 // one wave per listed destination cell, row by row; workgroups loop over the list
 template <bool kFill>
 __global__ void __launch_bounds__(kQueryBlock) grid_long_kernel(GridView G, const float *dst_box, const std::int32_t *long_rows, const int *n_long,
-                                                                index_t *count, const index_t *cand_off, std::int32_t *cand_src) {
+                                                                index_t *count, const index_t *cand_off, std::int32_t *cand_src,
+                                                                std::int32_t *pair_row) {
     extern __shared__ double s_tab[];
     const Tables T = load_tables(G, s_tab);
     const int lane = threadIdx.x & 63;
@@ -468,7 +496,11 @@ __global__ void __launch_bounds__(kQueryBlock) grid_long_kernel(GridView G, cons
                         hit = boxes_meet(G.src_box + 6 * static_cast<std::int64_t>(c), D.bx);
                     }
                     const std::uint64_t m = __ballot(hit);
-                    if (kFill && hit) cand_src[pos + __popcll(m & ((std::uint64_t(1) << lane) - 1))] = c;
+                    if (kFill && hit) {
+                        const index_t w = pos + __popcll(m & ((std::uint64_t(1) << lane) - 1));
+                        cand_src[w] = c;
+                        pair_row[w] = static_cast<std::int32_t>(d);
+                    }
                     pos += __popcll(m);
                 }
         }
@@ -481,6 +513,9 @@ __global__ void fill_u32_kernel(unsigned *p, std::int64_t n, unsigned v) {
     if (i < n) p[i] = v;
 }
 
+unsigned count_blocks(index_t n_dst) {
+    return static_cast<unsigned>(std::clamp<index_t>((n_dst + kQueryBlock - 1) / kQueryBlock, 1, kCountBlocks));
+}
 std::size_t rows_lds(int nrow) { return sizeof(double) * ((kRowFields * static_cast<std::size_t>(nrow) + 1) / 2); }
 bool lon_fits(int nlon, int nrow) { return rows_lds(nrow) + sizeof(double) * nlon <= kMaxLds; }
 std::size_t query_lds(int nlon, int nrow) { return rows_lds(nrow) + (lon_fits(nlon, nrow) ? sizeof(double) * nlon : 0); }
@@ -542,11 +577,11 @@ bool grid_index_build(const Mesh &src, const GridAxes &ax, const float *src_boxe
     return G.ready;
 }
 
-void grid_query(const GridIndex &G, const float *dst_boxes, index_t n_dst, DeviceBuffer<index_t> &cand_off, DeviceBuffer<std::int32_t> &cand_src,
-                hipStream_t s) {
+index_t grid_query(const GridIndex &G, const float *dst_boxes, index_t n_dst, DeviceBuffer<index_t> &cand_off, DeviceBuffer<std::int32_t> &cand_src,
+                   DeviceBuffer<std::int32_t> &pair_row, hipStream_t s) {
     if (!G.ready) throw InvalidArgument("grid_query: no grid index");
     PhaseTrace tr("broad_phase", s);
-    const double u0 = G.ulon0, du = G.dlon;
+    const double u0 = G.ulon0, du = 1.0 / G.dlon;
     const GridView V{G.axes.get(), G.rows.get(), G.nlon, G.nlat - 1, lon_fits(G.nlon, G.nlat - 1), u0, du, G.grid_order ? nullptr : G.cell_at.get(),
                      G.src_boxes};
     const std::size_t lds = query_lds(G.nlon, G.nlat - 1);
@@ -557,10 +592,11 @@ void grid_query(const GridIndex &G, const float *dst_boxes, index_t n_dst, Devic
     DeviceBuffer<int> n_long(1);
     AXIS_HIP_CHECK(hipMemsetAsync(n_long.get(), 0, sizeof(int), s));
     if (n_dst > 0) {
-        grid_count_kernel<false><<<grid_for(n_dst, kQueryBlock), kQueryBlock, lds, s>>>(V, dst_boxes, n_dst, count.get(), nullptr, nullptr,
-                                                                                      long_rows.get(), n_long.get());
+        grid_count_kernel<false><<<count_blocks(n_dst), kQueryBlock, lds, s>>>(
+            V, dst_boxes, n_dst, count.get(), nullptr, nullptr, nullptr, long_rows.get(), n_long.get());
         AXIS_LAUNCH_CHECK();
-        grid_long_kernel<false><<<kLongBlocks, kQueryBlock, lds, s>>>(V, dst_boxes, long_rows.get(), n_long.get(), count.get(), nullptr, nullptr);
+        grid_long_kernel<false>
+            <<<kLongBlocks, kQueryBlock, lds, s>>>(V, dst_boxes, long_rows.get(), n_long.get(), count.get(), nullptr, nullptr, nullptr);
         AXIS_LAUNCH_CHECK();
     }
     exclusive_scan_i64(count.get(), cand_off.get(), n_dst, s);
@@ -573,15 +609,17 @@ void grid_query(const GridIndex &G, const float *dst_boxes, index_t n_dst, Devic
     // the reference stores candidate offsets as int (weight_generator.cpp:247-254)
     if (total >= (index_t(1) << 31)) throw RuntimeError("broad phase: candidate pair count exceeds 2^31");
     cand_src.resize(std::max<index_t>(total, 1));
+    pair_row.resize(std::max<index_t>(total, 1));
     if (n_dst > 0 && total > 0) {
-        grid_count_kernel<true><<<grid_for(n_dst, kQueryBlock), kQueryBlock, lds, s>>>(V, dst_boxes, n_dst, nullptr, cand_off.get(), cand_src.get(),
-                                                                                     nullptr, nullptr);
+        grid_count_kernel<true><<<count_blocks(n_dst), kQueryBlock, lds, s>>>(
+            V, dst_boxes, n_dst, nullptr, cand_off.get(), cand_src.get(), pair_row.get(), nullptr, nullptr);
         AXIS_LAUNCH_CHECK();
         grid_long_kernel<true><<<kLongBlocks, kQueryBlock, lds, s>>>(V, dst_boxes, long_rows.get(), n_long.get(), nullptr, cand_off.get(),
-                                                                     cand_src.get());
+                                                                     cand_src.get(), pair_row.get());
         AXIS_LAUNCH_CHECK();
     }
     tr.mark("grid_fill");
+    return total;
 }
 
 }  // namespace axis_hip::detail

@@ -951,6 +951,16 @@ bool node_xyz_device(const Mesh &m, const GridAxes &ax, MeshGeom &g, hipStream_t
     return true;
 }
 
+// host threads for the libm angles beside a generation (AXIS_TRIG_THREADS)
+static int trig_threads() {
+    static const int n = [] {
+        if (const char *e = std::getenv("AXIS_TRIG_THREADS"))
+            if (int v = std::atoi(e); v > 0) return v;
+        return std::max(1, host_threads() - 3);
+    }();
+    return n;
+}
+
 bool node_xyz_trig(const Mesh &m, double *d_out, hipStream_t st, const std::atomic<bool> *cancel) {
     if (m.coord_system() != CoordinateSystem::SphericalDeg || std::getenv("AXIS_HOST_NODE_XYZ") != nullptr) return false;
     const index_t n = m.n_nodes();
@@ -961,11 +971,14 @@ bool node_xyz_trig(const Mesh &m, double *d_out, hipStream_t st, const std::atom
     DeviceBuffer<std::uint64_t> bits(6 * static_cast<std::size_t>(words));
     // (nodes on the multiples of 90 degrees: a few per thousand on a cubed sphere)
     const int full_cap = static_cast<int>(std::min<index_t>(n / 16 + kXyzFixCap, index_t(1) << 30));
+    const int full_first = std::min(full_cap, 1 << 16);  // list entries fetched with the flags
     DeviceBuffer<index_t> full(full_cap);
     DeviceBuffer<int> n_full(1);
     PinnedArray<std::uint64_t> hb;
     hb.resize(6 * static_cast<std::size_t>(words) + 1);
     std::uint64_t *flags = hb.data(), *res = flags + 2 * words, *h_nfull = flags + 6 * words;
+    PinnedArray<index_t> h_full;
+    h_full.resize(full_cap);
     AXIS_HIP_CHECK(hipMemsetAsync(n_full.get(), 0, sizeof(int), st));
     const double *d_lon = m.d_coords.get(), *d_lat = d_lon + n;
     trig_xyz_kernel<<<grid_for(n, 256), 256, 0, st>>>(d_lon, d_lat, n, words, d_out, bits.get(), bits.get() + words, full.get(), n_full.get(),
@@ -974,19 +987,36 @@ bool node_xyz_trig(const Mesh &m, double *d_out, hipStream_t st, const std::atom
     AXIS_HIP_CHECK(hipMemcpyAsync(flags, bits.get(), 2 * sizeof(std::uint64_t) * words, hipMemcpyDeviceToHost, st));
     *h_nfull = 0;
     AXIS_HIP_CHECK(hipMemcpyAsync(h_nfull, n_full.get(), sizeof(int), hipMemcpyDeviceToHost, st));
+    AXIS_HIP_CHECK(hipMemcpyAsync(h_full.data(), full.get(), sizeof(index_t) * full_first, hipMemcpyDeviceToHost, st));
     AXIS_HIP_CHECK(hipStreamSynchronize(st));
     const int nf = static_cast<int>(*h_nfull & 0xffffffffu);
     if (nf > full_cap) return false;
-    // host libm for the flagged angles: one bit per sine and cosine
+    if (nf > full_first) {
+        AXIS_HIP_CHECK(hipMemcpyAsync(h_full.data() + full_first, full.get() + full_first, sizeof(index_t) * (nf - full_first),
+                                      hipMemcpyDeviceToHost, st));
+        AXIS_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    // host libm: one bit per sine and cosine of the flagged angles, every
+    // value of the listed nodes (one pass: words first, then list chunks);
+    // three cores stay free for the generating thread and the HIP runtime
     const double *lon = m.h_coords.data(), *lat = lon + n;
+    PinnedArray<double> h_vals;
+    h_vals.resize(3 * static_cast<std::size_t>(std::max(nf, 1)));
+    constexpr index_t kListChunk = 64;  // list entries per word-sized work item
+    const index_t list_items = (nf + kListChunk - 1) / kListChunk;
     std::atomic<bool> stopped{false};
     parallel_range(
-        words,
+        words + list_items,
         [&](index_t b, index_t e) {
             for (index_t w = b; w < e; ++w) {
                 if (cancel && (w & 63) == 0 && cancel->load(std::memory_order_relaxed)) {
                     stopped.store(true);
                     return;
+                }
+                if (w >= words) {
+                    const index_t k0 = (w - words) * kListChunk, k1 = std::min<index_t>(k0 + kListChunk, nf);
+                    for (index_t k = k0; k < k1; ++k) node_xyz_one(m, h_full[k], &h_vals[3 * static_cast<std::size_t>(k)]);
+                    continue;
                 }
                 std::uint64_t r[4] = {0, 0, 0, 0};
                 for (int a = 0; a < 2; ++a) {
@@ -1002,26 +1032,19 @@ bool node_xyz_trig(const Mesh &m, double *d_out, hipStream_t st, const std::atom
                 for (int q = 0; q < 4; ++q) res[q * words + w] = r[q];
             }
         },
-        256, std::max(1, host_threads() - 2));
+        256, trig_threads());
     if (stopped.load()) return false;
     AXIS_HIP_CHECK(hipMemcpyAsync(bits.get() + 2 * words, res, 4 * sizeof(std::uint64_t) * words, hipMemcpyHostToDevice, st));
     trig_fix_kernel<<<grid_for(n, 256), 256, 0, st>>>(d_lon, d_lat, n, words, bits.get(), bits.get() + words, bits.get() + 2 * words, d_out);
     AXIS_LAUNCH_CHECK();
-    if (nf > 0) {  // every value of these nodes from host libm
-        std::vector<index_t> nodes(nf);
-        full.download(nodes.data(), nf, st);
-        AXIS_HIP_CHECK(hipStreamSynchronize(st));
-        std::vector<double> vals(3 * static_cast<std::size_t>(nf));
-        for (int k = 0; k < nf; ++k) node_xyz_one(m, nodes[k], &vals[3 * static_cast<std::size_t>(k)]);
-        DeviceBuffer<index_t> d_nodes(nf);
-        DeviceBuffer<double> d_vals(vals.size());
-        d_nodes.upload(nodes.data(), nf, st);
-        d_vals.upload(vals.data(), vals.size(), st);
-        scatter_xyz_kernel<<<grid_for(nf, 256), 256, 0, st>>>(d_nodes.get(), d_vals.get(), nf, d_out);
+    if (nf > 0) {  // the listed nodes' xyz (list order kept on the device)
+        DeviceBuffer<double> d_vals(3 * static_cast<std::size_t>(nf));
+        AXIS_HIP_CHECK(hipMemcpyAsync(d_vals.get(), h_vals.data(), sizeof(double) * 3 * nf, hipMemcpyHostToDevice, st));
+        scatter_xyz_kernel<<<grid_for(nf, 256), 256, 0, st>>>(full.get(), d_vals.get(), nf, d_out);
         AXIS_LAUNCH_CHECK();
     }
-    // the pinned words go back to their cache on return: the copies are done first
-    AXIS_HIP_CHECK(hipStreamSynchronize(st));
+    // (the pinned arrays go back to their cache pending the next device
+    // synchronisation, so the copies above read them intact)
     tr.mark("trig_xyz");
     return true;
 }
