@@ -91,9 +91,25 @@ __device__ inline int cell_nverts(const Side &m, index_t c) {
     return static_cast<int>(nv < 32 ? nv : 32);
 }
 
+// A pair for host libm: its index in arb (the count in n_arb), and (pair,
+// destination row, source cell) in trip while the count is below kTripCap,
+// so the host fetches what it needs with the count in one readback.
+constexpr int kTripCap = 1 << 16;
+
+__device__ inline void queue_for_host(index_t p, index_t j, index_t i, std::int32_t *arb, int *n_arb, std::int32_t *trip) {
+    const int k = atomicAdd(n_arb, 1);
+    arb[k] = static_cast<std::int32_t>(p);
+    if (k < kTripCap) {
+        trip[3 * k] = static_cast<std::int32_t>(p);
+        trip[3 * k + 1] = static_cast<std::int32_t>(j);
+        trip[3 * k + 2] = static_cast<std::int32_t>(i);
+    }
+}
+
 template <int CAP>
 __global__ void __launch_bounds__(kWave) overlap_kernel(index_t p0, index_t p1, const std::int32_t *pair_row, const std::int32_t *pair_src,
-                                                        Side src, Side dst, double *overlap, std::int32_t *arb, int *n_arb, int *n_tie) {
+                                                        Side src, Side dst, double *overlap, std::int32_t *arb, int *n_arb, int *n_tie,
+                                                        std::int32_t *trip) {
     __shared__ double buf[3][CAP][kWave];  // one lane-private polygon per lane, clipped in place
     const int lane = threadIdx.x;
     const index_t p = p0 + static_cast<index_t>(blockIdx.x) * kWave + lane;
@@ -135,10 +151,7 @@ __global__ void __launch_bounds__(kWave) overlap_kernel(index_t p0, index_t p1, 
         }
     }
     overlap[p] = ov;
-    if (to_host) {
-        int k = atomicAdd(n_arb, 1);
-        arb[k] = static_cast<std::int32_t>(p);
-    }
+    if (to_host) queue_for_host(p, j, i, arb, n_arb, trip);
 }
 
 // Two-kernel form of overlap_kernel for pair ranges below 2^31: the flags and
@@ -204,7 +217,7 @@ __global__ void __launch_bounds__(kSelThreads) cap_select_kernel(index_t p0, ind
 template <int CAP>
 __global__ void __launch_bounds__(kWave) clip_live_kernel(index_t p0, const std::uint32_t *live, const std::int32_t *n_live, unsigned n_chunks,
                                                           const std::int32_t *pair_row, const std::int32_t *pair_src, Side src, Side dst,
-                                                          double *overlap, std::int32_t *arb, int *n_arb) {
+                                                          double *overlap, std::int32_t *arb, int *n_arb, std::int32_t *trip) {
     __shared__ double buf[3][CAP][kWave];
     const int lane = threadIdx.x;
     // block b clips wave b / n_chunks of chunk b % n_chunks: the blocks with
@@ -237,10 +250,7 @@ __global__ void __launch_bounds__(kWave) clip_live_kernel(index_t p0, const std:
     }
     if (overflow) to_host = true;
     overlap[p] = ov;
-    if (to_host) {
-        int k = atomicAdd(n_arb, 1);
-        arb[k] = static_cast<std::int32_t>(p);
-    }
+    if (to_host) queue_for_host(p, j, i, arb, n_arb, trip);
 }
 
 // Rows the proposer's round must hand to the LBVH: rejected by the guard, or
@@ -434,65 +444,47 @@ constexpr int kLongRowMax = 4096; // sorted by one workgroup in LDS
 constexpr int kLongRowBlock = 256;
 constexpr int kMidRow = 64;        // sorted by one wave in a small LDS tile
 
-__global__ void compact_grouped_kernel(const double *overlap, const index_t *pos, index_t n, const std::int32_t *pair_src, std::int32_t *col_c,
-                                       double *ov_c) {
+constexpr int kHistMax = 16;  // row-length histogram: lengths 0 .. kHistMax, then longer
+
+// Nonzero overlaps per destination row (row_ptr after a scan): one thread per
+// pair.  A row's pairs are contiguous and rows ascend, so each wave splits
+// into runs of one row; the first lane of a run adds the run's count.
+__global__ void __launch_bounds__(256) nz_count_kernel(const double *overlap, const std::int32_t *pair_row, index_t n, unsigned long long *rnz) {
     const index_t p = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (p >= n || !(overlap[p] > 0.0)) return;
-    const index_t q = pos[p];
-    col_c[q] = pair_src[p];
-    ov_c[q] = overlap[p];
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const bool in = p < n;
+    const std::int32_t r = in ? pair_row[p] : -1;
+    const bool nz = in && overlap[p] > 0.0;
+    const std::int32_t prev = __shfl_up(r, 1);
+    const bool head = in && (lane == 0 || prev != r);
+    const std::uint64_t heads = __ballot(head), nzm = __ballot(nz);
+    if (!head) return;
+    const std::uint64_t after = lane == 63 ? 0 : heads >> (lane + 1);
+    const int next = after ? lane + 1 + __builtin_ctzll(after) : 64;
+    const std::uint64_t run = (next == 64 ? ~0ull : ((1ull << next) - 1)) & ~((1ull << lane) - 1);
+    const int cnt = __popcll(nzm & run);
+    if (cnt > 0) atomicAdd(&rnz[r], static_cast<unsigned long long>(cnt));
 }
 
-__global__ void __launch_bounds__(256) short_rows_kernel(const index_t *cand_off, const index_t *pos, index_t n_dst, const std::int32_t *col_c,
-                                                         const double *ov_c, const double *area_dst, int frac_area, index_t *row_ptr,
-                                                         index_t *col, double *val, double *ov_s, double *frac_b,
-                                                         int *max_row, std::int32_t *long_rows, int *n_long, unsigned long long *len_hist) {
-    // row-length histogram for the apply (lengths 0 .. kHistMax, then longer)
-    constexpr int kHistMax = 16;
+// Per destination row: the longest row, the row-length histogram for the
+// apply, and the rows too long for one thread, listed (up to kMidRow entries
+// from the front of the list, longer ones from the back).
+__global__ void __launch_bounds__(256) row_lengths_kernel(const unsigned long long *rnz, index_t n_dst, int *max_row, std::int32_t *long_rows,
+                                                          int *n_long, unsigned long long *len_hist) {
     __shared__ unsigned hist[kHistMax + 2];
     if (threadIdx.x < kHistMax + 2) hist[threadIdx.x] = 0;
     __syncthreads();
     int longest = 0;
     for (index_t r = static_cast<index_t>(blockIdx.x) * 256 + threadIdx.x; r < n_dst; r += static_cast<index_t>(gridDim.x) * 256) {
-        const index_t b = pos[cand_off[r]], e = pos[cand_off[r + 1]];
-        row_ptr[r] = b;
-        if (r == n_dst - 1) row_ptr[n_dst] = e;
-        const int len = static_cast<int>(min<index_t>(e - b, index_t(1) << 30));
+        const int len = static_cast<int>(min<unsigned long long>(rnz[r], 1ull << 30));
         longest = max(longest, len);
         atomicAdd(&hist[len > kHistMax ? kHistMax + 1 : len], 1u);
         if (len > kShortRow) {
-            // rows of up to kMidRow entries from the front of the list, longer ones from the back
             if (len <= kMidRow)
                 long_rows[atomicAdd(&n_long[0], 1)] = static_cast<std::int32_t>(r);
             else
                 long_rows[n_dst - 1 - atomicAdd(&n_long[1], 1)] = static_cast<std::int32_t>(r);
-            continue;
         }
-        std::int32_t cc[kShortRow];
-        double oo[kShortRow];
-        for (int k = 0; k < len; ++k) {
-            std::int32_t c = col_c[b + k];
-            double o = ov_c[b + k];
-            int m = k;
-            for (; m > 0 && cc[m - 1] > c; --m) {
-                cc[m] = cc[m - 1];
-                oo[m] = oo[m - 1];
-            }
-            cc[m] = c;
-            oo[m] = o;
-        }
-        const double a = area_dst[r];
-        double fb = 0.0;
-        for (int k = 0; k < len; ++k) fb += oo[k] / a;
-        const bool divide = frac_area && fb > 0.0;
-        for (int k = 0; k < len; ++k) {
-            double w = fmax(oo[k] / a, 0.0);
-            if (divide) w /= fb;
-            val[b + k] = w;
-            col[b + k] = cc[k];
-            ov_s[b + k] = oo[k];
-        }
-        frac_b[r] = fmin(fb, 1.0);
     }
     for (int d = 32; d >= 1; d >>= 1) longest = max(longest, __shfl_xor(longest, d, 64));
     __shared__ int part[4];
@@ -505,68 +497,127 @@ __global__ void __launch_bounds__(256) short_rows_kernel(const index_t *cand_off
     if (threadIdx.x < kHistMax + 2 && hist[threadIdx.x]) atomicAdd(&len_hist[threadIdx.x], static_cast<unsigned long long>(hist[threadIdx.x]));
 }
 
+// rows of at most kShortRow nonzeros, one thread each: the row's nonzero
+// candidates in column order (insertion sort), then weights and frac_b
+__global__ void __launch_bounds__(256) short_rows_kernel(const index_t *cand_off, const index_t *row_ptr, index_t n_dst, const std::int32_t *pair_src,
+                                                         const double *overlap, const double *area_dst, int frac_area, index_t *col, double *val,
+                                                         double *ov_s, double *frac_b) {
+    for (index_t r = static_cast<index_t>(blockIdx.x) * 256 + threadIdx.x; r < n_dst; r += static_cast<index_t>(gridDim.x) * 256) {
+        const index_t b = row_ptr[r];
+        const int len = static_cast<int>(min<index_t>(row_ptr[r + 1] - b, index_t(1) << 30));
+        if (len > kShortRow) continue;  // (a wave or a workgroup takes it)
+        std::int32_t cc[kShortRow];
+        double oo[kShortRow];
+        int k = 0;
+        for (index_t p = cand_off[r], e = cand_off[r + 1]; p < e && k < len; ++p) {
+            const double o = overlap[p];
+            if (!(o > 0.0)) continue;
+            const std::int32_t c = pair_src[p];
+            int m = k;
+            for (; m > 0 && cc[m - 1] > c; --m) {
+                cc[m] = cc[m - 1];
+                oo[m] = oo[m - 1];
+            }
+            cc[m] = c;
+            oo[m] = o;
+            ++k;
+        }
+        const double a = area_dst[r];
+        double fb = 0.0;
+        for (int q = 0; q < len; ++q) fb += oo[q] / a;
+        const bool divide = frac_area && fb > 0.0;
+        for (int q = 0; q < len; ++q) {
+            double w = fmax(oo[q] / a, 0.0);
+            if (divide) w /= fb;
+            val[b + q] = w;
+            col[b + q] = cc[q];
+            ov_s[b + q] = oo[q];
+        }
+        frac_b[r] = fmin(fb, 1.0);
+    }
+}
+
 // rows of kShortRow < len <= CAP: bitonic column sort in LDS, the ordered sum
 // by one thread, the rest in parallel (columns are distinct within a row, so
 // the sorted order does not depend on the sort).  CAP = kMidRow with one wave
 // for most long rows (LDS of a few hundred bytes: many rows per CU), CAP =
 // kLongRowMax with kLongRowBlock threads for the rest.
 template <int CAP, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) long_rows_kernel(const std::int32_t *long_rows, const index_t *row_ptr, const std::int32_t *col_c,
-                                                          const double *ov_c, const double *area_dst, int frac_area, index_t *col, double *val,
+__global__ void __launch_bounds__(BLOCK) long_rows_kernel(const std::int32_t *long_rows, const int *n_list, bool from_back, index_t n_dst,
+                                                          const index_t *cand_off, const index_t *row_ptr, const std::int32_t *pair_src,
+                                                          const double *overlap, const double *area_dst, int frac_area, index_t *col, double *val,
                                                           double *ov_s, double *frac_b) {
     __shared__ std::int32_t sc[CAP];
     __shared__ double so[CAP];
     __shared__ double s_fb;
+    __shared__ int s_n;
     const int tid = static_cast<int>(threadIdx.x);
-    const index_t r = long_rows[blockIdx.x];
-    const index_t b = row_ptr[r];
-    const int len = static_cast<int>(row_ptr[r + 1] - b);  // <= CAP (checked by the host)
-    int P = 2;
-    while (P < len) P <<= 1;
-    for (int k = tid; k < P; k += BLOCK) {
-        sc[k] = k < len ? col_c[b + k] : 0x7FFFFFFF;
-        so[k] = k < len ? ov_c[b + k] : 0.0;
-    }
-    __syncthreads();
-    for (int size = 2; size <= P; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int i = tid; i < P / 2; i += BLOCK) {
-                const int lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
-                const bool asc = (lo & size) == 0;
-                if ((sc[lo] > sc[hi]) == asc) {
-                    const std::int32_t tc = sc[lo];
-                    sc[lo] = sc[hi];
-                    sc[hi] = tc;
-                    const double to = so[lo];
-                    so[lo] = so[hi];
-                    so[hi] = to;
-                }
+    const int nl = *n_list;
+    for (int li = blockIdx.x; li < nl; li += gridDim.x) {  // (workgroups loop over the listed rows)
+        const index_t r = from_back ? long_rows[n_dst - 1 - li] : long_rows[li];
+        const index_t b = row_ptr[r];
+        const int len = static_cast<int>(row_ptr[r + 1] - b);  // <= CAP (checked by the host)
+        int P = 2;
+        while (P < len) P <<= 1;
+        // the row's nonzero candidates into LDS, in any order (columns are
+        // distinct within a row, so the sort below fixes the order)
+        if (tid == 0) s_n = 0;
+        __syncthreads();
+        for (index_t p = cand_off[r] + tid, e = cand_off[r + 1]; p < e; p += BLOCK) {
+            const double o = overlap[p];
+            if (o > 0.0) {
+                const int k = atomicAdd(&s_n, 1);
+                sc[k] = pair_src[p];
+                so[k] = o;
             }
-            __syncthreads();
         }
-    }
-    // sorted entries out, then the quotients overlap / area in place (in
-    // parallel), then their sum in column order by one thread
-    const double a = area_dst[r];
-    for (int k = tid; k < len; k += BLOCK) {
-        col[b + k] = sc[k];
-        ov_s[b + k] = so[k];
-        so[k] = so[k] / a;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        double fb = 0.0;
-        for (int k = 0; k < len; ++k) fb += so[k];
-        s_fb = fb;
-        frac_b[r] = fmin(fb, 1.0);
-    }
-    __syncthreads();
-    const double fb = s_fb;
-    const bool divide = frac_area && fb > 0.0;
-    for (int k = tid; k < len; k += BLOCK) {
-        double w = fmax(so[k], 0.0);
-        if (divide) w /= fb;
-        val[b + k] = w;
+        __syncthreads();
+        for (int k = len + tid; k < P; k += BLOCK) {
+            sc[k] = 0x7FFFFFFF;
+            so[k] = 0.0;
+        }
+        __syncthreads();
+        for (int size = 2; size <= P; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int i = tid; i < P / 2; i += BLOCK) {
+                    const int lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
+                    const bool asc = (lo & size) == 0;
+                    if ((sc[lo] > sc[hi]) == asc) {
+                        const std::int32_t tc = sc[lo];
+                        sc[lo] = sc[hi];
+                        sc[hi] = tc;
+                        const double to = so[lo];
+                        so[lo] = so[hi];
+                        so[hi] = to;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // sorted entries out, then the quotients overlap / area in place (in
+        // parallel), then their sum in column order by one thread
+        const double a = area_dst[r];
+        for (int k = tid; k < len; k += BLOCK) {
+            col[b + k] = sc[k];
+            ov_s[b + k] = so[k];
+            so[k] = so[k] / a;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double fb = 0.0;
+            for (int k = 0; k < len; ++k) fb += so[k];
+            s_fb = fb;
+            frac_b[r] = fmin(fb, 1.0);
+        }
+        __syncthreads();
+        const double fb = s_fb;
+        const bool divide = frac_area && fb > 0.0;
+        for (int k = tid; k < len; k += BLOCK) {
+            double w = fmax(so[k], 0.0);
+            if (divide) w /= fb;
+            val[b + k] = w;
+        }
+        __syncthreads();  // (the next row reuses the LDS)
     }
 }
 
@@ -919,6 +970,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
         cand_src = std::move(cs);
     }
     DeviceBuffer<int> counters(2);
+    DeviceBuffer<std::int32_t> trip(3 * static_cast<std::size_t>(kTripCap));
     AXIS_HIP_CHECK(hipMemsetAsync(counters.get(), 0, 2 * sizeof(int), s));
     if (n_dst > 0 && !grid_ok) {
         fill_rows_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(cand_off.get(), n_dst, pair_row.get());
@@ -959,61 +1011,82 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
             const unsigned cgrid = static_cast<unsigned>(n_chunks * (kSelChunk / kWave));
             if (need <= 8)
                 clip_live_kernel<8><<<cgrid, kWave, 0, s>>>(p0, live.get(), n_live.get(), static_cast<unsigned>(n_chunks), pair_row.get(), cand_src.get(), S, D, overlap.get(),
-                                                            arb.get(), counters.get());
+                                                            arb.get(), counters.get(), trip.get());
             else if (need <= 16)
                 clip_live_kernel<16><<<cgrid, kWave, 0, s>>>(p0, live.get(), n_live.get(), static_cast<unsigned>(n_chunks), pair_row.get(), cand_src.get(), S, D, overlap.get(),
-                                                             arb.get(), counters.get());
+                                                             arb.get(), counters.get(), trip.get());
             else
                 clip_live_kernel<24><<<cgrid, kWave, 0, s>>>(p0, live.get(), n_live.get(), static_cast<unsigned>(n_chunks), pair_row.get(), cand_src.get(), S, D, overlap.get(),
-                                                             arb.get(), counters.get());
+                                                             arb.get(), counters.get(), trip.get());
             AXIS_LAUNCH_CHECK();
             return;
         }
         if (need <= 8)
             overlap_kernel<8><<<grid, kWave, 0, s>>>(p0, p1, pair_row.get(), cand_src.get(), S, D, overlap.get(), arb.get(), counters.get(),
-                                                     counters.get() + 1);
+                                                     counters.get() + 1, trip.get());
         else if (need <= 16)
             overlap_kernel<16><<<grid, kWave, 0, s>>>(p0, p1, pair_row.get(), cand_src.get(), S, D, overlap.get(), arb.get(), counters.get(),
-                                                      counters.get() + 1);
+                                                      counters.get() + 1, trip.get());
         else
             overlap_kernel<24><<<grid, kWave, 0, s>>>(p0, p1, pair_row.get(), cand_src.get(), S, D, overlap.get(), arb.get(), counters.get(),
-                                                      counters.get() + 1);
+                                                      counters.get() + 1, trip.get());
         AXIS_LAUNCH_CHECK();
     };
     // host libm decides the queued pairs arb[a0 .. current count)
     int arb_done = 0;
     auto arbitrate = [&]() {
-        int h_cnt[2] = {0, 0};
-        AXIS_HIP_CHECK(hipMemcpyAsync(h_cnt, counters.get(), sizeof h_cnt, hipMemcpyDeviceToHost, s));
+        // the counters and the first queued triples in one readback
+        constexpr int kFetch = 4096;
+        const int fetch = std::max(0, std::min(kFetch, kTripCap - arb_done));
+        PinnedArray<std::int32_t> h;
+        h.resize(2 + 3 * static_cast<std::size_t>(kFetch));
+        AXIS_HIP_CHECK(hipMemcpyAsync(h.data(), counters.get(), 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+        if (fetch > 0)
+            AXIS_HIP_CHECK(hipMemcpyAsync(h.data() + 2, trip.get() + 3 * static_cast<std::size_t>(arb_done), 3 * sizeof(std::int32_t) * fetch,
+                                          hipMemcpyDeviceToHost, s));
         AXIS_HIP_CHECK(hipStreamSynchronize(s));
-        M->n_host_arbitrated = h_cnt[0];
-        M->n_cap_borderline = h_cnt[1];
-        const int na = h_cnt[0] - arb_done;
+        M->n_host_arbitrated = h[0];
+        M->n_cap_borderline = h[1];
+        const int na = h[0] - arb_done;
         if (na <= 0) return;
-        DeviceBuffer<std::int32_t> d_trip(3 * static_cast<std::size_t>(na));
-        gather_arbitrated_kernel<<<grid_for(na, 256), 256, 0, s>>>(arb.get() + arb_done, na, pair_row.get(), cand_src.get(), d_trip.get());
-        AXIS_LAUNCH_CHECK();
-        std::vector<std::int32_t> trip(3 * static_cast<std::size_t>(na)), pairs(na);
-        d_trip.download(trip.data(), 3 * static_cast<std::size_t>(na), s);
-        AXIS_HIP_CHECK(hipStreamSynchronize(s));
-        std::vector<double> vals(na);
+        const std::int32_t *tr3 = h.data() + 2;
+        std::vector<std::int32_t> more;  // beyond the first fetch
+        if (na > fetch) {
+            more.resize(3 * static_cast<std::size_t>(na));
+            if (arb_done + na <= kTripCap) {
+                AXIS_HIP_CHECK(hipMemcpyAsync(more.data(), trip.get() + 3 * static_cast<std::size_t>(arb_done), 3 * sizeof(std::int32_t) * na,
+                                              hipMemcpyDeviceToHost, s));
+            } else {  // (more than kTripCap queued: gathered from arb)
+                DeviceBuffer<std::int32_t> d_trip(3 * static_cast<std::size_t>(na));
+                gather_arbitrated_kernel<<<grid_for(na, 256), 256, 0, s>>>(arb.get() + arb_done, na, pair_row.get(), cand_src.get(), d_trip.get());
+                AXIS_LAUNCH_CHECK();
+                d_trip.download(more.data(), 3 * static_cast<std::size_t>(na), s);
+            }
+            AXIS_HIP_CHECK(hipStreamSynchronize(s));
+            tr3 = more.data();
+        }
+        PinnedArray<std::int32_t> pairs;
+        PinnedArray<double> vals;
+        pairs.resize(na);
+        vals.resize(na);
         HostSide HS{&src, &sg}, HD{&dst, &dg};
         parallel_range(
             na, [&](index_t b, index_t e) {
                 for (index_t k = b; k < e; ++k) {
-                    pairs[k] = trip[3 * k];
-                    vals[k] = host_pair_overlap(HS, HD, trip[3 * k + 1], trip[3 * k + 2]);
+                    pairs[k] = tr3[3 * k];
+                    vals[k] = host_pair_overlap(HS, HD, tr3[3 * k + 1], tr3[3 * k + 2]);
                 }
             },
-            64);
+            4);
+        // scattered in stream order (the pinned arrays are reused only after
+        // the next device synchronisation)
         DeviceBuffer<std::int32_t> d_idx(na);
         DeviceBuffer<double> d_vals(na);
-        d_idx.upload(pairs.data(), na, s);
-        d_vals.upload(vals.data(), na, s);
+        AXIS_HIP_CHECK(hipMemcpyAsync(d_idx.get(), pairs.data(), sizeof(std::int32_t) * na, hipMemcpyHostToDevice, s));
+        AXIS_HIP_CHECK(hipMemcpyAsync(d_vals.get(), vals.data(), sizeof(double) * na, hipMemcpyHostToDevice, s));
         scatter_overlaps_kernel<<<grid_for(na, 256), 256, 0, s>>>(d_idx.get(), d_vals.get(), na, overlap.get());
         AXIS_LAUNCH_CHECK();
-        AXIS_HIP_CHECK(hipStreamSynchronize(s));
-        arb_done = h_cnt[0];
+        arb_done = h[0];
     };
     run_overlaps(0, n_pairs);
     tr.mark("overlap");
@@ -1094,76 +1167,93 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
 
     tr.mark("arbitration");
     // ---- CSR assembly
-    DeviceBuffer<index_t> nzf(std::max<index_t>(n_pairs, 1)), pos(n_pairs + 1);
-    if (n_pairs > 0) {
-        nz_flag_kernel<<<grid_for(n_pairs, 256), 256, 0, s>>>(overlap.get(), n_pairs, nzf.get());
-        AXIS_LAUNCH_CHECK();
-    }
-    exclusive_scan_i64(nzf.get(), pos.get(), n_pairs, s);
-    index_t nnz = 0;
-    AXIS_HIP_CHECK(hipMemcpyAsync(&nnz, pos.get() + n_pairs, sizeof(index_t), hipMemcpyDeviceToHost, s));
-    AXIS_HIP_CHECK(hipStreamSynchronize(s));
-    M->nnz = nnz;
     const int col_bits = bits_for(std::max<index_t>(n_src, 2));
     const int row_bits = bits_for(std::max<index_t>(n_dst, 2));
-    const index_t nz1 = std::max<index_t>(nnz, 1);
+    const int frac_area = cfg.norm_type == NormType::FracArea ? 1 : 0;
+    constexpr int kH = sizeof(M->len_hist) / sizeof(M->len_hist[0]);
+    static_assert(kH == kHistMax + 2, "row-length histogram size");
     M->row_ptr.resize(n_dst + 1);
-    M->col.resize(nz1);
-    M->val.resize(nz1);
     M->frac_a.resize(std::max<index_t>(n_src, 1));
     M->frac_b.resize(std::max<index_t>(n_dst, 1));
-    M->area_a.resize(std::max<index_t>(n_src, 1));
-    M->area_b.resize(std::max<index_t>(n_dst, 1));
-    DeviceBuffer<double> ov(nz1);  // overlaps in CSR order
-    DeviceBuffer<std::uint64_t> key(nz1);
-    DeviceBuffer<std::uint32_t> perm(nz1);
     DeviceBuffer<int> max_row(1);
+    DeviceBuffer<unsigned long long> d_hist(kH);
     AXIS_HIP_CHECK(hipMemsetAsync(max_row.get(), 0, sizeof(int), s));
-    const int frac_area = cfg.norm_type == NormType::FracArea ? 1 : 0;
-    bool assembled = false;
-    if (!appended && n_dst > 0) {
-        DeviceBuffer<std::int32_t> col_c(nz1), long_rows(std::max<index_t>(n_dst, 1));
-        DeviceBuffer<double> ov_c(nz1);
-        DeviceBuffer<int> n_long(2);  // rows of up to kMidRow entries, longer rows
-        AXIS_HIP_CHECK(hipMemsetAsync(n_long.get(), 0, 2 * sizeof(int), s));
-        constexpr int kH = sizeof(M->len_hist) / sizeof(M->len_hist[0]);
-        DeviceBuffer<unsigned long long> d_hist(kH);
-        AXIS_HIP_CHECK(hipMemsetAsync(d_hist.get(), 0, sizeof(unsigned long long) * kH, s));
-        if (nnz > 0) {
-            compact_grouped_kernel<<<grid_for(n_pairs, 256), 256, 0, s>>>(overlap.get(), pos.get(), n_pairs, cand_src.get(), col_c.get(), ov_c.get());
+    AXIS_HIP_CHECK(hipMemsetAsync(d_hist.get(), 0, sizeof(unsigned long long) * kH, s));
+    // pairs grouped by row (every path but a proposer round 2 that appended
+    // pairs): per-row nonzero counts give row_ptr, then each row is sorted by
+    // column where it lies (one thread, one wave or one workgroup per row)
+    DeviceBuffer<std::int32_t> long_rows;
+    DeviceBuffer<int> n_long(2);  // rows of up to kMidRow entries, longer rows
+    index_t nnz = 0;
+    int h_max_row = 0;
+    bool grouped = !appended && n_dst > 0;
+    {
+        PinnedArray<index_t> h;
+        h.resize(2);
+        h[0] = 0;
+        h[1] = 0;
+        if (grouped) {
+            long_rows.resize(n_dst);
+            DeviceBuffer<unsigned long long> rnz(n_dst);
+            AXIS_HIP_CHECK(hipMemsetAsync(n_long.get(), 0, 2 * sizeof(int), s));
+            AXIS_HIP_CHECK(hipMemsetAsync(rnz.get(), 0, sizeof(unsigned long long) * n_dst, s));
+            if (n_pairs > 0) {
+                nz_count_kernel<<<grid_for(n_pairs, 256), 256, 0, s>>>(overlap.get(), pair_row.get(), n_pairs, rnz.get());
+                AXIS_LAUNCH_CHECK();
+            }
+            const unsigned grid = static_cast<unsigned>(std::min<index_t>((n_dst + 255) / 256, 2048));
+            row_lengths_kernel<<<grid, 256, 0, s>>>(rnz.get(), n_dst, max_row.get(), long_rows.get(), n_long.get(), d_hist.get());
+            AXIS_LAUNCH_CHECK();
+            exclusive_scan_i64(reinterpret_cast<const index_t *>(rnz.get()), M->row_ptr.get(), n_dst, s);
+            AXIS_HIP_CHECK(hipMemcpyAsync(h.data(), M->row_ptr.get() + n_dst, sizeof(index_t), hipMemcpyDeviceToHost, s));
+            AXIS_HIP_CHECK(hipMemcpyAsync(h.data() + 1, max_row.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+            AXIS_HIP_CHECK(hipStreamSynchronize(s));
+            nnz = h[0];
+            h_max_row = static_cast<int>(h[1] & 0xffffffff);
+            grouped = h_max_row <= kLongRowMax;
+        }
+    }
+    DeviceBuffer<index_t> pos;
+    if (!grouped) {  // nonzero flags over the pairs and their scan (the global sort below)
+        DeviceBuffer<index_t> nzf(std::max<index_t>(n_pairs, 1));
+        pos.resize(n_pairs + 1);
+        if (n_pairs > 0) {
+            nz_flag_kernel<<<grid_for(n_pairs, 256), 256, 0, s>>>(overlap.get(), n_pairs, nzf.get());
             AXIS_LAUNCH_CHECK();
         }
-        const unsigned grid = static_cast<unsigned>(std::min<index_t>((n_dst + 255) / 256, 2048));
-        short_rows_kernel<<<grid, 256, 0, s>>>(cand_off.get(), pos.get(), n_dst, col_c.get(), ov_c.get(), dg.d_area.get(), frac_area,
-                                               M->row_ptr.get(), M->col.get(), M->val.get(), ov.get(), M->frac_b.get(), max_row.get(),
-                                               long_rows.get(), n_long.get(), d_hist.get());
-        AXIS_LAUNCH_CHECK();
-        int h[3] = {0, 0, 0};
-        unsigned long long hh[kH];
-        AXIS_HIP_CHECK(hipMemcpyAsync(&h[0], max_row.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-        AXIS_HIP_CHECK(hipMemcpyAsync(&h[1], n_long.get(), 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-        AXIS_HIP_CHECK(hipMemcpyAsync(hh, d_hist.get(), sizeof hh, hipMemcpyDeviceToHost, s));
+        exclusive_scan_i64(nzf.get(), pos.get(), n_pairs, s);
+        AXIS_HIP_CHECK(hipMemcpyAsync(&nnz, pos.get() + n_pairs, sizeof(index_t), hipMemcpyDeviceToHost, s));
         AXIS_HIP_CHECK(hipStreamSynchronize(s));
-        for (int l = 0; l < kH; ++l) M->len_hist[l] = static_cast<index_t>(hh[l]);
-        M->len_hist_ready = true;
-        if (h[0] <= kLongRowMax) {
-            if (h[1] > 0) {
-                long_rows_kernel<kMidRow, kMidRow><<<static_cast<unsigned>(h[1]), kMidRow, 0, s>>>(
-                    long_rows.get(), M->row_ptr.get(), col_c.get(), ov_c.get(), dg.d_area.get(), frac_area, M->col.get(), M->val.get(), ov.get(),
-                    M->frac_b.get());
-                AXIS_LAUNCH_CHECK();
-            }
-            if (h[2] > 0) {
-                long_rows_kernel<kLongRowMax, kLongRowBlock><<<static_cast<unsigned>(h[2]), kLongRowBlock, 0, s>>>(
-                    long_rows.get() + (n_dst - h[2]), M->row_ptr.get(), col_c.get(), ov_c.get(), dg.d_area.get(), frac_area, M->col.get(),
-                    M->val.get(), ov.get(), M->frac_b.get());
-                AXIS_LAUNCH_CHECK();
-            }
-            assembled = true;
+    }
+    M->nnz = nnz;
+    const index_t nz1 = std::max<index_t>(nnz, 1);
+    M->col.resize(nz1);
+    M->val.resize(nz1);
+    DeviceBuffer<double> ov(nz1);  // overlaps in CSR order
+    DeviceBuffer<std::uint32_t> perm(nz1);
+    bool assembled = false;
+    if (grouped) {
+        const unsigned grid = static_cast<unsigned>(std::min<index_t>((n_dst + 255) / 256, 2048));
+        short_rows_kernel<<<grid, 256, 0, s>>>(cand_off.get(), M->row_ptr.get(), n_dst, cand_src.get(), overlap.get(), dg.d_area.get(), frac_area,
+                                               M->col.get(), M->val.get(), ov.get(), M->frac_b.get());
+        AXIS_LAUNCH_CHECK();
+        if (h_max_row > kShortRow) {  // the listed rows (their counts stay on the device)
+            long_rows_kernel<kMidRow, kMidRow><<<2048, kMidRow, 0, s>>>(long_rows.get(), n_long.get(), false, n_dst, cand_off.get(),
+                                                                        M->row_ptr.get(), cand_src.get(), overlap.get(), dg.d_area.get(),
+                                                                        frac_area, M->col.get(), M->val.get(), ov.get(), M->frac_b.get());
+            AXIS_LAUNCH_CHECK();
         }
+        if (h_max_row > kMidRow) {
+            long_rows_kernel<kLongRowMax, kLongRowBlock><<<512, kLongRowBlock, 0, s>>>(
+                long_rows.get(), n_long.get() + 1, true, n_dst, cand_off.get(), M->row_ptr.get(), cand_src.get(), overlap.get(), dg.d_area.get(),
+                frac_area, M->col.get(), M->val.get(), ov.get(), M->frac_b.get());
+            AXIS_LAUNCH_CHECK();
+        }
+        assembled = true;
     }
     if (!assembled) {
         // pairs in any order (or rows beyond the LDS sort): one global (row, col) sort
+        DeviceBuffer<std::uint64_t> key(nz1);
         DeviceBuffer<double> ov_c(nz1);
         if (nnz > 0) {
             compact_kernel<<<grid_for(n_pairs, 256), 256, 0, s>>>(overlap.get(), pos.get(), n_pairs, pair_row.get(), cand_src.get(), col_bits,
@@ -1194,11 +1284,15 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
         frac_a_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(ckey.get(), perm.get(), nnz, ov.get(), sg.d_area.get(), M->frac_a.get());
         AXIS_LAUNCH_CHECK();
     }
-    if (n_src > 0)
-        AXIS_HIP_CHECK(hipMemcpyAsync(M->area_a.get(), sg.d_area.get(), sizeof(double) * n_src, hipMemcpyDeviceToDevice, s));
-    if (n_dst > 0)
-        AXIS_HIP_CHECK(hipMemcpyAsync(M->area_b.get(), dg.d_area.get(), sizeof(double) * n_dst, hipMemcpyDeviceToDevice, s));
-    AXIS_HIP_CHECK(hipMemcpyAsync(&M->max_row_nnz, max_row.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    // the cell areas pass to the matrix (the geometry is done with them)
+    M->area_a = std::move(sg.d_area);
+    M->area_b = std::move(dg.d_area);
+    if (M->area_a.size() < static_cast<std::size_t>(std::max<index_t>(n_src, 1))) M->area_a.resize(std::max<index_t>(n_src, 1));
+    if (M->area_b.size() < static_cast<std::size_t>(std::max<index_t>(n_dst, 1))) M->area_b.resize(std::max<index_t>(n_dst, 1));
+    PinnedArray<unsigned long long> h_tail;
+    h_tail.resize(kH + 1);
+    AXIS_HIP_CHECK(hipMemcpyAsync(h_tail.data(), max_row.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    if (grouped) AXIS_HIP_CHECK(hipMemcpyAsync(h_tail.data() + 1, d_hist.get(), sizeof(unsigned long long) * kH, hipMemcpyDeviceToHost, s));
     tr.mark("csr");
     if (cfg.unmapped == UnmappedAction::Error && n_dst > 0) {
         DeviceBuffer<unsigned long long> first(1);
@@ -1212,6 +1306,11 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
             throw RuntimeError("WeightGenerator::generate_conservative: unmapped destination cell " + std::to_string(h_first));
     }
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    M->max_row_nnz = static_cast<int>(h_tail[0] & 0xffffffffu);
+    if (grouped) {
+        for (int l = 0; l < kH; ++l) M->len_hist[l] = static_cast<index_t>(h_tail[1 + l]);
+        M->len_hist_ready = true;
+    }
     M->gen_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return M;
 }
