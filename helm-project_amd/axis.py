@@ -183,6 +183,7 @@ def lib():
         "axis_matrix_csr_c": [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p],
         "axis_matrix_fractions_c": [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
         "axis_matrix_stats_c": [C.c_int, _I64, _D],
+        "axis_matrix_phase_ms_c": [C.c_int, _D, C.c_int],
         "axis_apply_c": [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int],
         "axis_batch_apply_c": [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_int,
                                C.c_void_p],
@@ -231,7 +232,7 @@ def exported_symbols():
             "axis_mesh_cell_centroids_c",
             "axis_generate_weights_c", "axis_generate_weights_config_c", "axis_matrix_from_coo_c",
             "axis_matrix_to_csr_c", "axis_matrix_info_c", "axis_matrix_csr_c", "axis_matrix_fractions_c",
-            "axis_matrix_stats_c", "axis_apply_c", "axis_batch_apply_c", "axis_batch_apply_async_c",
+            "axis_matrix_stats_c", "axis_matrix_phase_ms_c", "axis_apply_c", "axis_batch_apply_c", "axis_batch_apply_async_c",
             "axis_autotune_apply_c", "axis_check_conservation_c", "axis_destroy_mesh_c", "axis_destroy_matrix_c",
             "axis_batch_plan_create_c", "axis_batch_plan_run_c", "axis_batch_plan_info_c", "axis_destroy_batch_plan_c",
             "axis_mlp_create_c", "axis_mlp_params_c", "axis_mlp_set_params_c", "axis_mlp_forward_c",
@@ -426,6 +427,14 @@ class Matrix:
         _check(lib().axis_matrix_stats_c(self.handle, out.ctypes.data_as(_I64), C.byref(ms)))
         return {"candidates": int(out[0]), "host_arbitrated": int(out[1]), "cap_ties": int(out[2]),
                 "max_row_nnz": int(out[3]), "gen_ms": ms.value}
+
+    def phase_ms(self):
+        """GPU time of the generation's phases (axis_matrix_phase_ms_c); None
+        for a phase that did not run."""
+        out = np.zeros(4, dtype=np.float64)
+        _check(lib().axis_matrix_phase_ms_c(self.handle, out.ctypes.data_as(_D), 4))
+        names = ("broad_phase_query", "cap_filter_kernel", "clip_kernel", "csr_assembly")
+        return {k: (None if v < 0 else float(v)) for k, v in zip(names, out)}
 
     def apply(self, src, dst=None):
         """FP64 dst = S src through axis_apply_c (host numpy or device torch)."""

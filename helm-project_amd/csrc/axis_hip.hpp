@@ -141,6 +141,10 @@ class InterpolationMatrix {
     // diagnostics of the generation that produced it
     index_t n_candidates = 0, n_host_arbitrated = 0, n_cap_borderline = 0;
     double gen_ms = 0.0;
+    // GPU time of the general conservative path's phases (HIP events on the
+    // generation stream): broad-phase query, cap filter kernel, clip kernel,
+    // CSR assembly; -1 where the path did not run them
+    double phase_ms[4] = {-1.0, -1.0, -1.0, -1.0};
 
     /// Builds the apply layout once: FP32 hi/lo split, int32 columns and the
     /// segment list in spatial order.

@@ -485,6 +485,14 @@ int axis_matrix_stats_c(int matrix_handle, int64_t *out4, double *gen_ms) {
     });
 }
 
+int axis_matrix_phase_ms_c(int matrix_handle, double *out, int n) {
+    return guarded([&] {
+        auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
+        if (n < 0 || (n > 0 && !out)) throw InvalidArgument("axis_matrix_phase_ms_c: bad output array");
+        for (int k = 0; k < n; ++k) out[k] = k < 4 ? M->phase_ms[k] : -1.0;
+    });
+}
+
 int axis_apply_c(int matrix_handle, const double *src, double *dst, int n_src, int n_dst) {
     return guarded([&] {
         auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);

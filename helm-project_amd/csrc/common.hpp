@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <memory>
 #include <new>
 #include <stdexcept>
@@ -195,6 +196,27 @@ class DeviceBuffer {
     T *p_ = nullptr;
     std::size_t n_ = 0;
 };
+
+/// Copies n Ts from device memory at d to h (host memory of any kind) and
+/// waits for `s`.  Copies of up to 4 KiB go through a per-thread pinned
+/// staging buffer: a pageable destination makes the runtime stage the copy on
+/// its own threads, which a busy host delays.
+template <class T>
+inline void read_back(T *h, const T *d, std::size_t n, hipStream_t s) {
+    constexpr std::size_t kStage = 4096;
+    static thread_local void *stage = nullptr;  // (one per thread, kept until the process ends)
+    const std::size_t bytes = n * sizeof(T);
+    if (bytes == 0) return;
+    if (bytes > kStage) {
+        AXIS_HIP_CHECK(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, s));
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+        return;
+    }
+    if (!stage) AXIS_HIP_CHECK(hipHostMalloc(&stage, kStage, hipHostMallocDefault));
+    AXIS_HIP_CHECK(hipMemcpyAsync(stage, d, bytes, hipMemcpyDeviceToHost, s));
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    std::memcpy(h, stage, bytes);
+}
 
 inline unsigned grid_for(std::size_t n, unsigned block) {
     std::size_t g = (n + block - 1) / block;

@@ -32,6 +32,7 @@
 #include <cstring>
 #include <cmath>
 #include <limits>
+#include <map>
 #include <string>
 
 #include "axis_hip.hpp"
@@ -76,9 +77,12 @@ __device__ inline V3 trig_node(const Side &m, index_t a, index_t b) {
 // (weight_generator.cpp:483-520) or mesh nodes
 __device__ inline V3 cell_vertex(const Side &m, index_t c, int k) {
     if (m.trig) {
-        index_t ci = c % m.ni, cj = c / m.ni;
-        index_t a = (k == 1 || k == 2) ? ci + 1 : ci;
-        index_t b = (k >= 2) ? cj + 1 : cj;
+        // (32-bit division: cell counts stay below 2^31, and a 64-bit one
+        // costs many times more instructions)
+        const std::uint32_t cu = static_cast<std::uint32_t>(c), ni = static_cast<std::uint32_t>(m.ni);
+        const std::uint32_t cj = cu / ni, ci = cu - cj * ni;
+        const index_t a = (k == 1 || k == 2) ? ci + 1 : ci;
+        const index_t b = (k >= 2) ? cj + 1 : cj;
         return trig_node(m, a, b);
     }
     index_t nd = m.ind[m.off[c] + k];
@@ -756,8 +760,10 @@ double key_to_double(unsigned long long k) {
 // reductions, one download (an empty mesh gives lo = +max, hi = -max)
 void lon_ranges(const Mesh &a, const Mesh &b, double range[4], hipStream_t s) {
     DeviceBuffer<unsigned long long> r(4);
-    unsigned long long init[4] = {~0ull, 0ull, ~0ull, 0ull};
-    AXIS_HIP_CHECK(hipMemcpyAsync(r.get(), init, sizeof init, hipMemcpyHostToDevice, s));
+    // lows start at the largest key, highs at zero
+    AXIS_HIP_CHECK(hipMemsetAsync(r.get(), 0, sizeof(unsigned long long) * 4, s));
+    AXIS_HIP_CHECK(hipMemsetAsync(r.get(), 0xFF, sizeof(unsigned long long), s));
+    AXIS_HIP_CHECK(hipMemsetAsync(r.get() + 2, 0xFF, sizeof(unsigned long long), s));
     const Mesh *ms[2] = {&a, &b};
     for (int k = 0; k < 2; ++k) {
         const Mesh &m = *ms[k];
@@ -768,8 +774,7 @@ void lon_ranges(const Mesh &a, const Mesh &b, double range[4], hipStream_t s) {
         AXIS_LAUNCH_CHECK();
     }
     unsigned long long h[4];
-    AXIS_HIP_CHECK(hipMemcpyAsync(h, r.get(), sizeof h, hipMemcpyDeviceToHost, s));
-    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    read_back(h, r.get(), 4, s);
     for (int k = 0; k < 2; ++k) {
         if (h[2 * k] == ~0ull) {
             range[2 * k] = std::numeric_limits<double>::max();
@@ -800,10 +805,40 @@ void check_longitude_ranges(const Mesh &src, const Mesh &dst, bool structured_fa
                               "] are disjoint but would overlap after a shift of " + std::to_string(shift));
 }
 
+namespace {
+// HIP events timing one generation's phases on its stream, created once per
+// thread and device and read after the generation's last synchronisation
+struct PhaseEvents {
+    static constexpr int kN = 8;
+    hipEvent_t e[kN] = {};
+    bool on[kN] = {};
+    static PhaseEvents &get() {
+        static thread_local std::map<int, PhaseEvents> per_device;
+        int dev = 0;
+        AXIS_HIP_CHECK(hipGetDevice(&dev));
+        PhaseEvents &p = per_device[dev];
+        if (!p.e[0])
+            for (hipEvent_t &x : p.e) AXIS_HIP_CHECK(hipEventCreate(&x));
+        for (bool &b : p.on) b = false;
+        return p;
+    }
+    void rec(int k, hipStream_t s) {
+        AXIS_HIP_CHECK(hipEventRecord(e[k], s));
+        on[k] = true;
+    }
+    double ms(int a, int b) const {
+        float t = -1.0f;
+        if (!on[a] || !on[b] || hipEventElapsedTime(&t, e[a], e[b]) != hipSuccess) return -1.0;
+        return t;
+    }
+};
+}  // namespace
+
 std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, hipStream_t s,
                                                            const mlp::Model *model, ProposerStats *pst, double area_tol) {
     const auto t0 = std::chrono::steady_clock::now();
     PhaseTrace tr("conservative", s);
+    PhaseEvents &ev = PhaseEvents::get();
     // the host node-xyz task uploads on its own stream: this thread's
     // allocations must not wait for the device (and thereby for the task);
     // blocks released before this call are made reusable first
@@ -937,6 +972,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     DeviceBuffer<std::int32_t> cand_src, accepted;
     DeviceBuffer<std::int32_t> grid_rows;  // (the grid query writes each candidate's row)
     index_t n_pairs = 0;
+    ev.rec(0, s);
     if (grid_ok) {
         n_pairs = grid_query(grid, dg.d_box.get(), n_dst, cand_off, cand_src, grid_rows, s);
     } else {
@@ -944,9 +980,9 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
             proposer_candidates(sg, dg, *model, cand_off, cand_src, accepted, s);
         else
             lbvh_query(tree, dg.d_box.get(), n_dst, cand_off, cand_src, s);
-        AXIS_HIP_CHECK(hipMemcpyAsync(&n_pairs, cand_off.get() + n_dst, sizeof(index_t), hipMemcpyDeviceToHost, s));
-        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+        read_back(&n_pairs, cand_off.get() + n_dst, 1, s);
     }
+    ev.rec(1, s);
     tr.mark(model ? "propose" : "broad_phase");
 
     auto M = std::make_unique<InterpolationMatrix>();
@@ -1005,9 +1041,11 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
             if (n_chunks * (kSelChunk / kWave) >= (index_t(1) << 31)) throw InvalidArgument("generate: too many candidate pairs");
             DeviceBuffer<std::uint32_t> live(n_chunks * kSelChunk);
             DeviceBuffer<std::int32_t> n_live(n_chunks);
+            if (p0 == 0) ev.rec(2, s);
             cap_select_kernel<<<static_cast<unsigned>(n_chunks), kSelThreads, 0, s>>>(p0, p1, pair_row.get(), cand_src.get(), S, D, overlap.get(),
                                                                                      live.get(), n_live.get(), counters.get() + 1);
             AXIS_LAUNCH_CHECK();
+            if (p0 == 0) ev.rec(3, s);
             const unsigned cgrid = static_cast<unsigned>(n_chunks * (kSelChunk / kWave));
             if (need <= 8)
                 clip_live_kernel<8><<<cgrid, kWave, 0, s>>>(p0, live.get(), n_live.get(), static_cast<unsigned>(n_chunks), pair_row.get(), cand_src.get(), S, D, overlap.get(),
@@ -1019,6 +1057,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
                 clip_live_kernel<24><<<cgrid, kWave, 0, s>>>(p0, live.get(), n_live.get(), static_cast<unsigned>(n_chunks), pair_row.get(), cand_src.get(), S, D, overlap.get(),
                                                              arb.get(), counters.get(), trip.get());
             AXIS_LAUNCH_CHECK();
+            if (p0 == 0) ev.rec(4, s);
             return;
         }
         if (need <= 8)
@@ -1167,6 +1206,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
 
     tr.mark("arbitration");
     // ---- CSR assembly
+    ev.rec(5, s);
     const int col_bits = bits_for(std::max<index_t>(n_src, 2));
     const int row_bits = bits_for(std::max<index_t>(n_dst, 2));
     const int frac_area = cfg.norm_type == NormType::FracArea ? 1 : 0;
@@ -1222,8 +1262,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
             AXIS_LAUNCH_CHECK();
         }
         exclusive_scan_i64(nzf.get(), pos.get(), n_pairs, s);
-        AXIS_HIP_CHECK(hipMemcpyAsync(&nnz, pos.get() + n_pairs, sizeof(index_t), hipMemcpyDeviceToHost, s));
-        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+        read_back(&nnz, pos.get() + n_pairs, 1, s);
     }
     M->nnz = nnz;
     const index_t nz1 = std::max<index_t>(nnz, 1);
@@ -1284,6 +1323,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
         frac_a_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(ckey.get(), perm.get(), nnz, ov.get(), sg.d_area.get(), M->frac_a.get());
         AXIS_LAUNCH_CHECK();
     }
+    ev.rec(6, s);
     // the cell areas pass to the matrix (the geometry is done with them)
     M->area_a = std::move(sg.d_area);
     M->area_b = std::move(dg.d_area);
@@ -1306,6 +1346,10 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
             throw RuntimeError("WeightGenerator::generate_conservative: unmapped destination cell " + std::to_string(h_first));
     }
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    M->phase_ms[0] = ev.ms(0, 1);
+    M->phase_ms[1] = ev.ms(2, 3);
+    M->phase_ms[2] = ev.ms(3, 4);
+    M->phase_ms[3] = ev.ms(5, 6);
     M->max_row_nnz = static_cast<int>(h_tail[0] & 0xffffffffu);
     if (grouped) {
         for (int l = 0; l < kH; ++l) M->len_hist[l] = static_cast<index_t>(h_tail[1 + l]);

@@ -287,16 +287,14 @@ index_t unique_column(const double *d_col, const double *h_col, index_t n, bool 
     AXIS_LAUNCH_CHECK();
     exclusive_scan_i64(head.get(), pos.get(), n, s);
     index_t count = 0;
-    AXIS_HIP_CHECK(hipMemcpyAsync(&count, pos.get() + n, sizeof(index_t), hipMemcpyDeviceToHost, s));
-    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    read_back(&count, pos.get() + n, 1, s);
     DeviceBuffer<double> u(count);
     compact_kernel<<<grid_for(n, 256), 256, 0, s>>>(vals.get(), head.get(), pos.get(), n, u.get());
     AXIS_LAUNCH_CHECK();
     span_check_kernel<<<grid_for(n, 256), 256, 0, s>>>(vals.get(), head.get(), pos.get(), n, u.get(), bad.get());
     AXIS_LAUNCH_CHECK();
     int h_bad = 0;
-    AXIS_HIP_CHECK(hipMemcpyAsync(&h_bad, bad.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    read_back(&h_bad, bad.get(), 1, s);
     if (h_bad) {
         // a tolerance class wider than 1e-12: replay the reference's sequential filter exactly
         out = host_unique(std::vector<double>(h_col, h_col + n));
@@ -369,8 +367,7 @@ bool canonical_axes(const Mesh &m, index_t na, index_t nb, std::vector<double> &
         canonical_grid_kernel<<<grid_for(n, 256), 256, 0, s>>>(m.d_coords.get(), m.d_coords.get() + n, n, nlon, bad.get());
         AXIS_LAUNCH_CHECK();
         int h_bad = 1;
-        AXIS_HIP_CHECK(hipMemcpyAsync(&h_bad, bad.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+        read_back(&h_bad, bad.get(), 1, s);
         if (h_bad) continue;
         auto distinct = [](std::vector<double> v) {
             std::sort(v.begin(), v.end(), value_then_bits);
@@ -432,8 +429,7 @@ GridAxes unique_axes(const Mesh &m, hipStream_t s) {
             AXIS_LAUNCH_CHECK();
             count_bins_kernel<<<2 * kBinWords / 256, 256, 0, s>>>(bins.get(), dq.get());
             AXIS_LAUNCH_CHECK();
-            AXIS_HIP_CHECK(hipMemcpyAsync(&hq, dq.get(), sizeof hq, hipMemcpyDeviceToHost, s));
-            AXIS_HIP_CHECK(hipStreamSynchronize(s));
+            read_back(&hq, dq.get(), 1, s);
             tr.mark("bound");
             if (hq.nonfinite) {
                 run_distinct = false;  // (the bins are meaningless with non-finite coordinates)
@@ -452,8 +448,7 @@ GridAxes unique_axes(const Mesh &m, hipStream_t s) {
             AXIS_LAUNCH_CHECK();
             gather_set_kernel<<<2 * kSetSlots / 256, 256, 0, s>>>(set.get(), vals.get(), pos.get());
             AXIS_LAUNCH_CHECK();
-            AXIS_HIP_CHECK(hipMemcpyAsync(&hq, dq.get(), sizeof hq, hipMemcpyDeviceToHost, s));
-            AXIS_HIP_CHECK(hipStreamSynchronize(s));
+            read_back(&hq, dq.get(), 1, s);
             if (!hq.nonfinite && !hq.overflow[0] && !hq.overflow[1]) {
                 xlon.assign(hq.n_distinct[0], 0.0);
                 xlat.assign(hq.n_distinct[1], 0.0);
@@ -928,8 +923,7 @@ bool node_xyz_device(const Mesh &m, const GridAxes &ax, MeshGeom &g, hipStream_t
         AXIS_LAUNCH_CHECK();
     }
     int h_fix = 0;
-    AXIS_HIP_CHECK(hipMemcpyAsync(&h_fix, n_fix.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    read_back(&h_fix, n_fix.get(), 1, s);
     if (h_fix > kXyzFixCap) return false;
     if (h_fix > 0) {
         std::vector<index_t> nodes(h_fix);
@@ -1165,8 +1159,7 @@ void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s) {
     }
     tr.mark("upload_trig_cellprep");
     int h_n = 0;
-    AXIS_HIP_CHECK(hipMemcpyAsync(&h_n, n_recheck.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    read_back(&h_n, n_recheck.get(), 1, s);
     if (h_n > 0) {
         // replay near-zero-area cells with host libm (the reference's bits)
         std::vector<index_t> cells(h_n);

@@ -187,6 +187,13 @@ int axis_matrix_fractions_c(int matrix_handle, double *frac_a, double *frac_b, d
  * row; *gen_ms wall time of the generate call. */
 int axis_matrix_stats_c(int matrix_handle, int64_t *out4, double *gen_ms);
 
+/* Generation diagnostics (no reference counterpart): GPU time in ms, from HIP
+ * events on the generation stream, of the general conservative path's phases
+ * -- out[0] broad-phase query, out[1] cap filter kernel, out[2] clip kernel,
+ * out[3] CSR assembly -- -1 for a phase the generation did not run (other
+ * methods, analytic paths).  Fills out[0 .. n). */
+int axis_matrix_phase_ms_c(int matrix_handle, double *out, int n);
+
 /* axis_apply_c (axis_c_interop.cpp:217-236): dst = S * src, FP64.  src/dst may
  * be host or device memory. */
 int axis_apply_c(int matrix_handle, const double *src, double *dst, int n_src, int n_dst);

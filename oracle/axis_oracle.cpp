@@ -599,6 +599,67 @@ std::vector<i64> morton_order(const Mesh &m) {
 }
 
 // ------------------------------------------------------------ results
+// FP64 operation count of clip(subject, cp).area() for one pair, in the
+// formulation the GPU kernel runs (each half-space value computed once per
+// vertex): +, -, *, /, sqrt, atan2 one operation each.  Not part of the
+// reference; it prices the clip kernel against the FP64 roofline.
+struct OpCount {
+    double flops = 0.0;
+    i64 pairs = 0, edges = 0, intersections = 0, fan_terms = 0;
+};
+constexpr double kOpsCross = 9, kOpsDot = 5, kOpsNrm = 10;
+double count_clip_area_ops(const Poly &subject, const Poly &cp, OpCount &oc) {
+    double f = 0.0;
+    if (subject.n >= 3 && cp.n >= 3) {
+        Poly a = subject, b;
+        for (int ci = 0; ci < cp.n; ++ci) {
+            const V3 &ea = cp.v[ci];
+            const V3 &eb = cp.v[(ci + 1) % cp.n];
+            V3 normal = cross(ea, eb);
+            f += kOpsCross + 5;  // cross, lsq
+            if (lsq(normal) < 1.0e-30) continue;
+            ++oc.edges;
+            b.n = 0;
+            constexpr double eps = -1.0e-14;
+            f += kOpsDot;  // the first predecessor's value
+            for (int i = 0; i < a.n; ++i) {
+                int q = (i + a.n - 1) % a.n;
+                const V3 &cur = a.v[i];
+                const V3 &prv = a.v[q];
+                double dc = dot(normal, cur), dp = dot(normal, prv);
+                f += kOpsDot;
+                bool ci_in = dc >= eps, pi_in = dp >= eps;
+                auto inter = [&]() {
+                    ++oc.intersections;
+                    V3 d = cross(cross(ea, eb), cross(prv, cur));
+                    f += 3 * kOpsCross + 5;
+                    f += lsq(d) < 1.0e-30 ? 3 + kOpsNrm : kOpsNrm + 3 + kOpsNrm + kOpsDot;
+                    return gc_intersection(prv, cur, ea, eb);
+                };
+                if (ci_in) {
+                    if (!pi_in) b.push(inter());
+                    b.push(cur);
+                } else if (pi_in) {
+                    b.push(inter());
+                }
+            }
+            a.n = b.n;
+            for (int i = 0; i < b.n; ++i) a.v[i] = b.v[i];
+            if (a.n < 3) {
+                a.n = 0;
+                break;
+            }
+        }
+        if (a.n >= 3) {
+            oc.fan_terms += a.n - 2;
+            f += 35.0 * (a.n - 2);  // 4 dots, a cross, 3 adds, atan2, 2.0 *, the running sum
+        }
+    }
+    ++oc.pairs;
+    oc.flops += f;
+    return f;
+}
+
 struct Result {
     i64 n_src = 0, n_dst = 0;
     std::vector<i64> row_ptr, col;
@@ -606,6 +667,7 @@ struct Result {
     std::vector<double> frac_a, frac_b, area_a, area_b;
     // diagnostics
     i64 n_candidates = 0, n_cap_rejected = 0, n_clipped = 0;
+    OpCount ops;                              // optional FP64 operation count of the clips
     std::vector<i64> cand_offsets, cand_idx;  // optional broad phase dump
     std::string error;
     int path = 0;  // 0 general, 1 regular fast path, 2 rectilinear fast path
@@ -635,7 +697,8 @@ void to_csr(Result &r, const std::vector<i64> &rows, const std::vector<i64> &col
 // weight_generator.cpp:3304-3418 (setup) and :3586-3769 (loop, packing).
 void coastal(const Mesh &src, const Mesh &dst, int extrap, std::vector<i64> &rows, std::vector<i64> &cols, std::vector<double> &vals);
 
-void conservative(const Mesh &src, const Mesh &dst, int norm_type, int unmapped_error, int brute, int dump_candidates, int extrap, Result &r) {
+void conservative(const Mesh &src, const Mesh &dst, int norm_type, int unmapped_error, int brute, int dump_candidates, int extrap, Result &r,
+                  int parallel_clip, int count_ops) {
     const i64 n_src = src.n_cells, n_dst = dst.n_cells;
     r.n_src = n_src;
     r.n_dst = n_dst;
@@ -699,6 +762,31 @@ void conservative(const Mesh &src, const Mesh &dst, int norm_type, int unmapped_
     std::vector<double> fa(n_src, 0.0), fb(n_dst, 0.0);
     std::vector<i64> rows, cols;
     std::vector<double> vals;
+    // a pair the loop below clips (its filters, in its order)
+    auto clipped_pair = [&](i64 j, i64 i) {
+        if (has_smask && src.mask[i] == 0) return false;
+        if (sdeg[i] || sa[i] <= 0.0) return false;
+        return !cap_rejects(scen[i], srad[i], dcen[j], drad[j]);
+    };
+    auto row_clipped = [&](i64 j) { return !(has_dmask && dst.mask[j] == 0) && !ddeg[j] && da[j] > 0.0; };
+    // all-parallel variant: every row's overlaps computed in parallel first;
+    // the loop below then uses them in its own order (same results)
+    std::vector<std::vector<double>> pre;
+    if (parallel_clip) {
+        pre.resize(n_dst);
+#pragma omp parallel for schedule(dynamic, 64)
+        for (i64 j = 0; j < n_dst; ++j) {
+            if (!row_clipped(j)) continue;
+            pre[j].assign(cl[j].size(), 0.0);
+            Poly dp = dt.valid ? cell_poly_cached(dt, j) : cell_poly(dst, j);
+            for (size_t q = 0; q < cl[j].size(); ++q) {
+                const i64 i = cl[j][q];
+                if (!clipped_pair(j, i)) continue;
+                Poly sp = st.valid ? cell_poly_cached(st, i) : cell_poly(src, i);
+                pre[j][q] = clip(sp, dp).area();
+            }
+        }
+    }
     // sequential overlap loop in Morton order (:3610-3700)
     for (i64 k = 0; k < n_dst; ++k) {
         i64 j = order[k];
@@ -714,7 +802,8 @@ void conservative(const Mesh &src, const Mesh &dst, int norm_type, int unmapped_
         }
         bool has_entry = false;
         Poly dp = dt.valid ? cell_poly_cached(dt, j) : cell_poly(dst, j);
-        for (i64 i : cl[j]) {
+        for (size_t q = 0; q < cl[j].size(); ++q) {
+            const i64 i = cl[j][q];
             if (has_smask && src.mask[i] == 0) continue;
             if (sdeg[i]) continue;
             double asrc = sa[i];
@@ -723,9 +812,15 @@ void conservative(const Mesh &src, const Mesh &dst, int norm_type, int unmapped_
                 ++r.n_cap_rejected;
                 continue;
             }
-            Poly sp = st.valid ? cell_poly_cached(st, i) : cell_poly(src, i);
             ++r.n_clipped;
-            double ov = clip(sp, dp).area();
+            double ov;
+            if (parallel_clip) {
+                ov = pre[j][q];
+            } else {
+                Poly sp = st.valid ? cell_poly_cached(st, i) : cell_poly(src, i);
+                ov = clip(sp, dp).area();
+                if (count_ops) count_clip_area_ops(sp, dp, r.ops);
+            }
             if (ov <= 0.0) continue;
             double w = ov / adst;
             w = std::max(w, 0.0);
@@ -1493,7 +1588,7 @@ void *om_generate(const double *s_coords, int64_t s_nn, const int64_t *s_off, co
                 return r;
             }
         }
-        conservative(src, dst, norm_type, unmapped_error, flags & 1, flags & 2, extrap, *r);
+        conservative(src, dst, norm_type, unmapped_error, flags & 1, flags & 2, extrap, *r, flags & 4, flags & 8);
     } else if (method == 0) {
         // generate_bilinear, weight_generator.cpp:2306-2345
         const RegularInfo sg = detect_regular(src);
@@ -1548,6 +1643,14 @@ void *om_generate(const double *s_coords, int64_t s_nn, const int64_t *s_off, co
 const char *om_result_error(void *h) { return static_cast<Result *>(h)->error.c_str(); }
 int64_t om_result_nnz(void *h) { return (int64_t) static_cast<Result *>(h)->val.size(); }
 int om_result_path(void *h) { return static_cast<Result *>(h)->path; }
+void om_result_ops(void *h, double *out5) {
+    const OpCount &o = static_cast<Result *>(h)->ops;
+    out5[0] = o.flops;
+    out5[1] = static_cast<double>(o.pairs);
+    out5[2] = static_cast<double>(o.edges);
+    out5[3] = static_cast<double>(o.intersections);
+    out5[4] = static_cast<double>(o.fan_terms);
+}
 void om_result_stats(void *h, int64_t *out3) {
     Result *r = static_cast<Result *>(h);
     out3[0] = r->n_candidates;

@@ -58,6 +58,7 @@ def lib():
         L.om_result_path.restype = C.c_int
         L.om_result_path.argtypes = [C.c_void_p]
         L.om_result_stats.argtypes = [C.c_void_p, _I64]
+        L.om_result_ops.argtypes = [C.c_void_p, _D]
         L.om_result_csr.argtypes = [C.c_void_p, _I64, _I64, _D]
         L.om_result_fracs.argtypes = [C.c_void_p, _D, _D, _D, _D]
         L.om_result_n_candidates.restype = C.c_int64
@@ -187,8 +188,13 @@ class OracleError(RuntimeError):
 
 
 def generate(src, dst, method: int = 2, norm_type: int = 0, unmapped_error: bool = False, brute: bool = False,
-             keep_candidates: bool = False, coord_sys: int = 0, n_threads: int = 0, extrap: int = 1):
-    """Weights as a dict (CSR + fractions + areas + stats); raises OracleError like the reference throws."""
+             keep_candidates: bool = False, coord_sys: int = 0, n_threads: int = 0, extrap: int = 1, parallel_clip: bool = False,
+             count_ops: bool = False):
+    """Weights as a dict (CSR + fractions + areas + stats); raises OracleError like the reference throws.
+
+    parallel_clip: the all-parallel variant (every row's clips in parallel, then
+    the sequential accumulation; same results).  count_ops: with the
+    sequential loop, the FP64 operation count of the clips in out["ops"]."""
     sc, so, si = _mesh_args(src)
     dc, do, di = _mesh_args(dst)
     s_area = src.get("areas")
@@ -199,7 +205,7 @@ def generate(src, dst, method: int = 2, norm_type: int = 0, unmapped_error: bool
         s_mask = np.ascontiguousarray(s_mask, dtype=np.int32)
     if d_mask is not None:
         d_mask = np.ascontiguousarray(d_mask, dtype=np.int32)
-    flags = (1 if brute else 0) | (2 if keep_candidates else 0)
+    flags = (1 if brute else 0) | (2 if keep_candidates else 0) | (4 if parallel_clip else 0) | (8 if count_ops else 0)
     L = lib()
     h = L.om_generate(_p(sc, _D), src["coords"].shape[0], _p(so, _I64), _p(si, _I64), so.shape[0] - 1, _p(s_area, _D),
                       _p(s_mask, _I32), _p(dc, _D), dst["coords"].shape[0], _p(do, _I64), _p(di, _I64), do.shape[0] - 1,
@@ -223,6 +229,11 @@ def generate(src, dst, method: int = 2, norm_type: int = 0, unmapped_error: bool
         L.om_result_stats(h, _p(st, _I64))
         out["n_candidates"], out["n_cap_rejected"], out["n_clipped"] = (int(x) for x in st)
         out["path"] = ("general", "regular", "rectilinear")[L.om_result_path(h)]
+        if count_ops:
+            ops = np.zeros(5)
+            L.om_result_ops(h, _p(ops, _D))
+            out["ops"] = {"flops": float(ops[0]), "pairs": int(ops[1]), "edges": int(ops[2]), "intersections": int(ops[3]),
+                          "fan_terms": int(ops[4])}
         if keep_candidates:
             nc = L.om_result_n_candidates(h)
             co = np.zeros(n_dst + 1, dtype=np.int64)

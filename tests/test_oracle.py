@@ -680,3 +680,32 @@ def test_structured_corner_synthesis_hand_worked():
     # corners given: taken as they are
     c, _, _ = O.structured_to_unstructured(1, 1, [0.5], [0.5], [0.0, 1.0, 0.0, 1.0], [0.0, 0.0, 1.0, 1.0])
     assert c.tolist() == [[0.0, 0.0], [1.0, 0.0], [0.0, 1.0], [1.0, 1.0]]
+
+
+def test_parallel_clip_variant_equals_sequential():
+    """The all-parallel CPU variant (every row's clips in parallel, then the
+    reference's sequential accumulation) gives the sequential loop's bits."""
+    src, dst = grids.latlon_deg(4.0), grids.cubed_sphere(12)
+    a = O.generate(src, dst, 2)
+    b = O.generate(src, dst, 2, parallel_clip=True)
+    for k in ("row_ptr", "col", "val", "frac_a", "frac_b", "area_a", "area_b"):
+        assert np.array_equal(a[k], b[k]), k
+    assert a["n_clipped"] == b["n_clipped"] and a["n_cap_rejected"] == b["n_cap_rejected"]
+
+
+def test_clip_op_count_hand_worked():
+    """One quad clipped against itself: every vertex lies on every clip edge
+    (inside, no intersections), so each of the 4 edges costs cross + lsq (14),
+    one predecessor dot (5) and 4 vertex dots (20) = 39, and the 4-vertex
+    result's area 2 fan terms x 35: 4 x 39 + 70 = 226 operations."""
+    lon = np.array([10.0, 14.0, 15.0, 9.5])
+    lat = np.array([20.0, 21.0, 25.0, 24.0])
+    quad = grids._mesh(lon, lat, np.array([0, 4]), np.arange(4))
+    r = O.generate(quad, quad, 2, count_ops=True)
+    assert r["path"] == "general"
+    assert r["ops"] == {"flops": 226.0, "pairs": 1, "edges": 4, "intersections": 0, "fan_terms": 2}
+    # a quad well inside a larger one: again no vertex leaves any clip edge,
+    # the same 226 operations
+    big = grids._mesh(np.array([5.0, 20.0, 20.0, 5.0]), np.array([15.0, 15.0, 30.0, 30.0]), np.array([0, 4]), np.arange(4))
+    r2 = O.generate(quad, big, 2, count_ops=True)
+    assert r2["ops"] == {"flops": 226.0, "pairs": 1, "edges": 4, "intersections": 0, "fan_terms": 2}
