@@ -26,23 +26,39 @@ ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 vector
+# the apply launch the roofline record describes: compensated FP32 mode, FP32
+# fields, 4 fields per step, 4 field chunks, workgroup LDS tiles (the kernel
+# spmm_tile_kernel<16, 4, 1, float, float> on these weights); pinned so that
+# the profile in profiles/ and the timed launch are the same launch
+ROOFLINE_SHAPE = (4, 4, 1)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def profiled_traffic(name):
-    """HBM-side bytes per launch of the apply kernel, as measured by the
-    FETCH_SIZE / WRITE_SIZE counter runs summarised in profiles/<name>
-    (line 'traffic_bytes_per_launch = N'); None when absent."""
+def profiled_value(name, key):
+    """A figure recorded in profiles/<name> as a line 'key = N' (bytes per
+    launch from the FETCH_SIZE / WRITE_SIZE runs, operation counts, ...);
+    None when absent."""
     p = ROOT / "profiles" / name
     if not p.exists():
         return None
     for line in p.read_text().splitlines():
-        if line.strip().startswith("traffic_bytes_per_launch"):
+        if line.strip().startswith(key + " ") or line.strip().startswith(key + "="):
             return float(line.split("=")[1])
     return None
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def _timed(torch, stream, fn, reps=5):
@@ -55,6 +71,62 @@ def _timed(torch, stream, fn, reps=5):
     e1.record(stream)
     e1.synchronize()
     return e0.elapsed_time(e1) / reps
+
+
+def bench_config1(axis, grids, torch, O):
+    """The reference's CPU-path configuration: 2-degree lat-lon (180 x 90,
+    edges at multiples of 2 degrees) -> C24 conservative, FP64, applied to
+    f = 2 + cos^2(lat) cos(2 lon) + N(0, 0.01) (seed 1001).  Median of 5 runs
+    after 2 warm-ups, generation + apply, on the CPU restatement and on the
+    GPU, with the two results compared."""
+    import numpy as np
+
+    src, dst = grids.latlon_deg(2.0), grids.cubed_sphere(24)
+    cen = grids.cell_centroids_xyz(src)
+    lat = np.arcsin(np.clip(cen[:, 2], -1, 1))
+    lon = np.arctan2(cen[:, 1], cen[:, 0])
+    f = 2.0 + np.cos(lat) ** 2 * np.cos(2 * lon) + np.random.default_rng(1001).normal(0.0, 0.01, lat.size)
+
+    def cpu_once():
+        r = O.generate(src, dst, 2)
+        return r, O.batch_apply(r["row_ptr"], r["col"], r["val"], r["n_src"], f)
+
+    S, D = axis.Mesh(src), axis.Mesh(dst)
+    fd = torch.tensor(f, device="cuda")
+
+    def gpu_once():
+        m = axis.generate_weights(S, D, axis.CONSERVATIVE)
+        y = m.apply(fd)
+        torch.cuda.synchronize()
+        return m, y
+
+    def med(fn):
+        for _ in range(2):
+            out = fn()
+            if isinstance(out[0], axis.Matrix):
+                out[0].close()
+        ts = []
+        for _ in range(5):
+            t = time.perf_counter()
+            out = fn()
+            ts.append((time.perf_counter() - t) * 1000.0)
+            if isinstance(out[0], axis.Matrix) and len(ts) < 5:
+                out[0].close()
+        return sorted(ts)[2], out
+
+    t_cpu, (ref, y_cpu) = med(cpu_once)
+    t_gpu, (m, y_gpu) = med(gpu_once)
+    rp, col, val = m.csr()
+    yg = y_gpu.cpu().numpy()
+    res = {"workload": "2deg (180x90) -> C24 (3456 cells) conservative + FP64 apply of 1 field", "cpu_ms": t_cpu, "gpu_ms": t_gpu,
+           "cpu_cells_per_s": 3456 / (t_cpu * 1e-3), "gpu_cells_per_s": 3456 / (t_gpu * 1e-3),
+           "sparsity_equal": bool(np.array_equal(rp, ref["row_ptr"]) and np.array_equal(col, ref["col"])),
+           "max_weight_rel_err": float(np.max(np.abs(val - ref["val"]) / np.maximum(ref["val"], 1e-13))),
+           "max_field_rel_err": float(np.max(np.abs(yg - y_cpu)) / np.max(np.abs(y_cpu)))}
+    m.close()
+    S.close()
+    D.close()
+    return res
 
 
 def bench_config3(axis, grids, fields, torch, stream, m384, f384, n_src384, n_dst384):
@@ -305,9 +377,11 @@ def main():
         mb.close()
     torch.cuda.synchronize()
     log("timing", args.steps)
+    phase_runs = []  # GPU phase times of each timed generation (HIP events, read without a synchronisation)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         mc, mb = step()
+        phase_runs.append(mc.phase_ms())
         mc.close()
         mb.close()
     torch.cuda.synchronize()
@@ -367,11 +441,15 @@ def main():
     chosen = min(ok, key=lambda n: ok[n]["ms"])
     ms_mixed = modes[chosen]["ms"]
     rel_err = modes[chosen]["max_rel_err_vs_fp64"]
-    staged = modes[chosen]["shape"][2]
     bytes_mixed = apply_bytes(nnz_c, k, 4, 4)
     gbs_mixed = bytes_mixed / (ms_mixed * 1e-3) / 1e9
     gbs_plain = apply_bytes(nnz_c, k, 8, 8) / (ms_plain * 1e-3) / 1e9
-    traffic = profiled_traffic(f"apply_spmm_c384_k{k}_traffic.txt")
+    # the roofline launch: the pinned shape, timed here, profiled in profiles/
+    mc.set_shape(k, axis.F32, axis.F32, axis.MIXED, *ROOFLINE_SHAPE)
+    ms_roof = time_apply(mc, X32, Y32, k, axis.F32, axis.F32, axis.MIXED)
+    gbs_roof = bytes_mixed / (ms_roof * 1e-3) / 1e9
+    roof_profile = f"apply_spmm_tile_c384_k{k}_shape{''.join(map(str, ROOFLINE_SHAPE))}.md"
+    traffic = profiled_value(roof_profile, "traffic_bytes_per_launch")
     # measured copy ceiling: a 1 GiB device-to-device copy on the same stream
     ca = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device=dev)
     cb = torch.empty_like(ca)
@@ -451,7 +529,7 @@ def main():
             t = time.perf_counter()
             ref = O.generate(src_m, dst_m, 2)
             t_cpu = time.perf_counter() - t
-            cpu = {"value": n_dst / t_cpu, "unit": "cells/s", "cores": threads, "kind": "port",
+            cpu = {"value": n_dst / t_cpu, "unit": "cells/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
                    "sample": f"full conservative weight generation 0.25deg->C384 ({n_dst} cells, {len(ref['val'])} nnz) in "
                              f"{t_cpu:.2f} s: OpenMP broad phase/caps/areas on {threads} threads, sequential clip loop "
                              f"in Morton order as weight_generator.cpp:3610-3700"}
@@ -459,6 +537,35 @@ def main():
             cpu["sparsity_equal"] = bool(np.array_equal(rp, ref["row_ptr"]) and np.array_equal(col, ref["col"]))
             cpu["max_weight_rel_err"] = float(np.max(np.abs(val - ref["val"]) / np.maximum(ref["val"], 1e-13)))
             cpu["speedup_conservative_gen"] = t_cpu * 1000.0 / t_cons
+            # the all-parallel variant: every row's clips in parallel, then the
+            # sequential accumulation (the same weights)
+            t = time.perf_counter()
+            ref_p = O.generate(src_m, dst_m, 2, parallel_clip=True)
+            t_par = time.perf_counter() - t
+            cpu["all_parallel"] = {"value": n_dst / t_par, "unit": "cells/s", "cores": threads, "seconds": t_par,
+                                   "same_weights": bool(np.array_equal(ref_p["val"], ref["val"]) and np.array_equal(ref_p["col"], ref["col"])),
+                                   "speedup_conservative_gen": t_par * 1000.0 / t_cons}
+            del ref, ref_p
+            cpu["config1"] = bench_config1(axis, grids, torch, O)
+
+    # weight generation: the clip kernel against the FP64 VALU roofline, its
+    # operations counted by the oracle's instrumented loop (profiles/), its
+    # duration from the timed generations' HIP events
+    phase_med = {}
+    for key in ("broad_phase_query", "cap_filter_kernel", "clip_kernel", "csr_assembly"):
+        vals = sorted(r[key] for r in phase_runs if r.get(key) is not None)
+        phase_med[key] = vals[len(vals) // 2] if vals else None
+    clip_ops = profiled_value("clip_fp64_ops_c384.txt", "clip_fp64_ops_per_generation")
+    clip_ms = phase_med["clip_kernel"]
+    weightgen_roofline = None
+    if clip_ops and clip_ms:
+        achieved = clip_ops / (clip_ms * 1e-3) / 1e12
+        weightgen_roofline = {"bound": "fp64-valu", "kernel": "clip_live_kernel<8> (0.25deg->C384 conservative, one launch per generation)",
+                              "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                              "traffic": None, "ops_per_launch": clip_ops, "launch_ms": clip_ms,
+                              "valu_busy_frac": profiled_value("clip_live_c384_counters.md", "valu_busy_frac"),
+                              "hw_fp64_ops_per_launch": profiled_value("clip_live_c384_counters.md", "hw_fp64_lane_ops_per_launch"),
+                              "profile": "profiles/clip_fp64_ops_c384.txt, profiles/clip_live_c384_counters.md"}
 
     extra = {}
     sections = set(args.sections.split(",")) - {"none", ""}
@@ -494,7 +601,8 @@ def main():
         "phases_ms": {"conservative_gen": t_cons, "bilinear_gen": t_bil, "apply_fp64_1field": ms_apply1},
         "weightgen": {"conservative_cells_per_s": n_dst / (t_cons * 1e-3), "candidates": st["candidates"],
                       "host_arbitrated_pairs": st["host_arbitrated"], "cap_ties": st["cap_ties"],
-                      "max_row_nnz": st["max_row_nnz"], "conservation_rel_drift": cons["relative_error"]},
+                      "max_row_nnz": st["max_row_nnz"], "conservation_rel_drift": cons["relative_error"],
+                      "phase_gpu_ms": phase_med, "roofline": weightgen_roofline},
         "apply": {"fields": k, "chosen": chosen, "modes": modes, "chosen_ms": ms_mixed, "plain_f64_ms": ms_plain,
                   "plain_shape": shape_plain, "chosen_fields_per_s": k / (ms_mixed * 1e-3),
                   "plain_fields_per_s": k / (ms_plain * 1e-3), "chosen_GBs": gbs_mixed, "plain_GBs": gbs_plain,
@@ -503,11 +611,12 @@ def main():
         "nan_and_masked_apply": nan_masked,
         "masked_source_generation": masked_gen,
         "roofline": {"bound": "hbm",
-                     "kernel": ("spmm_tile_kernel" if staged else "spmm_seg_kernel")
-                     + f"<{chosen}, f32 fields> (0.25deg->C384 conservative weights, {k} fields)",
-                     "achieved": gbs_mixed, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs_mixed / HBM_PEAK_GBS,
-                     "traffic": traffic, "algorithmic_bytes": bytes_mixed, "copy_ceiling_GBs": copy_gbs,
-                     "frac_of_copy_ceiling": gbs_mixed / copy_gbs},
+                     "kernel": f"spmm_tile_kernel<16, 4, 1, float, float> + long_reduce_kernel<float>: mixed_comp32, FP32 fields, "
+                               f"shape {list(ROOFLINE_SHAPE)} (fields per step, field chunks, LDS tiles), 0.25deg->C384 conservative "
+                               f"weights, {k} fields",
+                     "achieved": gbs_roof, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs_roof / HBM_PEAK_GBS,
+                     "traffic": traffic, "algorithmic_bytes": bytes_mixed, "launch_ms": ms_roof, "profile": f"profiles/{roof_profile}",
+                     "copy_ceiling_GBs": copy_gbs, "frac_of_copy_ceiling": gbs_roof / copy_gbs},
         "cpu_baseline": cpu,
     }
     out.update(extra)
