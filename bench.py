@@ -216,19 +216,26 @@ def bench_config4(axis, grids, fields, torch, stream):
                 ms = _timed(torch, stream, lambda: m.batch_apply(Xs, Y, k, xt, yt, mode, stream=stream.cuda_stream, sync=False))
                 m.batch_apply(Xs, Y, k, xt, yt, mode)
                 err = ((Y.double() - Yref).abs().amax(dim=1) / Yref.abs().amax(dim=1)).max().item()
-                # global-integral drift |dst integral - src integral|, relative to the
-                # src integral of |f| (fronts integrate to ~0, which makes the
-                # reference's |d| / max(|src|, |dst|) meaningless for them), over
-                # two fronts and two harmonics
-                drift = 0.0
-                for fi in (0, 16, 32, 48):
+                # global-integral drift of every field: check_conservation's
+                # relative error (conservation.cpp:82-103, |dst - src| /
+                # max(|src|, |dst|)) and the same difference relative to the src
+                # integral of |f| (fronts integrate to ~0, where the reference's
+                # quotient says little)
+                ref_rel, l1_rel = [], []
+                for fi in range(k):
                     xf = Xs[fi].double().contiguous()
-                    rep = m.check_conservation(xf, Y[fi].double().contiguous())
-                    l1 = m.check_conservation(xf.abs(), Y[fi].double().abs().contiguous())["src_integral"]
-                    drift = max(drift, rep["absolute_error"] / l1)
+                    yf = Y[fi].double().contiguous()
+                    rep = m.check_conservation(xf, yf)
+                    l1 = m.check_conservation(xf.abs(), yf.abs().contiguous())["src_integral"]
+                    ref_rel.append(rep["relative_error"])
+                    l1_rel.append(rep["absolute_error"] / l1)
+                ref_rel.sort()
+                l1_rel.sort()
                 b = ns * k * Xs.element_size() + nd * k * Y.element_size() + nnz * 12 + (nd + 1) * 8
                 res[f"{sname}_in_{oname}_{mname}"] = {"ms": ms, "GBs": b / (ms * 1e-3) / 1e9, "frac_of_8TBs": b / (ms * 1e-3) / 8e12,
-                                                      "max_rel_err_vs_fp64": err, "conservation_drift_max": drift}
+                                                      "max_rel_err_vs_fp64": err,
+                                                      "conservation_rel_error_max": ref_rel[-1], "conservation_rel_error_median": ref_rel[k // 2],
+                                                      "conservation_l1_drift_max": l1_rel[-1], "conservation_l1_drift_median": l1_rel[k // 2]}
         del Xs
     ms_plain = _timed(torch, stream, lambda: m.batch_apply(X64, Yref, k, axis.F64, axis.F64, axis.PLAIN,
                                                            stream=stream.cuda_stream, sync=False))
