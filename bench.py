@@ -183,9 +183,12 @@ def bench_config3(axis, grids, fields, torch, stream, m384, f384, n_src384, n_ds
 
 def bench_config4(axis, grids, fields, torch, stream):
     """0.1deg (3600x1800) -> C768 conservative weights; 64 fields (32 fronts
-    tanh(lat/0.02)*cos(lon), 32 degree-32 harmonics) stored as BF16 and as
-    FP32, applied with FP32/compensated accumulation; error per field against
-    the FP64 apply of the same decoded inputs, and global-integral drift."""
+    2 + tanh(lat/0.02)*cos(lon), 32 degree-32 harmonics + 4: positive means,
+    as the 2 + cos^2(lat)cos(2 lon) field of the 2deg->C24 case, so that the
+    reference's relative conservation error, a quotient by the integral, is
+    informative) stored as BF16 and as FP32, applied with FP32/compensated
+    accumulation; error per field against the FP64 apply of the same decoded
+    inputs, and global-integral drift of every field."""
     import math
     dev = torch.device("cuda")
     s_m, d_m = grids.latlon_deg(0.1), grids.cubed_sphere(768)
@@ -201,14 +204,18 @@ def bench_config4(axis, grids, fields, torch, stream):
     k = 64
     X64 = torch.empty((k, ns), dtype=torch.float64, device=dev)
     for i in range(32):
-        X64[i] = torch.tanh((lat - 0.03 * (i - 16)) / 0.02) * torch.cos(lon + 0.2 * i)
-    X64[32:] = fields.harmonic_sums_torch(cen, [fields.harmonic_coeffs(1007 + i, lmax=32) for i in range(32)])
+        X64[i] = 2.0 + torch.tanh((lat - 0.03 * (i - 16)) / 0.02) * torch.cos(lon + 0.2 * i)
+    X64[32:] = 4.0 + fields.harmonic_sums_torch(cen, [fields.harmonic_coeffs(1007 + i, lmax=32) for i in range(32)])
     res = {"workload": "0.1deg (3600x1800) -> C768 (6x768x768), 64 fields", "n_src": ns, "n_dst": nd, "nnz": nnz,
            "conservative_gen_ms": t_gen}
     Yref = torch.empty((k, nd), dtype=torch.float64, device=dev)
     for sname, sdt, xt in (("bf16", torch.bfloat16, axis.BF16), ("f32", torch.float32, axis.F32)):
         Xs = X64.to(sdt).contiguous()
         m.batch_apply(Xs.double().contiguous(), Yref, k, axis.F64, axis.F64, axis.PLAIN)
+        # the FP64 apply's own conservation (the weights' share of any drift)
+        ref_int = [m.check_conservation(Xs[fi].double().contiguous(), Yref[fi].contiguous()) for fi in range(k)]
+        f64_rel = sorted(r["relative_error"] for r in ref_int)
+        res[f"{sname}_in_plain_f64_conservation_rel_error"] = {"max": f64_rel[-1], "median": f64_rel[k // 2]}
         for oname, odt, yt in (("f32_out", torch.float32, axis.F32), ("f64_out", torch.float64, axis.F64)):
             for mname, mode in (("comp32", axis.MIXED), ("acc64", axis.MIXED_ACC64)):
                 Y = torch.empty((k, nd), dtype=odt, device=dev)
@@ -218,10 +225,10 @@ def bench_config4(axis, grids, fields, torch, stream):
                 err = ((Y.double() - Yref).abs().amax(dim=1) / Yref.abs().amax(dim=1)).max().item()
                 # global-integral drift of every field: check_conservation's
                 # relative error (conservation.cpp:82-103, |dst - src| /
-                # max(|src|, |dst|)) and the same difference relative to the src
-                # integral of |f| (fronts integrate to ~0, where the reference's
-                # quotient says little)
-                ref_rel, l1_rel = [], []
+                # max(|src|, |dst|)), the same difference relative to the src
+                # integral of |f|, and the drift this apply adds to the FP64
+                # apply's (|dst - dst_fp64| / |src|)
+                ref_rel, l1_rel, add_rel = [], [], []
                 for fi in range(k):
                     xf = Xs[fi].double().contiguous()
                     yf = Y[fi].double().contiguous()
@@ -229,13 +236,17 @@ def bench_config4(axis, grids, fields, torch, stream):
                     l1 = m.check_conservation(xf.abs(), yf.abs().contiguous())["src_integral"]
                     ref_rel.append(rep["relative_error"])
                     l1_rel.append(rep["absolute_error"] / l1)
+                    add_rel.append(abs(rep["dst_integral"] - ref_int[fi]["dst_integral"]) / abs(rep["src_integral"]))
                 ref_rel.sort()
                 l1_rel.sort()
+                add_rel.sort()
                 b = ns * k * Xs.element_size() + nd * k * Y.element_size() + nnz * 12 + (nd + 1) * 8
                 res[f"{sname}_in_{oname}_{mname}"] = {"ms": ms, "GBs": b / (ms * 1e-3) / 1e9, "frac_of_8TBs": b / (ms * 1e-3) / 8e12,
                                                       "max_rel_err_vs_fp64": err,
                                                       "conservation_rel_error_max": ref_rel[-1], "conservation_rel_error_median": ref_rel[k // 2],
-                                                      "conservation_l1_drift_max": l1_rel[-1], "conservation_l1_drift_median": l1_rel[k // 2]}
+                                                      "conservation_l1_drift_max": l1_rel[-1], "conservation_l1_drift_median": l1_rel[k // 2],
+                                                      "drift_added_to_fp64_apply_max": add_rel[-1],
+                                                      "drift_added_to_fp64_apply_median": add_rel[k // 2]}
         del Xs
     ms_plain = _timed(torch, stream, lambda: m.batch_apply(X64, Yref, k, axis.F64, axis.F64, axis.PLAIN,
                                                            stream=stream.cuda_stream, sync=False))

@@ -6,7 +6,9 @@
 //
 //   detection ........ unique_axes / regular_from_axes        (mesh.hip)
 //   per-cell pass .... areas, degenerate flags, caps, boxes   (mesh.hip)
-//   broad phase ...... LBVH over float boxes                  (lbvh.hip)
+//   broad phase ...... float box overlaps: through the source's cell grid
+//                      when it is a rectilinear degree grid (gridquery.hip),
+//                      else an LBVH (lbvh.hip); the same candidate sets
 //   overlap pass ..... cap filter per (dst, candidate) pair, compacting the
 //                      survivors; then one lane per survivor: great-circle
 //                      Sutherland-Hodgman clip with the subject polygon in
@@ -14,9 +16,11 @@
 //   arbitration ...... the few pairs whose outcome hinges on libm ulps
 //                      (cap-test ties, atan2 cancellation, >CAP vertices)
 //                      are recomputed on the host with the reference's libm
-//   CSR assembly ..... nonzero compaction, (row, col) radix sort, row_ptr,
-//                      weights w = overlap / area_dst, frac_b (column order),
-//                      FracArea scaling, frac_a (row order), clamps
+//   CSR assembly ..... per-row nonzero counts, row_ptr by a scan, each row's
+//                      nonzeros sorted by column where they lie (candidates
+//                      are grouped by row), weights w = overlap / area_dst,
+//                      frac_b (column order), FracArea scaling, frac_a
+//                      (row order, by a stable column sort), clamps
 //
 // The sparsity pattern equals the reference's bit for bit: an entry exists
 // iff the clipped area is > 0, and that decision depends only on + - * /
@@ -143,7 +147,7 @@ __global__ void __launch_bounds__(kWave) overlap_kernel(index_t p0, index_t p1, 
                 for (int k = 0; k < ns; ++k) A.set(k, cell_vertex(src, i, k));
                 const int nc = cell_nverts(dst, j);
                 auto clipv = [&](int k) { return cell_vertex(dst, j, k); };
-                int n = geo::clip_polygon_inplace(A, ns, clipv, nc, CAP, &overflow);
+                int n = geo::clip_polygon_slots(A, ns, clipv, nc, CAP, &overflow);
                 if (!overflow && n >= 3) {
                     double abs_sum;
                     bool zero_num;
@@ -244,7 +248,7 @@ __global__ void __launch_bounds__(kWave) clip_live_kernel(index_t p0, const std:
         for (int k = 0; k < ns; ++k) A.set(k, cell_vertex(src, i, k));
         const int nc = cell_nverts(dst, j);
         auto clipv = [&](int k) { return cell_vertex(dst, j, k); };
-        int n = geo::clip_polygon_inplace(A, ns, clipv, nc, CAP, &overflow);
+        int n = geo::clip_polygon_slots(A, ns, clipv, nc, CAP, &overflow);
         if (!overflow && n >= 3) {
             double abs_sum;
             bool zero_num;

@@ -134,14 +134,49 @@ __host__ __device__ inline int clip_polygon(PolyRef a, int na, PolyRef b, ClipVe
     return na;
 }
 
-/// clip_polygon with one buffer: the walk over the subject's vertices reads
-/// one vertex ahead, and each step writes its (at most two) results at
-/// positions the walk has already read, which holds whenever the subject
-/// enters the clip half-space at most once per edge -- always for the convex
-/// cells a valid mesh has.  Anything else (a write that would land ahead of
-/// the walk, or more than `cap` vertices) sets *overflow, for the caller to
-/// decide the pair with clip_polygon on the host.  The arithmetic and the
-/// order of the results are those of clip_polygon.
+/// The inclusive half-space test of the clipper (spherical_clipper.hpp:
+/// 307-319): a vertex at signed distance d from the clip plane is kept when
+/// d >= -1e-14.
+__host__ __device__ inline bool inside_half(double d) { return d >= -1.0e-14; }
+
+/// One clip edge of clip_polygon on one buffer: the walk over the subject's
+/// vertices reads one vertex ahead, and each step writes its (at most two)
+/// results at positions the walk has already read, which holds whenever the
+/// subject enters the clip half-space at most once per edge -- always for the
+/// convex cells a valid mesh has.  Anything else (a write that would land
+/// ahead of the walk, or more than `cap` vertices) sets *overflow.  Returns
+/// the new vertex count.
+__host__ __device__ inline int clip_edge_walk(PolyRef a, int na, const V3 &ea, const V3 &eb, const V3 &normal, int cap, bool *overflow) {
+    int nb = 0;
+    V3 prev = a.get(na - 1);
+    double d_prev = dot(normal, prev);
+    V3 next = a.get(0);
+    for (int i = 0; i < na; ++i) {
+        const V3 cur = next;
+        if (i + 1 < na) next = a.get(i + 1);
+        const double d_cur = dot(normal, cur);
+        const bool cur_in = inside_half(d_cur);
+        const bool prev_in = inside_half(d_prev);
+        const int emit = cur_in ? (prev_in ? 1 : 2) : (prev_in ? 1 : 0);
+        if (emit > 0 && (nb + emit - 1 > i + 1 || nb + emit > cap)) {
+            *overflow = true;
+            return 0;
+        }
+        if (cur_in) {
+            if (!prev_in) a.set(nb++, gc_intersection(prev, cur, ea, eb));
+            a.set(nb++, cur);
+        } else if (prev_in) {
+            a.set(nb++, gc_intersection(prev, cur, ea, eb));
+        }
+        prev = cur;
+        d_prev = d_cur;
+    }
+    return nb;
+}
+
+/// clip_polygon on one buffer (clip_edge_walk per clip edge): on *overflow
+/// the caller decides the pair with clip_polygon on the host.  The
+/// arithmetic and the order of the results are those of clip_polygon.
 template <class ClipVertex>
 __host__ __device__ inline int clip_polygon_inplace(PolyRef a, int na, ClipVertex clip_vertex, int nc, int cap, bool *overflow) {
     if (na < 3 || nc < 3) return 0;
@@ -150,33 +185,75 @@ __host__ __device__ inline int clip_polygon_inplace(PolyRef a, int na, ClipVerte
         const V3 eb = clip_vertex(ci + 1 == nc ? 0 : ci + 1);
         V3 normal = cross(ea, eb);
         if (length_sq(normal) < 1.0e-30) continue;
-        int nb = 0;
-        constexpr double eps = -1.0e-14;
-        V3 prev = a.get(na - 1);
-        double d_prev = dot(normal, prev);
-        V3 next = a.get(0);
-        for (int i = 0; i < na; ++i) {
-            const V3 cur = next;
-            if (i + 1 < na) next = a.get(i + 1);
-            const double d_cur = dot(normal, cur);
-            const bool cur_in = d_cur >= eps;
-            const bool prev_in = d_prev >= eps;
-            const int emit = cur_in ? (prev_in ? 1 : 2) : (prev_in ? 1 : 0);
-            if (emit > 0 && (nb + emit - 1 > i + 1 || nb + emit > cap)) {
-                *overflow = true;
-                return 0;
+        na = clip_edge_walk(a, na, ea, eb, normal, cap, overflow);
+        if (*overflow || na < 3) return 0;
+    }
+    return na;
+}
+
+/// clip_polygon_inplace arranged for lockstep lanes.  Per clip edge: the
+/// side of every vertex first (a bit mask), then the intersections at the
+/// two places where a convex subject crosses the edge, then the surviving
+/// vertices moved into place.  A wave then runs the intersection code twice
+/// per edge, where the vertex walk runs it at every position at which any of
+/// its lanes crosses.  An edge that the subject crosses more than twice (not
+/// convex) takes clip_edge_walk.  Every intersection has the walk's operands
+/// and the output its order, so the results are clip_polygon_inplace's bit
+/// for bit, overflow included.  Needs cap < 32.
+template <class ClipVertex>
+__host__ __device__ inline int clip_polygon_slots(PolyRef a, int na, ClipVertex clip_vertex, int nc, int cap, bool *overflow) {
+    if (na < 3 || nc < 3) return 0;
+    if (na > cap) {
+        *overflow = true;
+        return 0;
+    }
+    for (int ci = 0; ci < nc; ++ci) {
+        const V3 ea = clip_vertex(ci);
+        const V3 eb = clip_vertex(ci + 1 == nc ? 0 : ci + 1);
+        const V3 normal = cross(ea, eb);
+        if (length_sq(normal) < 1.0e-30) continue;
+        unsigned in = 0;  // bit i: vertex i inside (na <= cap < 32)
+        for (int i = 0; i < na; ++i) in |= static_cast<unsigned>(inside_half(dot(normal, a.get(i)))) << i;
+        if (in == 0) return 0;  // every vertex outside: nothing is emitted
+        const unsigned full = (1u << na) - 1u;
+        const unsigned prev_in = ((in << 1) | (in >> (na - 1))) & full;  // bit i: vertex i-1 (cyclic) inside
+        const unsigned x = in ^ prev_in;                                // bit i: the arc (i-1, i) crosses
+        if (x == 0) continue;  // every vertex inside: the walk rewrites the polygon unchanged
+        if (__builtin_popcount(x) != 2) {
+            na = clip_edge_walk(a, na, ea, eb, normal, cap, overflow);
+            if (*overflow || na < 3) return 0;
+            continue;
+        }
+        const int i0 = __builtin_ctz(x), i1 = __builtin_ctz(x & (x - 1u));
+        const int nb = __builtin_popcount(in) + 2;
+        if (nb > cap) {
+            *overflow = true;
+            return 0;
+        }
+        const V3 x0 = gc_intersection(a.get(i0 == 0 ? na - 1 : i0 - 1), a.get(i0), ea, eb);
+        const V3 x1 = gc_intersection(a.get(i1 - 1), a.get(i1), ea, eb);
+        if ((in >> i0) & 1u) {
+            // enters at i0, leaves at i1: X0, v[i0 .. i1-1], X1
+            const int m = i1 - i0;
+            if (i0 == 0) {
+                for (int k = m - 1; k >= 0; --k) a.set(k + 1, a.get(k));
+            } else {
+                for (int k = 0; k < m; ++k) a.set(k + 1, a.get(i0 + k));
             }
-            if (cur_in) {
-                if (!prev_in) a.set(nb++, gc_intersection(prev, cur, ea, eb));
-                a.set(nb++, cur);
-            } else if (prev_in) {
-                a.set(nb++, gc_intersection(prev, cur, ea, eb));
+            a.set(0, x0);
+            a.set(m + 1, x1);
+        } else {
+            // leaves at i0, enters at i1: v[0 .. i0-1], X0, X1, v[i1 .. na-1]
+            const int t = na - i1;
+            if (i1 == i0 + 1) {
+                for (int k = t - 1; k >= 0; --k) a.set(i0 + 2 + k, a.get(i1 + k));
+            } else {
+                for (int k = 0; k < t; ++k) a.set(i0 + 2 + k, a.get(i1 + k));
             }
-            prev = cur;
-            d_prev = d_cur;
+            a.set(i0, x0);
+            a.set(i0 + 1, x1);
         }
         na = nb;
-        if (na < 3) return 0;
     }
     return na;
 }

@@ -4,9 +4,11 @@
 Builds the meshes of a headline pair once, then runs --reps generations of
 the requested method (after --warmup untimed ones) and prints one JSON line
 with the wall time of each, so that a kernel/runtime trace of the run covers
-a known number of generations:
+a known number of generations.  --gap-ms idles the GPU between generations
+(outside the timed calls), so that a kernel trace splits into one cluster
+per generation (profile_kernels.py --gap-ms):
 
-    python3 helm-project_amd/profile_generate.py --pair c384 --method conservative --reps 3
+    python3 helm-project_amd/profile_generate.py --pair c384 --method conservative --reps 3 --gap-ms 50
 """
 import argparse
 import importlib.util
@@ -34,6 +36,7 @@ def main():
     ap.add_argument("--method", default="conservative", choices=["conservative", "bilinear"])
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--gap-ms", type=float, default=0.0, help="idle time after each generation (not timed)")
     args = ap.parse_args()
 
     import torch
@@ -53,15 +56,21 @@ def main():
     for _ in range(args.warmup):
         axis.generate_weights(S, D, method).close()
     torch.cuda.synchronize()
-    times = []
+    if args.gap_ms > 0:
+        time.sleep(args.gap_ms * 1e-3)
+    times, phases = [], []
     for _ in range(args.reps):
         t = time.perf_counter()
         m = axis.generate_weights(S, D, method)
         torch.cuda.synchronize()
         times.append((time.perf_counter() - t) * 1e3)
         nnz = m.info()[2]
+        phases.append(m.phase_ms())
         m.close()
-    print(json.dumps({"pair": args.pair, "method": args.method, "ms": times, "nnz": nnz}), flush=True)
+        if args.gap_ms > 0:
+            torch.cuda.synchronize()
+            time.sleep(args.gap_ms * 1e-3)
+    print(json.dumps({"pair": args.pair, "method": args.method, "ms": times, "nnz": nnz, "phase_gpu_ms": phases}), flush=True)
 
 
 if __name__ == "__main__":
