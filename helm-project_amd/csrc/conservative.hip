@@ -450,6 +450,8 @@ __global__ void unmapped_kernel(const index_t *row_ptr, index_t n_dst, const std
 constexpr int kShortRow = 16;     // sorted by one thread
 constexpr int kLongRowMax = 4096; // sorted by one workgroup in LDS
 constexpr int kLongRowBlock = 256;
+constexpr int kHugeRowMax = 8192; // sorted by one 1024-thread workgroup in 96 KiB of LDS
+constexpr int kHugeRowBlock = 1024;
 constexpr int kMidRow = 64;        // sorted by one wave in a small LDS tile
 
 constexpr int kHistMax = 16;  // row-length histogram: lengths 0 .. kHistMax, then longer
@@ -545,13 +547,14 @@ __global__ void __launch_bounds__(256) short_rows_kernel(const index_t *cand_off
     }
 }
 
-// rows of kShortRow < len <= CAP: bitonic column sort in LDS, the ordered sum
-// by one thread, the rest in parallel (columns are distinct within a row, so
-// the sorted order does not depend on the sort).  CAP = kMidRow with one wave
-// for most long rows (LDS of a few hundred bytes: many rows per CU), CAP =
-// kLongRowMax with kLongRowBlock threads for the rest.
+// listed rows of min_len < len <= CAP: bitonic column sort in LDS, the
+// ordered sum by one thread, the rest in parallel (columns are distinct
+// within a row, so the sorted order does not depend on the sort).  CAP =
+// kMidRow with one wave for most long rows (LDS of a few hundred bytes: many
+// rows per CU), CAP = kLongRowMax with kLongRowBlock threads for the rest up
+// to that length, CAP = kHugeRowMax (one workgroup per CU) beyond.
 template <int CAP, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) long_rows_kernel(const std::int32_t *long_rows, const int *n_list, bool from_back, index_t n_dst,
+__global__ void __launch_bounds__(BLOCK) long_rows_kernel(const std::int32_t *long_rows, const int *n_list, bool from_back, int min_len, index_t n_dst,
                                                           const index_t *cand_off, const index_t *row_ptr, const std::int32_t *pair_src,
                                                           const double *overlap, const double *area_dst, int frac_area, index_t *col, double *val,
                                                           double *ov_s, double *frac_b) {
@@ -564,7 +567,8 @@ __global__ void __launch_bounds__(BLOCK) long_rows_kernel(const std::int32_t *lo
     for (int li = blockIdx.x; li < nl; li += gridDim.x) {  // (workgroups loop over the listed rows)
         const index_t r = from_back ? long_rows[n_dst - 1 - li] : long_rows[li];
         const index_t b = row_ptr[r];
-        const int len = static_cast<int>(row_ptr[r + 1] - b);  // <= CAP (checked by the host)
+        const int len = static_cast<int>(row_ptr[r + 1] - b);  // <= the longest listed row (checked by the host)
+        if (len <= min_len || len > CAP) continue;           // (another launch takes it; uniform over the workgroup)
         int P = 2;
         while (P < len) P <<= 1;
         // the row's nonzero candidates into LDS, in any order (columns are
@@ -1254,7 +1258,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
             AXIS_HIP_CHECK(hipStreamSynchronize(s));
             nnz = h[0];
             h_max_row = static_cast<int>(h[1] & 0xffffffff);
-            grouped = h_max_row <= kLongRowMax;
+            grouped = h_max_row <= kHugeRowMax;
         }
     }
     DeviceBuffer<index_t> pos;
@@ -1281,15 +1285,21 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
                                                M->col.get(), M->val.get(), ov.get(), M->frac_b.get());
         AXIS_LAUNCH_CHECK();
         if (h_max_row > kShortRow) {  // the listed rows (their counts stay on the device)
-            long_rows_kernel<kMidRow, kMidRow><<<2048, kMidRow, 0, s>>>(long_rows.get(), n_long.get(), false, n_dst, cand_off.get(),
+            long_rows_kernel<kMidRow, kMidRow><<<2048, kMidRow, 0, s>>>(long_rows.get(), n_long.get(), false, kShortRow, n_dst, cand_off.get(),
                                                                         M->row_ptr.get(), cand_src.get(), overlap.get(), dg.d_area.get(),
                                                                         frac_area, M->col.get(), M->val.get(), ov.get(), M->frac_b.get());
             AXIS_LAUNCH_CHECK();
         }
         if (h_max_row > kMidRow) {
             long_rows_kernel<kLongRowMax, kLongRowBlock><<<512, kLongRowBlock, 0, s>>>(
-                long_rows.get(), n_long.get() + 1, true, n_dst, cand_off.get(), M->row_ptr.get(), cand_src.get(), overlap.get(), dg.d_area.get(),
-                frac_area, M->col.get(), M->val.get(), ov.get(), M->frac_b.get());
+                long_rows.get(), n_long.get() + 1, true, kMidRow, n_dst, cand_off.get(), M->row_ptr.get(), cand_src.get(), overlap.get(),
+                dg.d_area.get(), frac_area, M->col.get(), M->val.get(), ov.get(), M->frac_b.get());
+            AXIS_LAUNCH_CHECK();
+        }
+        if (h_max_row > kLongRowMax) {
+            long_rows_kernel<kHugeRowMax, kHugeRowBlock><<<256, kHugeRowBlock, 0, s>>>(
+                long_rows.get(), n_long.get() + 1, true, kLongRowMax, n_dst, cand_off.get(), M->row_ptr.get(), cand_src.get(), overlap.get(),
+                dg.d_area.get(), frac_area, M->col.get(), M->val.get(), ov.get(), M->frac_b.get());
             AXIS_LAUNCH_CHECK();
         }
         assembled = true;
