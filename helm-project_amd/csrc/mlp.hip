@@ -9,7 +9,8 @@
 // pass, and W2^T, W3^T for the backward pass.
 //
 // One workgroup (4 waves) handles a tile of 32 samples.  Every product is a
-// sequence of 16x16x16 BF16 MFMA tiles whose operands are both K-contiguous
+// sequence of 16x16x32 BF16 MFMA tiles (16x16x16 for a last K step of 16)
+// whose operands are both K-contiguous
 // (A row-major M x K, B given as B^T row-major N x K); activations are kept in
 // LDS in both orientations so that forward and backward products read
 // K-contiguous rows:
@@ -52,12 +53,25 @@ constexpr int kPacked = bW3T + kH * kOutP;
 __device__ inline std::uint16_t f2bf(float f) { return round_bf16(f); }
 __device__ inline float bf2f(std::uint16_t h) { return __uint_as_float(static_cast<unsigned>(h) << 16); }
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
 // C tile (16 x 16) += A[16 x K] * B[K x 16]; A row-major (lda), B given as
-// B^T row-major (ldb).  Lane l feeds A[l % 16][4 (l / 16) ..+3] and
-// B[4 (l / 16) ..+3][l % 16]; it receives C[4 (l / 16) + i][l % 16], i < 4.
+// B^T row-major (ldb), K a multiple of 16, lda and ldb multiples of 8.  Steps
+// of 32 go through gfx950's 16x16x32 BF16 MFMA (lane l feeds A[l % 16][8
+// (l / 16) ..+7] and B[8 (l / 16) ..+7][l % 16]), a remaining 16 through the
+// 16x16x16 one (4 (l / 16) ..+3); lane l receives C[4 (l / 16) + i][l % 16],
+// i < 4, from both.
 __device__ inline f32x4 mma_tile(const std::uint16_t *A, int lda, const std::uint16_t *BT, int ldb, int K, f32x4 acc) {
-    const int lane = static_cast<int>(threadIdx.x) & 63, r = lane & 15, kq = (lane >> 4) * 4;
-    for (int k0 = 0; k0 < K; k0 += 16) {
+    const int lane = static_cast<int>(threadIdx.x) & 63, r = lane & 15;
+    int k0 = 0;
+    for (; k0 + 32 <= K; k0 += 32) {
+        const int kq = (lane >> 4) * 8;
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(A + r * lda + k0 + kq);
+        const bf16x8 b = *reinterpret_cast<const bf16x8 *>(BT + r * ldb + k0 + kq);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    }
+    for (; k0 < K; k0 += 16) {
+        const int kq = (lane >> 4) * 4;
         const bf16x4 a = *reinterpret_cast<const bf16x4 *>(A + r * lda + k0 + kq);
         const bf16x4 b = *reinterpret_cast<const bf16x4 *>(BT + r * ldb + k0 + kq);
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
