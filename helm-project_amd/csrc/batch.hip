@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <numeric>
 
 #include "apply_kernels.hpp"
@@ -344,6 +346,7 @@ BatchPlan::BatchPlan(std::vector<std::shared_ptr<InterpolationMatrix>> mats, std
                 if (fu > fpi) continue;
                 I.record(BatchShape{fu, fpi}, s);
                 const float ms = time_graph();
+                if (std::getenv("AXIS_TRACE")) std::fprintf(stderr, "axis[trace] batch_plan.persistent fu %d fields/item %d: %.3f ms per 3 replays\n", fu, fpi, ms);
                 if (ms < best_ms) {
                     best_ms = ms;
                     best = BatchShape{fu, fpi};
@@ -351,6 +354,7 @@ BatchPlan::BatchPlan(std::vector<std::shared_ptr<InterpolationMatrix>> mats, std
             }
         I.record_pairs();
         const float ms = time_graph();
+        if (std::getenv("AXIS_TRACE")) std::fprintf(stderr, "axis[trace] batch_plan.pair_launches: %.3f ms per 3 replays\n", ms);
         if (ms < best_ms) best = I.shape;
         AXIS_HIP_CHECK(hipEventDestroy(e0));
         AXIS_HIP_CHECK(hipEventDestroy(e1));
