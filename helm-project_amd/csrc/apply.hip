@@ -560,6 +560,54 @@ void batch_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, F
     apply_detail::launch(A, x, xt, y, yt, k, mode, L, s);
 }
 
+// ------------------------------------------------------ field_view entry points
+namespace {
+[[noreturn]] void extent_error(const char *fn, const char *what, std::size_t got, const char *whose, index_t want) {
+    throw InvalidArgument(std::string("axis::solver::") + fn + ": " + what + " (" + std::to_string(got) + ") != " + whose + " (" +
+                          std::to_string(want) + ")");
+}
+}  // namespace
+
+void apply(InterpolationMatrix &A, field_view<const double, 1> src, field_view<double, 1> dst, hipStream_t s) {
+    // apply.hpp:63-71: validation before dst is touched
+    if (src.extent(0) != static_cast<std::size_t>(A.n_src)) extent_error("apply", "src.extent(0)", src.extent(0), "matrix.n_src()", A.n_src);
+    if (dst.extent(0) != static_cast<std::size_t>(A.n_dst)) extent_error("apply", "dst.extent(0)", dst.extent(0), "matrix.n_dst()", A.n_dst);
+    batch_apply(A, src.data_handle(), FieldType::F64, dst.data_handle(), FieldType::F64, A.n_src, A.n_dst, 1, ApplyMode::Plain, s);
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void apply(InterpolationMatrix &A, field_view<const double, 1> src, field_view<double, 1> dst, field_view<const std::int32_t, 1> dst_mask,
+           hipStream_t s) {
+    if (src.extent(0) != static_cast<std::size_t>(A.n_src))
+        extent_error("apply (masked)", "src.extent(0)", src.extent(0), "matrix.n_src()", A.n_src);
+    if (dst.extent(0) != static_cast<std::size_t>(A.n_dst))
+        extent_error("apply (masked)", "dst.extent(0)", dst.extent(0), "matrix.n_dst()", A.n_dst);
+    masked_apply(A, src.data_handle(), dst.data_handle(), dst_mask.data_handle(), static_cast<index_t>(dst_mask.extent(0)), A.n_src, A.n_dst, 1, s);
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void batch_apply(InterpolationMatrix &A, field_view<const double, 2> src, field_view<double, 2> dst, hipStream_t s) {
+    // apply.hpp:280-292
+    if (src.extent(0) != static_cast<std::size_t>(A.n_src))
+        extent_error("batch_apply", "src.extent(0)", src.extent(0), "matrix.n_src()", A.n_src);
+    if (dst.extent(0) != static_cast<std::size_t>(A.n_dst))
+        extent_error("batch_apply", "dst.extent(0)", dst.extent(0), "matrix.n_dst()", A.n_dst);
+    if (src.extent(1) != dst.extent(1))
+        extent_error("batch_apply", "src.extent(1)", src.extent(1), "dst.extent(1)", static_cast<index_t>(dst.extent(1)));
+    batch_apply(A, src.data_handle(), FieldType::F64, dst.data_handle(), FieldType::F64, A.n_src, A.n_dst, static_cast<index_t>(src.extent(1)),
+                ApplyMode::Plain, s);
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void nan_batch_apply(InterpolationMatrix &A, field_view<const double, 2> src, field_view<double, 2> dst, double na_thres, hipStream_t s) {
+    // apply.hpp:363-371
+    if (src.extent(0) != static_cast<std::size_t>(A.n_src)) throw InvalidArgument("axis::solver::nan_batch_apply: src.extent(0) != matrix.n_src()");
+    if (dst.extent(0) != static_cast<std::size_t>(A.n_dst)) throw InvalidArgument("axis::solver::nan_batch_apply: dst.extent(0) != matrix.n_dst()");
+    if (src.extent(1) != dst.extent(1)) throw InvalidArgument("axis::solver::nan_batch_apply: src.extent(1) != dst.extent(1)");
+    nan_batch_apply(A, src.data_handle(), dst.data_handle(), A.n_src, A.n_dst, static_cast<index_t>(src.extent(1)), na_thres, s);
+    AXIS_HIP_CHECK(hipStreamSynchronize(s));
+}
+
 // ------------------------------------------------ apply without allocating
 namespace {
 

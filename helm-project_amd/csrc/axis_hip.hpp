@@ -13,6 +13,7 @@
 // at construction.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 #include <atomic>
 #include <map>
@@ -244,6 +245,38 @@ void batch_apply_noalloc(InterpolationMatrix &A, const void *x, FieldType xt, vo
 std::size_t noalloc_scratch_doubles();
 
 ConservationReport check_conservation(const InterpolationMatrix &A, const double *d_src, const double *d_dst, NormType norm, hipStream_t s);
+
+/// Non-owning layout_left view of a field array, the shape of the
+/// reference's field_view<T, Rank> (types.hpp:69-70: an mdspan with dynamic
+/// extents and layout_left): element (i, j) of a rank-2 view is data[i + j *
+/// extent(0)], so column j (one field) is contiguous.  Device memory here.
+template <class T, std::size_t Rank>
+class field_view {
+    static_assert(Rank == 1 || Rank == 2, "field_view: rank 1 or 2");
+
+   public:
+    field_view() = default;
+    field_view(T *data, std::size_t n0) requires(Rank == 1) : p_(data), e_{n0} {}
+    field_view(T *data, std::size_t n0, std::size_t n1) requires(Rank == 2) : p_(data), e_{n0, n1} {}
+    T *data_handle() const { return p_; }
+    std::size_t extent(std::size_t r) const { return r < Rank ? e_[r] : 1; }
+    static constexpr std::size_t rank() { return Rank; }
+
+   private:
+    T *p_ = nullptr;
+    std::size_t e_[Rank] = {};
+};
+
+/// apply (apply.hpp:61-119): dst = A src for one FP64 field; extents checked
+/// before dst is touched (the reference's messages); returns when done.
+void apply(InterpolationMatrix &A, field_view<const double, 1> src, field_view<double, 1> dst, hipStream_t s = nullptr);
+/// Masked apply (apply.hpp:457-484): rows with dst_mask 0 keep their values.
+void apply(InterpolationMatrix &A, field_view<const double, 1> src, field_view<double, 1> dst, field_view<const std::int32_t, 1> dst_mask,
+           hipStream_t s = nullptr);
+/// batch_apply (apply.hpp:275-343): dst(:, v) = A src(:, v) for every field v.
+void batch_apply(InterpolationMatrix &A, field_view<const double, 2> src, field_view<double, 2> dst, hipStream_t s = nullptr);
+/// nan_batch_apply (apply.hpp:359-436) on views.
+void nan_batch_apply(InterpolationMatrix &A, field_view<const double, 2> src, field_view<double, 2> dst, double na_thres, hipStream_t s = nullptr);
 
 /// Launch shape of the apply: fields per step, field chunks (more chunks =
 /// more workgroups, more re-reads of the matrix) and whether tiles stage

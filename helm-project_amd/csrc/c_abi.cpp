@@ -512,7 +512,7 @@ int axis_apply_c(int matrix_handle, const double *src, double *dst, int n_src, i
             ybuf.resize(n_dst);
             y = ybuf.get();
         }
-        batch_apply(*M, x.ptr, FieldType::F64, y, FieldType::F64, n_src, n_dst, 1, ApplyMode::Plain, s);
+        apply(*M, field_view<const double, 1>(x.ptr, static_cast<std::size_t>(n_src)), field_view<double, 1>(y, static_cast<std::size_t>(n_dst)), s);
         if (!dev_out && n_dst > 0) ybuf.download(dst, n_dst, s);
         AXIS_HIP_CHECK(hipStreamSynchronize(s));
     });
@@ -539,6 +539,12 @@ int axis_batch_apply_c(int matrix_handle, const void *src, int src_dtype, void *
     return guarded([&] {
         auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
         auto s = static_cast<hipStream_t>(stream);
+        if (amode(mode) == ApplyMode::Plain && ftype(src_dtype) == FieldType::F64 && ftype(dst_dtype) == FieldType::F64 && n_src >= 0 && n_dst >= 0 &&
+            n_fields >= 0) {  // the reference's batch_apply on field views (apply.hpp:336-343)
+            batch_apply(*M, field_view<const double, 2>(static_cast<const double *>(src), static_cast<std::size_t>(n_src), static_cast<std::size_t>(n_fields)),
+                        field_view<double, 2>(static_cast<double *>(dst), static_cast<std::size_t>(n_dst), static_cast<std::size_t>(n_fields)), s);
+            return;
+        }
         batch_apply(*M, src, ftype(src_dtype), dst, ftype(dst_dtype), n_src, n_dst, n_fields, amode(mode), s);
         AXIS_HIP_CHECK(hipStreamSynchronize(s));
     });
@@ -549,8 +555,9 @@ int axis_nan_batch_apply_c(int matrix_handle, const double *src, double *dst, in
     return guarded([&] {
         auto M = Registry::get().find<InterpolationMatrix>(matrix_handle);
         auto s = static_cast<hipStream_t>(stream);
-        nan_batch_apply(*M, src, dst, n_src, n_dst, n_fields, na_thres, s);
-        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+        if (n_src < 0 || n_dst < 0 || n_fields < 0) throw InvalidArgument("axis::solver::nan_batch_apply: negative extent");
+        nan_batch_apply(*M, field_view<const double, 2>(src, static_cast<std::size_t>(n_src), static_cast<std::size_t>(n_fields)),
+                        field_view<double, 2>(dst, static_cast<std::size_t>(n_dst), static_cast<std::size_t>(n_fields)), na_thres, s);
     });
 }
 
