@@ -74,7 +74,10 @@ __global__ void __launch_bounds__(kTileSeg, 5) persistent_apply_kernel(const Pai
             __syncthreads();
             if (it < 0) break;  // queue q drained (whole workgroup)
             const Item I = items[it];
-            const PairDesc &D = pairs[I.pair];
+            // the pair's operands copied into registers once per item (read
+            // through the pointer, every use inside the field loop would be
+            // reloaded: the kernel's stores could alias them)
+            const PairDesc D = pairs[I.pair];
             apply_detail::tile_apply<E, FU, NUM, XT, YT>(D.A, static_cast<const XT *>(D.X), static_cast<YT *>(D.Y), D.P, I.tile, I.f0, I.f1,
                                                          S);
         }
