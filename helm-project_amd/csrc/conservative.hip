@@ -432,6 +432,113 @@ __global__ void frac_a_kernel(const std::uint32_t *key, const std::uint32_t *per
     frac_a[c] = fmin(s, 1.0);
 }
 
+// frac_a without a global sort.  Each source column's CSR positions are
+// gathered by counting (column counts, their scan, an atomic scatter that
+// leaves each column's positions in arbitrary order), then each column sorts
+// its few positions and sums in ascending position, which is the row order
+// of frac_a_kernel's stable sort: the same additions in the same order.
+constexpr int kColNet = 16;       // columns of up to kColNet entries: one thread, a sorting network
+constexpr int kColLds = 4096;     // longer columns: one workgroup, a bitonic sort in LDS
+constexpr int kColBlock = 256;
+
+__global__ void col_count_kernel(const index_t *col, index_t nnz, unsigned long long *cnt) {
+    const index_t e = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e < nnz) atomicAdd(&cnt[col[e]], 1ull);
+}
+
+// cnt[c] counts down to zero (and is zero again afterwards)
+__global__ void col_scatter_kernel(const index_t *col, index_t nnz, const index_t *cp, unsigned long long *cnt, std::uint32_t *cs) {
+    const index_t e = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    const index_t c = col[e];
+    const unsigned long long left = atomicAdd(&cnt[c], ~0ull);  // (adds -1)
+    cs[cp[c] + static_cast<index_t>(left) - 1] = static_cast<std::uint32_t>(e);
+}
+
+__global__ void __launch_bounds__(kColBlock) frac_a_short_kernel(const index_t *cp, const std::uint32_t *cs, index_t n_src, const double *ov,
+                                                                 const double *area_src, double *frac_a, std::int32_t *long_cols, int *n_long) {
+    const index_t c = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= n_src) return;
+    const index_t b = cp[c];
+    const index_t n = cp[c + 1] - b;
+    if (n > kColNet) {
+        long_cols[atomicAdd(n_long, 1)] = static_cast<std::int32_t>(c);
+        return;
+    }
+    if (n == 0) {
+        frac_a[c] = 0.0;
+        return;
+    }
+    std::uint32_t v[kColNet];
+#pragma unroll
+    for (int k = 0; k < kColNet; ++k) v[k] = k < n ? cs[b + k] : 0xFFFFFFFFu;
+    // bitonic sorting network, ascending (constant indices: the values stay in registers)
+#pragma unroll
+    for (int k = 2; k <= kColNet; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < kColNet; ++i) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const std::uint32_t x = v[i], y = v[l];
+                    const bool up = (i & k) == 0;
+                    v[i] = up ? min(x, y) : max(x, y);
+                    v[l] = up ? max(x, y) : min(x, y);
+                }
+            }
+    const double a = area_src[c];
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < kColNet; ++k)
+        if (k < n) s += ov[v[k]] / a;
+    frac_a[c] = fmin(s, 1.0);
+}
+
+// one workgroup per listed column; a column beyond kColLds entries sets
+// *overflow (its frac_a is then taken from the global sort)
+__global__ void __launch_bounds__(kColBlock) frac_a_long_kernel(const index_t *cp, const std::uint32_t *cs, const std::int32_t *long_cols,
+                                                                const int *n_long, const double *ov, const double *area_src, double *frac_a,
+                                                                int *overflow) {
+    __shared__ std::uint32_t sv[kColLds];
+    const int nl = *n_long;
+    for (int li = blockIdx.x; li < nl; li += gridDim.x) {
+        const index_t c = long_cols[li];
+        const index_t b = cp[c];
+        const int n = static_cast<int>(cp[c + 1] - b);
+        if (n > kColLds) {
+            if (threadIdx.x == 0) *overflow = 1;
+            continue;
+        }
+        int P = 1;
+        while (P < n) P <<= 1;
+        for (int k = threadIdx.x; k < P; k += kColBlock) sv[k] = k < n ? cs[b + k] : 0xFFFFFFFFu;
+        __syncthreads();
+        for (int k = 2; k <= P; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = threadIdx.x; i < P; i += kColBlock) {
+                    const int l = i ^ j;
+                    if (l > i) {
+                        const std::uint32_t x = sv[i], y = sv[l];
+                        const bool up = (i & k) == 0;
+                        if (up ? x > y : x < y) {
+                            sv[i] = y;
+                            sv[l] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        if (threadIdx.x == 0) {
+            const double a = area_src[c];
+            double s = 0.0;
+            for (int k = 0; k < n; ++k) s += ov[sv[k]] / a;
+            frac_a[c] = fmin(s, 1.0);
+        }
+        __syncthreads();  // (sv is reused by the next column)
+    }
+}
+
 // first unmapped destination under UnmappedAction::Error
 __global__ void unmapped_kernel(const index_t *row_ptr, index_t n_dst, const std::int32_t *dflags, unsigned long long *first) {
     index_t r = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1327,16 +1434,50 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
             AXIS_LAUNCH_CHECK();
         }
     }
-    AXIS_HIP_CHECK(hipMemsetAsync(M->frac_a.get(), 0, sizeof(double) * std::max<index_t>(n_src, 1), s));
-    if (nnz > 0) {
-        // stable sort of the CSR entries by column alone (LSD radix)
-        DeviceBuffer<std::uint32_t> ckey(nz1);
-        column_keys_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(M->col.get(), nnz, ckey.get(), perm.get());
+    // frac_a: each column's CSR positions gathered by counting and summed in
+    // position order; a column of more than kColLds entries (a source cell
+    // under that many destination cells) sets the flag, and the stable
+    // global column sort below then gives every column's sum
+    DeviceBuffer<int> col_flags(2);  // long-column count, overflow
+    AXIS_HIP_CHECK(hipMemsetAsync(col_flags.get(), 0, 2 * sizeof(int), s));
+    const char *fa = std::getenv("AXIS_FRAC_A");  // "sort": the global column sort for every column
+    const bool by_count = n_src > 0 && nnz < (index_t(1) << 32) && !(fa && std::string(fa) == "sort");
+    if (by_count) {
+        DeviceBuffer<unsigned long long> cnt(n_src);
+        DeviceBuffer<index_t> cp(n_src + 1);
+        DeviceBuffer<std::uint32_t> cs(nz1);
+        DeviceBuffer<std::int32_t> long_cols(n_src);
+        AXIS_HIP_CHECK(hipMemsetAsync(cnt.get(), 0, sizeof(unsigned long long) * n_src, s));
+        if (nnz > 0) {
+            col_count_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(M->col.get(), nnz, cnt.get());
+            AXIS_LAUNCH_CHECK();
+        }
+        exclusive_scan_i64(reinterpret_cast<const index_t *>(cnt.get()), cp.get(), n_src, s);
+        if (nnz > 0) {
+            col_scatter_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(M->col.get(), nnz, cp.get(), cnt.get(), cs.get());
+            AXIS_LAUNCH_CHECK();
+        }
+        frac_a_short_kernel<<<grid_for(n_src, kColBlock), kColBlock, 0, s>>>(cp.get(), cs.get(), n_src, ov.get(), sg.d_area.get(), M->frac_a.get(),
+                                                                            long_cols.get(), col_flags.get());
         AXIS_LAUNCH_CHECK();
-        radix_sort_pairs_u32(ckey.get(), perm.get(), nnz, col_bits, s);
-        frac_a_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(ckey.get(), perm.get(), nnz, ov.get(), sg.d_area.get(), M->frac_a.get());
+        frac_a_long_kernel<<<1024, kColBlock, 0, s>>>(cp.get(), cs.get(), long_cols.get(), col_flags.get(), ov.get(), sg.d_area.get(),
+                                                      M->frac_a.get(), col_flags.get() + 1);
         AXIS_LAUNCH_CHECK();
     }
+    // stable sort of the CSR entries by column alone (LSD radix), then one
+    // thread per column run
+    auto frac_a_sorted = [&](const double *area_src) {
+        AXIS_HIP_CHECK(hipMemsetAsync(M->frac_a.get(), 0, sizeof(double) * std::max<index_t>(n_src, 1), s));
+        if (nnz > 0) {
+            DeviceBuffer<std::uint32_t> ckey(nz1);
+            column_keys_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(M->col.get(), nnz, ckey.get(), perm.get());
+            AXIS_LAUNCH_CHECK();
+            radix_sort_pairs_u32(ckey.get(), perm.get(), nnz, col_bits, s);
+            frac_a_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(ckey.get(), perm.get(), nnz, ov.get(), area_src, M->frac_a.get());
+            AXIS_LAUNCH_CHECK();
+        }
+    };
+    if (!by_count) frac_a_sorted(sg.d_area.get());
     ev.rec(6, s);
     // the cell areas pass to the matrix (the geometry is done with them)
     M->area_a = std::move(sg.d_area);
@@ -1344,8 +1485,9 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     if (M->area_a.size() < static_cast<std::size_t>(std::max<index_t>(n_src, 1))) M->area_a.resize(std::max<index_t>(n_src, 1));
     if (M->area_b.size() < static_cast<std::size_t>(std::max<index_t>(n_dst, 1))) M->area_b.resize(std::max<index_t>(n_dst, 1));
     PinnedArray<unsigned long long> h_tail;
-    h_tail.resize(kH + 1);
+    h_tail.resize(kH + 2);
     AXIS_HIP_CHECK(hipMemcpyAsync(h_tail.data(), max_row.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    AXIS_HIP_CHECK(hipMemcpyAsync(h_tail.data() + kH + 1, col_flags.get(), 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     if (grouped) AXIS_HIP_CHECK(hipMemcpyAsync(h_tail.data() + 1, d_hist.get(), sizeof(unsigned long long) * kH, hipMemcpyDeviceToHost, s));
     tr.mark("csr");
     if (cfg.unmapped == UnmappedAction::Error && n_dst > 0) {
@@ -1360,6 +1502,10 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
             throw RuntimeError("WeightGenerator::generate_conservative: unmapped destination cell " + std::to_string(h_first));
     }
     AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    if (by_count && (h_tail[kH + 1] >> 32) != 0) {  // a column of more than kColLds entries
+        frac_a_sorted(M->area_a.get());
+        AXIS_HIP_CHECK(hipStreamSynchronize(s));
+    }
     M->phase_ms[0] = ev.ms(0, 1);
     M->phase_ms[1] = ev.ms(2, 3);
     M->phase_ms[2] = ev.ms(3, 4);
