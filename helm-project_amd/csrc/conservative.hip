@@ -774,7 +774,38 @@ struct HostSide {
         if (g->regular) return 4;
         return static_cast<int>(std::min<index_t>(m->h_offsets[c + 1] - m->h_offsets[c], 32));
     }
-    // precompute_cap_data for one cell with host libm (spherical_cap_filter.hpp:85-167)
+    // the cell's nodes, each evaluated once (node() may call host libm);
+    // false for a cell of more than kMaxNodes nodes
+    static constexpr int kMaxNodes = 32;
+    bool nodes(index_t c, V3 *out, int &n) const {
+        const index_t b = m->h_offsets[c], e = m->h_offsets[c + 1];
+        if (e - b > kMaxNodes) return false;
+        n = static_cast<int>(e - b);
+        for (int k = 0; k < n; ++k) out[k] = node(m->h_indices[b + k]);
+        return true;
+    }
+    // precompute_cap_data for one cell with host libm (spherical_cap_filter.hpp:85-167),
+    // over the cell's nodes p[0 .. n)
+    static void cap_of(const V3 *p, int n, V3 &cen, double &rad) {
+        cen = V3{0, 0, 0};
+        rad = 0.0;
+        if (n < 1) return;
+        double sx = 0, sy = 0, sz = 0;
+        for (int k = 0; k < n; ++k) {
+            sx += p[k].x;
+            sy += p[k].y;
+            sz += p[k].z;
+        }
+        double inv = 1.0 / static_cast<double>(n);
+        cen = geo::normalize(V3{sx * inv, sy * inv, sz * inv});
+        for (int k = 0; k < n; ++k) {
+            double d = geo::dot(cen, p[k]);
+            d = std::fmin(1.0, std::fmax(-1.0, d));
+            double ang = std::acos(d);
+            if (ang > rad) rad = ang;
+        }
+    }
+    // the same for any cell size, node by node
     void cap(index_t c, V3 &cen, double &rad) const {
         const index_t b = m->h_offsets[c], e = m->h_offsets[c + 1];
         cen = V3{0, 0, 0};
@@ -801,20 +832,32 @@ struct HostSide {
 // The reference's inner-loop body for one (dst j, src i) pair with host
 // libm: cap test, clip with SphericalPolygon<32> capacity, fan area.
 double host_pair_overlap(const HostSide &S, const HostSide &D, index_t j, index_t i) {
+    // each node's xyz once per pair (the same values node() gives each call)
+    V3 sn[HostSide::kMaxNodes], dn[HostSide::kMaxNodes];
+    int n_sn = 0, n_dn = 0;
+    const bool s_nodes = S.nodes(i, sn, n_sn), d_nodes = D.nodes(j, dn, n_dn);
     V3 cs, cd;
     double rs, rd;
-    S.cap(i, cs, rs);
-    D.cap(j, cd, rd);
+    if (s_nodes)
+        HostSide::cap_of(sn, n_sn, cs, rs);
+    else
+        S.cap(i, cs, rs);
+    if (d_nodes)
+        HostSide::cap_of(dn, n_dn, cd, rd);
+    else
+        D.cap(j, cd, rd);
     double c = geo::dot(cs, cd);
     c = std::fmin(1.0, std::fmax(-1.0, c));
     if (std::acos(c) > (rs + rd)) return 0.0;
     double ax[32], ay[32], az[32], bx[32], by[32], bz[32];
     PolyRef A{ax, ay, az, 1}, B{bx, by, bz, 1};
+    // a non-regular cell's vertex k is its node k
+    const bool s_cached = s_nodes && !S.g->regular, d_cached = d_nodes && !D.g->regular;
     const int ns = S.nverts(i);
-    for (int k = 0; k < ns; ++k) A.set(k, S.vertex(i, k));
+    for (int k = 0; k < ns; ++k) A.set(k, s_cached ? sn[k] : S.vertex(i, k));
     const int nc = D.nverts(j);
     bool overflow = false;
-    auto clipv = [&](int k) { return D.vertex(j, k); };
+    auto clipv = [&](int k) { return d_cached ? dn[k] : D.vertex(j, k); };
     int n = geo::clip_polygon(A, ns, B, clipv, nc, 32, &overflow);
     double abs_sum;
     bool zero_num;
