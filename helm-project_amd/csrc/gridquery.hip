@@ -37,7 +37,8 @@
 //   4. grid_count_kernel: one lane per destination cell counts its hits
 //      (the row tables and, when it fits, the longitude axis in LDS); cells
 //      with more than kLaneItems grid cells to test are listed for
-//      grid_long_kernel (one wave per cell, row by row); then an exclusive
+//      grid_long_kernel (one wave per cell: lanes over its rows, then over
+//      its rows' grid cells taken together); then an exclusive
 //      scan of the counts, and the same two kernels fill the candidate list
 //      and each candidate's destination row (writes ordered by ballot).
 #include <hip/hip_runtime.h>
@@ -62,7 +63,7 @@ constexpr float kDstLonMarginDeg = 1e-3f;  // destination spans (single precisio
 constexpr float kAllLon = 1e30f;      // row overhang meaning "every cell of the row"
 constexpr double kRad2Deg = 180.0 / kPi;
 constexpr int kQueryBlock = 256;
-constexpr int kLongBlocks = 1024;     // workgroups of grid_long_kernel (they loop over the list)
+constexpr int kLongBlocks = 512;      // workgroups of grid_long_kernel (they loop over the list)
 constexpr index_t kCountBlocks = 2048;  // workgroups of grid_count_kernel (they loop over the cells)
 constexpr std::size_t kMaxLds = 64 * 1024;
 // per cell row: z-band suffix minimum and prefix maximum, longitude overhang,
@@ -467,44 +468,226 @@ __global__ void __launch_bounds__(kQueryBlock) grid_count_kernel(GridView G, con
         count_one<kFill>(G, T, dst_box, d, count, cand_off, cand_src, pair_row, long_rows, n_long);
 }
 
-// one wave per listed destination cell, row by row; workgroups loop over the list
+// One wave per listed destination cell (more than kLaneItems grid cells to
+// test; near the source's poles: at 0.1 deg -> C768, 121,632 cells of ~10
+// rows and ~115 grid cells each).  The lanes first take one row each (its
+// longitude ranges and grid-cell count, kept in LDS, and a prefix scan of the
+// counts), then one grid cell each of the cell's rows taken together, found
+// by a binary search over the row offsets; the hits come out in row, range
+// and column order, written in that order (ballot).  Workgroups loop over
+// the list.
+constexpr int kLongBlock = 1024;  // (16 waves: per-wave LDS beside the tables, two workgroups per CU)
+constexpr int kLongWaves = kLongBlock / 64;
+constexpr int kHeavyItems = 512;  // more grid cells than this: a workgroup per cell (grid_heavy_kernel)
+
+// a compiler barrier: LDS written by some lanes of a wave and read by others
+// after it, the wave's LDS instructions being executed in program order
+__device__ inline void wave_lds_fence() { asm volatile("" ::: "memory"); }
+
 template <bool kFill>
-__global__ void __launch_bounds__(kQueryBlock) grid_long_kernel(GridView G, const float *dst_box, const std::int32_t *long_rows, const int *n_long,
+__global__ void __launch_bounds__(kLongBlock) grid_long_kernel(GridView G, const float *dst_box, const std::int32_t *long_rows, const int *n_long,
                                                                 index_t *count, const index_t *cand_off, std::int32_t *cand_src,
-                                                                std::int32_t *pair_row) {
+                                                                std::int32_t *pair_row, std::int32_t *heavy_rows, int *n_heavy) {
     extern __shared__ double s_tab[];
+    __shared__ int s_off[kLongWaves][65];                    // grid cells before row lane L
+    __shared__ int s_lo[kLongWaves][kMaxLonRanges][64], s_hi[kLongWaves][kMaxLonRanges][64];
+    __shared__ int s_nq[kLongWaves][64];
     const Tables T = load_tables(G, s_tab);
-    const int lane = threadIdx.x & 63;
-    const int waves = kQueryBlock / 64;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int nl = *n_long;
-    for (int li = blockIdx.x * waves + (threadIdx.x >> 6); li < nl; li += gridDim.x * waves) {
+    for (int li = blockIdx.x * kLongWaves + wv; li < nl; li += gridDim.x * kLongWaves) {
         const index_t d = long_rows[li];
         Dst D;
         dst_setup(G, T, dst_box, d, D);
         index_t pos = kFill ? cand_off[d] : 0;
-        int r0[kMaxLonRanges], r1[kMaxLonRanges];
-        for (int j = D.j0; j <= D.j1; ++j) {
-            const int n = row_ranges(G, T, j, D, r0, r1);
-            const std::int64_t base = static_cast<std::int64_t>(j) * (G.nlon - 1);
-            for (int q = 0; q < n; ++q)
-                for (int a0 = r0[q]; a0 <= r1[q]; a0 += 64) {
-                    const int a = a0 + lane;
-                    bool hit = false;
-                    std::int32_t c = 0;
-                    if (a <= r1[q]) {
-                        c = cell_of(G, base + a);
-                        hit = boxes_meet(G.src_box + 6 * static_cast<std::int64_t>(c), D.bx);
-                    }
-                    const std::uint64_t m = __ballot(hit);
-                    if (kFill && hit) {
-                        const index_t w = pos + __popcll(m & ((std::uint64_t(1) << lane) - 1));
-                        cand_src[w] = c;
-                        pair_row[w] = static_cast<std::int32_t>(d);
-                    }
-                    pos += __popcll(m);
+        for (int jb = D.j0; jb <= D.j1; jb += 64) {
+            // one row per lane
+            int r0[kMaxLonRanges], r1[kMaxLonRanges];
+            int nq = 0, cnt = 0;
+            if (jb + lane <= D.j1) {
+                nq = row_ranges(G, T, jb + lane, D, r0, r1);
+                for (int q = 0; q < nq; ++q) cnt += r1[q] - r0[q] + 1;
+            }
+            int inc = cnt;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(inc, o, 64);
+                if (lane >= o) inc += t;
+            }
+            const int total = __shfl(inc, 63, 64);
+            if (jb == D.j0 && (D.j1 - D.j0 >= 64 || total > kHeavyItems)) {
+                // (listed again for grid_heavy_kernel by the counting pass; uniform over the wave)
+                if (!kFill && lane == 0) heavy_rows[atomicAdd(n_heavy, 1)] = static_cast<std::int32_t>(d);
+                pos = -1;
+                break;
+            }
+            wave_lds_fence();
+            s_nq[wv][lane] = nq;
+            for (int q = 0; q < kMaxLonRanges; ++q)
+                if (q < nq) {
+                    s_lo[wv][q][lane] = r0[q];
+                    s_hi[wv][q][lane] = r1[q];
                 }
+            s_off[wv][lane + 1] = inc;
+            if (lane == 0) s_off[wv][0] = 0;
+            wave_lds_fence();
+            // one grid cell per lane over the rows' cells taken together
+            for (int b = 0; b < total; b += 64) {
+                const int k = b + lane;
+                bool hit = false;
+                std::int32_t c = 0;
+                if (k < total) {
+                    int lo = 0, hi = 64;  // the row lane L with s_off[L] <= k < s_off[L + 1]
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (s_off[wv][mid] <= k)
+                            lo = mid;
+                        else
+                            hi = mid;
+                    }
+                    int u = k - s_off[wv][lo];
+                    int a = 0;
+                    const int n = s_nq[wv][lo];
+                    for (int q = 0; q < n; ++q) {
+                        const int qlo = s_lo[wv][q][lo], len = s_hi[wv][q][lo] - qlo + 1;
+                        if (u < len) {
+                            a = qlo + u;
+                            break;
+                        }
+                        u -= len;
+                    }
+                    c = cell_of(G, static_cast<std::int64_t>(jb + lo) * (G.nlon - 1) + a);
+                    hit = boxes_meet(G.src_box + 6 * static_cast<std::int64_t>(c), D.bx);
+                }
+                const std::uint64_t m = __ballot(hit);
+                if (kFill && hit) {
+                    const index_t w = pos + __popcll(m & ((std::uint64_t(1) << lane) - 1));
+                    cand_src[w] = c;
+                    pair_row[w] = static_cast<std::int32_t>(d);
+                }
+                pos += __popcll(m);
+            }
+            wave_lds_fence();
         }
-        if (!kFill && lane == 0) count[d] = pos;
+        if (!kFill && lane == 0 && pos >= 0) count[d] = pos;
+    }
+}
+
+// block-wide inclusive prefix sum of one int per thread (every thread calls it)
+template <int BLOCK>
+__device__ inline int block_inclusive_scan(int v, int *s_part) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    __syncthreads();  // (s_part may still be read by a previous scan)
+    if (lane == 63) s_part[wv] = v;
+    __syncthreads();
+    for (int w = 0; w < wv; ++w) v += s_part[w];
+    return v;
+}
+
+// One workgroup of kHeavyBlock threads per listed cell of more than 64 rows
+// or more than kHeavyItems grid cells (the cells over and around the source
+// grid's poles: at most 10,800 grid cells at 0.1 deg -> C768).  Per group of
+// kHeavyBlock rows: one row per thread (ranges and counts in LDS, their
+// prefix scan); then passes of up to kHeavyChunks 64-cell chunks: the 16
+// waves test the chunks in turn, keeping each chunk's hit mask in LDS; a
+// prefix scan of the chunks' hit counts places each chunk, and the filling
+// pass writes the hits in row, range and column order, as grid_long_kernel.
+constexpr int kHeavyBlock = 1024;
+constexpr int kHeavyWaves = kHeavyBlock / 64;
+constexpr int kHeavyChunks = kHeavyBlock;  // chunks per pass: one per thread for the scan
+constexpr int kHeavyGrid = 256;
+
+template <bool kFill>
+__global__ void __launch_bounds__(kHeavyBlock) grid_heavy_kernel(GridView G, const float *dst_box, const std::int32_t *heavy_rows, const int *n_heavy,
+                                                                 index_t *count, const index_t *cand_off, std::int32_t *cand_src,
+                                                                 std::int32_t *pair_row) {
+    extern __shared__ double s_tab[];
+    __shared__ int s_off[kHeavyBlock + 1];  // grid cells before row thread t
+    __shared__ int s_lo[kMaxLonRanges][kHeavyBlock], s_hi[kMaxLonRanges][kHeavyBlock];
+    __shared__ int s_nq[kHeavyBlock];
+    __shared__ unsigned long long s_mask[kHeavyChunks];
+    __shared__ int s_cpre[kHeavyChunks + 1];  // hits before chunk c of the pass
+    __shared__ int s_part[kHeavyWaves];
+    const Tables T = load_tables(G, s_tab);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nh = *n_heavy;
+    // (global cell k of the row group -> its source cell)
+    auto cell_at_k = [&](int jb, int k) {
+        int lo = 0, hi = kHeavyBlock;  // the row thread L with s_off[L] <= k < s_off[L + 1]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_off[mid] <= k)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        int u = k - s_off[lo], a = 0;
+        const int n = s_nq[lo];
+        for (int q = 0; q < n; ++q) {
+            const int qlo = s_lo[q][lo], len = s_hi[q][lo] - qlo + 1;
+            if (u < len) {
+                a = qlo + u;
+                break;
+            }
+            u -= len;
+        }
+        return cell_of(G, static_cast<std::int64_t>(jb + lo) * (G.nlon - 1) + a);
+    };
+    for (int hc = blockIdx.x; hc < nh; hc += gridDim.x) {  // (uniform over the workgroup)
+        const index_t d = heavy_rows[hc];
+        Dst D;
+        dst_setup(G, T, dst_box, d, D);
+        index_t pos = kFill ? cand_off[d] : 0;
+        for (int jb = D.j0; jb <= D.j1; jb += kHeavyBlock) {
+            int r0[kMaxLonRanges], r1[kMaxLonRanges];
+            int nq = 0, cnt = 0;
+            if (jb + tid <= D.j1) {
+                nq = row_ranges(G, T, jb + tid, D, r0, r1);
+                for (int q = 0; q < nq; ++q) cnt += r1[q] - r0[q] + 1;
+            }
+            s_nq[tid] = nq;
+            for (int q = 0; q < kMaxLonRanges; ++q)
+                if (q < nq) {
+                    s_lo[q][tid] = r0[q];
+                    s_hi[q][tid] = r1[q];
+                }
+            const int inc = block_inclusive_scan<kHeavyBlock>(cnt, s_part);
+            s_off[tid + 1] = inc;
+            if (tid == 0) s_off[0] = 0;
+            __syncthreads();
+            const int total = s_off[kHeavyBlock];
+            for (int cb = 0; cb < total; cb += 64 * kHeavyChunks) {
+                const int nch = min(kHeavyChunks, (total - cb + 63) / 64);
+                for (int c = wv; c < nch; c += kHeavyWaves) {
+                    const int k = cb + 64 * c + lane;
+                    bool hit = false;
+                    if (k < total) hit = boxes_meet(G.src_box + 6 * static_cast<std::int64_t>(cell_at_k(jb, k)), D.bx);
+                    const std::uint64_t m = __ballot(hit);
+                    if (lane == 0) s_mask[c] = m;
+                }
+                __syncthreads();
+                const int hits = tid < nch ? __popcll(s_mask[tid]) : 0;
+                const int hinc = block_inclusive_scan<kHeavyBlock>(hits, s_part);
+                s_cpre[tid + 1] = hinc;
+                if (tid == 0) s_cpre[0] = 0;
+                __syncthreads();
+                if (kFill)
+                    for (int c = wv; c < nch; c += kHeavyWaves) {
+                        const std::uint64_t m = s_mask[c];
+                        if ((m >> lane) & 1u) {
+                            const index_t w = pos + s_cpre[c] + __popcll(m & ((std::uint64_t(1) << lane) - 1));
+                            cand_src[w] = cell_at_k(jb, cb + 64 * c + lane);
+                            pair_row[w] = static_cast<std::int32_t>(d);
+                        }
+                    }
+                pos += s_cpre[nch];
+                __syncthreads();  // (s_mask and s_cpre are rewritten by the next pass)
+            }
+        }
+        if (!kFill && tid == 0) count[d] = pos;
     }
 }
 
@@ -589,14 +772,19 @@ index_t grid_query(const GridIndex &G, const float *dst_boxes, index_t n_dst, De
     cand_off.resize(n_dst + 1);
     DeviceBuffer<index_t> count(nd1);
     DeviceBuffer<std::int32_t> long_rows(nd1);
-    DeviceBuffer<int> n_long(1);
-    AXIS_HIP_CHECK(hipMemsetAsync(n_long.get(), 0, sizeof(int), s));
+    DeviceBuffer<std::int32_t> heavy_rows(nd1);
+    DeviceBuffer<int> n_lists(2);  // listed cells, of which heavy
+    int *n_long = n_lists.get(), *n_heavy = n_lists.get() + 1;
+    AXIS_HIP_CHECK(hipMemsetAsync(n_lists.get(), 0, 2 * sizeof(int), s));
     if (n_dst > 0) {
-        grid_count_kernel<false><<<count_blocks(n_dst), kQueryBlock, lds, s>>>(
-            V, dst_boxes, n_dst, count.get(), nullptr, nullptr, nullptr, long_rows.get(), n_long.get());
+        grid_count_kernel<false><<<count_blocks(n_dst), kQueryBlock, lds, s>>>(V, dst_boxes, n_dst, count.get(), nullptr, nullptr, nullptr,
+                                                                              long_rows.get(), n_long);
         AXIS_LAUNCH_CHECK();
-        grid_long_kernel<false>
-            <<<kLongBlocks, kQueryBlock, lds, s>>>(V, dst_boxes, long_rows.get(), n_long.get(), count.get(), nullptr, nullptr, nullptr);
+        grid_long_kernel<false><<<kLongBlocks, kLongBlock, lds, s>>>(V, dst_boxes, long_rows.get(), n_long, count.get(), nullptr, nullptr, nullptr,
+                                                                      heavy_rows.get(), n_heavy);
+        AXIS_LAUNCH_CHECK();
+        grid_heavy_kernel<false><<<kHeavyGrid, kHeavyBlock, lds, s>>>(V, dst_boxes, heavy_rows.get(), n_heavy, count.get(), nullptr, nullptr,
+                                                                      nullptr);
         AXIS_LAUNCH_CHECK();
     }
     exclusive_scan_i64(count.get(), cand_off.get(), n_dst, s);
@@ -614,7 +802,10 @@ index_t grid_query(const GridIndex &G, const float *dst_boxes, index_t n_dst, De
         grid_count_kernel<true><<<count_blocks(n_dst), kQueryBlock, lds, s>>>(
             V, dst_boxes, n_dst, nullptr, cand_off.get(), cand_src.get(), pair_row.get(), nullptr, nullptr);
         AXIS_LAUNCH_CHECK();
-        grid_long_kernel<true><<<kLongBlocks, kQueryBlock, lds, s>>>(V, dst_boxes, long_rows.get(), n_long.get(), nullptr, cand_off.get(),
+        grid_long_kernel<true><<<kLongBlocks, kLongBlock, lds, s>>>(V, dst_boxes, long_rows.get(), n_long, nullptr, cand_off.get(), cand_src.get(),
+                                                                     pair_row.get(), nullptr, nullptr);
+        AXIS_LAUNCH_CHECK();
+        grid_heavy_kernel<true><<<kHeavyGrid, kHeavyBlock, lds, s>>>(V, dst_boxes, heavy_rows.get(), n_heavy, nullptr, cand_off.get(),
                                                                      cand_src.get(), pair_row.get());
         AXIS_LAUNCH_CHECK();
     }
