@@ -38,9 +38,11 @@
 //      (the row tables and, when it fits, the longitude axis in LDS); cells
 //      with more than kLaneItems grid cells to test are listed for
 //      grid_long_kernel (one wave per cell: lanes over its rows, then over
-//      its rows' grid cells taken together); then an exclusive
-//      scan of the counts, and the same two kernels fill the candidate list
-//      and each candidate's destination row (writes ordered by ballot).
+//      its rows' grid cells taken together), which passes cells of more than
+//      64 rows or kHeavyItems grid cells on to grid_heavy_kernel (one
+//      1024-thread workgroup per cell); then an exclusive scan of the counts,
+//      and the same kernels fill the candidate list and each candidate's
+//      destination row, every cell's hits in row, range and column order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -66,6 +68,7 @@ constexpr int kQueryBlock = 256;
 constexpr int kLongBlocks = 512;      // workgroups of grid_long_kernel (they loop over the list)
 constexpr index_t kCountBlocks = 2048;  // workgroups of grid_count_kernel (they loop over the cells)
 constexpr std::size_t kMaxLds = 64 * 1024;
+constexpr std::size_t kBigTables = 32 * 1024;  // LDS tables beyond this: the counting pass takes 1024-thread workgroups
 // per cell row: z-band suffix minimum and prefix maximum, longitude overhang,
 // radial band (distance of the boxes' xy-rectangles from the polar axis)
 constexpr int kRowFields = 5;
@@ -458,13 +461,13 @@ __device__ inline void count_one(const GridView &G, const Tables &T, const float
 
 // one lane per destination cell; workgroups loop over the cells (the tables
 // are loaded once per workgroup)
-template <bool kFill>
-__global__ void __launch_bounds__(kQueryBlock) grid_count_kernel(GridView G, const float *dst_box, index_t n_dst, index_t *count,
+template <bool kFill, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) grid_count_kernel(GridView G, const float *dst_box, index_t n_dst, index_t *count,
                                                                  const index_t *cand_off, std::int32_t *cand_src, std::int32_t *pair_row,
                                                                  std::int32_t *long_rows, int *n_long) {
     extern __shared__ double s_tab[];
     const Tables T = load_tables(G, s_tab);
-    for (index_t d = static_cast<index_t>(blockIdx.x) * kQueryBlock + threadIdx.x; d < n_dst; d += static_cast<index_t>(gridDim.x) * kQueryBlock)
+    for (index_t d = static_cast<index_t>(blockIdx.x) * BLOCK + threadIdx.x; d < n_dst; d += static_cast<index_t>(gridDim.x) * BLOCK)
         count_one<kFill>(G, T, dst_box, d, count, cand_off, cand_src, pair_row, long_rows, n_long);
 }
 
@@ -696,8 +699,8 @@ __global__ void fill_u32_kernel(unsigned *p, std::int64_t n, unsigned v) {
     if (i < n) p[i] = v;
 }
 
-unsigned count_blocks(index_t n_dst) {
-    return static_cast<unsigned>(std::clamp<index_t>((n_dst + kQueryBlock - 1) / kQueryBlock, 1, kCountBlocks));
+unsigned count_blocks(index_t n_dst, int block) {
+    return static_cast<unsigned>(std::clamp<index_t>((n_dst + block - 1) / block, 1, kCountBlocks * kQueryBlock / block));
 }
 std::size_t rows_lds(int nrow) { return sizeof(double) * ((kRowFields * static_cast<std::size_t>(nrow) + 1) / 2); }
 bool lon_fits(int nlon, int nrow) { return rows_lds(nrow) + sizeof(double) * nlon <= kMaxLds; }
@@ -777,8 +780,14 @@ index_t grid_query(const GridIndex &G, const float *dst_boxes, index_t n_dst, De
     int *n_long = n_lists.get(), *n_heavy = n_lists.get() + 1;
     AXIS_HIP_CHECK(hipMemsetAsync(n_lists.get(), 0, 2 * sizeof(int), s));
     if (n_dst > 0) {
-        grid_count_kernel<false><<<count_blocks(n_dst), kQueryBlock, lds, s>>>(V, dst_boxes, n_dst, count.get(), nullptr, nullptr, nullptr,
-                                                                              long_rows.get(), n_long);
+        // counting: 1024-thread workgroups when the tables leave room for
+        // fewer than five 256-thread workgroups per CU (more waves resident)
+        if (lds > kBigTables)
+            grid_count_kernel<false, kLongBlock><<<count_blocks(n_dst, kLongBlock), kLongBlock, lds, s>>>(
+                V, dst_boxes, n_dst, count.get(), nullptr, nullptr, nullptr, long_rows.get(), n_long);
+        else
+            grid_count_kernel<false, kQueryBlock><<<count_blocks(n_dst, kQueryBlock), kQueryBlock, lds, s>>>(
+                V, dst_boxes, n_dst, count.get(), nullptr, nullptr, nullptr, long_rows.get(), n_long);
         AXIS_LAUNCH_CHECK();
         grid_long_kernel<false><<<kLongBlocks, kLongBlock, lds, s>>>(V, dst_boxes, long_rows.get(), n_long, count.get(), nullptr, nullptr, nullptr,
                                                                       heavy_rows.get(), n_heavy);
@@ -799,7 +808,7 @@ index_t grid_query(const GridIndex &G, const float *dst_boxes, index_t n_dst, De
     cand_src.resize(std::max<index_t>(total, 1));
     pair_row.resize(std::max<index_t>(total, 1));
     if (n_dst > 0 && total > 0) {
-        grid_count_kernel<true><<<count_blocks(n_dst), kQueryBlock, lds, s>>>(
+        grid_count_kernel<true, kQueryBlock><<<count_blocks(n_dst, kQueryBlock), kQueryBlock, lds, s>>>(
             V, dst_boxes, n_dst, nullptr, cand_off.get(), cand_src.get(), pair_row.get(), nullptr, nullptr);
         AXIS_LAUNCH_CHECK();
         grid_long_kernel<true><<<kLongBlocks, kLongBlock, lds, s>>>(V, dst_boxes, long_rows.get(), n_long, nullptr, cand_off.get(), cand_src.get(),
