@@ -47,6 +47,17 @@ __host__ __device__ inline double fan_term(const V3 &a, const V3 &b, const V3 &c
 
 /// Intersection of arc v1->v2 with the great circle through (ea, eb),
 /// spherical_clipper.hpp:263-286.
+/// `n1` is the clip edge's normal cross(ea, eb), computed once per edge by the
+/// callers (the same operands, so the same bits as computing it here).
+__host__ __device__ inline V3 gc_intersection_n(const V3 &v1, const V3 &v2, const V3 &n1) {
+    V3 n2 = cross(v1, v2);
+    V3 d = cross(n1, n2);
+    if (length_sq(d) < 1.0e-30) return normalize(add(v1, v2));
+    V3 cand = normalize(d);
+    V3 mid_dir = normalize(add(v1, v2));
+    if (dot(cand, mid_dir) < 0.0) cand = V3{-cand.x, -cand.y, -cand.z};
+    return cand;
+}
 __host__ __device__ inline V3 gc_intersection(const V3 &v1, const V3 &v2, const V3 &ea, const V3 &eb) {
     V3 n1 = cross(ea, eb);
     V3 n2 = cross(v1, v2);
@@ -144,9 +155,9 @@ __host__ __device__ inline bool inside_half(double d) { return d >= -1.0e-14; }
 /// results at positions the walk has already read, which holds whenever the
 /// subject enters the clip half-space at most once per edge -- always for the
 /// convex cells a valid mesh has.  Anything else (a write that would land
-/// ahead of the walk, or more than `cap` vertices) sets *overflow.  Returns
-/// the new vertex count.
-__host__ __device__ inline int clip_edge_walk(PolyRef a, int na, const V3 &ea, const V3 &eb, const V3 &normal, int cap, bool *overflow) {
+/// ahead of the walk, or more than `cap` vertices) sets *overflow.  `normal`
+/// is the clip edge's cross(ea, eb).  Returns the new vertex count.
+__host__ __device__ inline int clip_edge_walk(PolyRef a, int na, const V3 &normal, int cap, bool *overflow) {
     int nb = 0;
     V3 prev = a.get(na - 1);
     double d_prev = dot(normal, prev);
@@ -163,10 +174,10 @@ __host__ __device__ inline int clip_edge_walk(PolyRef a, int na, const V3 &ea, c
             return 0;
         }
         if (cur_in) {
-            if (!prev_in) a.set(nb++, gc_intersection(prev, cur, ea, eb));
+            if (!prev_in) a.set(nb++, gc_intersection_n(prev, cur, normal));
             a.set(nb++, cur);
         } else if (prev_in) {
-            a.set(nb++, gc_intersection(prev, cur, ea, eb));
+            a.set(nb++, gc_intersection_n(prev, cur, normal));
         }
         prev = cur;
         d_prev = d_cur;
@@ -185,7 +196,7 @@ __host__ __device__ inline int clip_polygon_inplace(PolyRef a, int na, ClipVerte
         const V3 eb = clip_vertex(ci + 1 == nc ? 0 : ci + 1);
         V3 normal = cross(ea, eb);
         if (length_sq(normal) < 1.0e-30) continue;
-        na = clip_edge_walk(a, na, ea, eb, normal, cap, overflow);
+        na = clip_edge_walk(a, na, normal, cap, overflow);
         if (*overflow || na < 3) return 0;
     }
     return na;
@@ -220,7 +231,7 @@ __host__ __device__ inline int clip_polygon_slots(PolyRef a, int na, ClipVertex 
         const unsigned x = in ^ prev_in;                                // bit i: the arc (i-1, i) crosses
         if (x == 0) continue;  // every vertex inside: the walk rewrites the polygon unchanged
         if (__builtin_popcount(x) != 2) {
-            na = clip_edge_walk(a, na, ea, eb, normal, cap, overflow);
+            na = clip_edge_walk(a, na, normal, cap, overflow);
             if (*overflow || na < 3) return 0;
             continue;
         }
@@ -230,8 +241,8 @@ __host__ __device__ inline int clip_polygon_slots(PolyRef a, int na, ClipVertex 
             *overflow = true;
             return 0;
         }
-        const V3 x0 = gc_intersection(a.get(i0 == 0 ? na - 1 : i0 - 1), a.get(i0), ea, eb);
-        const V3 x1 = gc_intersection(a.get(i1 - 1), a.get(i1), ea, eb);
+        const V3 x0 = gc_intersection_n(a.get(i0 == 0 ? na - 1 : i0 - 1), a.get(i0), normal);
+        const V3 x1 = gc_intersection_n(a.get(i1 - 1), a.get(i1), normal);
         if ((in >> i0) & 1u) {
             // enters at i0, leaves at i1: X0, v[i0 .. i1-1], X1
             const int m = i1 - i0;
