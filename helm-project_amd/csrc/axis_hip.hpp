@@ -410,6 +410,12 @@ void cell_morton_keys(const Mesh &m, DeviceBuffer<std::uint32_t> &key, hipStream
 
 // scan / sort primitives (scan.hip, sort.hip)
 void exclusive_scan_i64(const index_t *in, index_t *out, index_t n, hipStream_t s);  // out[n] = total
+/// Copies `bytes` between device memory and pinned host memory (PinnedCache
+/// blocks: mapped, device-accessible) with a copy kernel on `s` instead of a
+/// DMA engine.  The node-xyz side stream copies this way: with both the side
+/// stream and the generating stream on the DMA engines, a copy of the
+/// generating stream was seen to wait ~5 ms about once in 30 generations.
+void kernel_copy(void *dst, const void *src, std::size_t bytes, hipStream_t s);
 void radix_sort_pairs_u32(std::uint32_t *keys, std::uint32_t *vals, std::size_t n, int bits, hipStream_t s);
 /// Stable LSD sort on key bits [first_bit, bits) (first_bit a multiple of 8):
 /// keys whose lower bits are already ascending need only the upper passes.

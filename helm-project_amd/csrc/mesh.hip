@@ -978,16 +978,16 @@ bool node_xyz_trig(const Mesh &m, double *d_out, hipStream_t st, const std::atom
     trig_xyz_kernel<<<grid_for(n, 256), 256, 0, st>>>(d_lon, d_lat, n, words, d_out, bits.get(), bits.get() + words, full.get(), n_full.get(),
                                                       full_cap);
     AXIS_LAUNCH_CHECK();
-    AXIS_HIP_CHECK(hipMemcpyAsync(flags, bits.get(), 2 * sizeof(std::uint64_t) * words, hipMemcpyDeviceToHost, st));
+    // (copy kernels on this side stream, no DMA engine: kernel_copy)
+    kernel_copy(flags, bits.get(), 2 * sizeof(std::uint64_t) * words, st);
     *h_nfull = 0;
-    AXIS_HIP_CHECK(hipMemcpyAsync(h_nfull, n_full.get(), sizeof(int), hipMemcpyDeviceToHost, st));
-    AXIS_HIP_CHECK(hipMemcpyAsync(h_full.data(), full.get(), sizeof(index_t) * full_first, hipMemcpyDeviceToHost, st));
+    kernel_copy(h_nfull, n_full.get(), sizeof(int), st);
+    kernel_copy(h_full.data(), full.get(), sizeof(index_t) * full_first, st);
     AXIS_HIP_CHECK(hipStreamSynchronize(st));
     const int nf = static_cast<int>(*h_nfull & 0xffffffffu);
     if (nf > full_cap) return false;
     if (nf > full_first) {
-        AXIS_HIP_CHECK(hipMemcpyAsync(h_full.data() + full_first, full.get() + full_first, sizeof(index_t) * (nf - full_first),
-                                      hipMemcpyDeviceToHost, st));
+        kernel_copy(h_full.data() + full_first, full.get() + full_first, sizeof(index_t) * (nf - full_first), st);
         AXIS_HIP_CHECK(hipStreamSynchronize(st));
     }
     // host libm: one bit per sine and cosine of the flagged angles, every
@@ -1028,12 +1028,12 @@ bool node_xyz_trig(const Mesh &m, double *d_out, hipStream_t st, const std::atom
         },
         256, trig_threads());
     if (stopped.load()) return false;
-    AXIS_HIP_CHECK(hipMemcpyAsync(bits.get() + 2 * words, res, 4 * sizeof(std::uint64_t) * words, hipMemcpyHostToDevice, st));
+    kernel_copy(bits.get() + 2 * words, res, 4 * sizeof(std::uint64_t) * words, st);
     trig_fix_kernel<<<grid_for(n, 256), 256, 0, st>>>(d_lon, d_lat, n, words, bits.get(), bits.get() + words, bits.get() + 2 * words, d_out);
     AXIS_LAUNCH_CHECK();
     if (nf > 0) {  // the listed nodes' xyz (list order kept on the device)
         DeviceBuffer<double> d_vals(3 * static_cast<std::size_t>(nf));
-        AXIS_HIP_CHECK(hipMemcpyAsync(d_vals.get(), h_vals.data(), sizeof(double) * 3 * nf, hipMemcpyHostToDevice, st));
+        kernel_copy(d_vals.get(), h_vals.data(), sizeof(double) * 3 * nf, st);
         scatter_xyz_kernel<<<grid_for(nf, 256), 256, 0, st>>>(full.get(), d_vals.get(), nf, d_out);
         AXIS_LAUNCH_CHECK();
     }
@@ -1069,6 +1069,8 @@ bool node_xyz_host(const Mesh &m, MeshGeom &g, const std::atomic<bool> *cancel, 
     // beside a running generation (upload given), two cores stay free for the
     // generating thread and the HIP runtime's own threads
     const int max_threads = upload ? std::max(1, host_threads() - 2) : 0;
+    int dev = 0;
+    AXIS_HIP_CHECK(hipGetDevice(&dev));
     parallel_range(n_nodes, [&](index_t b, index_t e) {
         SinCosMemo memo;
         for (index_t i = b; i < e; ++i) {
@@ -1086,15 +1088,18 @@ bool node_xyz_host(const Mesh &m, MeshGeom &g, const std::atomic<bool> *cancel, 
         }
         // this slice to the device now (errors are reported after the join:
         // a worker thread must not throw)
-        if (upload && e > b) {
-            const hipError_t r = hipMemcpyAsync(d_out + 3 * b, g.h_xyz.data() + 3 * b, 3 * sizeof(double) * static_cast<std::size_t>(e - b),
-                                                hipMemcpyHostToDevice, up);
-            if (r != hipSuccess) up_err.store(static_cast<int>(r));
+        if (upload && e > b) {  // (a copy kernel, as the side stream's other copies)
+            try {
+                AXIS_HIP_CHECK(hipSetDevice(dev));  // (a pool thread: the generating thread's device)
+                kernel_copy(d_out + 3 * b, g.h_xyz.data() + 3 * b, 3 * sizeof(double) * static_cast<std::size_t>(e - b), up);
+            } catch (...) {
+                up_err.store(1);
+            }
         }
     }, 4096, max_threads);
     const bool complete = !(cancel && cancel->load());
     g.xyz_ready = complete;
-    if (up_err.load() != 0) AXIS_HIP_CHECK(static_cast<hipError_t>(up_err.load()));
+    if (up_err.load() != 0) throw RuntimeError("node xyz: a slice upload failed");
     if (!upload || !complete) return false;
     AXIS_HIP_CHECK(hipEventRecord(done, up));
     return true;

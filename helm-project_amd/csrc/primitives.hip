@@ -234,6 +234,30 @@ __global__ void __launch_bounds__(kScanBlock) scan_down_kernel(const index_t *in
 
 }  // namespace
 
+namespace {
+// grid-stride copies; a host pointer is a mapped pinned block
+__global__ void copy_u64_kernel(std::uint64_t *dst, const std::uint64_t *src, std::size_t n) {
+    for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<std::size_t>(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+__global__ void copy_u8_kernel(unsigned char *dst, const unsigned char *src, std::size_t n) {
+    for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<std::size_t>(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+}  // namespace
+
+void kernel_copy(void *dst, const void *src, std::size_t bytes, hipStream_t s) {
+    if (bytes == 0) return;
+    const bool words = (reinterpret_cast<std::uintptr_t>(dst) % 8) == 0 && (reinterpret_cast<std::uintptr_t>(src) % 8) == 0 && bytes % 8 == 0;
+    const std::size_t n = words ? bytes / 8 : bytes;
+    const unsigned grid = static_cast<unsigned>(std::min<std::size_t>((n + 255) / 256, 1024));
+    if (words)
+        copy_u64_kernel<<<grid, 256, 0, s>>>(static_cast<std::uint64_t *>(dst), static_cast<const std::uint64_t *>(src), n);
+    else
+        copy_u8_kernel<<<grid, 256, 0, s>>>(static_cast<unsigned char *>(dst), static_cast<const unsigned char *>(src), n);
+    AXIS_LAUNCH_CHECK();
+}
+
 void exclusive_scan_i64(const index_t *in, index_t *out, index_t n, hipStream_t s) {
     if (n <= 0) {
         AXIS_HIP_CHECK(hipMemsetAsync(out, 0, sizeof(index_t), s));
