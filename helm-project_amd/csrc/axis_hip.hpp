@@ -456,11 +456,33 @@ void lbvh_query(const LbvhTree &t, const float *dst_boxes, index_t n_dst, Device
 void lbvh_candidates(const float *src_boxes, index_t n_src, const float *dst_boxes, index_t n_dst, DeviceBuffer<index_t> &cand_off,
                      DeviceBuffer<std::int32_t> &cand_src, hipStream_t s);
 
+/// WeightGenerator::generate's longitude-disjoint check in two halves:
+/// launch() queues both meshes' longitude ranges and their readback on `s`
+/// without waiting; finish() waits for that readback (normally already done
+/// by a wait of the generation itself) and throws the reference's mismatch
+/// error.  A generation that meets an error of its own before finish() runs
+/// reports that one instead.
+class LonCheck {
+   public:
+    void launch(const Mesh &src, const Mesh &dst, hipStream_t s);
+    void finish(bool structured_fast_path);
+    ~LonCheck();
+
+   private:
+    const Mesh *src_ = nullptr, *dst_ = nullptr;
+    DeviceBuffer<unsigned long long> d_;
+    PinnedArray<unsigned long long> h_;
+    hipEvent_t done_ = nullptr;
+    bool armed_ = false;
+};
+
 /// First-order conservative weights; with a proposer network the broad
 /// phase is the network's guarded search caps, with LBVH fallback rows.
+/// `lon` (launched by the caller) is finished after the source's grid
+/// detection.
 std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, hipStream_t s,
                                                            const mlp::Model *model = nullptr, ProposerStats *pst = nullptr,
-                                                           double area_tol = 1e-12);
+                                                           double area_tol = 1e-12, LonCheck *lon = nullptr);
 std::unique_ptr<InterpolationMatrix> generate_bilinear_rect(const Mesh &src, const Mesh &dst, const GridAxes &src_axes, hipStream_t s);
 void check_longitude_ranges(const Mesh &src, const Mesh &dst, bool structured_fast_path, hipStream_t s);
 /// coastal_renormalize_and_extrapolate (weight_generator.cpp:1736-1881):

@@ -147,14 +147,18 @@ std::unique_ptr<InterpolationMatrix> WeightGenerator::generate_proposed(const Me
     std::unique_ptr<InterpolationMatrix> M;
     switch (cfg.method) {
         case InterpolationMethod::Conservative1stOrder: {
-            detail::check_longitude_ranges(src, dst, false, s);
-            M = detail::generate_conservative(src, dst, cfg, s, proposer, stats, area_tol);
+            detail::LonCheck lon;  // (finished after the source's grid detection)
+            lon.launch(src, dst, s);
+            M = detail::generate_conservative(src, dst, cfg, s, proposer, stats, area_tol, &lon);
+            lon.finish(false);  // (a no-op once finished)
             break;
         }
         case InterpolationMethod::Bilinear: {
+            detail::LonCheck lon;
+            lon.launch(src, dst, s);
             detail::GridAxes sax = detail::unique_axes(src, s);
             const bool s_rect = detail::rectilinear_from_axes(src, sax);
-            detail::check_longitude_ranges(src, dst, s_rect, s);
+            lon.finish(s_rect);
             detail::MeshGeom sg, dg;
             if (detail::regular_from_axes(src, sax, sg)) {
                 detail::GridAxes dax = detail::unique_axes(dst, s);

@@ -915,24 +915,23 @@ double key_to_double(unsigned long long k) {
 }
 
 // longitude ranges of the vertices of both meshes' cells: one pair of
-// reductions, one download (an empty mesh gives lo = +max, hi = -max)
-void lon_ranges(const Mesh &a, const Mesh &b, double range[4], hipStream_t s) {
-    DeviceBuffer<unsigned long long> r(4);
+// reductions into r[4] (an empty mesh leaves lo = all ones, hi = 0)
+void lon_ranges_launch(const Mesh &a, const Mesh &b, unsigned long long *r, hipStream_t s) {
     // lows start at the largest key, highs at zero
-    AXIS_HIP_CHECK(hipMemsetAsync(r.get(), 0, sizeof(unsigned long long) * 4, s));
-    AXIS_HIP_CHECK(hipMemsetAsync(r.get(), 0xFF, sizeof(unsigned long long), s));
-    AXIS_HIP_CHECK(hipMemsetAsync(r.get() + 2, 0xFF, sizeof(unsigned long long), s));
+    AXIS_HIP_CHECK(hipMemsetAsync(r, 0, sizeof(unsigned long long) * 4, s));
+    AXIS_HIP_CHECK(hipMemsetAsync(r, 0xFF, sizeof(unsigned long long), s));
+    AXIS_HIP_CHECK(hipMemsetAsync(r + 2, 0xFF, sizeof(unsigned long long), s));
     const Mesh *ms[2] = {&a, &b};
     for (int k = 0; k < 2; ++k) {
         const Mesh &m = *ms[k];
         if (m.n_cells() == 0) continue;
         const unsigned grid = static_cast<unsigned>(std::min<index_t>((m.n_cells() + kRangeBlock - 1) / kRangeBlock, 1024));
-        lon_range_kernel<<<grid, kRangeBlock, 0, s>>>(m.d_offsets.get(), m.d_indices.get(), m.d_coords.get(), m.n_cells(), r.get() + 2 * k,
-                                                      r.get() + 2 * k + 1);
+        lon_range_kernel<<<grid, kRangeBlock, 0, s>>>(m.d_offsets.get(), m.d_indices.get(), m.d_coords.get(), m.n_cells(), r + 2 * k, r + 2 * k + 1);
         AXIS_LAUNCH_CHECK();
     }
-    unsigned long long h[4];
-    read_back(h, r.get(), 4, s);
+}
+
+void lon_ranges_decode(const unsigned long long h[4], double range[4]) {
     for (int k = 0; k < 2; ++k) {
         if (h[2 * k] == ~0ull) {
             range[2 * k] = std::numeric_limits<double>::max();
@@ -944,13 +943,16 @@ void lon_ranges(const Mesh &a, const Mesh &b, double range[4], hipStream_t s) {
     }
 }
 
-}  // namespace
+void lon_ranges(const Mesh &a, const Mesh &b, double range[4], hipStream_t s) {
+    DeviceBuffer<unsigned long long> r(4);
+    lon_ranges_launch(a, b, r.get(), s);
+    unsigned long long h[4];
+    read_back(h, r.get(), 4, s);
+    lon_ranges_decode(h, range);
+}
 
-void check_longitude_ranges(const Mesh &src, const Mesh &dst, bool structured_fast_path, hipStream_t s) {
-    const auto cs = src.coord_system();
-    if (cs != CoordinateSystem::SphericalDeg && cs != CoordinateSystem::SphericalRad) return;
-    double range[4];
-    lon_ranges(src, dst, range, s);
+// the reference's decision on the two ranges (weight_generator.cpp:1898-1992)
+void lon_mismatch_check(CoordinateSystem cs, const double range[4], bool structured_fast_path) {
     const double s0 = range[0], s1 = range[1], d0 = range[2], d1 = range[3];
     bool disjoint = (s1 < d0) || (d1 < s0);
     if (!disjoint) return;
@@ -961,6 +963,46 @@ void check_longitude_ranges(const Mesh &src, const Mesh &dst, bool structured_fa
         throw InvalidArgument("WeightGenerator::generate: Spherical longitude coordinate range mismatch: source [" + std::to_string(s0) + ", " +
                               std::to_string(s1) + "] and destination [" + std::to_string(d0) + ", " + std::to_string(d1) +
                               "] are disjoint but would overlap after a shift of " + std::to_string(shift));
+}
+
+}  // namespace
+
+void check_longitude_ranges(const Mesh &src, const Mesh &dst, bool structured_fast_path, hipStream_t s) {
+    const auto cs = src.coord_system();
+    if (cs != CoordinateSystem::SphericalDeg && cs != CoordinateSystem::SphericalRad) return;
+    double range[4];
+    lon_ranges(src, dst, range, s);
+    lon_mismatch_check(cs, range, structured_fast_path);
+}
+
+void LonCheck::launch(const Mesh &src, const Mesh &dst, hipStream_t s) {
+    const auto cs = src.coord_system();
+    armed_ = cs == CoordinateSystem::SphericalDeg || cs == CoordinateSystem::SphericalRad;
+    if (!armed_) return;
+    src_ = &src;
+    dst_ = &dst;
+    d_.resize(4);
+    h_.resize(4);
+    if (!done_) AXIS_HIP_CHECK(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
+    lon_ranges_launch(src, dst, d_.get(), s);
+    AXIS_HIP_CHECK(hipMemcpyAsync(h_.data(), d_.get(), 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    AXIS_HIP_CHECK(hipEventRecord(done_, s));
+}
+
+void LonCheck::finish(bool structured_fast_path) {
+    if (!armed_) return;
+    armed_ = false;
+    AXIS_HIP_CHECK(hipEventSynchronize(done_));
+    double range[4];
+    lon_ranges_decode(h_.data(), range);
+    lon_mismatch_check(src_->coord_system(), range, structured_fast_path);
+}
+
+LonCheck::~LonCheck() {
+    if (done_) {
+        (void)hipEventSynchronize(done_);  // (the readback lands in a block the cache may hand out again)
+        (void)hipEventDestroy(done_);
+    }
 }
 
 namespace {
@@ -993,7 +1035,7 @@ struct PhaseEvents {
 }  // namespace
 
 std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, const Mesh &dst, const RegridConfig &cfg, hipStream_t s,
-                                                           const mlp::Model *model, ProposerStats *pst, double area_tol) {
+                                                           const mlp::Model *model, ProposerStats *pst, double area_tol, LonCheck *lon) {
     const auto t0 = std::chrono::steady_clock::now();
     PhaseTrace tr("conservative", s);
     PhaseEvents &ev = PhaseEvents::get();
@@ -1068,6 +1110,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
         src_done.set_value();
     }
     GridAxes sax = unique_axes(src, s);
+    if (lon) lon->finish(false);  // (its readback was queued before the detection's)
     const bool src_dev_xyz = spherical && node_xyz_device_eligible(src, sax);
     if (src_dev_xyz) skip_src.store(true);
     GridAxes dax = unique_axes(dst, s);

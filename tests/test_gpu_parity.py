@@ -120,6 +120,13 @@ def test_rejections_match_reference():
     far["coords"] = np.asfortranarray(far["coords"] + np.array([720.0, 0.0]))
     with pytest.raises(axis.AxisError, match="longitude"):
         axis.generate_weights(s, axis.Mesh(far), axis.CONSERVATIVE)
+    # bilinear: the check comes before the path's own requirements (a
+    # Voronoi source is no lat-lon grid), as in the reference
+    with pytest.raises(axis.AxisError, match="longitude"):
+        axis.generate_weights(axis.Mesh(grids.jittered_voronoi(2, 5)), axis.Mesh(far), axis.BILINEAR)
+    # a good pair afterwards on the same stream
+    m = axis.generate_weights(s, axis.Mesh(grids.cubed_sphere(4)), axis.CONSERVATIVE)
+    assert m.nnz > 0
 
 
 @pytest.mark.parametrize("which", ["2deg->C24", "nonuniform->C16"])
