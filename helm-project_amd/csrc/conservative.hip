@@ -925,7 +925,7 @@ void lon_ranges_launch(const Mesh &a, const Mesh &b, unsigned long long *r, hipS
     for (int k = 0; k < 2; ++k) {
         const Mesh &m = *ms[k];
         if (m.n_cells() == 0) continue;
-        const unsigned grid = static_cast<unsigned>(std::min<index_t>((m.n_cells() + kRangeBlock - 1) / kRangeBlock, 1024));
+        const unsigned grid = static_cast<unsigned>(std::min<index_t>((m.n_cells() + kRangeBlock - 1) / kRangeBlock, 256));
         lon_range_kernel<<<grid, kRangeBlock, 0, s>>>(m.d_offsets.get(), m.d_indices.get(), m.d_coords.get(), m.n_cells(), r + 2 * k, r + 2 * k + 1);
         AXIS_LAUNCH_CHECK();
     }

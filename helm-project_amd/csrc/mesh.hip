@@ -208,12 +208,18 @@ __global__ void __launch_bounds__(kQuickBlock) sample_bins_kernel(const double *
     }
 }
 
-// occupied-bin counts (one wave-reduced atomic per wave)
-__global__ void count_bins_kernel(const std::uint32_t *bins, QuickAxes *q) {
-    const int w = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;  // < 2 * kBinWords
-    int c = __popc(bins[w]);
+// occupied-bin counts: each thread a run of words, one atomic per workgroup
+// (a workgroup's words lie in one column's half)
+constexpr int kCountWords = 16;  // words per thread
+__global__ void __launch_bounds__(256) count_bins_kernel(const std::uint32_t *bins, QuickAxes *q) {
+    const int w0 = (static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x)) * kCountWords;  // < 2 * kBinWords
+    int c = 0;
+    for (int k = 0; k < kCountWords; ++k) c += __popc(bins[w0 + k]);
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&q->occupied[w / kBinWords], c);
+    __shared__ int part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(&q->occupied[(static_cast<int>(blockIdx.x) * 256 * kCountWords) / kBinWords], part[0] + part[1] + part[2] + part[3]);
 }
 
 // distinct values of both columns (finite inputs: order keys are then never
@@ -421,13 +427,14 @@ GridAxes unique_axes(const Mesh &m, hipStream_t s) {
         if (!counts) {
             DeviceBuffer<std::uint32_t> bins(2 * kBinWords);
             AXIS_HIP_CHECK(hipMemsetAsync(bins.get(), 0, sizeof(std::uint32_t) * 2 * kBinWords, s));
-            const unsigned mm_grid = static_cast<unsigned>(std::min<index_t>((n + kQuickBlock - 1) / kQuickBlock, 1024));
+            // (256 workgroups: their closing atomics on four words serialise)
+            const unsigned mm_grid = static_cast<unsigned>(std::min<index_t>((n + kQuickBlock - 1) / kQuickBlock, 256));
             minmax_kernel<<<mm_grid, kQuickBlock, 0, s>>>(lon, lat, n, dq.get());
             AXIS_LAUNCH_CHECK();
             const index_t ns = std::min(n, kSample);
             sample_bins_kernel<<<grid_for(ns, kQuickBlock), kQuickBlock, 0, s>>>(lon, lat, ns, dq.get(), bins.get());
             AXIS_LAUNCH_CHECK();
-            count_bins_kernel<<<2 * kBinWords / 256, 256, 0, s>>>(bins.get(), dq.get());
+            count_bins_kernel<<<2 * kBinWords / (256 * kCountWords), 256, 0, s>>>(bins.get(), dq.get());
             AXIS_LAUNCH_CHECK();
             read_back(&hq, dq.get(), 1, s);
             tr.mark("bound");
