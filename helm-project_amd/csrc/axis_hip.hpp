@@ -401,7 +401,11 @@ bool node_xyz_device_eligible(const Mesh &m, const GridAxes &ax);
 /// are computed on the host.  Returns false (and leaves g.dev_xyz unset and
 /// g.d_xyz untouched) when the mesh is not eligible or too many nodes need the
 /// host.  The caller must not have slice uploads into g.d_xyz still to queue.
-bool node_xyz_device(const Mesh &m, const GridAxes &ax, MeshGeom &g, hipStream_t s);
+/// `tables`: axis_trig_tables(ax), when the caller computed them beforehand.
+bool node_xyz_device(const Mesh &m, const GridAxes &ax, MeshGeom &g, hipStream_t s, PinnedArray<double> *tables = nullptr);
+/// The host-libm sin/cos tables of a grid's axis values that node_xyz_device
+/// uploads (pure host work: may run on another thread meanwhile).
+PinnedArray<double> axis_trig_tables(const GridAxes &ax);
 void prepare_geometry(const Mesh &m, MeshGeom &g, hipStream_t s);
 
 /// 30-bit Morton code of every cell's centroid (normalised mean of its vertex

@@ -1113,6 +1113,14 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     if (lon) lon->finish(false);  // (its readback was queued before the detection's)
     const bool src_dev_xyz = spherical && node_xyz_device_eligible(src, sax);
     if (src_dev_xyz) skip_src.store(true);
+    // the source's axis sin/cos tables (host libm, ~2,000 angles at 0.25deg)
+    // on a helper thread while the destination's detection runs
+    std::future<PinnedArray<double>> src_tables;
+    if (src_dev_xyz)
+        src_tables = std::async(std::launch::async, [&sax] {
+            BlockCache::NoWaitScope no_wait_here;  // (its pinned block)
+            return axis_trig_tables(sax);
+        });
     GridAxes dax = unique_axes(dst, s);
     const bool s_reg = regular_from_axes(src, sax, sg), d_reg = regular_from_axes(dst, dax, dg);
     // generate_conservative's dispatch (weight_generator.cpp:3259-3296): the
@@ -1144,7 +1152,10 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     // skip_src) before the device build may replace sg.d_xyz, so no slice
     // upload is queued into a buffer after its release
     src_ready.get();
-    if (src_dev_xyz) node_xyz_device(src, sax, sg, s);  // (false: host xyz below)
+    if (src_dev_xyz) {
+        PinnedArray<double> tables = src_tables.get();
+        node_xyz_device(src, sax, sg, s, &tables);  // (false: host xyz below)
+    }
     if (!sg.dev_xyz && src_uploaded.load()) {
         AXIS_HIP_CHECK(hipStreamWaitEvent(s, src_up.e, 0));
         (src_trig.load() ? sg.dev_xyz : sg.d_xyz_ready) = true;

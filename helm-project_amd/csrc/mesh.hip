@@ -895,11 +895,7 @@ bool node_xyz_device_eligible(const Mesh &m, const GridAxes &ax) {
            ax.xlon.size() < (std::size_t(1) << 30) && ax.xlat.size() < (std::size_t(1) << 30);
 }
 
-bool node_xyz_device(const Mesh &m, const GridAxes &ax, MeshGeom &g, hipStream_t s) {
-    g.dev_xyz = false;
-    if (!node_xyz_device_eligible(m, ax)) return false;
-    PhaseTrace tr("geometry", s);
-    const index_t n = m.n_nodes();
+PinnedArray<double> axis_trig_tables(const GridAxes &ax) {
     // exact distinct coordinates where the detection kept them (else the
     // tolerance-unique ones: nodes off those values go to the host)
     const std::vector<double> &vlon = ax.xlon.empty() ? ax.ulon : ax.xlon, &vlat = ax.xlat.empty() ? ax.ulat : ax.xlat;
@@ -916,6 +912,21 @@ bool node_xyz_device(const Mesh &m, const GridAxes &ax, MeshGeom &g, hipStream_t
         ulat[b] = vlat[b];
         sin_cos(vlat[b] * kDeg2Rad, slat[b], clat[b]);
     }
+    return tab;
+}
+
+bool node_xyz_device(const Mesh &m, const GridAxes &ax, MeshGeom &g, hipStream_t s, PinnedArray<double> *tables) {
+    g.dev_xyz = false;
+    if (!node_xyz_device_eligible(m, ax)) return false;
+    PhaseTrace tr("geometry", s);
+    const index_t n = m.n_nodes();
+    const int nlon = static_cast<int>((ax.xlon.empty() ? ax.ulon : ax.xlon).size()), nlat = static_cast<int>((ax.xlat.empty() ? ax.ulat : ax.xlat).size());
+    PinnedArray<double> own;
+    if (!tables) {
+        own = axis_trig_tables(ax);
+        tables = &own;
+    }
+    PinnedArray<double> &tab = *tables;
     DeviceBuffer<double> d_tab(tab.size());
     d_tab.upload(tab.data(), tab.size(), s);
     // a fresh buffer, moved into g.d_xyz only on success: on failure g.d_xyz
