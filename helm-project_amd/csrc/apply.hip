@@ -114,21 +114,41 @@ __global__ void seg_fill_kernel(const std::uint64_t *key, const index_t *rp, ind
 // row is cut into E-entry segments, each an FMA chain from zero, and the
 // segment sums are added in order from zero (one wave per row), exactly as
 // the segmented kernels and long_reduce_kernel do.
+// (E a template constant: a row's column and weight loads, then its field
+// gathers, are all issued before the FMA chain needs them)
+template <int E>
 __global__ void __launch_bounds__(256) direct_rows_kernel(const index_t *__restrict__ rp, const index_t *__restrict__ col,
                                                           const double *__restrict__ val, index_t n_dst, index_t n_src, const double *__restrict__ X,
-                                                          double *__restrict__ Y, index_t k, int E, std::int32_t *__restrict__ long_list,
+                                                          double *__restrict__ Y, index_t k, std::int32_t *__restrict__ long_list,
                                                           int *__restrict__ n_long) {
     const index_t r = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (r >= n_dst) return;
-    const index_t b = rp[r], len = rp[r + 1] - b;
+    const index_t b = rp[r];
+    const int len = static_cast<int>(min<index_t>(rp[r + 1] - b, E + 1));
     if (len > E) {
         long_list[atomicAdd(n_long, 1)] = static_cast<std::int32_t>(r);
         return;
     }
+    index_t c[E];
+    double v[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        c[q] = 0;
+        v[q] = 0.0;
+        if (q < len) {
+            c[q] = col[b + q];
+            v[q] = val[b + q];
+        }
+    }
     for (index_t f = 0; f < k; ++f) {
         const double *xf = X + f * n_src;
+        double xv[E];
+#pragma unroll
+        for (int q = 0; q < E; ++q) xv[q] = q < len ? xf[c[q]] : 0.0;
         double acc = 0.0;
-        for (index_t e = b; e < b + len; ++e) acc = fma(val[e], xf[col[e]], acc);
+#pragma unroll
+        for (int q = 0; q < E; ++q)
+            if (q < len) acc = fma(v[q], xv[q], acc);
         Y[f * n_dst + r] = acc;
     }
 }
@@ -541,9 +561,18 @@ void batch_apply(InterpolationMatrix &A, const void *x, FieldType xt, void *y, F
         DeviceBuffer<std::int32_t> list(std::max<index_t>(n_long, 1));
         DeviceBuffer<int> cnt(1);
         AXIS_HIP_CHECK(hipMemsetAsync(cnt.get(), 0, sizeof(int), s));
-        direct_rows_kernel<<<grid_for(n_dst, 256), 256, 0, s>>>(A.row_ptr.get(), A.col.get(), A.val.get(), n_dst, n_src, static_cast<const double *>(x),
-                                                                static_cast<double *>(y), k, E, list.get(), cnt.get());
-        AXIS_LAUNCH_CHECK();
+        auto rows = [&](auto kern) {
+            kern<<<grid_for(n_dst, 256), 256, 0, s>>>(A.row_ptr.get(), A.col.get(), A.val.get(), n_dst, n_src, static_cast<const double *>(x),
+                                                      static_cast<double *>(y), k, list.get(), cnt.get());
+            AXIS_LAUNCH_CHECK();
+        };
+        switch (E) {
+            case 4: rows(direct_rows_kernel<4>); break;
+            case 8: rows(direct_rows_kernel<8>); break;
+            case 12: rows(direct_rows_kernel<12>); break;
+            case 16: rows(direct_rows_kernel<16>); break;
+            default: throw RuntimeError("batch_apply: unexpected segment length " + std::to_string(E));
+        }
         if (n_long > 0) {
             direct_long_kernel<<<static_cast<unsigned>((n_long + 3) / 4), 256, 0, s>>>(A.row_ptr.get(), A.col.get(), A.val.get(), n_dst, n_src,
                                                                                         static_cast<const double *>(x), static_cast<double *>(y), k, E,
