@@ -555,6 +555,7 @@ __global__ void unmapped_kernel(const index_t *row_ptr, index_t n_dst, const std
 // (weight_generator.cpp:3682-3724: w = max(ov / area_b, 0), frac_b = Σ ov /
 // area_b in column order, FracArea division by the unclamped sum).
 constexpr int kShortRow = 16;     // sorted by one thread
+constexpr int kShortBatch = 4;    // candidates a thread loads at once
 constexpr int kLongRowMax = 4096; // sorted by one workgroup in LDS
 constexpr int kLongRowBlock = 256;
 constexpr int kHugeRowMax = 8192; // sorted by one 1024-thread workgroup in 96 KiB of LDS
@@ -626,18 +627,35 @@ __global__ void __launch_bounds__(256) short_rows_kernel(const index_t *cand_off
         std::int32_t cc[kShortRow];
         double oo[kShortRow];
         int k = 0;
-        for (index_t p = cand_off[r], e = cand_off[r + 1]; p < e && k < len; ++p) {
-            const double o = overlap[p];
-            if (!(o > 0.0)) continue;
-            const std::int32_t c = pair_src[p];
-            int m = k;
-            for (; m > 0 && cc[m - 1] > c; --m) {
-                cc[m] = cc[m - 1];
-                oo[m] = oo[m - 1];
+        // candidates kShortBatch at a time: their overlaps and sources loaded
+        // together (the sources whatever the overlap), then filed in order
+        const index_t e = cand_off[r + 1];
+        for (index_t p0 = cand_off[r]; p0 < e && k < len; p0 += kShortBatch) {
+            double ob[kShortBatch];
+            std::int32_t sb[kShortBatch];
+#pragma unroll
+            for (int j = 0; j < kShortBatch; ++j) {
+                ob[j] = 0.0;
+                sb[j] = 0;
+                if (p0 + j < e) {
+                    ob[j] = overlap[p0 + j];
+                    sb[j] = pair_src[p0 + j];
+                }
             }
-            cc[m] = c;
-            oo[m] = o;
-            ++k;
+#pragma unroll
+            for (int j = 0; j < kShortBatch; ++j) {
+                const double o = ob[j];
+                if (!(o > 0.0) || k >= len) continue;
+                const std::int32_t c = sb[j];
+                int m = k;
+                for (; m > 0 && cc[m - 1] > c; --m) {
+                    cc[m] = cc[m - 1];
+                    oo[m] = oo[m - 1];
+                }
+                cc[m] = c;
+                oo[m] = o;
+                ++k;
+            }
         }
         const double a = area_dst[r];
         double fb = 0.0;
