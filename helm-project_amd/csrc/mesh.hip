@@ -560,17 +560,30 @@ constexpr int kCellBlock = 256;
 
 __device__ inline V3 node(const double *xyz, index_t n) { return V3{xyz[3 * n], xyz[3 * n + 1], xyz[3 * n + 2]}; }
 
+// A cell's vertices: through its node indices, or held in registers (a
+// quad's four, loaded together up front).
+struct NodeVtx {
+    const double *xyz;
+    const index_t *idx;
+    __device__ V3 operator()(int k) const { return node(xyz, idx[k]); }
+};
+struct QuadVtx {
+    V3 v[4];
+    __device__ V3 operator()(int k) const { return v[k]; }
+};
+
 // DegenerateCellHandler::classify on the first min(nv, 32) vertices.
 // Returns the classification and the fan area it used.
-__device__ int classify_cell(const double *xyz, const index_t *idx, int nv, double &fan_area) {
+template <class Vtx>
+__device__ int classify_cell(const Vtx &P, int nv, double &fan_area) {
     fan_area = 0.0;
     if (nv < 3) return 1;
     // area (spherical_clipper.hpp:112-137)
     double total = 0.0;
-    V3 a = node(xyz, idx[0]);
-    V3 b = node(xyz, idx[1]);
+    V3 a = P(0);
+    V3 b = P(1);
     for (int i = 1; i < nv - 1; ++i) {
-        V3 c = node(xyz, idx[i + 1]);
+        V3 c = P(i + 1);
         double num;
         total += geo::fan_term(a, b, c, num);
         b = c;
@@ -581,10 +594,10 @@ __device__ int classify_cell(const double *xyz, const index_t *idx, int nv, doub
     const double t2 = 1e-15 * 1e-15;
     int uniq = 1;
     for (int i = 1; i < nv; ++i) {
-        V3 vi = node(xyz, idx[i]);
+        V3 vi = P(i);
         bool dup = false;
         for (int j = 0; j < i; ++j) {
-            V3 vj = node(xyz, idx[j]);
+            V3 vj = P(j);
             V3 d{vi.x - vj.x, vi.y - vj.y, vi.z - vj.z};
             if (geo::length_sq(d) < t2) {
                 dup = true;
@@ -598,12 +611,12 @@ __device__ int classify_cell(const double *xyz, const index_t *idx, int nv, doub
     if (nv >= 4) {
         for (int i = 0; i < nv; ++i) {
             int in = (i + 1) % nv;
-            V3 pa = node(xyz, idx[i]), pb = node(xyz, idx[in]);
+            V3 pa = P(i), pb = P(in);
             V3 nab = geo::cross(pa, pb);
             for (int j = i + 2; j < nv; ++j) {
                 int jn = (j + 1) % nv;
                 if (jn == i) continue;
-                V3 pc = node(xyz, idx[j]), pd = node(xyz, idx[jn]);
+                V3 pc = P(j), pd = P(jn);
                 double dc = geo::dot(nab, pc), dd = geo::dot(nab, pd);
                 if (dc * dd >= 0.0) continue;
                 V3 ncd = geo::cross(pc, pd);
@@ -619,32 +632,32 @@ __device__ int classify_cell(const double *xyz, const index_t *idx, int nv, doub
 __device__ inline float f_down(double v) { return nextafterf(__double2float_rd(v), -INFINITY); }
 __device__ inline float f_up(double v) { return nextafterf(__double2float_ru(v), INFINITY); }
 
-__global__ void __launch_bounds__(kCellBlock) cell_prep_kernel(index_t n_cells, const index_t *off, const index_t *ind, const double *xyz,
-                                                               const double *xyz_deg, const double *mesh_areas, const std::int32_t *mask,
-                                                               double *area_out, std::int32_t *flags_out, double *cap_out, float *box_out,
-                                                               index_t *recheck, int *n_recheck) {
-    index_t c = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (c >= n_cells) return;
-    const index_t b = off[c];
-    const index_t nv_all = off[c + 1] - b;
-    const int nv = static_cast<int>(nv_all < 32 ? nv_all : 32);
-    const index_t *idx = ind + b;
+// one cell's flags, area, cap and box; A gives its vertices' xyz, Dg those
+// of the degree reading of its coordinates (the same when `same`); NV, when
+// not 0, is the cell's vertex count known at compile time (all loops then
+// unroll over registers)
+template <int NV, class VA, class VD>
+__device__ inline void cell_prep_one(index_t c, int nv_dyn, index_t nv_all_dyn, const VA &A, const VD &Dg, bool same, const double *mesh_areas,
+                                     const std::int32_t *mask, double *area_out, std::int32_t *flags_out, double *cap_out, float *box_out,
+                                     index_t *recheck, int *n_recheck) {
+    const int nv = NV ? NV : nv_dyn;
+    const index_t nv_all = NV ? NV : nv_all_dyn;
     // degenerate scan (always on the degree interpretation of the coordinates)
     double fan_deg;
-    int cls = classify_cell(xyz_deg, idx, nv, fan_deg);
+    int cls = classify_cell(Dg, nv, fan_deg);
     // cell area: mesh areas when given, else the fan of the polygon
     double area;
     if (mesh_areas) {
         area = mesh_areas[c];
-    } else if (xyz_deg == xyz) {
+    } else if (same) {
         area = fan_deg;
     } else {
         double total = 0.0;
         if (nv >= 3) {
-            V3 a = node(xyz, idx[0]);
-            V3 bb = node(xyz, idx[1]);
+            V3 a = A(0);
+            V3 bb = A(1);
             for (int i = 1; i < nv - 1; ++i) {
-                V3 cc = node(xyz, idx[i + 1]);
+                V3 cc = A(i + 1);
                 double num;
                 total += geo::fan_term(a, bb, cc, num);
                 bb = cc;
@@ -666,7 +679,7 @@ __global__ void __launch_bounds__(kCellBlock) cell_prep_kernel(index_t n_cells, 
     if (nv_all >= 1) {
         double sx = 0, sy = 0, sz = 0;
         for (index_t k = 0; k < nv_all; ++k) {
-            V3 p = node(xyz, idx[k]);
+            V3 p = A(static_cast<int>(k));
             sx += p.x;
             sy += p.y;
             sz += p.z;
@@ -674,7 +687,7 @@ __global__ void __launch_bounds__(kCellBlock) cell_prep_kernel(index_t n_cells, 
         double inv = 1.0 / static_cast<double>(nv_all);
         cen = geo::normalize(V3{sx * inv, sy * inv, sz * inv});
         for (index_t k = 0; k < nv_all; ++k) {
-            double d = geo::dot(cen, node(xyz, idx[k]));
+            double d = geo::dot(cen, A(static_cast<int>(k)));
             d = fmin(1.0, fmax(-1.0, d));
             double ang = acos(d);
             if (ang > rad) rad = ang;
@@ -688,7 +701,7 @@ __global__ void __launch_bounds__(kCellBlock) cell_prep_kernel(index_t n_cells, 
     // outward one extra float ulp so it contains the reference's box
     double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
     for (index_t k = 0; k < nv_all; ++k) {
-        V3 p = node(xyz, idx[k]);
+        V3 p = A(static_cast<int>(k));
         mn[0] = fmin(mn[0], p.x);
         mn[1] = fmin(mn[1], p.y);
         mn[2] = fmin(mn[2], p.z);
@@ -701,6 +714,32 @@ __global__ void __launch_bounds__(kCellBlock) cell_prep_kernel(index_t n_cells, 
     for (int a = 0; a < 3; ++a) {
         box_out[6 * c + a] = nv_all ? f_down(mn[a] - eps) : INFINITY;
         box_out[6 * c + 3 + a] = nv_all ? f_up(mx[a] + eps) : -INFINITY;
+    }
+}
+
+__global__ void __launch_bounds__(kCellBlock) cell_prep_kernel(index_t n_cells, const index_t *off, const index_t *ind, const double *xyz,
+                                                               const double *xyz_deg, const double *mesh_areas, const std::int32_t *mask,
+                                                               double *area_out, std::int32_t *flags_out, double *cap_out, float *box_out,
+                                                               index_t *recheck, int *n_recheck) {
+    index_t c = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= n_cells) return;
+    const index_t b = off[c];
+    const index_t nv_all = off[c + 1] - b;
+    const int nv = static_cast<int>(nv_all < 32 ? nv_all : 32);
+    const index_t *idx = ind + b;
+    const bool same = xyz_deg == xyz;
+    if (same && nv_all == 4) {
+        // a quad: its four vertices loaded together, once
+        QuadVtx Q;
+        const index_t i0 = idx[0], i1 = idx[1], i2 = idx[2], i3 = idx[3];
+        Q.v[0] = node(xyz, i0);
+        Q.v[1] = node(xyz, i1);
+        Q.v[2] = node(xyz, i2);
+        Q.v[3] = node(xyz, i3);
+        cell_prep_one<4>(c, nv, nv_all, Q, Q, true, mesh_areas, mask, area_out, flags_out, cap_out, box_out, recheck, n_recheck);
+    } else {
+        cell_prep_one<0>(c, nv, nv_all, NodeVtx{xyz, idx}, NodeVtx{xyz_deg, idx}, same, mesh_areas, mask, area_out, flags_out, cap_out, box_out, recheck,
+                      n_recheck);
     }
 }
 
