@@ -245,10 +245,24 @@ __global__ void __launch_bounds__(kWave) clip_live_kernel(index_t p0, const std:
     const int ns = cell_nverts(src, i);
     bool overflow = ns > CAP;
     if (!overflow) {
-        for (int k = 0; k < ns; ++k) A.set(k, cell_vertex(src, i, k));
         const int nc = cell_nverts(dst, j);
-        auto clipv = [&](int k) { return cell_vertex(dst, j, k); };
-        int n = geo::clip_polygon_slots(A, ns, clipv, nc, CAP, &overflow);
+        int n;
+        if (ns == 4 && nc == 4) {
+            // two quads: all eight vertices loaded together, once (the clip
+            // cell's vertices then come from registers)
+            const V3 s0 = cell_vertex(src, i, 0), s1 = cell_vertex(src, i, 1), s2 = cell_vertex(src, i, 2), s3 = cell_vertex(src, i, 3);
+            const V3 q0 = cell_vertex(dst, j, 0), q1 = cell_vertex(dst, j, 1), q2 = cell_vertex(dst, j, 2), q3 = cell_vertex(dst, j, 3);
+            A.set(0, s0);
+            A.set(1, s1);
+            A.set(2, s2);
+            A.set(3, s3);
+            auto clipq = [&](int k) { return k == 0 ? q0 : (k == 1 ? q1 : (k == 2 ? q2 : q3)); };
+            n = geo::clip_polygon_slots(A, 4, clipq, 4, CAP, &overflow);
+        } else {
+            for (int k = 0; k < ns; ++k) A.set(k, cell_vertex(src, i, k));
+            auto clipv = [&](int k) { return cell_vertex(dst, j, k); };
+            n = geo::clip_polygon_slots(A, ns, clipv, nc, CAP, &overflow);
+        }
         if (!overflow && n >= 3) {
             double abs_sum;
             bool zero_num;
