@@ -446,24 +446,30 @@ __global__ void frac_a_kernel(const std::uint32_t *key, const std::uint32_t *per
     frac_a[c] = fmin(s, 1.0);
 }
 
-// frac_a without a global sort.  Each source column's CSR positions are
-// gathered by counting (column counts, their scan, an atomic scatter that
-// leaves each column's positions in arbitrary order), then each column sorts
-// its few positions and sums in ascending position, which is the row order
-// of frac_a_kernel's stable sort: the same additions in the same order.
+// frac_a without a global sort.  Each source column's entries are gathered
+// by counting (column counts, their scan, an atomic scatter that leaves each
+// column's entry indices in arbitrary order), then each column sorts its few
+// indices and sums in ascending index, which is the row order of
+// frac_a_kernel's stable sort: the same additions in the same order.  The
+// entries are the CSR's, or, where the candidate pairs are grouped by row
+// with rows ascending, the candidates of positive overlap (a row holds a
+// column at most once, so ascending candidate index is ascending row too).
 constexpr int kColNet = 16;       // columns of up to kColNet entries: one thread, a sorting network
 constexpr int kColLds = 4096;     // longer columns: one workgroup, a bitonic sort in LDS
 constexpr int kColBlock = 256;
 
-__global__ void col_count_kernel(const index_t *col, index_t nnz, unsigned long long *cnt) {
+// keep (optional): entries of keep[e] > 0 only
+template <class C>
+__global__ void col_count_kernel(const C *col, const double *keep, index_t n, unsigned long long *cnt) {
     const index_t e = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (e < nnz) atomicAdd(&cnt[col[e]], 1ull);
+    if (e < n && (!keep || keep[e] > 0.0)) atomicAdd(&cnt[col[e]], 1ull);
 }
 
 // cnt[c] counts down to zero (and is zero again afterwards)
-__global__ void col_scatter_kernel(const index_t *col, index_t nnz, const index_t *cp, unsigned long long *cnt, std::uint32_t *cs) {
+template <class C>
+__global__ void col_scatter_kernel(const C *col, const double *keep, index_t n, const index_t *cp, unsigned long long *cnt, std::uint32_t *cs) {
     const index_t e = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (e >= nnz) return;
+    if (e >= n || (keep && !(keep[e] > 0.0))) return;
     const index_t c = col[e];
     const unsigned long long left = atomicAdd(&cnt[c], ~0ull);  // (adds -1)
     cs[cp[c] + static_cast<index_t>(left) - 1] = static_cast<std::uint32_t>(e);
@@ -1463,6 +1469,73 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     DeviceBuffer<unsigned long long> d_hist(kH);
     AXIS_HIP_CHECK(hipMemsetAsync(max_row.get(), 0, sizeof(int), s));
     AXIS_HIP_CHECK(hipMemsetAsync(d_hist.get(), 0, sizeof(unsigned long long) * kH, s));
+    // frac_a: each column's entries gathered by counting and summed in
+    // ascending entry (row) order; a column of more than kColLds entries (a
+    // source cell under that many destination cells) sets the flag, and the
+    // stable global column sort below then gives every column's sum.  Where
+    // the pairs are grouped by row with rows ascending, the entries are the
+    // candidates of positive overlap, counted on a side stream beside the
+    // CSR assembly; otherwise the CSR's entries, after it.
+    DeviceBuffer<int> col_flags(2);  // long-column count, overflow
+    const char *fa = std::getenv("AXIS_FRAC_A");  // "sort": the global column sort for every column
+    const bool fa_sort = fa && std::string(fa) == "sort";
+    const bool by_cand = !appended && n_src > 0 && n_pairs < (index_t(1) << 32) && !fa_sort;
+    hipStream_t fs = s;
+    struct FracEvents {
+        hipEvent_t ready = nullptr, done = nullptr;
+        ~FracEvents() {
+            if (ready) (void)hipEventDestroy(ready);
+            if (done) (void)hipEventDestroy(done);
+        }
+    } fev;
+    DeviceBuffer<unsigned long long> col_cnt;
+    DeviceBuffer<index_t> col_ptr;
+    DeviceBuffer<std::uint32_t> col_ent;
+    DeviceBuffer<std::int32_t> long_cols;
+    const double *ov_csr = nullptr;  // (the CSR-order overlaps, once they exist)
+    auto frac_a_counted = [&](hipStream_t st, bool cand) {
+        const index_t n_ent = cand ? n_pairs : M->nnz;
+        const double *keep = cand ? overlap.get() : nullptr;
+        const double *ov_e = cand ? overlap.get() : ov_csr;
+        col_cnt.resize(n_src);
+        col_ptr.resize(n_src + 1);
+        col_ent.resize(std::max<index_t>(n_ent, 1));
+        long_cols.resize(n_src);
+        AXIS_HIP_CHECK(hipMemsetAsync(col_cnt.get(), 0, sizeof(unsigned long long) * n_src, st));
+        if (n_ent > 0) {
+            if (cand)
+                col_count_kernel<<<grid_for(n_ent, 256), 256, 0, st>>>(cand_src.get(), keep, n_ent, col_cnt.get());
+            else
+                col_count_kernel<<<grid_for(n_ent, 256), 256, 0, st>>>(M->col.get(), keep, n_ent, col_cnt.get());
+            AXIS_LAUNCH_CHECK();
+        }
+        exclusive_scan_i64(reinterpret_cast<const index_t *>(col_cnt.get()), col_ptr.get(), n_src, st);
+        if (n_ent > 0) {
+            if (cand)
+                col_scatter_kernel<<<grid_for(n_ent, 256), 256, 0, st>>>(cand_src.get(), keep, n_ent, col_ptr.get(), col_cnt.get(), col_ent.get());
+            else
+                col_scatter_kernel<<<grid_for(n_ent, 256), 256, 0, st>>>(M->col.get(), keep, n_ent, col_ptr.get(), col_cnt.get(), col_ent.get());
+            AXIS_LAUNCH_CHECK();
+        }
+        frac_a_short_kernel<<<grid_for(n_src, kColBlock), kColBlock, 0, st>>>(col_ptr.get(), col_ent.get(), n_src, ov_e, sg.d_area.get(),
+                                                                             M->frac_a.get(), long_cols.get(), col_flags.get());
+        AXIS_LAUNCH_CHECK();
+        frac_a_long_kernel<<<1024, kColBlock, 0, st>>>(col_ptr.get(), col_ent.get(), long_cols.get(), col_flags.get(), ov_e, sg.d_area.get(),
+                                                       M->frac_a.get(), col_flags.get() + 1);
+        AXIS_LAUNCH_CHECK();
+    };
+    if (by_cand) {
+        fs = upload_stream();
+        AXIS_HIP_CHECK(hipEventCreateWithFlags(&fev.ready, hipEventDisableTiming));
+        AXIS_HIP_CHECK(hipEventCreateWithFlags(&fev.done, hipEventDisableTiming));
+        AXIS_HIP_CHECK(hipEventRecord(fev.ready, s));
+        AXIS_HIP_CHECK(hipStreamWaitEvent(fs, fev.ready, 0));
+        AXIS_HIP_CHECK(hipMemsetAsync(col_flags.get(), 0, 2 * sizeof(int), fs));
+        frac_a_counted(fs, true);
+        AXIS_HIP_CHECK(hipEventRecord(fev.done, fs));
+    } else {
+        AXIS_HIP_CHECK(hipMemsetAsync(col_flags.get(), 0, 2 * sizeof(int), s));
+    }
     // pairs grouped by row (every path but a proposer round 2 that appended
     // pairs): per-row nonzero counts give row_ptr, then each row is sorted by
     // column where it lies (one thread, one wave or one workgroup per row)
@@ -1563,36 +1636,9 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
             AXIS_LAUNCH_CHECK();
         }
     }
-    // frac_a: each column's CSR positions gathered by counting and summed in
-    // position order; a column of more than kColLds entries (a source cell
-    // under that many destination cells) sets the flag, and the stable
-    // global column sort below then gives every column's sum
-    DeviceBuffer<int> col_flags(2);  // long-column count, overflow
-    AXIS_HIP_CHECK(hipMemsetAsync(col_flags.get(), 0, 2 * sizeof(int), s));
-    const char *fa = std::getenv("AXIS_FRAC_A");  // "sort": the global column sort for every column
-    const bool by_count = n_src > 0 && nnz < (index_t(1) << 32) && !(fa && std::string(fa) == "sort");
-    if (by_count) {
-        DeviceBuffer<unsigned long long> cnt(n_src);
-        DeviceBuffer<index_t> cp(n_src + 1);
-        DeviceBuffer<std::uint32_t> cs(nz1);
-        DeviceBuffer<std::int32_t> long_cols(n_src);
-        AXIS_HIP_CHECK(hipMemsetAsync(cnt.get(), 0, sizeof(unsigned long long) * n_src, s));
-        if (nnz > 0) {
-            col_count_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(M->col.get(), nnz, cnt.get());
-            AXIS_LAUNCH_CHECK();
-        }
-        exclusive_scan_i64(reinterpret_cast<const index_t *>(cnt.get()), cp.get(), n_src, s);
-        if (nnz > 0) {
-            col_scatter_kernel<<<grid_for(nnz, 256), 256, 0, s>>>(M->col.get(), nnz, cp.get(), cnt.get(), cs.get());
-            AXIS_LAUNCH_CHECK();
-        }
-        frac_a_short_kernel<<<grid_for(n_src, kColBlock), kColBlock, 0, s>>>(cp.get(), cs.get(), n_src, ov.get(), sg.d_area.get(), M->frac_a.get(),
-                                                                            long_cols.get(), col_flags.get());
-        AXIS_LAUNCH_CHECK();
-        frac_a_long_kernel<<<1024, kColBlock, 0, s>>>(cp.get(), cs.get(), long_cols.get(), col_flags.get(), ov.get(), sg.d_area.get(),
-                                                      M->frac_a.get(), col_flags.get() + 1);
-        AXIS_LAUNCH_CHECK();
-    }
+    ov_csr = ov.get();
+    const bool by_count = by_cand || (n_src > 0 && nnz < (index_t(1) << 32) && !fa_sort);
+    if (by_count && !by_cand) frac_a_counted(s, false);
     // stable sort of the CSR entries by column alone (LSD radix), then one
     // thread per column run
     auto frac_a_sorted = [&](const double *area_src) {
@@ -1607,6 +1653,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
         }
     };
     if (!by_count) frac_a_sorted(sg.d_area.get());
+    if (by_cand) AXIS_HIP_CHECK(hipStreamWaitEvent(s, fev.done, 0));
     ev.rec(6, s);
     // the cell areas pass to the matrix (the geometry is done with them)
     M->area_a = std::move(sg.d_area);
