@@ -69,6 +69,9 @@ constexpr int kLongBlocks = 512;      // workgroups of grid_long_kernel (they lo
 constexpr index_t kCountBlocks = 2048;  // workgroups of grid_count_kernel (they loop over the cells)
 constexpr std::size_t kMaxLds = 64 * 1024;
 constexpr std::size_t kBigTables = 32 * 1024;  // LDS tables beyond this: the counting pass takes 1024-thread workgroups
+constexpr int kFillWin = 512;      // candidate-list window per wave of the filling pass (entries), small tables
+constexpr int kFillWinBig = 1024;  // the same beside tables of more than kBigTables (one 1024-thread workgroup per CU)
+constexpr std::size_t fill_lds(int block, int win) { return static_cast<std::size_t>(block / 64) * (win + 64) * sizeof(std::int32_t); }
 // per cell row: z-band suffix minimum and prefix maximum, longitude overhang,
 // radial band (distance of the boxes' xy-rectangles from the polar axis)
 constexpr int kRowFields = 5;
@@ -408,9 +411,15 @@ __device__ inline bool boxes_meet(const float *a, const float *b) {
 
 __device__ inline std::int32_t cell_of(const GridView &G, std::int64_t k) { return G.cell_at ? G.cell_at[k] : static_cast<std::int32_t>(k); }
 
-template <bool kFill>
+// orders the LDS accesses of a wave's lanes around it (a wave's LDS
+// operations complete in program order; this keeps the compiler's order)
+__device__ inline void wave_lds_fence() { asm volatile("" ::: "memory"); }
+
+// kFill: every hit goes to put(position, source cell), positions from
+// cand_off[d] on
+template <bool kFill, class Put>
 __device__ inline void count_one(const GridView &G, const Tables &T, const float *dst_box, index_t d, index_t *count, const index_t *cand_off,
-                                 std::int32_t *cand_src, std::int32_t *pair_row, std::int32_t *long_rows, int *n_long) {
+                                 std::int32_t *long_rows, int *n_long, Put &&put) {
     Dst D;
     dst_setup(G, T, dst_box, d, D);
     int r0[kMaxLonRanges], r1[kMaxLonRanges];
@@ -448,10 +457,7 @@ __device__ inline void count_one(const GridView &G, const Tables &T, const float
 #pragma unroll
                 for (int u = 0; u < kUnroll; ++u)
                     if (cs[u] >= 0 && boxes_meet(bs[u], D.bx)) {
-                        if (kFill) {
-                            cand_src[pos] = cs[u];
-                            pair_row[pos] = static_cast<std::int32_t>(d);
-                        }
+                        if (kFill) put(pos, cs[u]);
                         ++pos;
                     }
             }
@@ -459,16 +465,64 @@ __device__ inline void count_one(const GridView &G, const Tables &T, const float
     if (!kFill) count[d] = pos;
 }
 
-// one lane per destination cell; workgroups loop over the cells (the tables
-// are loaded once per workgroup)
-template <bool kFill, int BLOCK>
+// counting: one lane per destination cell; workgroups loop over the cells
+// (the tables are loaded once per workgroup)
+template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) grid_count_kernel(GridView G, const float *dst_box, index_t n_dst, index_t *count,
-                                                                 const index_t *cand_off, std::int32_t *cand_src, std::int32_t *pair_row,
                                                                  std::int32_t *long_rows, int *n_long) {
     extern __shared__ double s_tab[];
     const Tables T = load_tables(G, s_tab);
     for (index_t d = static_cast<index_t>(blockIdx.x) * BLOCK + threadIdx.x; d < n_dst; d += static_cast<index_t>(gridDim.x) * BLOCK)
-        count_one<kFill>(G, T, dst_box, d, count, cand_off, cand_src, pair_row, long_rows, n_long);
+        count_one<false>(G, T, dst_box, d, count, nullptr, long_rows, n_long, [](index_t, std::int32_t) {});
+}
+
+// filling: one lane per destination cell as in the count, the hits of a
+// wave's 64 cells staged in a window of W entries in LDS (the cells' ranges
+// of the candidate list are consecutive, so position - the first cell's
+// offset indexes the window), then written out in position order by the
+// wave's lanes together (coalesced), each entry's row found by a binary
+// search over the 64 offsets.  Hits past the window are stored directly.
+// Entries of the listed cells (their hits come from grid_long_kernel) stay
+// marked -1 in the window and are not written.
+template <int BLOCK, int W>
+__global__ void __launch_bounds__(BLOCK) grid_fill_kernel(GridView G, const float *dst_box, index_t n_dst, const index_t *cand_off,
+                                                                std::int32_t *cand_src, std::int32_t *pair_row, unsigned tab_bytes) {
+    extern __shared__ double s_tab[];
+    const Tables T = load_tables(G, s_tab);
+    const int lane = static_cast<int>(threadIdx.x & 63), wv = static_cast<int>(threadIdx.x >> 6);
+    std::int32_t *win = reinterpret_cast<std::int32_t *>(reinterpret_cast<char *>(s_tab) + tab_bytes) + wv * (W + 64);
+    std::int32_t *offs = win + W;
+    for (index_t d0 = static_cast<index_t>(blockIdx.x) * BLOCK + wv * 64; d0 < n_dst; d0 += static_cast<index_t>(gridDim.x) * BLOCK) {
+        const index_t d = d0 + lane;
+        const bool in = d < n_dst;
+        const index_t base = cand_off[d0];
+        const int span = static_cast<int>(min<index_t>(cand_off[min<index_t>(d0 + 64, n_dst)] - base, W));
+        for (int i = lane; i < span; i += 64) win[i] = -1;
+        offs[lane] = in ? static_cast<std::int32_t>(min<index_t>(cand_off[d] - base, index_t(0x7fffffff))) : 0x7fffffff;
+        wave_lds_fence();
+        if (in)
+            count_one<true>(G, T, dst_box, d, nullptr, cand_off, nullptr, nullptr, [&](index_t pos, std::int32_t c) {
+                const index_t k = pos - base;
+                if (k < W) {
+                    win[k] = c;
+                } else {
+                    cand_src[pos] = c;
+                    pair_row[pos] = static_cast<std::int32_t>(d);
+                }
+            });
+        wave_lds_fence();
+        for (int i = lane; i < span; i += 64) {
+            const std::int32_t c = win[i];
+            if (c < 0) continue;
+            int r = 0;  // the last lane whose offset is at most i
+#pragma unroll
+            for (int step = 32; step >= 1; step >>= 1)
+                if (offs[r + step] <= i) r += step;
+            cand_src[base + i] = c;
+            pair_row[base + i] = static_cast<std::int32_t>(d0 + r);
+        }
+        wave_lds_fence();
+    }
 }
 
 // One wave per listed destination cell (more than kLaneItems grid cells to
@@ -485,7 +539,6 @@ constexpr int kHeavyItems = 512;  // more grid cells than this: a workgroup per 
 
 // a compiler barrier: LDS written by some lanes of a wave and read by others
 // after it, the wave's LDS instructions being executed in program order
-__device__ inline void wave_lds_fence() { asm volatile("" ::: "memory"); }
 
 template <bool kFill>
 __global__ void __launch_bounds__(kLongBlock) grid_long_kernel(GridView G, const float *dst_box, const std::int32_t *long_rows, const int *n_long,
@@ -783,11 +836,11 @@ index_t grid_query(const GridIndex &G, const float *dst_boxes, index_t n_dst, De
         // counting: 1024-thread workgroups when the tables leave room for
         // fewer than five 256-thread workgroups per CU (more waves resident)
         if (lds > kBigTables)
-            grid_count_kernel<false, kLongBlock><<<count_blocks(n_dst, kLongBlock), kLongBlock, lds, s>>>(
-                V, dst_boxes, n_dst, count.get(), nullptr, nullptr, nullptr, long_rows.get(), n_long);
+            grid_count_kernel<kLongBlock><<<count_blocks(n_dst, kLongBlock), kLongBlock, lds, s>>>(V, dst_boxes, n_dst, count.get(),
+                                                                                                  long_rows.get(), n_long);
         else
-            grid_count_kernel<false, kQueryBlock><<<count_blocks(n_dst, kQueryBlock), kQueryBlock, lds, s>>>(
-                V, dst_boxes, n_dst, count.get(), nullptr, nullptr, nullptr, long_rows.get(), n_long);
+            grid_count_kernel<kQueryBlock><<<count_blocks(n_dst, kQueryBlock), kQueryBlock, lds, s>>>(V, dst_boxes, n_dst, count.get(),
+                                                                                                     long_rows.get(), n_long);
         AXIS_LAUNCH_CHECK();
         grid_long_kernel<false><<<kLongBlocks, kLongBlock, lds, s>>>(V, dst_boxes, long_rows.get(), n_long, count.get(), nullptr, nullptr, nullptr,
                                                                       heavy_rows.get(), n_heavy);
@@ -808,8 +861,15 @@ index_t grid_query(const GridIndex &G, const float *dst_boxes, index_t n_dst, De
     cand_src.resize(std::max<index_t>(total, 1));
     pair_row.resize(std::max<index_t>(total, 1));
     if (n_dst > 0 && total > 0) {
-        grid_count_kernel<true, kQueryBlock><<<count_blocks(n_dst, kQueryBlock), kQueryBlock, lds, s>>>(
-            V, dst_boxes, n_dst, nullptr, cand_off.get(), cand_src.get(), pair_row.get(), nullptr, nullptr);
+        // the window beside the tables: 256-thread workgroups with small
+        // tables, else 1024-thread ones (as the count)
+        const unsigned tab = static_cast<unsigned>(lds);
+        if (lds > kBigTables)
+            grid_fill_kernel<kLongBlock, kFillWinBig><<<count_blocks(n_dst, kLongBlock), kLongBlock, lds + fill_lds(kLongBlock, kFillWinBig), s>>>(
+                V, dst_boxes, n_dst, cand_off.get(), cand_src.get(), pair_row.get(), tab);
+        else
+            grid_fill_kernel<kQueryBlock, kFillWin><<<count_blocks(n_dst, kQueryBlock), kQueryBlock, lds + fill_lds(kQueryBlock, kFillWin), s>>>(
+                V, dst_boxes, n_dst, cand_off.get(), cand_src.get(), pair_row.get(), tab);
         AXIS_LAUNCH_CHECK();
         grid_long_kernel<true><<<kLongBlocks, kLongBlock, lds, s>>>(V, dst_boxes, long_rows.get(), n_long, nullptr, cand_off.get(), cand_src.get(),
                                                                      pair_row.get(), nullptr, nullptr);
