@@ -906,7 +906,13 @@ double host_pair_overlap(const HostSide &S, const HostSide &D, index_t j, index_
 // min / max of the cells' mean vertex longitude as order-preserving integer
 // keys: grid-stride over the cells, wave then workgroup reduction, one atomic
 // per workgroup (per-wave atomics on two addresses serialise in L2)
-constexpr int kRangeBlock = 256;
+constexpr int kRangeBlock = 1024;
+constexpr index_t kRangeGrid = 256;  // workgroups: one per CU (each ends in an atomic pair on the same two words)
+
+// lows start at the largest key, highs at zero
+__global__ void range_init_kernel(unsigned long long *r) {
+    if (threadIdx.x < 4) r[threadIdx.x] = (threadIdx.x & 1) ? 0ull : ~0ull;
+}
 
 __global__ void __launch_bounds__(kRangeBlock) lon_range_kernel(const index_t *off, const index_t *ind, const double *lon, index_t n_cells,
                                                                 unsigned long long *mn, unsigned long long *mx) {
@@ -916,7 +922,16 @@ __global__ void __launch_bounds__(kRangeBlock) lon_range_kernel(const index_t *o
         index_t b = off[c], e = off[c + 1];
         if (e > b) {
             double s = 0.0;
-            for (index_t k = b; k < e; ++k) s += lon[ind[k]];
+            if (e - b == 4) {  // a quad: its four indices, then its four longitudes, loaded together
+                const index_t i0 = ind[b], i1 = ind[b + 1], i2 = ind[b + 2], i3 = ind[b + 3];
+                const double l0 = lon[i0], l1 = lon[i1], l2 = lon[i2], l3 = lon[i3];
+                s += l0;
+                s += l1;
+                s += l2;
+                s += l3;
+            } else {
+                for (index_t k = b; k < e; ++k) s += lon[ind[k]];
+            }
             double cx = s / static_cast<double>(e - b);
             std::uint64_t bits = static_cast<std::uint64_t>(__double_as_longlong(cx));
             const unsigned long long key = (bits >> 63) ? ~bits : (bits | 0x8000000000000000ull);
@@ -955,15 +970,13 @@ double key_to_double(unsigned long long k) {
 // longitude ranges of the vertices of both meshes' cells: one pair of
 // reductions into r[4] (an empty mesh leaves lo = all ones, hi = 0)
 void lon_ranges_launch(const Mesh &a, const Mesh &b, unsigned long long *r, hipStream_t s) {
-    // lows start at the largest key, highs at zero
-    AXIS_HIP_CHECK(hipMemsetAsync(r, 0, sizeof(unsigned long long) * 4, s));
-    AXIS_HIP_CHECK(hipMemsetAsync(r, 0xFF, sizeof(unsigned long long), s));
-    AXIS_HIP_CHECK(hipMemsetAsync(r + 2, 0xFF, sizeof(unsigned long long), s));
+    range_init_kernel<<<1, 64, 0, s>>>(r);
+    AXIS_LAUNCH_CHECK();
     const Mesh *ms[2] = {&a, &b};
     for (int k = 0; k < 2; ++k) {
         const Mesh &m = *ms[k];
         if (m.n_cells() == 0) continue;
-        const unsigned grid = static_cast<unsigned>(std::min<index_t>((m.n_cells() + kRangeBlock - 1) / kRangeBlock, 256));
+        const unsigned grid = static_cast<unsigned>(std::min<index_t>((m.n_cells() + kRangeBlock - 1) / kRangeBlock, kRangeGrid));
         lon_range_kernel<<<grid, kRangeBlock, 0, s>>>(m.d_offsets.get(), m.d_indices.get(), m.d_coords.get(), m.n_cells(), r + 2 * k, r + 2 * k + 1);
         AXIS_LAUNCH_CHECK();
     }
