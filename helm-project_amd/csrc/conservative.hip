@@ -447,8 +447,9 @@ __global__ void frac_a_kernel(const std::uint32_t *key, const std::uint32_t *per
 }
 
 // frac_a without a global sort.  Each source column's entries are gathered
-// by counting (column counts, their scan, an atomic scatter that leaves each
-// column's entry indices in arbitrary order), then each column sorts its few
+// by counting (column counts whose atomics give each entry a rank within its
+// column, their scan, a scatter to offset + rank that leaves each column's
+// entry indices in arbitrary order), then each column sorts its few
 // indices and sums in ascending index, which is the row order of
 // frac_a_kernel's stable sort: the same additions in the same order.  The
 // entries are the CSR's, or, where the candidate pairs are grouped by row
@@ -458,21 +459,20 @@ constexpr int kColNet = 16;       // columns of up to kColNet entries: one threa
 constexpr int kColLds = 4096;     // longer columns: one workgroup, a bitonic sort in LDS
 constexpr int kColBlock = 256;
 
-// keep (optional): entries of keep[e] > 0 only
+// keep (optional): entries of keep[e] > 0 only.  Each entry takes the
+// column's count before its own increment as its rank (an arbitrary order
+// within the column: the column's entries are sorted afterwards).
 template <class C>
-__global__ void col_count_kernel(const C *col, const double *keep, index_t n, unsigned long long *cnt) {
+__global__ void col_count_kernel(const C *col, const double *keep, index_t n, unsigned long long *cnt, std::uint32_t *rank) {
     const index_t e = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (e < n && (!keep || keep[e] > 0.0)) atomicAdd(&cnt[col[e]], 1ull);
+    if (e < n && (!keep || keep[e] > 0.0)) rank[e] = static_cast<std::uint32_t>(atomicAdd(&cnt[col[e]], 1ull));
 }
 
-// cnt[c] counts down to zero (and is zero again afterwards)
 template <class C>
-__global__ void col_scatter_kernel(const C *col, const double *keep, index_t n, const index_t *cp, unsigned long long *cnt, std::uint32_t *cs) {
+__global__ void col_scatter_kernel(const C *col, const double *keep, index_t n, const index_t *cp, const std::uint32_t *rank, std::uint32_t *cs) {
     const index_t e = static_cast<index_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= n || (keep && !(keep[e] > 0.0))) return;
-    const index_t c = col[e];
-    const unsigned long long left = atomicAdd(&cnt[c], ~0ull);  // (adds -1)
-    cs[cp[c] + static_cast<index_t>(left) - 1] = static_cast<std::uint32_t>(e);
+    cs[cp[col[e]] + rank[e]] = static_cast<std::uint32_t>(e);
 }
 
 __global__ void __launch_bounds__(kColBlock) frac_a_short_kernel(const index_t *cp, const std::uint32_t *cs, index_t n_src, const double *ov,
@@ -1505,6 +1505,7 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     DeviceBuffer<index_t> col_ptr;
     DeviceBuffer<std::uint32_t> col_ent;
     DeviceBuffer<std::int32_t> long_cols;
+    DeviceBuffer<std::uint32_t> col_rank;
     const double *ov_csr = nullptr;  // (the CSR-order overlaps, once they exist)
     auto frac_a_counted = [&](hipStream_t st, bool cand) {
         const index_t n_ent = cand ? n_pairs : M->nnz;
@@ -1513,21 +1514,22 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
         col_cnt.resize(n_src);
         col_ptr.resize(n_src + 1);
         col_ent.resize(std::max<index_t>(n_ent, 1));
+        col_rank.resize(std::max<index_t>(n_ent, 1));
         long_cols.resize(n_src);
         AXIS_HIP_CHECK(hipMemsetAsync(col_cnt.get(), 0, sizeof(unsigned long long) * n_src, st));
         if (n_ent > 0) {
             if (cand)
-                col_count_kernel<<<grid_for(n_ent, 256), 256, 0, st>>>(cand_src.get(), keep, n_ent, col_cnt.get());
+                col_count_kernel<<<grid_for(n_ent, 256), 256, 0, st>>>(cand_src.get(), keep, n_ent, col_cnt.get(), col_rank.get());
             else
-                col_count_kernel<<<grid_for(n_ent, 256), 256, 0, st>>>(M->col.get(), keep, n_ent, col_cnt.get());
+                col_count_kernel<<<grid_for(n_ent, 256), 256, 0, st>>>(M->col.get(), keep, n_ent, col_cnt.get(), col_rank.get());
             AXIS_LAUNCH_CHECK();
         }
         exclusive_scan_i64(reinterpret_cast<const index_t *>(col_cnt.get()), col_ptr.get(), n_src, st);
         if (n_ent > 0) {
             if (cand)
-                col_scatter_kernel<<<grid_for(n_ent, 256), 256, 0, st>>>(cand_src.get(), keep, n_ent, col_ptr.get(), col_cnt.get(), col_ent.get());
+                col_scatter_kernel<<<grid_for(n_ent, 256), 256, 0, st>>>(cand_src.get(), keep, n_ent, col_ptr.get(), col_rank.get(), col_ent.get());
             else
-                col_scatter_kernel<<<grid_for(n_ent, 256), 256, 0, st>>>(M->col.get(), keep, n_ent, col_ptr.get(), col_cnt.get(), col_ent.get());
+                col_scatter_kernel<<<grid_for(n_ent, 256), 256, 0, st>>>(M->col.get(), keep, n_ent, col_ptr.get(), col_rank.get(), col_ent.get());
             AXIS_LAUNCH_CHECK();
         }
         frac_a_short_kernel<<<grid_for(n_src, kColBlock), kColBlock, 0, st>>>(col_ptr.get(), col_ent.get(), n_src, ov_e, sg.d_area.get(),
