@@ -386,6 +386,8 @@ bool node_xyz_trig(const Mesh &m, double *d_out, hipStream_t st, const std::atom
 /// A non-blocking stream of the current device for host-to-device uploads
 /// beside the caller's stream (created once per device, default priority).
 hipStream_t upload_stream();
+/// A second such stream: the long rows of the CSR assembly beside the short ones.
+hipStream_t rows_stream();
 /// xyz of one node as node_xyz_host computes it (same bits).
 void node_xyz_one(const Mesh &m, index_t node, double out[3]);
 /// Host-only necessary condition for node_xyz_device_eligible: degree

@@ -1604,27 +1604,42 @@ std::unique_ptr<InterpolationMatrix> generate_conservative(const Mesh &src, cons
     DeviceBuffer<std::uint32_t> perm(nz1);
     bool assembled = false;
     if (grouped) {
+        // the listed rows on a second stream, beside the short rows (they
+        // write other rows); this stream waits for them
+        hipStream_t ls = s;
+        FracEvents lev;
+        if (h_max_row > kShortRow) {
+            ls = rows_stream();
+            AXIS_HIP_CHECK(hipEventCreateWithFlags(&lev.ready, hipEventDisableTiming));
+            AXIS_HIP_CHECK(hipEventCreateWithFlags(&lev.done, hipEventDisableTiming));
+            AXIS_HIP_CHECK(hipEventRecord(lev.ready, s));
+            AXIS_HIP_CHECK(hipStreamWaitEvent(ls, lev.ready, 0));
+        }
         const unsigned grid = static_cast<unsigned>(std::min<index_t>((n_dst + 255) / 256, 2048));
         short_rows_kernel<<<grid, 256, 0, s>>>(cand_off.get(), M->row_ptr.get(), n_dst, cand_src.get(), overlap.get(), dg.d_area.get(), frac_area,
                                                M->col.get(), M->val.get(), ov.get(), M->frac_b.get());
         AXIS_LAUNCH_CHECK();
         if (h_max_row > kShortRow) {  // the listed rows (their counts stay on the device)
-            long_rows_kernel<kMidRow, kMidRow><<<2048, kMidRow, 0, s>>>(long_rows.get(), n_long.get(), false, kShortRow, n_dst, cand_off.get(),
+            long_rows_kernel<kMidRow, kMidRow><<<2048, kMidRow, 0, ls>>>(long_rows.get(), n_long.get(), false, kShortRow, n_dst, cand_off.get(),
                                                                         M->row_ptr.get(), cand_src.get(), overlap.get(), dg.d_area.get(),
                                                                         frac_area, M->col.get(), M->val.get(), ov.get(), M->frac_b.get());
             AXIS_LAUNCH_CHECK();
         }
         if (h_max_row > kMidRow) {
-            long_rows_kernel<kLongRowMax, kLongRowBlock><<<512, kLongRowBlock, 0, s>>>(
+            long_rows_kernel<kLongRowMax, kLongRowBlock><<<512, kLongRowBlock, 0, ls>>>(
                 long_rows.get(), n_long.get() + 1, true, kMidRow, n_dst, cand_off.get(), M->row_ptr.get(), cand_src.get(), overlap.get(),
                 dg.d_area.get(), frac_area, M->col.get(), M->val.get(), ov.get(), M->frac_b.get());
             AXIS_LAUNCH_CHECK();
         }
         if (h_max_row > kLongRowMax) {
-            long_rows_kernel<kHugeRowMax, kHugeRowBlock><<<256, kHugeRowBlock, 0, s>>>(
+            long_rows_kernel<kHugeRowMax, kHugeRowBlock><<<256, kHugeRowBlock, 0, ls>>>(
                 long_rows.get(), n_long.get() + 1, true, kLongRowMax, n_dst, cand_off.get(), M->row_ptr.get(), cand_src.get(), overlap.get(),
                 dg.d_area.get(), frac_area, M->col.get(), M->val.get(), ov.get(), M->frac_b.get());
             AXIS_LAUNCH_CHECK();
+        }
+        if (ls != s) {
+            AXIS_HIP_CHECK(hipEventRecord(lev.done, ls));
+            AXIS_HIP_CHECK(hipStreamWaitEvent(s, lev.done, 0));
         }
         assembled = true;
     }

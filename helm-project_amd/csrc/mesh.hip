@@ -1100,19 +1100,26 @@ bool node_xyz_trig(const Mesh &m, double *d_out, hipStream_t st, const std::atom
     return true;
 }
 
-hipStream_t upload_stream() {
+namespace {
+// the current device's side stream number k (non-blocking, default priority,
+// created on first use and kept until the process ends)
+hipStream_t side_stream(int k) {
     static std::mutex mu;
-    static std::map<int, hipStream_t> streams;
+    static std::map<std::pair<int, int>, hipStream_t> streams;
     int dev = 0;
     AXIS_HIP_CHECK(hipGetDevice(&dev));
     std::lock_guard<std::mutex> g(mu);
-    auto it = streams.find(dev);
+    auto it = streams.find({dev, k});
     if (it != streams.end()) return it->second;
     hipStream_t st = nullptr;
     AXIS_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    streams[dev] = st;
+    streams[{dev, k}] = st;
     return st;
 }
+}  // namespace
+
+hipStream_t upload_stream() { return side_stream(0); }
+hipStream_t rows_stream() { return side_stream(1); }
 
 bool node_xyz_host(const Mesh &m, MeshGeom &g, const std::atomic<bool> *cancel, hipStream_t up, hipEvent_t done, double *d_out) {
     const index_t n_nodes = m.n_nodes();
